@@ -1,0 +1,96 @@
+/*
+ * tc_oracle.h -- CPU restatement of bahvalo/TurbChannel's time stepper.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the parity checker for the HIP
+ * engine in turbchannel_amd/. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product never does.
+ *
+ * Everything is FP64 and follows the reference's CPU (OpenMP) path:
+ *   staggered/func_global.h, staggered/main.cpp:12-185,
+ *   collocated_AB3/func_global.h, collocated_AB3/func_fluxes.h,
+ *   collocated_AB3/main.cpp:197-301.
+ * Index helpers are generalised from the reference's power-of-2 bitmask form
+ * (staggered/data_global_inc.h:4-9) to any NX, NY; or_index_selfcheck()
+ * compares both forms at power-of-2 sizes.
+ *
+ * Field layout is the reference's: real3 AoS, cell index
+ * in = ix + NX*(iy + NY*iz), ix fastest (staggered/func_cuda.cpp:68).
+ */
+#ifndef TC_ORACLE_H
+#define TC_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_SCHEME_HW = 0, OR_SCHEME_VV = 1, OR_SCHEME_AB3 = 2 };
+enum { OR_KIND_VEL = 0, OR_KIND_FLUX = 1, OR_KIND_PRES = 2, OR_KIND_NUT = 3 };
+
+typedef struct or_params {
+    int scheme;        /* OR_SCHEME_* */
+    int nx, ny, nz;    /* staggered: nz cells; AB3: nz nodes (walls included) */
+    int turb_visc;     /* 1: nu_t array + Smagorinsky-van Driest (ENABLE_TURB_VISC) */
+    int visc_type;     /* staggered: 0 CROSS, 1 NAIVE; AB3: 0 FD Laplacian, 1 Galerkin */
+    int stab_type;     /* AB3: 0 or 1 (collocated_AB3/func_global.h:424-449) */
+    int fd_mode;       /* AB3: FDmode (collocated_AB3/config.h:55) */
+    double visc, sour_dpdx, inv_length_scale;
+    double hx, hy, xmin, ymin;
+    double cfr_ubulk;  /* >0: hold the bulk velocity at this value (new numerics) */
+} or_params;
+
+typedef struct or_solver or_solver;
+
+/* zz: staggered nz+1 face coordinates, AB3 nz node coordinates. NULL on error. */
+or_solver* or_create(const or_params* p, const double* zz);
+void or_destroy(or_solver* s);
+const char* or_last_error(void);
+
+/* Mesh-derived arrays (staggered/main.cpp:112-156; collocated_AB3/main.cpp:295-300).
+ * name: "ZZ","DZ","DZW","GC1","GC2","GCZ","MAT_A","MAT_D","KERG". Returns length, -1 if absent. */
+int or_mesh_array(const or_solver* s, const char* name, double* out, int cap);
+double or_kerg_norm2(const or_solver* s);
+/* FillMatrix (staggered/main.cpp:12-48): NZ rows of 7 entries. */
+void or_fill_matrix(const or_solver* s, double* M);
+/* solve_banded (staggered/main.cpp:73-109). Returns its error code. */
+int or_solve_banded(int N, int NB, double* M, double* x, double tiny);
+
+/* Field operators on caller arrays (real3 AoS = 3*N3D doubles). */
+void or_explicit_term(or_solver* s, const double* u, double* kuhat);
+void or_flux_term(or_solver* s, const double* u, int do_conv, int do_visc, double* kuhat);
+void or_visc_flux_naive(or_solver* s, const double* u, double* f);  /* staggered ViscFlux<NAIVE> */
+void or_apply_gradient(or_solver* s, const double* p, double* g);
+void or_apply_divergence(or_solver* s, const double* v, double* dv);
+void or_solve_pressure(or_solver* s, double* x, const double* f);   /* LinearSystemSolveFFT */
+void or_unapply_gradient(or_solver* s, const double* v, double* p);
+void or_normalize_pressure(or_solver* s, double* p);
+void or_recalc_turb_visc(or_solver* s, const double* u);            /* writes the internal nu_t */
+void or_calc_abs_s(or_solver* s, const double* u, double* absS);    /* staggered CalcAbsS */
+void or_integral(or_solver* s, const double* u, double out[3]);
+double or_kinetic_energy(or_solver* s, const double* u);
+/* Profiles: staggered 12 doubles/plane (u3, uu6, nu_t, p, pp); AB3 10 (u3, uu6, nu_t). */
+int or_profiles(or_solver* s, const double* u, const double* p, double* av);
+double or_cfl(or_solver* s, const double* u, double tau, int mode);
+
+/* Ring-buffer state, as the reference's data::vel/flux/pres[4] and nu_t. */
+int or_set_field(or_solver* s, int kind, int slot, const double* src);
+int or_get_field(or_solver* s, int kind, int slot, double* dst);
+int or_step(or_solver* s, double t, double tau, int cur, int bdf_order);
+/* Bulk-flow correction used by or_step when cfr_ubulk > 0 (returns the shift applied). */
+double or_cfr_correct(or_solver* s, double* u);
+
+/* New numerics: tanh wall-normal mesh and the seeded initial field (before projection). */
+void or_tanh_mesh(int n_intervals, double gamma, double* zz);
+void or_initial_field(or_solver* s, unsigned long long seed, double ubulk, double noise_rms, double* u);
+
+/* Plain complex FFT (for pinning against numpy): n points, sign -1 forward / +1 backward. */
+void or_fft(int n, const double* in_re_im, double* out_re_im, int sign);
+
+/* Compares generalised and power-of-2 bitmask index helpers; returns #mismatches. */
+int or_index_selfcheck(int nx, int ny, int nz);
+
+void or_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
