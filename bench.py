@@ -1,0 +1,206 @@
+"""Benchmark of the MI355X time stepper on one GPU.
+
+Default workload: the fourth-order (Verstappen-Veldman) staggered DNS on a
+192 x 160 x 128 grid (x, span, wall-normal), tanh wall-normal stretching 1.8,
+box 2 pi x pi x 2, bulk Re 5600 held by the constant-flow-rate correction,
+seeded divergence-free initial field, dt = 1e-3, FP64. A step is one
+tc_step (explicit term + BDF3/EX3, pressure solve, projection); the fields are
+resident in HBM for the whole timed region.
+
+Prints one JSON line with the metric (grid-point steps per second), the
+roofline of the dominant kernel (measured live with HIP events on the engine's
+stream) and the CPU baseline (the FP64 OpenMP oracle on the host cores, on a
+bounded number of steps of the same workload).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import turbchannel_amd as tca  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+# Algorithmic reals moved per grid point by each kernel class (each array
+# element counted once; halos and cache reuse not counted).
+REALS_PER_POINT = {
+    "flux": 21,      # reads u_n, u_n-1, u_n-2, F_n-1, F_n-2 (15); writes F_n, r (6)
+    "div": 4,        # reads r (3); writes div (1)
+    "fft_x": 2,      # reads div (1); writes the half spectrum (~1)
+    "fft_y": 2,
+    "zsolve": 4,     # two passes over the spectrum: read + write each
+    "ifft_y": 2,
+    "ifft_x": 2,
+    "project": 7,    # reads p (1), r (3); writes u_n+1 (3)
+}
+
+WORKLOADS = {
+    # name: scheme, nx, ny, nz, gamma, Re_b, Re_tau, dt
+    "vv_192x160x128": (tca.SCHEME_VV, 192, 160, 128, 1.8, 5600.0, 180.0, 1e-3),
+    "hw_64x64x64": (tca.SCHEME_HW, 64, 64, 64, 1.8, 5600.0, 180.0, 2e-3),
+    "ab3_384x384x192": (tca.SCHEME_AB3, 384, 384, 192, 2.0, 13750.0, 395.0, 5e-4),
+    "vv_384x384x192": (tca.SCHEME_VV, 384, 384, 192, 2.0, 13750.0, 395.0, 5e-4),
+}
+
+
+def problem(name, precision):
+    scheme, nx, ny, nz, gamma, re_b, re_tau, dt = WORKLOADS[name]
+    if scheme == tca.SCHEME_AB3:
+        k = np.arange(nz)
+        zz = 1.0 + np.tanh(gamma * (2.0 * k / (nz - 1) - 1.0)) / math.tanh(gamma)
+    else:
+        k = np.arange(nz + 1)
+        zz = 1.0 + np.tanh(gamma * (2.0 * k / nz - 1.0)) / math.tanh(gamma)
+    zz[0], zz[-1] = 0.0, 2.0
+    visc = 2.0 / re_b
+    u_tau = re_tau * visc
+    kw = dict(visc=visc, sour_dpdx=u_tau * u_tau, inv_length_scale=u_tau / visc, hx=2 * math.pi / nx,
+              hy=math.pi / ny, cfr_ubulk=1.0, precision=precision)
+    if scheme == tca.SCHEME_AB3:
+        kw.update(stab_type=1, fd_mode=1, visc_type=0)
+    return scheme, nx, ny, nz, zz, kw, dt
+
+
+def cpu_baseline(name, seconds_target=15.0):
+    """FP64 OpenMP oracle on the host cores, on a bounded number of steps."""
+    import oracle
+
+    scheme, nx, ny, nz, zz, kw, dt = problem(name, tca.FP64)
+    kw.pop("precision")
+    threads = min(os.cpu_count() or 1, 16)
+    oracle.set_threads(threads)
+    sol = oracle.Solver(scheme, nx, ny, nz, zz, **kw)
+    u0 = sol.project(sol.initial_field(seed=42))
+    sol.set_field(oracle.KIND_VEL, 0, u0)
+    cur = 0
+    for it in range(3):  # order ramp, untimed
+        sol.step(it * dt, dt, cur, it + 1)
+        cur = (cur + 1) % 4
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        sol.step(n * dt, dt, cur, 3)
+        cur = (cur + 1) % 4
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds_target or n >= 50:
+            break
+    return {"value": nx * ny * nz * n / el, "unit": "grid-pt steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} BDF3 steps of {name} (FP64 oracle, OpenMP {threads} threads, {el:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="vv_192x160x128", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    precision = tca.FP64 if args.precision == "fp64" else tca.FP32_BULK
+    scheme, nx, ny, nz, zz, kw, dt = problem(args.workload, precision)
+    n3d = nx * ny * nz
+    eng = tca.Engine(scheme, nx, ny, nz, zz, device=0, **kw)
+    eng.initial_field(0, seed=42, ubulk=1.0, noise_rms=0.1)
+    cur = 0
+    for it in range(max(args.warmup, 3)):
+        eng.step(it * dt, dt, cur, min(it + 1, 3))
+        cur = (cur + 1) % 4
+    eng.sync()
+
+    try:
+        import torch
+
+        sync_torch = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only
+        sync_torch = lambda: None  # noqa: E731
+
+    # timed region 1: steps only, with per-kernel HIP events on the engine's stream
+    eng.timing_reset()
+    eng.timing(True)
+    eng.sync()
+    sync_torch()
+    t0 = time.perf_counter()
+    for it in range(args.steps):
+        eng.step(it * dt, dt, cur, 3)
+        cur = (cur + 1) % 4
+    eng.sync()
+    sync_torch()
+    el = time.perf_counter() - t0
+    eng.timing(False)
+    kt = eng.timing_read()
+
+    # timed region 2: steps plus the reference loop's per-step diagnostics
+    # (flow rate, plane statistics, CFL: staggered/main.cpp:312-338)
+    nd = max(1, args.steps // 4)
+    eng.sync()
+    t1 = time.perf_counter()
+    for it in range(nd):
+        eng.step(it * dt, dt, cur, 3)
+        cur = (cur + 1) % 4
+        eng.integral((tca.VEL, cur))
+        eng.profiles((tca.VEL, cur), cur)
+        eng.cfl((tca.VEL, cur), dt, 3)
+    eng.sync()
+    el_diag = time.perf_counter() - t1
+
+    rsz = 8 if precision == tca.FP64 else 4
+    per_kernel = {}
+    for k, (ms, n) in kt.items():
+        if n == 0:
+            continue
+        avg = ms / n
+        d = {"avg_us": round(1e3 * avg, 2), "launches": n}
+        if k in REALS_PER_POINT:
+            gbs = REALS_PER_POINT[k] * rsz * n3d / (avg * 1e-3) / 1e9
+            d["gbs"] = round(gbs, 1)
+            d["frac_hbm"] = round(gbs / PEAK_HBM_GBS, 4)
+        per_kernel[k] = d
+    dom = max((k for k in per_kernel if k in REALS_PER_POINT), key=lambda k: kt[k][0])
+    dom_avg_s = kt[dom][0] / kt[dom][1] * 1e-3
+    achieved = REALS_PER_POINT[dom] * rsz * n3d / dom_avg_s / 1e9
+
+    value = n3d * args.steps / el
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.workload, args.cpu_seconds)
+    rec = {
+        "metric": "grid-pt steps/s",
+        "value": value,
+        "unit": "grid-pt steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == tca.FP64 else "f32 (f64 low-mode solves, normalisation, flow rate, statistics)",
+        "data": "synthetic (seeded divergence-free initial field)",
+        "config": {"workload": args.workload, "scheme": ["HW", "VV", "AB3"][scheme], "grid": [nx, ny, nz],
+                   "dt": dt, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "with_diagnostics": {"value": n3d * nd / el_diag, "ms_per_step": 1e3 * el_diag / nd, "steps": nd},
+        "kernels": per_kernel,
+    }
+    eng.close()
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,131 @@
+/*
+ * turbchannel.h -- C ABI of the MI355X time stepper for TurbChannel's
+ * staggered (Harlow-Welch 2nd order, Verstappen-Veldman 4th order) and
+ * collocated BDF3/extrapolation solvers.
+ *
+ * One opaque engine per device. It owns every device buffer, as the
+ * reference's data_cuda::data does (staggered/func_global.h:944-974).
+ * Host buffers use the reference's layout: real3 AoS, cell index
+ * in = ix + NX*(iy + NY*iz), ix fastest (staggered/func_cuda.cpp:68), where
+ * real is double for TC_FP64 and float for TC_FP32_BULK (tc_real_size()).
+ *
+ * Every call returns TC_OK or an error code; tc_last_error() gives the text.
+ * The reference prints and calls exit(0) instead (staggered/func_cuda.cpp:23-35);
+ * a driver maps a non-zero status to that. Work is queued on the engine's
+ * stream; calls that return host data (tc_get_field, tc_integral, tc_profiles,
+ * tc_cfl, ...) synchronise first, and tc_sync() waits for everything queued.
+ *
+ * A field is named by (kind, slot): the reference's vel[4], flux[4], pres[4]
+ * ring buffers (staggered/data_global_inc.h:126-130) and nu_t (slot 0).
+ */
+#ifndef TURBCHANNEL_H
+#define TURBCHANNEL_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tc_engine tc_engine;
+
+enum tc_scheme { TC_SCHEME_HW = 0, TC_SCHEME_VV = 1, TC_SCHEME_AB3 = 2 };
+enum tc_precision { TC_FP64 = 0, TC_FP32_BULK = 1 };
+enum tc_kind { TC_VEL = 0, TC_FLUX = 1, TC_PRES = 2, TC_NUT = 3 };
+enum tc_status { TC_OK = 0, TC_ERR_ARG = 1, TC_ERR_HIP = 2, TC_ERR_STATE = 3, TC_ERR_NUMERIC = 4 };
+
+/* Kernel classes reported by tc_timing_read(). */
+enum tc_kernel {
+    TC_K_TURB_VISC = 0, /* Smagorinsky-van Driest nu_t */
+    TC_K_FLUX = 1,      /* explicit term + BDF3/EX3 right-hand side */
+    TC_K_DIV = 2,       /* divergence of the right-hand side */
+    TC_K_FFT_X = 3,     /* forward real FFT along x */
+    TC_K_FFT_Y = 4,     /* forward complex FFT along y */
+    TC_K_ZSOLVE = 5,    /* banded wall-normal solve per (kx, ky) */
+    TC_K_IFFT_Y = 6,    /* inverse complex FFT along y */
+    TC_K_IFFT_X = 7,    /* inverse real FFT along x */
+    TC_K_PROJECT = 8,   /* gradient + velocity correction (+ flow-rate shift) */
+    TC_K_OTHER = 9,     /* reductions and small kernels */
+    TC_K_COUNT = 10
+};
+
+/* Problem definition: the reference's main_params (staggered/config.h:47-73,
+ * collocated_AB3/config.h:39-56) plus the wall-normal mesh and this engine's
+ * scheme/precision switches, which the reference fixes at compile time
+ * (CONV_APPR_TYPE, VISC_APPR_TYPE, ENABLE_TURB_VISC, USE_DOUBLE_PRECISION). */
+typedef struct tc_params {
+    int scheme;            /* tc_scheme */
+    int precision;         /* tc_precision */
+    int nx, ny, nz;        /* staggered: nz cells; collocated: nz nodes incl. walls. nx, ny even */
+    int turb_visc;         /* allocate nu_t and run the Smagorinsky-van Driest model */
+    int visc_type;         /* staggered: 0 CROSS, 1 NAIVE; collocated: 0 FD Laplacian, 1 Galerkin */
+    int stab_type;         /* collocated: StabType 0 or 1 */
+    int fd_mode;           /* collocated: FDmode */
+    double visc, sour_dpdx, inv_length_scale;
+    double hx, hy, xmin, ymin;
+    double cfr_ubulk;      /* > 0: hold the bulk velocity at this value (constant flow rate) */
+    const double* zz;      /* staggered: nz+1 face coordinates; collocated: nz node coordinates */
+} tc_params;
+
+/* Lifecycle. Replaces data::alloc_all + InitFourier + CopyDataToDevice
+ * (staggered/func_global.h:944-963, :197-212; staggered/func_cuda.cpp:102-104)
+ * and the mesh set-up InitDerivativeMeshData (staggered/main.cpp:112-156). */
+int tc_create(const tc_params* p, int device, tc_engine** out);
+int tc_destroy(tc_engine* e);
+const char* tc_last_error(void);
+int tc_real_size(const tc_engine* e);           /* sizeof(real) of host buffers: 8 or 4 */
+int tc_sync(tc_engine* e);
+
+/* Host <-> device copies of one field; replaces the cudaMemcpy calls on
+ * vel[i], pres[i], nu_t in staggered/main.cpp:283-286, :348-349, :382-383. */
+int tc_set_field(tc_engine* e, int kind, int slot, const void* host, size_t bytes);
+int tc_get_field(tc_engine* e, int kind, int slot, void* host, size_t bytes);
+int tc_field_device_ptr(tc_engine* e, int kind, int slot, void** dev_ptr);
+
+/* One time step: data::Step(t, tau, current_field, bdf_order)
+ * (staggered/func_global.h:897-942; collocated_AB3/func_global.h:387-450).
+ * Reads slots cur, cur-1, cur-2 (mod 4) and writes slot (cur+1)%4. */
+int tc_step(tc_engine* e, double t, double tau, int cur, int bdf_order);
+
+/* Operators used by the reference's self-tests (staggered/selftest.cpp:360-431). */
+int tc_explicit_term(tc_engine* e, int in_kind, int in_slot, int out_kind, int out_slot);     /* data::ExplicitTerm */
+int tc_apply_gradient(tc_engine* e, int p_kind, int p_slot, int out_kind, int out_slot);      /* data::ApplyGradient */
+int tc_unapply_gradient(tc_engine* e, int v_kind, int v_slot, int p_kind, int p_slot);        /* data::UnapplyGradient */
+int tc_normalize_pressure(tc_engine* e, int kind, int slot);                                  /* data::NormalizePressure */
+int tc_recalc_turb_visc(tc_engine* e, int kind, int slot);                                    /* data::RecalcTurbVisc */
+
+/* Reductions, synchronous (staggered/func_global.h:677-808). */
+int tc_integral(tc_engine* e, int kind, int slot, double out[3]);                            /* data::CalcIntegral */
+int tc_kinetic_energy(tc_engine* e, int kind, int slot, double* out);                        /* data::CalcKineticEnergy */
+int tc_cfl(tc_engine* e, int kind, int slot, double tau, int mode, double* out);             /* data::CalcCFL */
+/* data::CalcProfiles / CalcVelocityProfile: per plane, FP64, staggered 12 values
+ * (u[3], uu[6], nu_t, p, pp) and collocated 10 (u[3], uu[6], nu_t). */
+int tc_profiles(tc_engine* e, int vel_kind, int vel_slot, int p_slot, double* host_av, size_t count);
+
+/* Constant-flow-rate correction of one velocity field (no reference counterpart;
+ * tc_step applies it itself when cfr_ubulk > 0). shift may be NULL. */
+int tc_cfr_correct(tc_engine* e, int kind, int slot, double* shift);
+/* u <- u - G L^-1 D u: the projection of tc_step without the time-step terms. */
+int tc_project(tc_engine* e, int kind, int slot);
+
+/* Seeded initial field (no reference counterpart): laminar parabola with bulk
+ * velocity ubulk plus the discrete curl of a smoothed Gaussian vector potential
+ * scaled to noise_rms, on the host in FP64, not projected. out: 3*nx*ny*nz doubles.
+ * tc_initial_field uploads it to a velocity slot and projects it. */
+int tc_make_initial_field(const tc_params* p, unsigned long long seed, double ubulk, double noise_rms, double* out);
+int tc_initial_field(tc_engine* e, int slot, unsigned long long seed, double ubulk, double noise_rms);
+
+/* Mesh data derived on the host from tc_params (staggered/main.cpp:112-156,
+ * collocated_AB3/main.cpp:295-300). name: "ZZ","DZ","DZW","GC1","GC2","GCZ",
+ * "MAT_A","MAT_D","KERG","KERG_norm2". Returns the length, or -1. No device needed. */
+int tc_mesh_array(const tc_params* p, const char* name, double* out, int cap);
+
+/* Per-kernel timing with HIP events on the engine's stream. */
+int tc_timing_enable(tc_engine* e, int on);
+int tc_timing_read(tc_engine* e, int kernel, double* total_ms, long long* launches);
+int tc_timing_reset(tc_engine* e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,67 @@
+"""CPU checks of the C ABI library: it loads, exports every function of
+include/turbchannel.h, and its host-side set-up (mesh data, initial field)
+agrees with the oracle. No compute call needs a GPU here."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import turbchannel_amd as tca
+from tests import parity
+
+
+def test_library_exports_every_declared_function():
+    L = tca.lib()
+    names = tca.header_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_no_fallback_when_library_missing(monkeypatch, tmp_path):
+    monkeypatch.setattr(tca, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(tca, "_lib", None)
+    with pytest.raises(tca.TCError):
+        tca.lib()
+
+
+@pytest.mark.parametrize("name", ["hw", "vv", "vv_odd", "ab3"])
+def test_mesh_data_matches_oracle(name):
+    p = parity.make_params(name)
+    o = parity.make_oracle(name)
+    names = ["ZZ", "DZ", "MAT_A"]
+    if p.scheme == tca.SCHEME_VV:
+        names += ["DZW", "GC1", "GC2", "GCZ", "MAT_D", "KERG"]
+    for n in names:
+        np.testing.assert_allclose(tca.mesh_array(p, n), o.mesh(n), rtol=1e-14, atol=1e-300, err_msg=n)
+    if p.scheme == tca.SCHEME_VV:
+        np.testing.assert_allclose(tca.mesh_array(p, "KERG_norm2")[0], o.kerg_norm2(), rtol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["hw", "vv_odd", "ab3"])
+def test_initial_field_matches_oracle(name):
+    p = parity.make_params(name)
+    o = parity.make_oracle(name)
+    for seed in (42, 7):
+        a = tca.make_initial_field(p, seed=seed, ubulk=1.0, noise_rms=0.1)
+        b = o.initial_field(seed=seed, ubulk=1.0, noise_rms=0.1)
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-14)
+
+
+def test_create_rejects_bad_arguments():
+    zz = np.linspace(0, 2, 9)
+    p = tca.make_params(tca.SCHEME_VV, 15, 8, 8, zz, visc=0.01, hx=0.1, hy=0.1)  # odd nx
+    h = ctypes.c_void_p()
+    rc = tca.lib().tc_create(ctypes.byref(p), 0, ctypes.byref(h))
+    assert rc != 0 and not h.value
+    assert tca.lib().tc_last_error().decode()
+    p = tca.make_params(tca.SCHEME_VV, 16, 8, 8, zz, visc=0.01, hx=0.1, hy=0.1, turb_visc=1, visc_type=0)
+    rc = tca.lib().tc_create(ctypes.byref(p), 0, ctypes.byref(h))
+    assert rc == 1 and "CROSS" in tca.lib().tc_last_error().decode()
+
+
+def test_mesh_array_rejects_bad_mesh():
+    zz = np.array([0.0, 0.5, 0.4, 1.0, 1.5, 1.8, 2.0])  # not increasing
+    p = tca.make_params(tca.SCHEME_HW, 8, 8, 6, zz, visc=0.01, hx=0.1, hy=0.1)
+    assert tca.lib().tc_mesh_array(ctypes.byref(p), b"DZ", None, 0) == -1

@@ -1,0 +1,844 @@
+// C ABI of the MI355X time stepper (include/turbchannel.h).
+//
+// The engine owns all device buffers and one HIP stream (default priority).
+// FP64 engines run everything in double. FP32-bulk engines keep fields,
+// stencils and FFTs in float and use double for the lowest-wavenumber
+// wall-normal solves, the (0,0)-mode normalisation, the flow-rate correction
+// and all reductions.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+using namespace tc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) return fail(TC_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kLowK = 2;  // FP32-bulk: modes with kx <= 2 and |ky| <= 2 are solved in FP64
+
+struct Timing {
+    bool on = false;
+    struct Rec {
+        int k;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    double total_ms[TC_K_COUNT] = {0};
+    long long count[TC_K_COUNT] = {0};
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+};
+
+}  // namespace
+
+struct tc_engine {
+    tc_params P{};
+    std::vector<double> zz;
+    HostMesh hm;
+    ZFactors zf;
+    int dev = 0;
+    hipStream_t s = nullptr;
+    bool fp32 = false;
+    int nx = 0, ny = 0, nz = 0, nxy = 0, nxh = 0, nm = 0;
+    size_t n3d = 0, rsz = 8;
+    void* vel[4] = {};
+    void* flux[4] = {};
+    void* pres[4] = {};
+    void* nut = nullptr;
+    void* divv = nullptr;
+    void* dp = nullptr;   // pressure increment (collocated StabType 1) / scratch pressure
+    void* spec = nullptr;
+    void* dmesh = nullptr;  // mesh arrays in T
+    size_t mstride = 0;     // elements per mesh array
+    double* d_wplane = nullptr;  // normalisation weights per plane
+    void* d_kerg = nullptr;      // KERG in T (plane shift weight, VV)
+    void* zfac = nullptr;        // factors in T
+    double* zfac_low = nullptr;  // factors of the low modes in double
+    int* d_low = nullptr;
+    int nlow = 0;
+    double* ylow = nullptr;
+    double *d_Wt = nullptr, *d_C = nullptr, *d_s00 = nullptr;
+    FFTPlan plx, ply;
+    void *twx = nullptr, *twy = nullptr;
+    double* d_partial = nullptr;
+    size_t npartial = 0;
+    double* d_out = nullptr;    // small results (16 doubles)
+    double* d_shift = nullptr;  // flow-rate shift
+    double* d_prof = nullptr;   // profiles result (nz * 12)
+    Timing tim;
+    std::vector<void*> allocs;
+
+    template <typename T>
+    Mesh<T> mesh() const {
+        Mesh<T> m;
+        m.nx = nx;
+        m.ny = ny;
+        m.nz = nz;
+        m.nxy = nxy;
+        m.n3d = (int)n3d;
+        m.nxh = nxh;
+        m.hx = (T)P.hx;
+        m.hy = (T)P.hy;
+        m.inv_hx = (T)(1.0 / P.hx);
+        m.inv_hy = (T)(1.0 / P.hy);
+        m.visc = (T)P.visc;
+        m.sour = (T)P.sour_dpdx;
+        m.inv_len = (T)P.inv_length_scale;
+        m.zlo = (T)hm.ZZ.front();
+        m.zhi = (T)hm.ZZ.back();
+        const T* b = (const T*)dmesh;
+        m.ZZ = b;
+        m.DZ = b + mstride;
+        m.DZW = b + 2 * mstride;
+        m.GC1 = b + 3 * mstride;
+        m.GC2 = b + 4 * mstride;
+        m.GCZ = b + 5 * mstride;
+        m.cellh = b + 6 * mstride;
+        return m;
+    }
+
+    void tbeg(int k, hipEvent_t& a) {
+        a = nullptr;
+        if (!tim.on) return;
+        a = tim.get();
+        if (a) hipEventRecord(a, s);
+    }
+    void tend(int k, hipEvent_t a) {
+        if (!tim.on || !a) return;
+        hipEvent_t b = tim.get();
+        if (!b) return;
+        hipEventRecord(b, s);
+        tim.recs.push_back({k, a, b});
+    }
+};
+
+namespace {
+
+template <typename T>
+int dalloc(tc_engine* e, T** p, size_t n) {
+    void* q = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc(&q, n * sizeof(T)));
+    HIPCHK(hipMemset(q, 0, n * sizeof(T)));
+    e->allocs.push_back(q);
+    *p = (T*)q;
+    return TC_OK;
+}
+
+int dalloc_bytes(tc_engine* e, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    HIPCHK(hipMalloc(p, bytes));
+    HIPCHK(hipMemset(*p, 0, bytes));
+    e->allocs.push_back(*p);
+    return TC_OK;
+}
+
+template <typename T>
+int upload_as(tc_engine* e, void* dst, const double* src, size_t n) {
+    std::vector<T> tmp(n);
+    for (size_t i = 0; i < n; i++) tmp[i] = (T)src[i];
+    HIPCHK(hipMemcpy(dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice));
+    return TC_OK;
+}
+
+void* field_ptr(tc_engine* e, int kind, int slot) {
+    if (slot < 0 || slot > 3) return nullptr;
+    switch (kind) {
+        case TC_VEL: return e->vel[slot];
+        case TC_FLUX: return e->flux[slot];
+        case TC_PRES: return e->pres[slot];
+        case TC_NUT: return slot == 0 ? e->nut : nullptr;
+        default: return nullptr;
+    }
+}
+bool is_vec(int kind) { return kind == TC_VEL || kind == TC_FLUX; }
+bool is_scal(int kind) { return kind == TC_PRES || kind == TC_NUT; }
+
+template <typename T>
+int setup_device(tc_engine* e) {
+    const size_t n3 = e->n3d;
+    for (int i = 0; i < 4; i++) {
+        T* p;
+        if (int r = dalloc(e, &p, 3 * n3)) return r;
+        e->vel[i] = p;
+        if (int r = dalloc(e, &p, 3 * n3)) return r;
+        e->flux[i] = p;
+        if (int r = dalloc(e, &p, n3)) return r;
+        e->pres[i] = p;
+    }
+    T* p;
+    if (e->P.turb_visc) {
+        if (int r = dalloc(e, &p, n3)) return r;
+        e->nut = p;
+    }
+    if (int r = dalloc(e, &p, n3)) return r;
+    e->divv = p;
+    if (int r = dalloc(e, &p, n3)) return r;
+    e->dp = p;
+    if (int r = dalloc_bytes(e, &e->spec, (size_t)e->nm * e->nz * sizeof(Cplx<T>))) return r;
+    // mesh arrays: ZZ, DZ, DZW, GC1, GC2, GCZ, cellh, each mstride long
+    const HostMesh& hm = e->hm;
+    e->mstride = (size_t)e->nz + 8;
+    std::vector<double> mh(7 * e->mstride, 0.0);
+    auto put = [&](int k, const std::vector<double>& v) {
+        for (size_t i = 0; i < v.size() && i < e->mstride; i++) mh[k * e->mstride + i] = v[i];
+    };
+    put(0, hm.ZZ);
+    put(1, hm.DZ);
+    put(2, hm.DZW);
+    put(3, hm.GC1);
+    put(4, hm.GC2);
+    put(5, hm.GCZ);
+    put(6, hm.cellh);
+    if (int r = dalloc_bytes(e, &e->dmesh, mh.size() * sizeof(T))) return r;
+    if (int r = upload_as<T>(e, e->dmesh, mh.data(), mh.size())) return r;
+    // normalisation weights (NormalizePressure)
+    std::vector<double> wpl(e->nz);
+    for (int iz = 0; iz < e->nz; iz++) {
+        if (e->P.scheme == TC_SCHEME_HW) wpl[iz] = hm.DZ[iz];
+        else if (e->P.scheme == TC_SCHEME_VV) wpl[iz] = hm.KERG[iz] * hm.DZ[iz];
+        else wpl[iz] = hm.cellh[iz] * e->P.hx * e->P.hy;
+    }
+    if (int r = dalloc(e, &e->d_wplane, e->nz)) return r;
+    HIPCHK(hipMemcpy(e->d_wplane, wpl.data(), e->nz * sizeof(double), hipMemcpyHostToDevice));
+    if (int r = dalloc_bytes(e, &e->d_kerg, e->nz * sizeof(T))) return r;
+    if (int r = upload_as<T>(e, e->d_kerg, hm.KERG.data(), e->nz)) return r;
+    // wall-normal factors
+    const ZFactors& zf = e->zf;
+    const size_t fz = (size_t)e->nz * e->nm;
+    const int ncoef = 2 * zf.W + 1;
+    if (int r = dalloc_bytes(e, &e->zfac, ncoef * fz * sizeof(T))) return r;
+    {
+        std::vector<T> tmp(ncoef * fz);
+        int k = 0;
+        for (int c = 0; c < zf.W; c++, k++)
+            for (size_t i = 0; i < fz; i++) tmp[k * fz + i] = (T)zf.M[c][i];
+        for (size_t i = 0; i < fz; i++) tmp[k * fz + i] = (T)zf.D[i];
+        k++;
+        for (int c = 0; c < zf.W; c++, k++)
+            for (size_t i = 0; i < fz; i++) tmp[k * fz + i] = (T)zf.U[c][i];
+        HIPCHK(hipMemcpy(e->zfac, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+    if (int r = dalloc(e, &e->d_Wt, e->nz)) return r;
+    if (int r = dalloc(e, &e->d_C, e->nz)) return r;
+    if (int r = dalloc(e, &e->d_s00, e->nz)) return r;
+    HIPCHK(hipMemcpy(e->d_Wt, zf.Wt.data(), e->nz * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_C, zf.C.data(), e->nz * sizeof(double), hipMemcpyHostToDevice));
+    if (e->fp32) {  // low-mode list and its double factors
+        std::vector<int> low;
+        for (int ky = 0; ky < e->ny; ky++)
+            for (int kx = 0; kx <= kLowK && kx < e->nxh; kx++)
+                if (ky <= kLowK || ky >= e->ny - kLowK) low.push_back(ky * e->nxh + kx);
+        e->nlow = (int)low.size();
+        if (int r = dalloc(e, &e->d_low, low.size())) return r;
+        HIPCHK(hipMemcpy(e->d_low, low.data(), low.size() * sizeof(int), hipMemcpyHostToDevice));
+        const size_t lz = (size_t)e->nz * e->nlow;
+        std::vector<double> lf(ncoef * lz);
+        auto col = [&](const std::vector<double>& a, int k) {
+            for (int iz = 0; iz < e->nz; iz++)
+                for (int li = 0; li < e->nlow; li++) lf[k * lz + (size_t)iz * e->nlow + li] = a[(size_t)iz * e->nm + low[li]];
+        };
+        int k = 0;
+        for (int c = 0; c < zf.W; c++) col(zf.M[c], k++);
+        col(zf.D, k++);
+        for (int c = 0; c < zf.W; c++) col(zf.U[c], k++);
+        if (int r = dalloc(e, &e->zfac_low, lf.size())) return r;
+        HIPCHK(hipMemcpy(e->zfac_low, lf.data(), lf.size() * sizeof(double), hipMemcpyHostToDevice));
+        if (int r = dalloc(e, &e->ylow, 2 * lz)) return r;
+    }
+    // FFT twiddles exp(-2 pi i k / n)
+    auto tw = [&](int n, void** dst) -> int {
+        std::vector<Cplx<T>> t(n);
+        for (int k = 0; k < n; k++) {
+            const double a = 2.0 * 3.14159265358979323846 * (double)k / (double)n;
+            t[k] = {(T)std::cos(a), (T)(-std::sin(a))};
+        }
+        if (int r = dalloc_bytes(e, dst, n * sizeof(Cplx<T>))) return r;
+        HIPCHK(hipMemcpy(*dst, t.data(), n * sizeof(Cplx<T>), hipMemcpyHostToDevice));
+        return TC_OK;
+    };
+    if (int r = tw(e->nx, &e->twx)) return r;
+    if (int r = tw(e->ny, &e->twy)) return r;
+    e->npartial = std::max<size_t>((size_t)kRedBlocks * 3, (size_t)flux_grid_blocks(e->nxy, e->nz));
+    e->npartial = std::max<size_t>(e->npartial, (size_t)profiles_chunks(e->nxy) * e->nz * 12);
+    if (int r = dalloc(e, &e->d_partial, e->npartial)) return r;
+    if (int r = dalloc(e, &e->d_out, 16)) return r;
+    if (int r = dalloc(e, &e->d_shift, 4)) return r;
+    if (int r = dalloc(e, &e->d_prof, (size_t)e->nz * 12)) return r;
+    return TC_OK;
+}
+
+template <typename T>
+ZFacDev<T> zfac_dev(tc_engine* e) {
+    ZFacDev<T> f;
+    f.W = e->zf.W;
+    f.nz = e->nz;
+    f.ncols = e->nm;
+    const size_t fz = (size_t)e->nz * e->nm;
+    const T* b = (const T*)e->zfac;
+    int k = 0;
+    f.M1 = b + (k++) * fz;
+    if (f.W == 3) {
+        f.M2 = b + (k++) * fz;
+        f.M3 = b + (k++) * fz;
+    }
+    f.D = b + (k++) * fz;
+    f.U1 = b + (k++) * fz;
+    if (f.W == 3) {
+        f.U2 = b + (k++) * fz;
+        f.U3 = b + (k++) * fz;
+    }
+    return f;
+}
+
+ZFacDev<double> zfac_low_dev(tc_engine* e) {
+    ZFacDev<double> f;
+    f.W = e->zf.W;
+    f.nz = e->nz;
+    f.ncols = e->nlow;
+    const size_t lz = (size_t)e->nz * e->nlow;
+    const double* b = e->zfac_low;
+    int k = 0;
+    f.M1 = b + (k++) * lz;
+    if (f.W == 3) {
+        f.M2 = b + (k++) * lz;
+        f.M3 = b + (k++) * lz;
+    }
+    f.D = b + (k++) * lz;
+    f.U1 = b + (k++) * lz;
+    if (f.W == 3) {
+        f.U2 = b + (k++) * lz;
+        f.U3 = b + (k++) * lz;
+    }
+    return f;
+}
+
+// Pressure solve f -> x (LinearSystemSolveFFT incl. NormalizePressure).
+template <typename T>
+int solve(tc_engine* e, const T* f, T* x) {
+    hipEvent_t ev;
+    Cplx<T>* spec = (Cplx<T>*)e->spec;
+    const int nrows = e->ny * e->nz;
+    e->tbeg(TC_K_FFT_X, ev);
+    HIPCHK(launch_fft_x_fwd<T>(e->s, e->plx, (const Cplx<T>*)e->twx, f, spec, nrows));
+    e->tend(TC_K_FFT_X, ev);
+    e->tbeg(TC_K_FFT_Y, ev);
+    HIPCHK(launch_fft_y<T>(e->s, e->ply, (const Cplx<T>*)e->twy, spec, e->nxh, e->nz, 0));
+    e->tend(TC_K_FFT_Y, ev);
+    e->tbeg(TC_K_ZSOLVE, ev);
+    ZNorm norm;
+    norm.Wt = e->d_Wt;
+    norm.C = e->d_C;
+    if (e->fp32) {
+        HIPCHK(launch_zsolve<T>(e->s, spec, zfac_dev<T>(e), e->nxh, e->ny, 1, kLowK, ZNorm{}));
+        HIPCHK(launch_zsolve_low<T>(e->s, spec, zfac_low_dev(e), e->d_low, e->nlow, e->nm, e->ylow, norm));
+    } else {
+        norm.s00 = e->d_s00;
+        HIPCHK(launch_zsolve<T>(e->s, spec, zfac_dev<T>(e), e->nxh, e->ny, 0, kLowK, norm));
+    }
+    e->tend(TC_K_ZSOLVE, ev);
+    e->tbeg(TC_K_IFFT_Y, ev);
+    HIPCHK(launch_fft_y<T>(e->s, e->ply, (const Cplx<T>*)e->twy, spec, e->nxh, e->nz, 1));
+    e->tend(TC_K_IFFT_Y, ev);
+    e->tbeg(TC_K_IFFT_X, ev);
+    HIPCHK(launch_fft_x_inv<T>(e->s, e->plx, (const Cplx<T>*)e->twx, spec, x, nrows, (T)(1.0 / ((double)e->nx * e->ny))));
+    e->tend(TC_K_IFFT_X, ev);
+    return TC_OK;
+}
+
+template <typename T>
+int unapply(tc_engine* e, const T* v, T* p) {
+    hipEvent_t ev;
+    e->tbeg(TC_K_DIV, ev);
+    HIPCHK(launch_divergence<T>(e->s, e->mesh<T>(), e->P.scheme, v, (T*)e->divv));
+    e->tend(TC_K_DIV, ev);
+    return solve<T>(e, (const T*)e->divv, p);
+}
+
+template <typename T>
+int step(tc_engine* e, double tau, int cur, int order) {
+    const int next = (cur + 1) % 4, prev = (cur + 3) % 4, prev2 = (cur + 2) % 4;
+    const double bd0 = (order == 3) ? (1.5 + 1.0 / 3.0) : (order == 2) ? 1.5 : 1.0;
+    const Mesh<T> m = e->mesh<T>();
+    const bool stab = e->P.scheme == TC_SCHEME_AB3 && e->P.stab_type != 0;
+    const bool cfr = e->P.cfr_ubulk > 0;
+    hipEvent_t ev;
+    if (e->P.turb_visc) {
+        e->tbeg(TC_K_TURB_VISC, ev);
+        HIPCHK(launch_turb_visc<T>(e->s, m, e->P.scheme, (const T*)e->vel[cur], (T*)e->nut));
+        e->tend(TC_K_TURB_VISC, ev);
+    }
+    FluxArgs<T> a;
+    a.scheme = e->P.scheme;
+    a.visc_type = e->P.visc_type;
+    a.turb = e->P.turb_visc;
+    a.order = order;
+    a.stab = stab;
+    a.vel = (const T*)e->vel[cur];
+    a.nut = (const T*)e->nut;
+    a.fcur = (T*)e->flux[cur];
+    a.vprev = (const T*)e->vel[prev];
+    a.vprev2 = (const T*)e->vel[prev2];
+    a.fprev = (const T*)e->flux[prev];
+    a.fprev2 = (const T*)e->flux[prev2];
+    a.fnext = (T*)e->flux[next];
+    a.pcur = (const T*)e->pres[cur];
+    a.inv_tau = (T)(1.0 / tau);
+    a.cfr_partial = cfr ? e->d_partial : nullptr;
+    e->tbeg(TC_K_FLUX, ev);
+    HIPCHK(launch_flux<T>(e->s, m, a));
+    e->tend(TC_K_FLUX, ev);
+    T* ptarget = stab ? (T*)e->dp : (T*)e->pres[next];
+    if (int r = unapply<T>(e, (const T*)e->flux[next], ptarget)) return r;
+    if (cfr) {
+        e->tbeg(TC_K_OTHER, ev);
+        HIPCHK(launch_cfr_shift(e->s, e->d_partial, flux_grid_blocks(e->nxy, e->nz), tau / bd0,
+                                e->P.cfr_ubulk * e->hm.area, e->hm.vol_u, e->d_shift));
+        e->tend(TC_K_OTHER, ev);
+    }
+    ProjectArgs<T> pa;
+    pa.scheme = e->P.scheme;
+    pa.p = ptarget;
+    pa.r = (const T*)e->flux[next];
+    pa.vnext = (T*)e->vel[next];
+    pa.tau_over_bd0 = (T)(tau / bd0);
+    pa.shift = cfr ? e->d_shift : nullptr;
+    if (stab) {
+        pa.pcur = (const T*)e->pres[cur];
+        pa.pnext = (T*)e->pres[next];
+    }
+    e->tbeg(TC_K_PROJECT, ev);
+    HIPCHK(launch_project<T>(e->s, m, pa));
+    e->tend(TC_K_PROJECT, ev);
+    return TC_OK;
+}
+
+template <typename T>
+int normalize(tc_engine* e, T* p) {
+    const Mesh<T> m = e->mesh<T>();
+    RedArgs<T> r;
+    r.op = RED_PSUM;
+    r.scheme = e->P.scheme;
+    r.p = p;
+    r.wplane = e->d_wplane;
+    HIPCHK(launch_reduce<T>(e->s, m, r, e->d_partial, e->d_out));
+    double coef;
+    const T* w = nullptr;
+    const double nxy = (double)e->nxy;
+    if (e->P.scheme == TC_SCHEME_HW) coef = 1.0 / ((e->hm.ZZ[e->nz] - e->hm.ZZ[0]) * nxy);
+    else if (e->P.scheme == TC_SCHEME_VV) {
+        coef = 1.0 / (e->hm.KERG_norm2 * nxy);
+        w = (const T*)e->d_kerg;
+    } else coef = e->P.hx * e->P.hy / e->hm.area;
+    HIPCHK(launch_add_scaled_plane<T>(e->s, m, p, e->d_out, coef, w));
+    return TC_OK;
+}
+
+template <typename T>
+int reduce_to_host(tc_engine* e, RedArgs<T> r, double* out, int n) {
+    HIPCHK(launch_reduce<T>(e->s, e->mesh<T>(), r, e->d_partial, e->d_out));
+    HIPCHK(hipMemcpyAsync(out, e->d_out, n * sizeof(double), hipMemcpyDeviceToHost, e->s));
+    HIPCHK(hipStreamSynchronize(e->s));
+    return TC_OK;
+}
+
+template <typename T>
+int project(tc_engine* e, T* v) {
+    if (int r = unapply<T>(e, v, (T*)e->dp)) return r;
+    ProjectArgs<T> pa;
+    pa.scheme = e->P.scheme;
+    pa.p = (const T*)e->dp;
+    pa.r = v;
+    pa.vnext = v;
+    pa.tau_over_bd0 = T(1);
+    HIPCHK(launch_project<T>(e->s, e->mesh<T>(), pa));
+    return TC_OK;
+}
+
+#define DISPATCH(e, call) ((e)->fp32 ? call<float> : call<double>)
+
+int check(tc_engine* e) {
+    if (!e) return fail(TC_ERR_ARG, "engine is NULL");
+    if (hipSetDevice(e->dev) != hipSuccess) return fail(TC_ERR_HIP, "hipSetDevice failed");
+    return TC_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* tc_last_error(void) { return g_err.c_str(); }
+
+int tc_mesh_array(const tc_params* p, const char* name, double* out, int cap) {
+    if (!p || !name) return -1;
+    HostMesh m;
+    if (!build_mesh(*p, m).empty()) return -1;
+    std::vector<double> v;
+    if (!strcmp(name, "ZZ")) v = m.ZZ;
+    else if (!strcmp(name, "DZ")) v.assign(m.DZ.begin(), m.DZ.begin() + (p->scheme == TC_SCHEME_AB3 ? p->nz - 1 : p->nz));
+    else if (!strcmp(name, "MAT_A")) v = m.MAT_A;
+    else if (!strcmp(name, "KERG_norm2") && p->scheme == TC_SCHEME_VV) v = {m.KERG_norm2};
+    else if (p->scheme == TC_SCHEME_VV) {
+        if (!strcmp(name, "DZW")) v = m.DZW;
+        else if (!strcmp(name, "GC1")) v = m.GC1;
+        else if (!strcmp(name, "GC2")) v = m.GC2;
+        else if (!strcmp(name, "GCZ")) v = m.GCZ;
+        else if (!strcmp(name, "MAT_D")) v.assign(m.MAT_D, m.MAT_D + 6);
+        else if (!strcmp(name, "KERG")) v = m.KERG;
+        else return -1;
+    } else return -1;
+    if (out)
+        for (int i = 0; i < (int)v.size() && i < cap; i++) out[i] = v[i];
+    return (int)v.size();
+}
+
+int tc_make_initial_field(const tc_params* p, unsigned long long seed, double ubulk, double noise_rms, double* out) {
+    if (!p || !out) return fail(TC_ERR_ARG, "NULL argument");
+    HostMesh m;
+    std::string err = build_mesh(*p, m);
+    if (!err.empty()) return fail(TC_ERR_ARG, err);
+    make_initial_field(*p, m, seed, ubulk, noise_rms, out);
+    return TC_OK;
+}
+
+int tc_create(const tc_params* p, int device, tc_engine** out) {
+    if (!p || !out) return fail(TC_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (p->precision != TC_FP64 && p->precision != TC_FP32_BULK) return fail(TC_ERR_ARG, "unknown precision");
+    if (p->visc_type < 0 || p->visc_type > 1) return fail(TC_ERR_ARG, "visc_type must be 0 or 1");
+    if (p->scheme != TC_SCHEME_AB3 && p->turb_visc && p->visc_type == 0)
+        return fail(TC_ERR_ARG, "tViscApproximation::CROSS does not work for a turbulence model");
+    const double n3 = (double)p->nx * p->ny * p->nz;
+    if (n3 * 3 > 2.0e9) return fail(TC_ERR_ARG, "grid too large for 32-bit cell indexing");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(TC_ERR_HIP, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(TC_ERR_ARG, "DEVICE ID OUT OF RANGE");
+    std::unique_ptr<tc_engine> e(new tc_engine());
+    e->P = *p;
+    e->dev = device;
+    std::string err = build_mesh(*p, e->hm);
+    if (!err.empty()) return fail(TC_ERR_ARG, err);
+    e->zz = e->hm.ZZ;
+    e->P.zz = e->zz.data();
+    err = build_zfactors(e->P, e->hm, e->zf);
+    if (!err.empty()) return fail(TC_ERR_NUMERIC, err);
+    if (!make_plan(p->nx, e->plx) || !make_plan(p->ny, e->ply)) return fail(TC_ERR_ARG, "nx and ny must have only the prime factors 2, 3 and 5");
+    e->fp32 = p->precision == TC_FP32_BULK;
+    e->rsz = e->fp32 ? 4 : 8;
+    e->nx = p->nx;
+    e->ny = p->ny;
+    e->nz = p->nz;
+    e->nxy = p->nx * p->ny;
+    e->nxh = p->nx / 2 + 1;
+    e->nm = e->nxh * p->ny;
+    e->n3d = (size_t)e->nxy * p->nz;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking));
+    int r = e->fp32 ? setup_device<float>(e.get()) : setup_device<double>(e.get());
+    if (r) {
+        tc_destroy(e.release());
+        return r;
+    }
+    HIPCHK(hipDeviceSynchronize());
+    *out = e.release();
+    return TC_OK;
+}
+
+int tc_destroy(tc_engine* e) {
+    if (!e) return TC_OK;
+    hipSetDevice(e->dev);
+    if (e->s) hipStreamSynchronize(e->s);
+    for (auto& rc : e->tim.recs) {
+        hipEventDestroy(rc.a);
+        hipEventDestroy(rc.b);
+    }
+    for (auto ev : e->tim.pool) hipEventDestroy(ev);
+    for (void* p : e->allocs) hipFree(p);
+    if (e->s) hipStreamDestroy(e->s);
+    delete e;
+    return TC_OK;
+}
+
+int tc_real_size(const tc_engine* e) { return e ? (int)e->rsz : 0; }
+
+int tc_sync(tc_engine* e) {
+    if (int r = check(e)) return r;
+    HIPCHK(hipStreamSynchronize(e->s));
+    return TC_OK;
+}
+
+int tc_set_field(tc_engine* e, int kind, int slot, const void* host, size_t bytes) {
+    if (int r = check(e)) return r;
+    void* d = field_ptr(e, kind, slot);
+    if (!d || !host) return fail(TC_ERR_ARG, "bad field");
+    const size_t need = (is_vec(kind) ? 3 : 1) * e->n3d * e->rsz;
+    if (bytes != need) return fail(TC_ERR_ARG, "size mismatch: expected " + std::to_string(need) + " bytes");
+    HIPCHK(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, e->s));
+    HIPCHK(hipStreamSynchronize(e->s));
+    return TC_OK;
+}
+
+int tc_get_field(tc_engine* e, int kind, int slot, void* host, size_t bytes) {
+    if (int r = check(e)) return r;
+    void* d = field_ptr(e, kind, slot);
+    if (!d || !host) return fail(TC_ERR_ARG, "bad field");
+    const size_t need = (is_vec(kind) ? 3 : 1) * e->n3d * e->rsz;
+    if (bytes != need) return fail(TC_ERR_ARG, "size mismatch: expected " + std::to_string(need) + " bytes");
+    HIPCHK(hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, e->s));
+    HIPCHK(hipStreamSynchronize(e->s));
+    return TC_OK;
+}
+
+int tc_field_device_ptr(tc_engine* e, int kind, int slot, void** dev_ptr) {
+    if (!e || !dev_ptr) return fail(TC_ERR_ARG, "NULL argument");
+    void* d = field_ptr(e, kind, slot);
+    if (!d) return fail(TC_ERR_ARG, "bad field");
+    *dev_ptr = d;
+    return TC_OK;
+}
+
+int tc_step(tc_engine* e, double t, double tau, int cur, int bdf_order) {
+    (void)t;
+    if (int r = check(e)) return r;
+    if (bdf_order < 1 || bdf_order > 3) return fail(TC_ERR_ARG, "Wrong bdf_order");
+    if (cur < 0 || cur > 3) return fail(TC_ERR_ARG, "bad current_field");
+    if (!(tau > 0)) return fail(TC_ERR_ARG, "tau must be positive");
+    return e->fp32 ? step<float>(e, tau, cur, bdf_order) : step<double>(e, tau, cur, bdf_order);
+}
+
+int tc_explicit_term(tc_engine* e, int in_kind, int in_slot, int out_kind, int out_slot) {
+    if (int r = check(e)) return r;
+    void* in = field_ptr(e, in_kind, in_slot);
+    void* out = field_ptr(e, out_kind, out_slot);
+    if (!in || !out || !is_vec(in_kind) || !is_vec(out_kind) || in == out) return fail(TC_ERR_ARG, "bad field");
+    auto run = [&](auto zero) -> int {
+        using T = decltype(zero);
+        FluxArgs<T> a;
+        a.scheme = e->P.scheme;
+        a.visc_type = e->P.visc_type;
+        a.turb = e->P.turb_visc;
+        a.order = 0;
+        a.vel = (const T*)in;
+        a.nut = (const T*)e->nut;
+        a.fcur = (T*)out;
+        hipEvent_t ev;
+        e->tbeg(TC_K_FLUX, ev);
+        HIPCHK(launch_flux<T>(e->s, e->mesh<T>(), a));
+        e->tend(TC_K_FLUX, ev);
+        return TC_OK;
+    };
+    return e->fp32 ? run(0.0f) : run(0.0);
+}
+
+int tc_apply_gradient(tc_engine* e, int p_kind, int p_slot, int out_kind, int out_slot) {
+    if (int r = check(e)) return r;
+    void* p = field_ptr(e, p_kind, p_slot);
+    void* out = field_ptr(e, out_kind, out_slot);
+    if (!p || !out || !is_scal(p_kind) || !is_vec(out_kind)) return fail(TC_ERR_ARG, "bad field");
+    if (e->fp32) HIPCHK(launch_gradient<float>(e->s, e->mesh<float>(), e->P.scheme, (const float*)p, (float*)out));
+    else HIPCHK(launch_gradient<double>(e->s, e->mesh<double>(), e->P.scheme, (const double*)p, (double*)out));
+    return TC_OK;
+}
+
+int tc_unapply_gradient(tc_engine* e, int v_kind, int v_slot, int p_kind, int p_slot) {
+    if (int r = check(e)) return r;
+    void* v = field_ptr(e, v_kind, v_slot);
+    void* p = field_ptr(e, p_kind, p_slot);
+    if (!v || !p || !is_vec(v_kind) || !is_scal(p_kind)) return fail(TC_ERR_ARG, "bad field");
+    return e->fp32 ? unapply<float>(e, (const float*)v, (float*)p) : unapply<double>(e, (const double*)v, (double*)p);
+}
+
+int tc_normalize_pressure(tc_engine* e, int kind, int slot) {
+    if (int r = check(e)) return r;
+    void* p = field_ptr(e, kind, slot);
+    if (!p || !is_scal(kind)) return fail(TC_ERR_ARG, "bad field");
+    return e->fp32 ? normalize<float>(e, (float*)p) : normalize<double>(e, (double*)p);
+}
+
+int tc_recalc_turb_visc(tc_engine* e, int kind, int slot) {
+    if (int r = check(e)) return r;
+    void* v = field_ptr(e, kind, slot);
+    if (!v || !is_vec(kind)) return fail(TC_ERR_ARG, "bad field");
+    if (!e->nut) return TC_OK;  // RecalcTurbVisc returns when nu_t is not allocated
+    if (e->fp32) HIPCHK(launch_turb_visc<float>(e->s, e->mesh<float>(), e->P.scheme, (const float*)v, (float*)e->nut));
+    else HIPCHK(launch_turb_visc<double>(e->s, e->mesh<double>(), e->P.scheme, (const double*)v, (double*)e->nut));
+    return TC_OK;
+}
+
+int tc_integral(tc_engine* e, int kind, int slot, double out[3]) {
+    if (int r = check(e)) return r;
+    void* v = field_ptr(e, kind, slot);
+    if (!v || !is_vec(kind) || !out) return fail(TC_ERR_ARG, "bad field");
+    auto run = [&](auto zero) -> int {
+        using T = decltype(zero);
+        RedArgs<T> a;
+        a.op = RED_INTEGRAL;
+        a.scheme = e->P.scheme;
+        a.v = (const T*)v;
+        return reduce_to_host<T>(e, a, out, 3);
+    };
+    return e->fp32 ? run(0.0f) : run(0.0);
+}
+
+int tc_kinetic_energy(tc_engine* e, int kind, int slot, double* out) {
+    if (int r = check(e)) return r;
+    void* v = field_ptr(e, kind, slot);
+    if (!v || !is_vec(kind) || !out) return fail(TC_ERR_ARG, "bad field");
+    auto run = [&](auto zero) -> int {
+        using T = decltype(zero);
+        RedArgs<T> a;
+        a.op = RED_KE;
+        a.scheme = e->P.scheme;
+        a.v = (const T*)v;
+        double o;
+        if (int r = reduce_to_host<T>(e, a, &o, 1)) return r;
+        *out = 0.5 * o;
+        return TC_OK;
+    };
+    return e->fp32 ? run(0.0f) : run(0.0);
+}
+
+int tc_cfl(tc_engine* e, int kind, int slot, double tau, int mode, double* out) {
+    if (int r = check(e)) return r;
+    void* v = field_ptr(e, kind, slot);
+    if (!v || !is_vec(kind) || !out) return fail(TC_ERR_ARG, "bad field");
+    auto run = [&](auto zero) -> int {
+        using T = decltype(zero);
+        RedArgs<T> a;
+        a.op = RED_CFL;
+        a.scheme = e->P.scheme;
+        a.v = (const T*)v;
+        a.nut = (const T*)e->nut;
+        a.tau = tau;
+        a.mode = mode;
+        a.visc_fac = e->P.scheme == TC_SCHEME_AB3 ? 4.0 * (e->P.visc_type + 1) : 4.0 * ((e->P.visc_type != 0) + 1);
+        return reduce_to_host<T>(e, a, out, 1);
+    };
+    return e->fp32 ? run(0.0f) : run(0.0);
+}
+
+int tc_profiles(tc_engine* e, int vel_kind, int vel_slot, int p_slot, double* host_av, size_t count) {
+    if (int r = check(e)) return r;
+    void* v = field_ptr(e, vel_kind, vel_slot);
+    const int nv = e->P.scheme == TC_SCHEME_AB3 ? 10 : 12;
+    void* p = e->P.scheme == TC_SCHEME_AB3 ? nullptr : field_ptr(e, TC_PRES, p_slot);
+    if (!v || !is_vec(vel_kind) || !host_av) return fail(TC_ERR_ARG, "bad field");
+    if (e->P.scheme != TC_SCHEME_AB3 && !p) return fail(TC_ERR_ARG, "bad pressure slot");
+    if (count != (size_t)nv * e->nz) return fail(TC_ERR_ARG, "count must be nz*" + std::to_string(nv));
+    if (e->fp32)
+        HIPCHK(launch_profiles<float>(e->s, e->mesh<float>(), e->P.scheme, (const float*)v, (const float*)p, (const float*)e->nut,
+                                      e->d_partial, e->d_prof));
+    else
+        HIPCHK(launch_profiles<double>(e->s, e->mesh<double>(), e->P.scheme, (const double*)v, (const double*)p,
+                                       (const double*)e->nut, e->d_partial, e->d_prof));
+    HIPCHK(hipMemcpyAsync(host_av, e->d_prof, count * sizeof(double), hipMemcpyDeviceToHost, e->s));
+    HIPCHK(hipStreamSynchronize(e->s));
+    return TC_OK;
+}
+
+int tc_cfr_correct(tc_engine* e, int kind, int slot, double* shift) {
+    if (int r = check(e)) return r;
+    void* v = field_ptr(e, kind, slot);
+    if (!v || !is_vec(kind)) return fail(TC_ERR_ARG, "bad field");
+    if (!(e->P.cfr_ubulk > 0)) return fail(TC_ERR_STATE, "cfr_ubulk is not set");
+    double q[3];
+    if (int r = tc_integral(e, kind, slot, q)) return r;
+    const double sh = (e->P.cfr_ubulk * e->hm.area - q[0]) / e->hm.vol_u;
+    HIPCHK(hipMemcpyAsync(e->d_shift, &sh, sizeof(double), hipMemcpyHostToDevice, e->s));
+    if (e->fp32) HIPCHK(launch_add_u<float>(e->s, e->mesh<float>(), e->P.scheme, (float*)v, e->d_shift));
+    else HIPCHK(launch_add_u<double>(e->s, e->mesh<double>(), e->P.scheme, (double*)v, e->d_shift));
+    HIPCHK(hipStreamSynchronize(e->s));
+    if (shift) *shift = sh;
+    return TC_OK;
+}
+
+int tc_project(tc_engine* e, int kind, int slot) {
+    if (int r = check(e)) return r;
+    void* v = field_ptr(e, kind, slot);
+    if (!v || !is_vec(kind)) return fail(TC_ERR_ARG, "bad field");
+    return e->fp32 ? project<float>(e, (float*)v) : project<double>(e, (double*)v);
+}
+
+int tc_initial_field(tc_engine* e, int slot, unsigned long long seed, double ubulk, double noise_rms) {
+    if (int r = check(e)) return r;
+    void* v = field_ptr(e, TC_VEL, slot);
+    if (!v) return fail(TC_ERR_ARG, "bad slot");
+    std::vector<double> u(3 * e->n3d);
+    make_initial_field(e->P, e->hm, seed, ubulk, noise_rms, u.data());
+    if (e->fp32) {
+        if (int r = upload_as<float>(e, v, u.data(), u.size())) return r;
+    } else {
+        HIPCHK(hipMemcpy(v, u.data(), u.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return tc_project(e, TC_VEL, slot);
+}
+
+int tc_timing_enable(tc_engine* e, int on) {
+    if (!e) return fail(TC_ERR_ARG, "engine is NULL");
+    e->tim.on = on != 0;
+    return TC_OK;
+}
+
+int tc_timing_read(tc_engine* e, int kernel, double* total_ms, long long* launches) {
+    if (int r = check(e)) return r;
+    if (kernel < 0 || kernel >= TC_K_COUNT) return fail(TC_ERR_ARG, "bad kernel id");
+    HIPCHK(hipStreamSynchronize(e->s));
+    for (auto& rc : e->tim.recs) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, rc.a, rc.b));
+        e->tim.total_ms[rc.k] += ms;
+        e->tim.count[rc.k] += 1;
+        e->tim.pool.push_back(rc.a);
+        e->tim.pool.push_back(rc.b);
+    }
+    e->tim.recs.clear();
+    if (total_ms) *total_ms = e->tim.total_ms[kernel];
+    if (launches) *launches = e->tim.count[kernel];
+    return TC_OK;
+}
+
+int tc_timing_reset(tc_engine* e) {
+    if (int r = check(e)) return r;
+    HIPCHK(hipStreamSynchronize(e->s));
+    for (auto& rc : e->tim.recs) {
+        e->tim.pool.push_back(rc.a);
+        e->tim.pool.push_back(rc.b);
+    }
+    e->tim.recs.clear();
+    for (int k = 0; k < TC_K_COUNT; k++) {
+        e->tim.total_ms[k] = 0;
+        e->tim.count[k] = 0;
+    }
+    return TC_OK;
+}
+
+}  // extern "C"

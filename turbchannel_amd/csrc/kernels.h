@@ -1,0 +1,115 @@
+// Launchers of the engine's HIP kernels (one translation unit per kernel family).
+#pragma once
+
+#include "tc_internal.h"
+
+namespace tc {
+
+// Explicit term (convection + viscosity + source) fused with the BDF3/EX3
+// right-hand side. order == 0: write only kuhat = F(vel) to fcur.
+template <typename T>
+struct FluxArgs {
+    int scheme = 0, visc_type = 0, turb = 0, order = 0, stab = 0;
+    const T* vel = nullptr;   // u_n
+    const T* nut = nullptr;   // nu_t (turb)
+    T* fcur = nullptr;        // F_n (written)
+    const T *vprev = nullptr, *vprev2 = nullptr, *fprev = nullptr, *fprev2 = nullptr;
+    T* fnext = nullptr;       // right-hand side r (written when order > 0)
+    const T* pcur = nullptr;  // collocated StabType 1: r -= G p_n
+    T inv_tau = 0;
+    double* cfr_partial = nullptr;  // per-block sums of r_u * volume (nullptr: off)
+};
+template <typename T>
+hipError_t launch_flux(hipStream_t s, const Mesh<T>& m, const FluxArgs<T>& a);
+int flux_grid_blocks(int nxy, int nz);  // number of per-block partial sums written by launch_flux
+
+template <typename T>
+hipError_t launch_turb_visc(hipStream_t s, const Mesh<T>& m, int scheme, const T* vel, T* nut);
+template <typename T>
+hipError_t launch_divergence(hipStream_t s, const Mesh<T>& m, int scheme, const T* v, T* dv);
+template <typename T>
+hipError_t launch_gradient(hipStream_t s, const Mesh<T>& m, int scheme, const T* p, T* g);
+
+// u_next = (r - G p) * tau_over_bd0 (+ flow-rate shift on u); collocated StabType 1
+// also forms p_next = dp + p_cur.
+template <typename T>
+struct ProjectArgs {
+    int scheme = 0;
+    const T* p = nullptr;       // pressure (or pressure increment)
+    const T* r = nullptr;       // right-hand side
+    T* vnext = nullptr;
+    T tau_over_bd0 = 1;
+    const double* shift = nullptr;  // flow-rate shift on u (nullptr: none)
+    const T* pcur = nullptr;        // StabType 1
+    T* pnext = nullptr;             // StabType 1
+};
+template <typename T>
+hipError_t launch_project(hipStream_t s, const Mesh<T>& m, const ProjectArgs<T>& a);
+
+// Adds a device-held shift to the streamwise velocity (collocated walls excluded).
+template <typename T>
+hipError_t launch_add_u(hipStream_t s, const Mesh<T>& m, int scheme, T* v, const double* shift);
+// f[cell] -= (sum[0] * coef) * plane_w[iz]  (plane_w == nullptr: 1); NormalizePressure's shift.
+template <typename T>
+hipError_t launch_add_scaled_plane(hipStream_t s, const Mesh<T>& m, T* f, const double* sum, double coef, const T* plane_w);
+
+// ---------------------------------------------------------------------------
+// Spectral pressure solve
+// ---------------------------------------------------------------------------
+template <typename T>
+hipError_t launch_fft_x_fwd(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const T* in, Cplx<T>* spec,
+                            int nrows);
+template <typename T>
+hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cplx<T>* spec, int nxh, int nz,
+                        int inverse);
+template <typename T>
+hipError_t launch_fft_x_inv(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const Cplx<T>* spec, T* out,
+                            int nrows, T scale);
+
+// Factor arrays on the device, [coef][iz][mode] (or [iz][lowmode] for the list).
+template <typename CT>
+struct ZFacDev {
+    int W = 1, nz = 0, ncols = 0;
+    const CT *M1 = nullptr, *M2 = nullptr, *M3 = nullptr, *D = nullptr, *U1 = nullptr, *U2 = nullptr, *U3 = nullptr;
+};
+struct ZNorm {
+    const double *Wt = nullptr, *C = nullptr;
+    double* s00 = nullptr;  // nz doubles of scratch for the (0,0) column
+};
+// Solve every mode (skip_low: leave the low-mode set to launch_zsolve_low).
+template <typename T>
+hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, int nxh, int ny, int skip_low, int klow,
+                         ZNorm norm);
+// FP64 solve of the listed modes; ylow: nz * nlow * 2 doubles of scratch.
+template <typename T>
+hipError_t launch_zsolve_low(hipStream_t s, Cplx<T>* spec, const ZFacDev<double>& f, const int* modes, int nlow,
+                             int nm, double* ylow, ZNorm norm);
+
+// ---------------------------------------------------------------------------
+// Reductions (FP64 accumulation, deterministic two-stage)
+// ---------------------------------------------------------------------------
+enum RedOp { RED_INTEGRAL = 0, RED_KE = 1, RED_PSUM = 2, RED_CFL = 3 };
+template <typename T>
+struct RedArgs {
+    int op = 0, scheme = 0;
+    const T* v = nullptr;    // vector field
+    const T* p = nullptr;    // scalar field
+    const T* nut = nullptr;
+    const double* wplane = nullptr;  // per-plane weights (RED_PSUM)
+    double tau = 0, visc_fac = 0;
+    int mode = 3;
+};
+constexpr int kRedBlocks = 1024;
+// out: 3 doubles (integral), 1 (KE, PSUM, CFL). partial: kRedBlocks*3 doubles.
+template <typename T>
+hipError_t launch_reduce(hipStream_t s, const Mesh<T>& m, const RedArgs<T>& a, double* partial, double* out);
+// Final stage of the flow-rate correction: shift = (ub*V - q*scale)/V_u from partial sums.
+hipError_t launch_cfr_shift(hipStream_t s, const double* partial, int nblocks, double scale, double ub_vol,
+                            double vol_u, double* shift);
+// Plane statistics: out[iz*NV + k], NV = 12 (staggered) or 10 (collocated).
+template <typename T>
+hipError_t launch_profiles(hipStream_t s, const Mesh<T>& m, int scheme, const T* v, const T* p, const T* nut,
+                           double* partial, double* out);
+int profiles_chunks(int nxy);
+
+}  // namespace tc

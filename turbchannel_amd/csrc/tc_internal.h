@@ -1,0 +1,72 @@
+// Internal declarations shared by the host set-up, the kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/turbchannel.h"
+
+namespace tc {
+
+// ---------------------------------------------------------------------------
+// Host-side mesh data (FP64), derived from tc_params.
+// ---------------------------------------------------------------------------
+struct HostMesh {
+    int scheme = 0, nx = 0, ny = 0, nz = 0;
+    std::vector<double> ZZ, DZ, DZW, GC1, GC2, GCZ, MAT_A, KERG;
+    double MAT_D[6] = {0, 0, 0, 0, 0, 0};
+    double KERG_norm2 = 0;
+    double area = 0;   // box volume (the reference's `area`)
+    double vol_u = 0;  // volume carried by streamwise-velocity unknowns (flow-rate correction)
+    std::vector<double> cellh;  // collocated cell heights per node plane
+};
+
+// Returns "" on success, an error text otherwise.
+std::string build_mesh(const tc_params& p, HostMesh& m);
+int solve_banded(int N, int NB, double* M, double* x, double tiny);
+
+// Precomputed elimination of the wall-normal pressure systems, one column per
+// (kx, ky) mode with kx in [0, nx/2], ky in [0, ny). W = bandwidth (1 or 3).
+// Forward:  x[i] = (((x[i] + M3 x[i-3]) + M2 x[i-2]) + M1 x[i-1]) * D[i]
+// Backward: x[i] -= U1 x[i+1];  x[i] -= U2 x[i+2];  x[i] -= U3 x[i+3]
+// Arrays are [coef][iz][mode]. Mode-independent M1 of the tridiagonal solvers is
+// stored per mode too, to keep one kernel.
+struct ZFactors {
+    int W = 1, nz = 0, nmodes = 0;
+    std::vector<double> M[3], D, U[3];
+    // normalisation of the (0,0) mode in spectral space: X00[iz] -= psum * C[iz],
+    // psum = sum_iz Wt[iz] * X00[iz]
+    std::vector<double> Wt, C;
+};
+std::string build_zfactors(const tc_params& p, const HostMesh& m, ZFactors& f);
+
+void make_initial_field(const tc_params& p, const HostMesh& m, unsigned long long seed, double ubulk,
+                        double noise_rms, double* out);
+
+// ---------------------------------------------------------------------------
+// Device-side types
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Cplx {
+    T x, y;
+};
+
+constexpr int kMaxStages = 16;
+struct FFTPlan {
+    int n = 0, nst = 0;
+    int radix[kMaxStages] = {0};
+};
+bool make_plan(int n, FFTPlan& pl);
+
+// Constant data handed to kernels by value.
+template <typename T>
+struct Mesh {
+    int nx, ny, nz, nxy, n3d, nxh;
+    T hx, hy, inv_hx, inv_hy, visc, sour, inv_len;
+    T zlo, zhi;  // wall coordinates
+    const T *ZZ, *DZ, *DZW, *GC1, *GC2, *GCZ, *cellh;
+};
+
+}  // namespace tc
