@@ -104,10 +104,22 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 256 MiB device copy (torch) whose counter readings scale the traffic figures")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(256 * 2**20 // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     precision = tca.FP64 if args.precision == "fp64" else tca.FP32_BULK
     scheme, nx, ny, nz, zz, kw, dt = problem(args.workload, precision)

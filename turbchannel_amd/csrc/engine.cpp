@@ -80,7 +80,7 @@ struct tc_engine {
     void* d_kerg = nullptr;      // KERG in T (plane shift weight, VV)
     void* zfac = nullptr;        // factors in T
     double* zfac_low = nullptr;  // factors of the low modes in double
-    int* d_low = nullptr;
+    int* d_lowidx = nullptr;  // mode -> index in the FP64 low-mode list, or -1
     int nlow = 0;
     double* ylow = nullptr;
     double *d_Wt = nullptr, *d_C = nullptr, *d_s00 = nullptr;
@@ -256,8 +256,10 @@ int setup_device(tc_engine* e) {
             for (int kx = 0; kx <= kLowK && kx < e->nxh; kx++)
                 if (ky <= kLowK || ky >= e->ny - kLowK) low.push_back(ky * e->nxh + kx);
         e->nlow = (int)low.size();
-        if (int r = dalloc(e, &e->d_low, low.size())) return r;
-        HIPCHK(hipMemcpy(e->d_low, low.data(), low.size() * sizeof(int), hipMemcpyHostToDevice));
+        std::vector<int> lowidx(e->nm, -1);
+        for (int li = 0; li < e->nlow; li++) lowidx[low[li]] = li;
+        if (int r = dalloc(e, &e->d_lowidx, lowidx.size())) return r;
+        HIPCHK(hipMemcpy(e->d_lowidx, lowidx.data(), lowidx.size() * sizeof(int), hipMemcpyHostToDevice));
         const size_t lz = (size_t)e->nz * e->nlow;
         std::vector<double> lf(ncoef * lz);
         auto col = [&](const std::vector<double>& a, int k) {
@@ -285,7 +287,7 @@ int setup_device(tc_engine* e) {
     };
     if (int r = tw(e->nx, &e->twx)) return r;
     if (int r = tw(e->ny, &e->twy)) return r;
-    e->npartial = std::max<size_t>((size_t)kRedBlocks * 3, (size_t)flux_grid_blocks(e->nxy, e->nz));
+    e->npartial = std::max<size_t>((size_t)kRedBlocks * 3, (size_t)flux_grid_blocks(e->nx, e->ny, e->nz));
     e->npartial = std::max<size_t>(e->npartial, (size_t)profiles_chunks(e->nxy) * e->nz * 12);
     if (int r = dalloc(e, &e->d_partial, e->npartial)) return r;
     if (int r = dalloc(e, &e->d_out, 16)) return r;
@@ -356,11 +358,10 @@ int solve(tc_engine* e, const T* f, T* x) {
     norm.Wt = e->d_Wt;
     norm.C = e->d_C;
     if (e->fp32) {
-        HIPCHK(launch_zsolve<T>(e->s, spec, zfac_dev<T>(e), e->nxh, e->ny, 1, kLowK, ZNorm{}));
-        HIPCHK(launch_zsolve_low<T>(e->s, spec, zfac_low_dev(e), e->d_low, e->nlow, e->nm, e->ylow, norm));
+        HIPCHK(launch_zsolve<T>(e->s, spec, zfac_dev<T>(e), zfac_low_dev(e), e->d_lowidx, e->nlow, e->ylow, norm));
     } else {
         norm.s00 = e->d_s00;
-        HIPCHK(launch_zsolve<T>(e->s, spec, zfac_dev<T>(e), e->nxh, e->ny, 0, kLowK, norm));
+        HIPCHK(launch_zsolve<T>(e->s, spec, zfac_dev<T>(e), ZFacDev<double>{}, nullptr, 0, nullptr, norm));
     }
     e->tend(TC_K_ZSOLVE, ev);
     e->tbeg(TC_K_IFFT_Y, ev);
@@ -418,7 +419,7 @@ int step(tc_engine* e, double tau, int cur, int order) {
     if (int r = unapply<T>(e, (const T*)e->flux[next], ptarget)) return r;
     if (cfr) {
         e->tbeg(TC_K_OTHER, ev);
-        HIPCHK(launch_cfr_shift(e->s, e->d_partial, flux_grid_blocks(e->nxy, e->nz), tau / bd0,
+        HIPCHK(launch_cfr_shift(e->s, e->d_partial, flux_grid_blocks(e->nx, e->ny, e->nz), tau / bd0,
                                 e->P.cfr_ubulk * e->hm.area, e->hm.vol_u, e->d_shift));
         e->tend(TC_K_OTHER, ev);
     }

@@ -21,7 +21,7 @@ struct FluxArgs {
 };
 template <typename T>
 hipError_t launch_flux(hipStream_t s, const Mesh<T>& m, const FluxArgs<T>& a);
-int flux_grid_blocks(int nxy, int nz);  // number of per-block partial sums written by launch_flux
+int flux_grid_blocks(int nx, int ny, int nz);  // number of per-block partial sums written by launch_flux
 
 template <typename T>
 hipError_t launch_turb_visc(hipStream_t s, const Mesh<T>& m, int scheme, const T* vel, T* nut);
@@ -76,14 +76,13 @@ struct ZNorm {
     const double *Wt = nullptr, *C = nullptr;
     double* s00 = nullptr;  // nz doubles of scratch for the (0,0) column
 };
-// Solve every mode (skip_low: leave the low-mode set to launch_zsolve_low).
+// Solve every mode. FP32-bulk engines pass lowidx (mode -> index in the FP64
+// factor list, -1 for the FP32 modes), the FP64 factors of those modes and
+// ylow (nz * nlow * 2 doubles of scratch); FP64 engines pass lowidx = nullptr.
+// norm.s00 (FP64 engines) or the low-mode path normalises the (0,0) mode.
 template <typename T>
-hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, int nxh, int ny, int skip_low, int klow,
-                         ZNorm norm);
-// FP64 solve of the listed modes; ylow: nz * nlow * 2 doubles of scratch.
-template <typename T>
-hipError_t launch_zsolve_low(hipStream_t s, Cplx<T>* spec, const ZFacDev<double>& f, const int* modes, int nlow,
-                             int nm, double* ylow, ZNorm norm);
+hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, const ZFacDev<double>& flow,
+                         const int* lowidx, int nlow, double* ylow, ZNorm norm);
 
 // ---------------------------------------------------------------------------
 // Reductions (FP64 accumulation, deterministic two-stage)

@@ -25,8 +25,8 @@ template <typename T>
 __device__ __forceinline__ Cplx<T> cmul_si(Cplx<T> a, T s) { return {-s * a.y, s * a.x}; }
 
 // In-register DFT of length R (2, 3, 4, 5); s = -1 forward, +1 inverse.
-template <typename T>
-__device__ __forceinline__ void bfly(Cplx<T>* v, int R, T s) {
+template <typename T, int R>
+__device__ __forceinline__ void bfly(Cplx<T>* v, T s) {
     if (R == 2) {
         const Cplx<T> a = v[0], b = v[1];
         v[0] = cadd(a, b);
@@ -65,10 +65,48 @@ __device__ __forceinline__ void bfly(Cplx<T>* v, int R, T s) {
     }
 }
 
-// Stockham autosort transform of nlines lines of length N held in LDS.
-// Element e of line l lives at buf[l*lstride + e*estride]. tw[k] = exp(-2 pi i k/N).
-// Butterflies are numbered so that consecutive threads touch consecutive LDS
-// words: line-fastest when the lines are interleaved (estride > 1).
+// One Stockham pass of radix R over nlines lines of length N (Ns = product of
+// the radices already applied). Element e of line l lives at
+// buf[l*lstride + e*estride]; tw[k] = exp(-2 pi i k/N). Butterflies are
+// numbered so that consecutive threads touch consecutive LDS words:
+// line-fastest when the lines are interleaved (estride > 1).
+template <typename T, int R>
+__device__ __forceinline__ void stockham_pass(const Cplx<T>* __restrict__ src, Cplx<T>* __restrict__ dst, int N, int Ns,
+                                              int nlines, int lstride, int estride, const Cplx<T>* __restrict__ tw, T s) {
+    const int NR = N / R;
+    const int total = NR * nlines;
+    const int tstep = N / (Ns * R);
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        int l, j;
+        if (estride == 1) {
+            l = t / NR;
+            j = t - l * NR;
+        } else {
+            j = t / nlines;
+            l = t - j * nlines;
+        }
+        Cplx<T> v[R];
+        const Cplx<T>* sp = src + l * lstride;
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = sp[(j + r * NR) * estride];
+        const int jm = j % Ns;
+        if (jm != 0) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                Cplx<T> w = tw[r * jm * tstep];
+                w.y *= -s;  // forward table; conjugate for the inverse
+                v[r] = cmul(v[r], w);
+            }
+        }
+        bfly<T, R>(v, s);
+        const int o = (j - jm) * R + jm;
+        Cplx<T>* dp = dst + l * lstride;
+#pragma unroll
+        for (int r = 0; r < R; r++) dp[(o + r * Ns) * estride] = v[r];
+    }
+}
+
+// Stockham autosort transform held in LDS; returns the buffer with the result.
 template <typename T>
 __device__ Cplx<T>* lds_fft(Cplx<T>* src, Cplx<T>* dst, const FFTPlan& pl, int nlines, int lstride, int estride,
                             const Cplx<T>* __restrict__ tw, T s) {
@@ -76,33 +114,11 @@ __device__ Cplx<T>* lds_fft(Cplx<T>* src, Cplx<T>* dst, const FFTPlan& pl, int n
     int Ns = 1;
     for (int st = 0; st < pl.nst; st++) {
         const int R = pl.radix[st];
-        const int NR = N / R;
-        const int total = NR * nlines;
-        const int tstep = N / (Ns * R);
-        for (int t = threadIdx.x; t < total; t += blockDim.x) {
-            int l, j;
-            if (estride == 1) {
-                l = t / NR;
-                j = t - l * NR;
-            } else {
-                j = t / nlines;
-                l = t - j * nlines;
-            }
-            Cplx<T> v[5];
-            const Cplx<T>* sp = src + l * lstride;
-            for (int r = 0; r < R; r++) v[r] = sp[(j + r * NR) * estride];
-            const int jm = j % Ns;
-            if (jm != 0) {
-                for (int r = 1; r < R; r++) {
-                    Cplx<T> w = tw[r * jm * tstep];
-                    w.y *= -s;  // forward table; conjugate for the inverse
-                    v[r] = cmul(v[r], w);
-                }
-            }
-            bfly(v, R, s);
-            const int o = (j - jm) * R + jm;
-            Cplx<T>* dp = dst + l * lstride;
-            for (int r = 0; r < R; r++) dp[(o + r * Ns) * estride] = v[r];
+        switch (R) {
+            case 2: stockham_pass<T, 2>(src, dst, N, Ns, nlines, lstride, estride, tw, s); break;
+            case 3: stockham_pass<T, 3>(src, dst, N, Ns, nlines, lstride, estride, tw, s); break;
+            case 4: stockham_pass<T, 4>(src, dst, N, Ns, nlines, lstride, estride, tw, s); break;
+            default: stockham_pass<T, 5>(src, dst, N, Ns, nlines, lstride, estride, tw, s); break;
         }
         __syncthreads();
         Cplx<T>* tmp = src;
@@ -212,101 +228,190 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
 // collocated_AB3/func_global.h:8-61) with factors precomputed on the host.
 // One thread per (mode, real/imaginary part): the matrix is real.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool is_low_mode(int kx, int ky, int ny, int klow) {
-    return kx <= klow && (ky <= klow || ky >= ny - klow);
-}
+template <typename T, int W>
+__device__ void zsolve_low_lane(Cplx<T>* spec, const ZFacDev<double>& f, int li, int part, int nm, int mode,
+                                double* ylow, int nlow, const ZNorm& norm);
 
 template <typename T, int W>
-__global__ __launch_bounds__(256) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, int nxh, int ny, int skip_low, int klow,
-                                                 ZNorm norm) {
+__global__ __launch_bounds__(256) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFacDev<double> flow, const int* lowidx,
+                                                 int nlow, double* ylow, ZNorm norm) {
+    constexpr int B = 8;  // rows whose loads are issued together, ahead of the recurrence
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nm = f.ncols, nz = f.nz;
     const int mode = t >> 1, part = t & 1;
     if (mode >= nm) return;
-    const int kx = mode % nxh, ky = mode / nxh;
-    if (skip_low && is_low_mode(kx, ky, ny, klow)) return;
-    T* x = reinterpret_cast<T*>(spec) + 2 * mode + part;
+    if (lowidx) {  // FP32-bulk: the lowest wavenumbers take the FP64 path
+        const int li = lowidx[mode];
+        if (li >= 0) {
+            zsolve_low_lane<T, W>(spec, flow, li, part, nm, mode, ylow, nlow, norm);
+            return;
+        }
+    }
+    T* __restrict__ x = reinterpret_cast<T*>(spec) + 2 * mode + part;
     const size_t xs = 2 * (size_t)nm;  // stride between planes, in reals
+    const T* __restrict__ M1 = f.M1 + mode;
+    const T* __restrict__ M2 = f.M2 + mode;
+    const T* __restrict__ M3 = f.M3 + mode;
+    const T* __restrict__ D = f.D + mode;
+    const T* __restrict__ U1 = f.U1 + mode;
+    const T* __restrict__ U2 = f.U2 + mode;
+    const T* __restrict__ U3 = f.U3 + mode;
     T y1 = 0, y2 = 0, y3 = 0;
     // forward substitution
-    for (int i = 0; i < nz; i++) {
-        const size_t fi = (size_t)i * nm + mode;
-        T v = x[i * xs];
-        if (W == 3) v = v + f.M3[fi] * y3;
-        if (W == 3) v = v + f.M2[fi] * y2;
-        v = v + f.M1[fi] * y1;
-        v = v * f.D[fi];
-        x[i * xs] = v;
-        y3 = y2;
-        y2 = y1;
-        y1 = v;
+    for (int i0 = 0; i0 < nz; i0 += B) {
+        T b[B], m1[B], m2[B], m3[B], d[B];
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int i = i0 + k;
+            if (i < nz) {
+                const size_t fi = (size_t)i * nm;
+                b[k] = x[i * xs];
+                m1[k] = M1[fi];
+                d[k] = D[fi];
+                if (W == 3) {
+                    m2[k] = M2[fi];
+                    m3[k] = M3[fi];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int i = i0 + k;
+            if (i < nz) {
+                T v = b[k];
+                if (W == 3) v = v + m3[k] * y3;
+                if (W == 3) v = v + m2[k] * y2;
+                v = v + m1[k] * y1;
+                v = v * d[k];
+                x[i * xs] = v;
+                y3 = y2;
+                y2 = y1;
+                y1 = v;
+            }
+        }
     }
     // backward substitution
     const bool n00 = (mode == 0 && part == 0 && norm.s00 != nullptr);
     double psum = 0.0;
     T z1 = 0, z2 = 0, z3 = 0;
-    for (int i = nz - 1; i >= 0; i--) {
-        const size_t fi = (size_t)i * nm + mode;
-        T v = x[i * xs];
-        v = v - f.U1[fi] * z1;
-        if (W == 3) v = v - f.U2[fi] * z2;
-        if (W == 3) v = v - f.U3[fi] * z3;
-        x[i * xs] = v;
-        if (n00) {
-            psum += norm.Wt[i] * (double)v;
-            norm.s00[i] = (double)v;
+    for (int i0 = nz - 1; i0 >= 0; i0 -= B) {
+        T b[B], u1[B], u2[B], u3[B];
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int i = i0 - k;
+            if (i >= 0) {
+                const size_t fi = (size_t)i * nm;
+                b[k] = x[i * xs];
+                u1[k] = U1[fi];
+                if (W == 3) {
+                    u2[k] = U2[fi];
+                    u3[k] = U3[fi];
+                }
+            }
         }
-        z3 = z2;
-        z2 = z1;
-        z1 = v;
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int i = i0 - k;
+            if (i >= 0) {
+                T v = b[k];
+                v = v - u1[k] * z1;
+                if (W == 3) v = v - u2[k] * z2;
+                if (W == 3) v = v - u3[k] * z3;
+                x[i * xs] = v;
+                if (n00) {
+                    psum += norm.Wt[i] * (double)v;
+                    norm.s00[i] = (double)v;
+                }
+                z3 = z2;
+                z2 = z1;
+                z1 = v;
+            }
+        }
     }
     if (n00)  // NormalizePressure of the (0,0) mode, in spectral space
         for (int i = 0; i < nz; i++) x[i * xs] = T(norm.s00[i] - psum * norm.C[i]);
 }
 
-// FP64 solves of a short list of modes (the FP32-bulk engine's low wavenumbers).
+// FP64 solve of one listed mode (the FP32-bulk engine's low wavenumbers). The
+// intermediate of the forward pass stays in double (ylow: nz * nlow * 2).
 template <typename T, int W>
-__global__ __launch_bounds__(64) void k_zsolve_low(Cplx<T>* spec, ZFacDev<double> f, const int* __restrict__ modes,
-                                                   int nlow, int nm, double* ylow, ZNorm norm) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int li = t >> 1, part = t & 1;
-    if (li >= nlow) return;
-    const int mode = modes[li];
+__device__ void zsolve_low_lane(Cplx<T>* spec, const ZFacDev<double>& f, int li, int part, int nm, int mode,
+                                double* ylow, int nlow, const ZNorm& norm) {
+    constexpr int B = 8;
     const int nz = f.nz;
-    T* x = reinterpret_cast<T*>(spec) + 2 * mode + part;
+    T* __restrict__ x = reinterpret_cast<T*>(spec) + 2 * mode + part;
     const size_t xs = 2 * (size_t)nm;
-    double* y = ylow + t;
+    double* __restrict__ y = ylow + 2 * li + part;
     const size_t ys = 2 * (size_t)nlow;
     double y1 = 0, y2 = 0, y3 = 0;
-    for (int i = 0; i < nz; i++) {
-        const size_t fi = (size_t)i * nlow + li;
-        double v = (double)x[i * xs];
-        if (W == 3) v = v + f.M3[fi] * y3;
-        if (W == 3) v = v + f.M2[fi] * y2;
-        v = v + f.M1[fi] * y1;
-        v = v * f.D[fi];
-        y[i * ys] = v;
-        y3 = y2;
-        y2 = y1;
-        y1 = v;
+    for (int i0 = 0; i0 < nz; i0 += B) {
+        double b[B], m1[B], m2[B], m3[B], d[B];
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int i = i0 + k;
+            if (i < nz) {
+                const size_t fi = (size_t)i * nlow + li;
+                b[k] = (double)x[i * xs];
+                m1[k] = f.M1[fi];
+                d[k] = f.D[fi];
+                if (W == 3) {
+                    m2[k] = f.M2[fi];
+                    m3[k] = f.M3[fi];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int i = i0 + k;
+            if (i < nz) {
+                double v = b[k];
+                if (W == 3) v = v + m3[k] * y3;
+                if (W == 3) v = v + m2[k] * y2;
+                v = v + m1[k] * y1;
+                v = v * d[k];
+                y[i * ys] = v;
+                y3 = y2;
+                y2 = y1;
+                y1 = v;
+            }
+        }
     }
     const bool n00 = (mode == 0 && part == 0);
     double psum = 0.0, z1 = 0, z2 = 0, z3 = 0;
-    for (int i = nz - 1; i >= 0; i--) {
-        const size_t fi = (size_t)i * nlow + li;
-        double v = y[i * ys];
-        v = v - f.U1[fi] * z1;
-        if (W == 3) v = v - f.U2[fi] * z2;
-        if (W == 3) v = v - f.U3[fi] * z3;
-        y[i * ys] = v;
-        if (n00) psum += norm.Wt[i] * v;
-        z3 = z2;
-        z2 = z1;
-        z1 = v;
+    for (int i0 = nz - 1; i0 >= 0; i0 -= B) {
+        double b[B], u1[B], u2[B], u3[B];
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int i = i0 - k;
+            if (i >= 0) {
+                const size_t fi = (size_t)i * nlow + li;
+                b[k] = y[i * ys];
+                u1[k] = f.U1[fi];
+                if (W == 3) {
+                    u2[k] = f.U2[fi];
+                    u3[k] = f.U3[fi];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+            const int i = i0 - k;
+            if (i >= 0) {
+                double v = b[k];
+                v = v - u1[k] * z1;
+                if (W == 3) v = v - u2[k] * z2;
+                if (W == 3) v = v - u3[k] * z3;
+                y[i * ys] = v;
+                if (n00) psum += norm.Wt[i] * v;
+                z3 = z2;
+                z2 = z1;
+                z1 = v;
+            }
+        }
     }
     for (int i = 0; i < nz; i++) {
         double v = y[i * ys];
-        if (n00) v -= psum * norm.C[i];
+        if (n00) v -= psum * norm.C[i];  // NormalizePressure of the (0,0) mode
         x[i * xs] = T(v);
     }
 }
@@ -354,21 +459,12 @@ hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cpl
 }
 
 template <typename T>
-hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, int nxh, int ny, int skip_low, int klow,
-                         ZNorm norm) {
+hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, const ZFacDev<double>& flow,
+                         const int* lowidx, int nlow, double* ylow, ZNorm norm) {
     const int nthreads = 2 * f.ncols;
     const int grid = (nthreads + 255) / 256;
-    if (f.W == 3) hipLaunchKernelGGL((k_zsolve<T, 3>), dim3(grid), dim3(256), 0, s, spec, f, nxh, ny, skip_low, klow, norm);
-    else hipLaunchKernelGGL((k_zsolve<T, 1>), dim3(grid), dim3(256), 0, s, spec, f, nxh, ny, skip_low, klow, norm);
-    return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_zsolve_low(hipStream_t s, Cplx<T>* spec, const ZFacDev<double>& f, const int* modes, int nlow, int nm,
-                             double* ylow, ZNorm norm) {
-    const int grid = (2 * nlow + 63) / 64;
-    if (f.W == 3) hipLaunchKernelGGL((k_zsolve_low<T, 3>), dim3(grid), dim3(64), 0, s, spec, f, modes, nlow, nm, ylow, norm);
-    else hipLaunchKernelGGL((k_zsolve_low<T, 1>), dim3(grid), dim3(64), 0, s, spec, f, modes, nlow, nm, ylow, norm);
+    if (f.W == 3) hipLaunchKernelGGL((k_zsolve<T, 3>), dim3(grid), dim3(256), 0, s, spec, f, flow, lowidx, nlow, ylow, norm);
+    else hipLaunchKernelGGL((k_zsolve<T, 1>), dim3(grid), dim3(256), 0, s, spec, f, flow, lowidx, nlow, ylow, norm);
     return hipGetLastError();
 }
 
@@ -376,9 +472,8 @@ hipError_t launch_zsolve_low(hipStream_t s, Cplx<T>* spec, const ZFacDev<double>
     template hipError_t launch_fft_x_fwd<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const T*, Cplx<T>*, int);   \
     template hipError_t launch_fft_x_inv<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const Cplx<T>*, T*, int, T); \
     template hipError_t launch_fft_y<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, Cplx<T>*, int, int, int);       \
-    template hipError_t launch_zsolve<T>(hipStream_t, Cplx<T>*, const ZFacDev<T>&, int, int, int, int, ZNorm);      \
-    template hipError_t launch_zsolve_low<T>(hipStream_t, Cplx<T>*, const ZFacDev<double>&, const int*, int, int,   \
-                                             double*, ZNorm);
+    template hipError_t launch_zsolve<T>(hipStream_t, Cplx<T>*, const ZFacDev<T>&, const ZFacDev<double>&,         \
+                                         const int*, int, double*, ZNorm);
 INST(double)
 INST(float)
 #undef INST

@@ -1,10 +1,13 @@
 // Stencil kernels: explicit term + BDF3/EX3 right-hand side, turbulent
 // viscosity, divergence, gradient and the velocity correction.
 //
-// Launch shape: one thread per cell, blockIdx.y = wall-normal plane, so every
-// branch on iz (the near-wall closures of the fourth-order scheme) is uniform
-// across a wavefront and every mesh coefficient is a scalar load.
+// Every block works on one wall-normal plane at a time, so every branch on iz
+// (the near-wall closures of the fourth-order scheme) is uniform across a
+// wavefront and every mesh coefficient is a scalar load.
+#include <algorithm>
+
 #include "kernels.h"
+#include "stencil.h"
 
 namespace tc {
 
@@ -37,367 +40,6 @@ __device__ __forceinline__ void make_nb(Nb& nb, int ix, int iy, int iz, int nx, 
     }
     nb.nxy = nx * ny;
     nb.base = iz * nb.nxy;
-}
-
-// Field accessors for real3 AoS arrays.
-#define F3(a, c, dx, dy, dz) a[3 * nb.at(dx, dy, dz) + (c)]
-#define UU(dx, dy, dz) F3(vel, 0, dx, dy, dz)
-#define VV(dx, dy, dz) F3(vel, 1, dx, dy, dz)
-#define WW(dx, dy, dz) F3(vel, 2, dx, dy, dz)
-
-// ---------------------------------------------------------------------------
-// Convection, Harlow-Welch (staggered/func_global.h:345-370)
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void conv_hw(const Mesh<T>& m, const T* __restrict__ vel, const Nb& nb, int iz, T f[3]) {
-    const int NZ = m.nz;
-    const T dz = m.DZ[iz], dzm = iz == 0 ? T(0) : m.DZ[iz - 1];
-    const T inv_hz = T(1) / dz;
-    const T inv_hz_alt = T(2) / (dz + dzm);
-    const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
-    T f0 = 0, f1 = 0, f2 = 0;
-    f0 -= ((u0 + UU(1, 0, 0)) * (u0 + UU(1, 0, 0)) - (u0 + UU(-1, 0, 0)) * (u0 + UU(-1, 0, 0))) * m.inv_hx;
-    f0 -= ((u0 + UU(0, 1, 0)) * (VV(0, 1, 0) + VV(-1, 1, 0)) - (u0 + UU(0, -1, 0)) * (v0 + VV(-1, 0, 0))) * m.inv_hy;
-    if (iz != NZ - 1) f0 -= (u0 + UU(0, 0, 1)) * (WW(0, 0, 1) + WW(-1, 0, 1)) * inv_hz;
-    if (iz != 0) f0 += (u0 + UU(0, 0, -1)) * (w0 + WW(-1, 0, 0)) * inv_hz;
-
-    f1 -= ((v0 + VV(1, 0, 0)) * (UU(1, 0, 0) + UU(1, -1, 0)) - (v0 + VV(-1, 0, 0)) * (u0 + UU(0, -1, 0))) * m.inv_hx;
-    f1 -= ((v0 + VV(0, 1, 0)) * (v0 + VV(0, 1, 0)) - (v0 + VV(0, -1, 0)) * (v0 + VV(0, -1, 0))) * m.inv_hy;
-    if (iz != NZ - 1) f1 -= (v0 + VV(0, 0, 1)) * (WW(0, 0, 1) + WW(0, -1, 1)) * inv_hz;
-    if (iz != 0) f1 += (v0 + VV(0, 0, -1)) * (w0 + WW(0, -1, 0)) * inv_hz;
-
-    if (iz != 0) {
-        f2 -= ((w0 + WW(1, 0, 0)) * (UU(1, 0, 0) * dz + UU(1, 0, -1) * dzm) - (w0 + WW(-1, 0, 0)) * (u0 * dz + UU(0, 0, -1) * dzm)) *
-              m.inv_hx * inv_hz_alt;
-        f2 -= ((w0 + WW(0, 1, 0)) * (VV(0, 1, 0) * dz + VV(0, 1, -1) * dzm) - (w0 + WW(0, -1, 0)) * (v0 * dz + VV(0, 0, -1) * dzm)) *
-              m.inv_hy * inv_hz_alt;
-        const T wr = iz == NZ - 1 ? T(0) : WW(0, 0, 1);
-        f2 -= (w0 + wr) * (w0 + wr) * inv_hz_alt;
-        f2 += (w0 + WW(0, 0, -1)) * (w0 + WW(0, 0, -1)) * inv_hz_alt;
-    }
-    f[0] = f0 * T(0.25);
-    f[1] = f1 * T(0.25);
-    f[2] = f2 * T(0.25);
-}
-
-// ---------------------------------------------------------------------------
-// Convection, Verstappen-Veldman fourth order (staggered/func_global.h:374-529)
-// Mass fluxes through faces: "bar" (9/-1 interpolation, weight DZ) and
-// "barbar" (3h, weight DZW); near-wall mirror rules at iz in {0,1,2,NZ-3..NZ-1}.
-// ---------------------------------------------------------------------------
-// 4th-order interpolation of a transverse component along x at (dy, dz)
-#define XBAR(A, dy, dz) (T(9) * (A(0, dy, dz) + A(-1, dy, dz)) - (A(-2, dy, dz) + A(1, dy, dz)))
-#define YBAR(A, dx, dz) (T(9) * (A(dx, 0, dz) + A(dx, -1, dz)) - (A(dx, -2, dz) + A(dx, 1, dz)))
-
-template <typename T>
-__device__ __forceinline__ T zbar_vv(const Mesh<T>& m, const T* __restrict__ vel, const Nb& nb, int c, int dx, int dy,
-                                     int iz, const T* __restrict__ A) {
-    // 9*(F(0)*A[iz] + F(-1)*A[iz-1]) - F(-2)*A[iz-2] - F(+1)*A[iz+1] with the wall rules
-    const int NZ = m.nz;
-    const T f0 = F3(vel, c, dx, dy, 0), fm = F3(vel, c, dx, dy, -1);
-    T r = T(9) * (f0 * A[iz] + fm * A[iz - 1]);
-    r -= (iz == 1) ? -fm * A[0] : F3(vel, c, dx, dy, -2) * A[iz - 2];
-    r -= (iz == NZ - 1) ? -f0 * A[NZ - 1] : F3(vel, c, dx, dy, 1) * A[iz + 1];
-    return r;
-}
-
-template <typename T>
-__device__ __forceinline__ void conv_vv(const Mesh<T>& m, const T* __restrict__ vel, const Nb& nb, int iz, T f[3]) {
-    const int NZ = m.nz;
-    const T C27 = T(1.0 / 27.0);
-    const T hx = m.hx, hy = m.hy, hxy = hx * hy;
-    const T dz = m.DZ[iz], dzw = m.DZW[iz];
-    const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
-    T f0 = 0, f1 = 0, f2 = 0;
-    // ---- u-equation
-    {
-        const T br = dz * hy * (T(9) * (u0 + UU(1, 0, 0)) - (UU(-1, 0, 0) + UU(2, 0, 0)));
-        const T bl = dz * hy * (T(9) * (UU(-1, 0, 0) + u0) - (UU(-2, 0, 0) + UU(1, 0, 0)));
-        const T bbr = dzw * hy * (T(9) * (UU(1, 0, 0) + UU(2, 0, 0)) - (u0 + UU(3, 0, 0)));
-        const T bbl = dzw * hy * (T(9) * (UU(-2, 0, 0) + UU(-1, 0, 0)) - (UU(-3, 0, 0) + u0));
-        f0 += br * (u0 + UU(1, 0, 0)) - C27 * bbr * (u0 + UU(3, 0, 0)) - bl * (UU(-1, 0, 0) + u0) + C27 * bbl * (UU(-3, 0, 0) + u0);
-    }
-    {
-        const T br = dz * hx * XBAR(VV, 1, 0);
-        const T bl = dz * hx * XBAR(VV, 0, 0);
-        const T bbr = dzw * hx * XBAR(VV, 2, 0);
-        const T bbl = dzw * hx * XBAR(VV, -1, 0);
-        f0 += br * (u0 + UU(0, 1, 0)) - C27 * bbr * (u0 + UU(0, 3, 0)) - bl * (UU(0, -1, 0) + u0) + C27 * bbl * (UU(0, -3, 0) + u0);
-    }
-    {
-        const T bl = iz == 0 ? T(0) : hxy * XBAR(WW, 0, 0);
-        const T br = iz == NZ - 1 ? T(0) : hxy * XBAR(WW, 0, 1);
-        const T bbl = iz == 1 ? T(0) : iz == 0 ? br : hxy * XBAR(WW, 0, -1);
-        const T bbr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? bl : hxy * XBAR(WW, 0, 2);
-        const T q_r = iz == NZ - 1 ? T(0) : u0 + UU(0, 0, 1);
-        const T q_rr = u0 + (iz == NZ - 3 ? -UU(0, 0, 2) : iz == NZ - 2 ? T(0) : iz == NZ - 1 ? UU(0, 0, -2) : UU(0, 0, 3));
-        const T q_l = iz == 0 ? T(0) : UU(0, 0, -1) + u0;
-        const T q_ll = u0 + (iz == 2 ? -UU(0, 0, -2) : iz == 1 ? T(0) : iz == 0 ? UU(0, 0, 2) : UU(0, 0, -3));
-        f0 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
-    }
-    // ---- v-equation
-    {
-        const T br = dz * hy * YBAR(UU, 1, 0);
-        const T bl = dz * hy * YBAR(UU, 0, 0);
-        const T bbr = dzw * hy * YBAR(UU, 2, 0);
-        const T bbl = dzw * hy * YBAR(UU, -1, 0);
-        f1 += br * (v0 + VV(1, 0, 0)) - C27 * bbr * (v0 + VV(3, 0, 0)) - bl * (VV(-1, 0, 0) + v0) + C27 * bbl * (VV(-3, 0, 0) + v0);
-    }
-    {
-        const T br = dz * hx * (T(9) * (v0 + VV(0, 1, 0)) - (VV(0, -1, 0) + VV(0, 2, 0)));
-        const T bl = dz * hx * (T(9) * (VV(0, -1, 0) + v0) - (VV(0, -2, 0) + VV(0, 1, 0)));
-        const T bbr = dzw * hx * (T(9) * (VV(0, 1, 0) + VV(0, 2, 0)) - (v0 + VV(0, 3, 0)));
-        const T bbl = dzw * hx * (T(9) * (VV(0, -2, 0) + VV(0, -1, 0)) - (VV(0, -3, 0) + v0));
-        f1 += br * (v0 + VV(0, 1, 0)) - C27 * bbr * (v0 + VV(0, 3, 0)) - bl * (VV(0, -1, 0) + v0) + C27 * bbl * (VV(0, -3, 0) + v0);
-    }
-    {
-        const T bl = iz == 0 ? T(0) : hxy * YBAR(WW, 0, 0);
-        const T br = iz == NZ - 1 ? T(0) : hxy * YBAR(WW, 0, 1);
-        const T bbl = iz == 1 ? T(0) : iz == 0 ? br : hxy * YBAR(WW, 0, -1);
-        const T bbr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? bl : hxy * YBAR(WW, 0, 2);
-        const T q_r = iz == NZ - 1 ? T(0) : v0 + VV(0, 0, 1);
-        const T q_rr = v0 + (iz == NZ - 3 ? -VV(0, 0, 2) : iz == NZ - 2 ? T(0) : iz == NZ - 1 ? VV(0, 0, -2) : VV(0, 0, 3));
-        const T q_l = iz == 0 ? T(0) : VV(0, 0, -1) + v0;
-        const T q_ll = v0 + (iz == 2 ? -VV(0, 0, -2) : iz == 1 ? T(0) : iz == 0 ? VV(0, 0, 2) : VV(0, 0, -3));
-        f1 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
-    }
-    // ---- w-equation (no unknown at iz = 0)
-    if (iz != 0) {
-        {
-            const T br = hy * zbar_vv(m, vel, nb, 0, 1, 0, iz, m.DZ);
-            const T bl = hy * zbar_vv(m, vel, nb, 0, 0, 0, iz, m.DZ);
-            const T bbr = hy * zbar_vv(m, vel, nb, 0, 2, 0, iz, m.DZW);
-            const T bbl = hy * zbar_vv(m, vel, nb, 0, -1, 0, iz, m.DZW);
-            f2 += br * (w0 + WW(1, 0, 0)) - C27 * bbr * (w0 + WW(3, 0, 0)) - bl * (WW(-1, 0, 0) + w0) + C27 * bbl * (WW(-3, 0, 0) + w0);
-        }
-        {
-            // The reference scales this y-face flux by hy (staggered/func_global.h:475-487); kept.
-            const T br = hy * zbar_vv(m, vel, nb, 1, 0, 1, iz, m.DZ);
-            const T bl = hy * zbar_vv(m, vel, nb, 1, 0, 0, iz, m.DZ);
-            const T bbr = hy * zbar_vv(m, vel, nb, 1, 0, 2, iz, m.DZW);
-            const T bbl = hy * zbar_vv(m, vel, nb, 1, 0, -1, iz, m.DZW);
-            f2 += br * (w0 + WW(0, 1, 0)) - C27 * bbr * (w0 + WW(0, 3, 0)) - bl * (WW(0, -1, 0) + w0) + C27 * bbl * (WW(0, -3, 0) + w0);
-        }
-        {
-            const T wzl = WW(0, 0, -1);
-            const T wzr = iz == NZ - 1 ? T(0) : WW(0, 0, 1);
-            const T wzll = iz == 1 ? w0 : WW(0, 0, -2);
-            const T wzrr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? w0 : WW(0, 0, 2);
-            const T wzlll = iz == 1 ? WW(0, 0, 1) : iz == 2 ? WW(0, 0, -1) : WW(0, 0, -3);
-            const T wzrrr = iz == NZ - 3 ? T(0) : iz == NZ - 2 ? WW(0, 0, 1) : iz == NZ - 1 ? WW(0, 0, -1) : WW(0, 0, 3);
-            const T bl = hxy * (T(9) * (wzl + w0) - (wzll + wzr));
-            const T br = hxy * (T(9) * (w0 + wzr) - (wzl + wzrr));
-            const T bbl = hxy * (T(9) * (wzll + wzl) - (wzlll + w0));
-            const T bbr = hxy * (T(9) * (wzr + wzrr) - (w0 + wzrrr));
-            f2 += br * (w0 + wzr) - bl * (wzl + w0);
-            if (iz < NZ - 2) f2 -= C27 * bbr * (w0 + wzrrr);
-            else if (iz == NZ - 2) f2 -= C27 * bbr * (-WW(0, 0, 1) + w0);
-            else f2 -= C27 * bbr * (WW(0, 0, -1) + w0);
-            if (iz > 2) f2 += C27 * bbl * (wzlll + w0);
-            else if (iz == 2) f2 += C27 * bbl * (-WW(0, 0, -1) + w0);
-            else f2 += C27 * bbl * (WW(0, 0, 1) + w0);
-        }
-    }
-    const T minus_inv_vol = T(-1) / (T(32) * (dz - T(1.0 / 9.0) * dzw) * hx * hy);
-    f[0] = f0 * minus_inv_vol;
-    f[1] = f1 * minus_inv_vol;
-    f[2] = f2 * (-T(1. / 32.) * m.inv_hx * m.inv_hy * m.GCZ[iz]);
-}
-
-// ---------------------------------------------------------------------------
-// Viscous term, constant viscosity (staggered/func_global.h:533-562)
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void visc_cross(const Mesh<T>& m, const T* __restrict__ vel, const Nb& nb, int iz, T f[3]) {
-    const int NZ = m.nz;
-    const T dz = m.DZ[iz];
-    const T inv_hz = T(1) / dz;
-    const T inv_hz_alt = T(2) / (dz + (iz == 0 ? T(0) : m.DZ[iz - 1]));
-    const T ihx2 = m.inv_hx * m.inv_hx, ihy2 = m.inv_hy * m.inv_hy;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const T a0 = F3(vel, c, 0, 0, 0);
-        f[c] = ihx2 * (F3(vel, c, 1, 0, 0) + F3(vel, c, -1, 0, 0) - T(2) * a0);
-        f[c] += ihy2 * (F3(vel, c, 0, 1, 0) + F3(vel, c, 0, -1, 0) - T(2) * a0);
-    }
-    const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
-    if (iz != NZ - 1) {
-        const T inv_hz_up = T(2) / (dz + m.DZ[iz + 1]);
-        f[0] += (UU(0, 0, 1) - u0) * inv_hz_up * inv_hz;
-        f[1] += (VV(0, 0, 1) - v0) * inv_hz_up * inv_hz;
-        f[2] += (WW(0, 0, 1) - w0) * inv_hz * inv_hz_alt;
-    } else {
-        const T inv_hz_up = T(2) / dz;
-        f[0] += (-u0) * inv_hz_up * inv_hz;
-        f[1] += (-v0) * inv_hz_up * inv_hz;
-        f[2] += (-w0) * inv_hz * inv_hz_alt;
-    }
-    if (iz != 0) {
-        f[0] += (UU(0, 0, -1) - u0) * inv_hz_alt * inv_hz;
-        f[1] += (VV(0, 0, -1) - v0) * inv_hz_alt * inv_hz;
-        f[2] += (WW(0, 0, -1) - w0) * (T(1) / m.DZ[iz - 1]) * inv_hz_alt;
-    } else {
-        f[0] += (-u0) * inv_hz_alt * inv_hz;
-        f[1] += (-v0) * inv_hz_alt * inv_hz;
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) f[c] *= m.visc;
-}
-
-// ---------------------------------------------------------------------------
-// Viscous term, div(2 (nu + nu_t) S) (staggered/func_global.h:565-625),
-// including the edge-average of sxy_190 at :598 as the reference has it.
-// ---------------------------------------------------------------------------
-template <typename T, bool TURB>
-__device__ __forceinline__ void visc_naive(const Mesh<T>& m, const T* __restrict__ vel, const T* __restrict__ nut,
-                                           const Nb& nb, int iz, T f[3]) {
-    const int NZ = m.nz;
-    const T visc = m.visc;
-    auto mu = [&](int dx, int dy, int dz) -> T { return TURB ? visc + nut[nb.at(dx, dy, dz)] : visc; };
-    const T dz = m.DZ[iz];
-    const T inv_hz = T(1) / dz;
-    const T inv_hz_up = T(2) / (dz + (iz == NZ - 1 ? T(0) : m.DZ[iz + 1]));
-    const T inv_hz_down = T(2) / (dz + (iz == 0 ? T(0) : m.DZ[iz - 1]));
-    const T ihx = m.inv_hx, ihy = m.inv_hy;
-    const T q = T(0.25);
-    const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
-    const T mu0 = mu(0, 0, 0);
-    T f0 = 0, f1 = 0, f2 = 0;
-    const T sxx_000 = T(2) * (UU(1, 0, 0) - u0) * ihx * mu0;
-    const T sxx_800 = T(2) * (u0 - UU(-1, 0, 0)) * ihx * mu(-1, 0, 0);
-    f0 += (sxx_000 - sxx_800) * ihx;
-    const T sxy_910 = ((UU(0, 1, 0) - u0) * ihy + (VV(0, 1, 0) - VV(-1, 1, 0)) * ihx) * q * (mu0 + mu(-1, 0, 0) + mu(0, 1, 0) + mu(-1, 1, 0));
-    const T sxy_990 = ((u0 - UU(0, -1, 0)) * ihy + (v0 - VV(-1, 0, 0)) * ihx) * q * (mu(0, -1, 0) + mu(-1, -1, 0) + mu0 + mu(-1, 0, 0));
-    f0 += (sxy_910 - sxy_990) * ihy;
-    const T sxz_901 = (iz == NZ - 1) ? (T(-2) * u0 * inv_hz * visc)
-                                     : ((UU(0, 0, 1) - u0) * inv_hz_up + (WW(0, 0, 1) - WW(-1, 0, 1)) * ihx) * q *
-                                           (mu0 + mu(-1, 0, 0) + mu(0, 0, 1) + mu(-1, 0, 1));
-    const T sxz_909 = (iz == 0) ? (T(2) * u0 * inv_hz * visc)
-                                : ((u0 - UU(0, 0, -1)) * inv_hz_down + (w0 - WW(-1, 0, 0)) * ihx) * q *
-                                      (mu(0, 0, -1) + mu(-1, 0, -1) + mu0 + mu(-1, 0, 0));
-    f0 += (sxz_901 - sxz_909) * inv_hz;
-
-    const T sxy_190 = ((UU(1, 0, 0) - UU(1, -1, 0)) * ihy + (VV(1, 0, 0) - v0) * ihx) * q * (mu0 + mu(1, 0, 0) + mu(0, -1, 0) + mu(1, 1, 0));
-    f1 += (sxy_190 - sxy_990) * ihx;
-    const T syy_000 = T(2) * (VV(0, 1, 0) - v0) * ihy * mu0;
-    const T syy_080 = T(2) * (v0 - VV(0, -1, 0)) * ihy * mu(0, -1, 0);
-    f1 += (syy_000 - syy_080) * ihy;
-    const T syz_091 = (iz == NZ - 1) ? (T(-2) * v0 * inv_hz * visc)
-                                     : ((VV(0, 0, 1) - v0) * inv_hz_up + (WW(0, 0, 1) - WW(0, -1, 1)) * ihy) * q *
-                                           (mu0 + mu(0, -1, 0) + mu(0, 0, 1) + mu(0, -1, 1));
-    const T syz_099 = (iz == 0) ? (T(2) * v0 * inv_hz * visc)
-                                : ((v0 - VV(0, 0, -1)) * inv_hz_down + (w0 - WW(0, -1, 0)) * ihy) * q *
-                                      (mu(0, 0, -1) + mu(0, -1, -1) + mu0 + mu(0, -1, 0));
-    f1 += (syz_091 - syz_099) * inv_hz;
-
-    if (iz != 0) {
-        const T sxz_109 = ((UU(1, 0, 0) - UU(1, 0, -1)) * inv_hz_down + (WW(1, 0, 0) - w0) * ihx) * q *
-                          (mu(0, 0, -1) + mu(1, 0, -1) + mu0 + mu(1, 0, 0));
-        f2 += (sxz_109 - sxz_909) * ihx;
-        const T syz_019 = ((VV(0, 1, 0) - VV(0, 1, -1)) * inv_hz_down + (WW(0, 1, 0) - w0) * ihy) * q *
-                          (mu(0, 0, -1) + mu(0, 1, -1) + mu0 + mu(0, 1, 0));
-        f2 += (syz_019 - syz_099) * ihy;
-        const T szz_000 = T(2) * ((iz == NZ - 1 ? T(0) : WW(0, 0, 1)) - w0) * inv_hz * mu0;
-        const T szz_008 = T(2) * (w0 - WW(0, 0, -1)) * mu(0, 0, -1) / m.DZ[iz - 1];
-        f2 += (szz_000 - szz_008) * inv_hz_down;
-    }
-    f[0] = f0;
-    f[1] = f1;
-    f[2] = f2;
-}
-
-// ---------------------------------------------------------------------------
-// Collocated nodes: skew-symmetric central convection + viscous term
-// (collocated_AB3/func_fluxes.h:21-120). Interior nodes only.
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void conv_ab3(const Mesh<T>& m, const T* __restrict__ vel, const Nb& nb, int iz, T inv_hz, T f[3]) {
-    const T ih[3] = {m.inv_hx, m.inv_hy, inv_hz};
-    const T u0[3] = {UU(0, 0, 0), VV(0, 0, 0), WW(0, 0, 0)};
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const int jn = nb.at(d == 0 ? -1 : 0, d == 1 ? -1 : 0, d == 2 ? -1 : 0);
-        const int kn = nb.at(d == 0 ? 1 : 0, d == 1 ? 1 : 0, d == 2 ? 1 : 0);
-        const T uj = vel[3 * jn + d], uk = vel[3 * kn + d];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const T conv_div = T(0.5) * (vel[3 * kn + c] * uk - vel[3 * jn + c] * uj);
-            const T conv_adv = u0[d] * T(0.5) * (vel[3 * kn + c] - vel[3 * jn + c]);
-            f[c] -= T(0.5) * (conv_div + conv_adv) * ih[d];
-        }
-    }
-}
-
-// Q1-Galerkin viscous term with element nu_t (collocated_AB3/func_fluxes.h:21-85).
-template <typename T, bool TURB>
-__device__ __forceinline__ void visc_galerkin(const Mesh<T>& m, const T* __restrict__ vel, const T* __restrict__ nut,
-                                              const Nb& nb, int iz, T g_out[3]) {
-    const T C9 = T(1.0 / 9.0), C18 = T(1.0 / 18.0), C36 = T(1.0 / 36.0), C3 = T(1.0 / 3.0), C6 = T(1.0 / 6.0);
-    const T hx = m.hx, hy = m.hy;
-    T sum[3] = {0, 0, 0};
-#pragma unroll
-    for (int oz = 0; oz <= 1; oz++) {
-        const int sz = oz ? 1 : -1;
-        const T hz = oz == 0 ? m.DZ[iz - 1] : m.DZ[iz];
-        const T inv_hz = T(1) / hz;
-#pragma unroll
-        for (int oy = 0; oy <= 1; oy++) {
-            const int sy = oy ? 1 : -1;
-#pragma unroll
-            for (int ox = 0; ox <= 1; ox++) {
-                const int sx = ox ? 1 : -1;
-                const T mx = m.inv_hx * hy * hz;
-                const T my = m.inv_hy * hx * hz;
-                const T mz = inv_hz * hx * hy;
-                T U[8][3];  // corners: bit0 x, bit1 y, bit2 z
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const int id = nb.at((k & 1) ? sx : 0, (k & 2) ? sy : 0, (k & 4) ? sz : 0);
-                    U[k][0] = vel[3 * id];
-                    U[k][1] = vel[3 * id + 1];
-                    U[k][2] = vel[3 * id + 2];
-                }
-                // names as in the reference: in, in_x, in_y, in_xy, in_z, in_xz, in_yz, in_xyz
-#define Q(name, c) U[name][c]
-                const int I0 = 0, IX_ = 1, IY_ = 2, IXY = 3, IZ_ = 4, IXZ = 5, IYZ = 6, IXYZ = 7;
-                T g[3];
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-                    g[c] = (C9 * (Q(IX_, c) - Q(I0, c)) + C18 * (Q(IXY, c) - Q(IY_, c)) + C18 * (Q(IXZ, c) - Q(IZ_, c)) + C36 * (Q(IXYZ, c) - Q(IYZ, c))) * mx +
-                           (C9 * (Q(IY_, c) - Q(I0, c)) + C18 * (Q(IXY, c) - Q(IX_, c)) + C18 * (Q(IYZ, c) - Q(IZ_, c)) + C36 * (Q(IXYZ, c) - Q(IXZ, c))) * my +
-                           (C9 * (Q(IZ_, c) - Q(I0, c)) + C18 * (Q(IXZ, c) - Q(IX_, c)) + C18 * (Q(IYZ, c) - Q(IY_, c)) + C36 * (Q(IXYZ, c) - Q(IXY, c))) * mz;
-                T gxx = C9 * (Q(IX_, 0) - Q(I0, 0)) + C18 * (Q(IXY, 0) - Q(IY_, 0)) + C18 * (Q(IXZ, 0) - Q(IZ_, 0)) + C36 * (Q(IXYZ, 0) - Q(IYZ, 0));
-                T gxy = T(0.25) * (C3 * (Q(IX_, 1) - Q(I0, 1) + Q(IXY, 1) - Q(IY_, 1)) + C6 * (Q(IXZ, 1) - Q(IZ_, 1) + Q(IXYZ, 1) - Q(IYZ, 1)));
-                T gxz = T(0.25) * (C3 * (Q(IX_, 2) - Q(I0, 2) + Q(IXZ, 2) - Q(IZ_, 2)) + C6 * (Q(IXY, 2) - Q(IY_, 2) + Q(IXYZ, 2) - Q(IYZ, 2)));
-                T gyy = C9 * (Q(IY_, 1) - Q(I0, 1)) + C18 * (Q(IXY, 1) - Q(IX_, 1)) + C18 * (Q(IYZ, 1) - Q(IZ_, 1)) + C36 * (Q(IXYZ, 1) - Q(IXZ, 1));
-                T gyx = T(0.25) * (C3 * (Q(IY_, 0) - Q(I0, 0) + Q(IXY, 0) - Q(IX_, 0)) + C6 * (Q(IYZ, 0) - Q(IZ_, 0) + Q(IXYZ, 0) - Q(IXZ, 0)));
-                T gyz = T(0.25) * (C3 * (Q(IY_, 2) - Q(I0, 2) + Q(IYZ, 2) - Q(IZ_, 2)) + C6 * (Q(IXY, 2) - Q(IX_, 2) + Q(IXYZ, 2) - Q(IXZ, 2)));
-                T gzz = C9 * (Q(IZ_, 2) - Q(I0, 2)) + C18 * (Q(IYZ, 2) - Q(IY_, 2)) + C18 * (Q(IXZ, 2) - Q(IX_, 2)) + C36 * (Q(IXYZ, 2) - Q(IXY, 2));
-                T gzx = T(0.25) * (C3 * (Q(IZ_, 0) - Q(I0, 0) + Q(IXZ, 0) - Q(IX_, 0)) + C6 * (Q(IYZ, 0) - Q(IY_, 0) + Q(IXYZ, 0) - Q(IXY, 0)));
-                T gzy = T(0.25) * (C3 * (Q(IZ_, 1) - Q(I0, 1) + Q(IYZ, 1) - Q(IY_, 1)) + C6 * (Q(IXZ, 1) - Q(IX_, 1) + Q(IXYZ, 1) - Q(IXY, 1)));
-#undef Q
-                gxx *= mx;
-                gxy *= hz * T(sx * sy);
-                gxz *= hy * T(sx * sz);
-                gyy *= my;
-                gyx *= hz * T(sx * sy);
-                gyz *= hx * T(sy * sz);
-                gzz *= mz;
-                gzx *= hy * T(sx * sz);
-                gzy *= hx * T(sy * sz);
-                g[0] += gxx + gxy + gxz;
-                g[1] += gyy + gyx + gyz;
-                g[2] += gzz + gzx + gzy;
-                T vc = m.visc;
-                if (TURB) vc += nut[nb.at(ox ? 0 : -1, oy ? 0 : -1, oz ? 0 : -1)];
-#pragma unroll
-                for (int c = 0; c < 3; c++) sum[c] += g[c] * vc;
-            }
-        }
-    }
-    g_out[0] = sum[0];
-    g_out[1] = sum[1];
-    g_out[2] = sum[2];
 }
 
 // Gradient at one cell (staggered/func_global.h:297-303, :316-322; collocated_AB3/func_global.h:298-303).
@@ -453,88 +95,182 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 }
 
 // ---------------------------------------------------------------------------
-// K1: explicit term (+ BDF3/EX3 right-hand side)
+// K1: explicit term (+ BDF3/EX3 right-hand side), 2.5D-blocked.
+// A block owns a kTX x kTY tile of (x, y) columns and marches along z over a
+// chunk of planes. The velocity planes iz-RZ..iz+RZ of the tile plus a halo of
+// RXY cells sit in an LDS ring (one component per sub-array, so a wavefront
+// reads consecutive words); the fourth-order scheme's z+-3 values of the own
+// column come straight from global memory. Blocks are ordered so that each
+// XCD works on neighbouring tiles and shares their halos in its L2.
 // ---------------------------------------------------------------------------
+constexpr int kTX = 32, kTY = 8;
+
+__device__ __forceinline__ int xcd_swizzle(int b, int nb) {
+    const int q = nb / 8, r = nb % 8, x = b % 8, i = b / 8;
+    return x * q + min(x, r) + i;
+}
+
+template <typename T>
+struct GlobalNu {
+    const T* nut;
+    const Nb* nb;
+    __device__ __forceinline__ T operator()(int dx, int dy, int dz) const { return nut[nb->at(dx, dy, dz)]; }
+};
+
+template <typename T, int RXY, int RZ>
+struct RingAcc {
+    static constexpr int W = kTX + 2 * RXY, H = kTY + 2 * RXY, PS = W * H;
+    const T* pl[2 * RZ + 1];
+    int off;         // position of the cell in the padded tile
+    const T* gcol;   // the cell in global memory (own column beyond the ring)
+    int nxy3;
+    __device__ __forceinline__ T operator()(int c, int dx, int dy, int dz) const {
+        if (dz > RZ || dz < -RZ) return gcol[dz * nxy3 + c];
+        return pl[dz + RZ][c * PS + off + dy * W + dx];
+    }
+};
+
 template <typename T, int SCHEME, int VISC, bool TURB>
-__global__ __launch_bounds__(kBlock) void k_flux(Mesh<T> m, FluxArgs<T> a) {
-    __shared__ double sh[kBlock / 64];
-    const int iz = blockIdx.y;
-    int ix, iy;
-    const bool active = cell_of_thread(m.nx, m.nxy, ix, iy);
+__global__ __launch_bounds__(kTX * kTY) void k_flux(Mesh<T> m, FluxArgs<T> a, int ntx, int ntiles, int zchunk) {
+    constexpr int RXY = SCHEME == TC_SCHEME_VV ? 3 : 1;
+    constexpr int RZ = SCHEME == TC_SCHEME_VV ? 2 : 1;
+    constexpr int NS = 2 * RZ + 1;
+    using Acc = RingAcc<T, RXY, RZ>;
+    constexpr int W = Acc::W, PS = Acc::PS;
+    __shared__ T ring[NS * 3 * PS];
+    __shared__ double sh[kTX * kTY / 64];
+    const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int tile = lb % ntiles, chunk = lb / ntiles;
+    const int x0 = (tile % ntx) * kTX, y0 = (tile / ntx) * kTY;
+    const int z0 = chunk * zchunk, z1 = min(m.nz, z0 + zchunk);
+    const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+    const int ix = x0 + tx, iy = y0 + ty;
+    const bool active = ix < m.nx && iy < m.ny;
+    Nb nb;  // global neighbours for nu_t and the collocated pressure gradient
+    make_nb<1>(nb, active ? ix : 0, active ? iy : 0, 0, m.nx, m.ny);
+
+    auto load_plane = [&](int p) {
+        T* dst = ring + (p % NS) * 3 * PS;
+        const T* __restrict__ src = a.vel + (size_t)3 * p * m.nxy;
+        for (int e = threadIdx.x; e < PS; e += kTX * kTY) {
+            const int lx = e % W, ly = e / W;
+            int gx = (x0 + lx - RXY) % m.nx;
+            if (gx < 0) gx += m.nx;
+            int gy = (y0 + ly - RXY) % m.ny;
+            if (gy < 0) gy += m.ny;
+            const T* sp = src + 3 * (gy * m.nx + gx);
+            dst[e] = sp[0];
+            dst[PS + e] = sp[1];
+            dst[2 * PS + e] = sp[2];
+        }
+    };
+    for (int p = max(0, z0 - RZ); p < min(m.nz, z0 + RZ); p++) load_plane(p);
+
     double cfr = 0.0;
-    if (active) {
-        Nb nb;
-        make_nb<3>(nb, ix, iy, iz, m.nx, m.ny);
-        const T* __restrict__ vel = a.vel;
-        const int in = nb.at(0, 0, 0);
-        T k[3] = {0, 0, 0};
-        if (SCHEME == TC_SCHEME_AB3) {
-            if (iz != 0 && iz != m.nz - 1) {
-                const T inv_hz = T(2) / (m.DZ[iz - 1] + m.DZ[iz]);
-                conv_ab3(m, vel, nb, iz, inv_hz, k);
-                if (VISC == 1) {
-                    T g[3];
-                    visc_galerkin<T, TURB>(m, vel, a.nut, nb, iz, g);
-                    const T s = m.inv_hx * m.inv_hy * inv_hz;
+    for (int iz = z0; iz < z1; iz++) {
+        if (iz + RZ < m.nz) load_plane(iz + RZ);
+        __syncthreads();
+        if (active) {
+            Acc acc;
 #pragma unroll
-                    for (int c = 0; c < 3; c++) k[c] += g[c] * s;
-                } else {
-                    const T ihx2 = m.visc * m.inv_hx * m.inv_hx, ihy2 = m.visc * m.inv_hy * m.inv_hy;
+            for (int d = -RZ; d <= RZ; d++) acc.pl[d + RZ] = ring + (((iz + d) % NS + NS) % NS) * 3 * PS;
+            acc.off = (ty + RXY) * W + tx + RXY;
+            const int in = iz * m.nxy + iy * m.nx + ix;
+            acc.gcol = a.vel + 3 * in;
+            acc.nxy3 = 3 * m.nxy;
+            nb.base = iz * m.nxy;
+            GlobalNu<T> nu{a.nut, &nb};
+            T k[3] = {0, 0, 0};
+            if (SCHEME == TC_SCHEME_AB3) {
+                if (iz != 0 && iz != m.nz - 1) {
+                    const T inv_hz = T(2) / (m.DZ[iz - 1] + m.DZ[iz]);
+                    conv_ab3(m, acc, iz, inv_hz, k);
+                    if (VISC == 1) {
+                        T g[3];
+                        visc_galerkin<T, TURB>(m, acc, nu, iz, g);
+                        const T s = m.inv_hx * m.inv_hy * inv_hz;
 #pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        const T a0 = F3(vel, c, 0, 0, 0);
-                        k[c] += ihx2 * (F3(vel, c, 1, 0, 0) + F3(vel, c, -1, 0, 0) - T(2) * a0);
-                        k[c] += ihy2 * (F3(vel, c, 0, 1, 0) + F3(vel, c, 0, -1, 0) - T(2) * a0);
-                        k[c] += m.visc * ((F3(vel, c, 0, 0, 1) - a0) / m.DZ[iz] - (a0 - F3(vel, c, 0, 0, -1)) / m.DZ[iz - 1]) * inv_hz;
+                        for (int c = 0; c < 3; c++) k[c] += g[c] * s;
+                    } else {  // naive Laplacian (collocated_AB3/func_fluxes.h:110-113)
+                        const T ihx2 = m.visc * m.inv_hx * m.inv_hx, ihy2 = m.visc * m.inv_hy * m.inv_hy;
+#pragma unroll
+                        for (int c = 0; c < 3; c++) {
+                            const T a0 = acc(c, 0, 0, 0);
+                            k[c] += ihx2 * (acc(c, 1, 0, 0) + acc(c, -1, 0, 0) - T(2) * a0);
+                            k[c] += ihy2 * (acc(c, 0, 1, 0) + acc(c, 0, -1, 0) - T(2) * a0);
+                            k[c] += m.visc * ((acc(c, 0, 0, 1) - a0) / m.DZ[iz] - (a0 - acc(c, 0, 0, -1)) / m.DZ[iz - 1]) * inv_hz;
+                        }
                     }
+                    k[0] += m.sour;
                 }
-                k[0] += m.sour;
+            } else {  // ExplicitTerm (staggered/func_global.h:642-649)
+                T c[3], v[3];
+                if (SCHEME == TC_SCHEME_HW) conv_hw(m, acc, iz, c);
+                else conv_vv(m, acc, iz, c);
+                if (VISC == 1) visc_naive<T, TURB>(m, acc, nu, iz, v);
+                else visc_cross(m, acc, iz, v);
+                k[0] = (m.sour + c[0]) + v[0];
+                k[1] = (T(0) + c[1]) + v[1];
+                k[2] = iz == 0 ? T(0) : (T(0) + c[2]) + v[2];
             }
-        } else {
-            T c[3], v[3];
-            if (SCHEME == TC_SCHEME_HW) conv_hw(m, vel, nb, iz, c);
-            else conv_vv(m, vel, nb, iz, c);
-            if (VISC == 1) visc_naive<T, TURB>(m, vel, a.nut, nb, iz, v);
-            else visc_cross(m, vel, nb, iz, v);
-            k[0] = (m.sour + c[0]) + v[0];
-            k[1] = (T(0) + c[1]) + v[1];
-            k[2] = iz == 0 ? T(0) : (T(0) + c[2]) + v[2];
-        }
-        T* fc = a.fcur + 3 * in;
-        fc[0] = k[0];
-        fc[1] = k[1];
-        fc[2] = k[2];
-        if (a.order > 0) {
-            T r[3];
-            const T it = a.inv_tau;
+            T* fc = a.fcur + 3 * in;
+            fc[0] = k[0];
+            fc[1] = k[1];
+            fc[2] = k[2];
+            if (a.order > 0) {  // BDF/EX combine (staggered/func_global.h:909-932)
+                T r[3];
+                const T it = a.inv_tau;
 #pragma unroll
-            for (int cc = 0; cc < 3; cc++) {
-                const int j = 3 * in + cc;
-                if (a.order == 1) r[cc] = k[cc] + it * vel[j];
-                else if (a.order == 2) r[cc] = T(2) * k[cc] - a.fprev[j] + it * (T(2) * vel[j] - T(0.5) * a.vprev[j]);
-                else
-                    r[cc] = T(3) * k[cc] - T(3) * a.fprev[j] + a.fprev2[j] +
-                            it * (T(3) * vel[j] - T(1.5) * a.vprev[j] + T(1.0 / 3.0) * a.vprev2[j]);
+                for (int cc = 0; cc < 3; cc++) {
+                    const int j = 3 * in + cc;
+                    const T u0 = acc(cc, 0, 0, 0);
+                    if (a.order == 1) r[cc] = k[cc] + it * u0;
+                    else if (a.order == 2) r[cc] = T(2) * k[cc] - a.fprev[j] + it * (T(2) * u0 - T(0.5) * a.vprev[j]);
+                    else
+                        r[cc] = T(3) * k[cc] - T(3) * a.fprev[j] + a.fprev2[j] +
+                                it * (T(3) * u0 - T(1.5) * a.vprev[j] + T(1.0 / 3.0) * a.vprev2[j]);
+                }
+                if (SCHEME == TC_SCHEME_AB3 && a.stab) {  // collocated_AB3/func_global.h:435-437
+                    T g[3];
+                    grad_at<T, TC_SCHEME_AB3>(m, a.pcur, nb, iz, g);
+                    r[0] -= g[0];
+                    r[1] -= g[1];
+                    r[2] -= g[2];
+                }
+                T* fn = a.fnext + 3 * in;
+                fn[0] = r[0];
+                fn[1] = r[1];
+                fn[2] = r[2];
+                if (a.cfr_partial) cfr += (double)r[0] * u_volume<T, SCHEME>(m, iz);
             }
-            if (SCHEME == TC_SCHEME_AB3 && a.stab) {
-                T g[3];
-                grad_at<T, TC_SCHEME_AB3>(m, a.pcur, nb, iz, g);
-                r[0] -= g[0];
-                r[1] -= g[1];
-                r[2] -= g[2];
-            }
-            T* fn = a.fnext + 3 * in;
-            fn[0] = r[0];
-            fn[1] = r[1];
-            fn[2] = r[2];
-            if (a.cfr_partial) cfr = (double)r[0] * u_volume<T, SCHEME>(m, iz);
         }
+        __syncthreads();
     }
     if (a.cfr_partial) {
         const double s = block_sum(cfr, sh);
-        if (threadIdx.x == 0) a.cfr_partial[blockIdx.y * gridDim.x + blockIdx.x] = s;
+        if (threadIdx.x == 0) a.cfr_partial[blockIdx.x] = s;
     }
 }
+
+struct FluxGrid {
+    int ntx, ntiles, zchunk, nblocks;
+};
+inline FluxGrid flux_grid(int nx, int ny, int nz) {
+    FluxGrid g;
+    g.ntx = (nx + kTX - 1) / kTX;
+    g.ntiles = g.ntx * ((ny + kTY - 1) / kTY);
+    int nch = (512 + g.ntiles - 1) / g.ntiles;  // about two blocks per CU
+    nch = std::max(1, std::min(nch, std::max(1, nz / 8)));
+    g.zchunk = (nz + nch - 1) / nch;
+    g.nblocks = g.ntiles * ((nz + g.zchunk - 1) / g.zchunk);
+    return g;
+}
+
+// local accessors for the remaining direct-access kernels
+#define F3(a, c, dx, dy, dz) a[3 * nb.at(dx, dy, dz) + (c)]
+#define UU(dx, dy, dz) F3(vel, 0, dx, dy, dz)
+#define VV(dx, dy, dz) F3(vel, 1, dx, dy, dz)
+#define WW(dx, dy, dz) F3(vel, 2, dx, dy, dz)
 
 // ---------------------------------------------------------------------------
 // Turbulent viscosity
@@ -727,33 +463,36 @@ dim3 cell_grid(const Mesh<T>& m) {
 
 }  // namespace
 
-int flux_grid_blocks(int nxy, int nz) { return ((nxy + kBlock - 1) / kBlock) * nz; }
+int flux_grid_blocks(int nx, int ny, int nz) { return flux_grid(nx, ny, nz).nblocks; }
 
 template <typename T>
 hipError_t launch_flux(hipStream_t s, const Mesh<T>& m, const FluxArgs<T>& a) {
-    const dim3 g = cell_grid(m);
+    const FluxGrid fg = flux_grid(m.nx, m.ny, m.nz);
+    const dim3 g(fg.nblocks), b(kTX * kTY);
+#define L(SCH, VIS, TRB) hipLaunchKernelGGL((k_flux<T, SCH, VIS, TRB>), g, b, 0, s, m, a, fg.ntx, fg.ntiles, fg.zchunk)
     if (a.scheme == TC_SCHEME_HW) {
         if (a.visc_type == 1) {
-            if (a.turb) hipLaunchKernelGGL((k_flux<T, TC_SCHEME_HW, 1, true>), g, dim3(kBlock), 0, s, m, a);
-            else hipLaunchKernelGGL((k_flux<T, TC_SCHEME_HW, 1, false>), g, dim3(kBlock), 0, s, m, a);
+            if (a.turb) L(TC_SCHEME_HW, 1, true);
+            else L(TC_SCHEME_HW, 1, false);
         } else {
-            hipLaunchKernelGGL((k_flux<T, TC_SCHEME_HW, 0, false>), g, dim3(kBlock), 0, s, m, a);
+            L(TC_SCHEME_HW, 0, false);
         }
     } else if (a.scheme == TC_SCHEME_VV) {
         if (a.visc_type == 1) {
-            if (a.turb) hipLaunchKernelGGL((k_flux<T, TC_SCHEME_VV, 1, true>), g, dim3(kBlock), 0, s, m, a);
-            else hipLaunchKernelGGL((k_flux<T, TC_SCHEME_VV, 1, false>), g, dim3(kBlock), 0, s, m, a);
+            if (a.turb) L(TC_SCHEME_VV, 1, true);
+            else L(TC_SCHEME_VV, 1, false);
         } else {
-            hipLaunchKernelGGL((k_flux<T, TC_SCHEME_VV, 0, false>), g, dim3(kBlock), 0, s, m, a);
+            L(TC_SCHEME_VV, 0, false);
         }
     } else {
         if (a.visc_type == 1) {
-            if (a.turb) hipLaunchKernelGGL((k_flux<T, TC_SCHEME_AB3, 1, true>), g, dim3(kBlock), 0, s, m, a);
-            else hipLaunchKernelGGL((k_flux<T, TC_SCHEME_AB3, 1, false>), g, dim3(kBlock), 0, s, m, a);
+            if (a.turb) L(TC_SCHEME_AB3, 1, true);
+            else L(TC_SCHEME_AB3, 1, false);
         } else {
-            hipLaunchKernelGGL((k_flux<T, TC_SCHEME_AB3, 0, false>), g, dim3(kBlock), 0, s, m, a);
+            L(TC_SCHEME_AB3, 0, false);
         }
     }
+#undef L
     return hipGetLastError();
 }
 

@@ -1,0 +1,377 @@
+// Stencil formulas of the explicit term, written once against a field accessor
+// acc(c, dx, dy, dz) -> component c at offset (dx, dy, dz) from the cell, so the
+// same arithmetic serves the LDS-tiled kernels and plain global-memory access.
+// nu(dx, dy, dz) gives the turbulent viscosity at an offset.
+#pragma once
+
+#include "tc_internal.h"
+
+namespace tc {
+
+// Field accessors for real3 AoS arrays.
+#define F3(a, c, dx, dy, dz) acc(c, dx, dy, dz)
+#define UU(dx, dy, dz) acc(0, dx, dy, dz)
+#define VV(dx, dy, dz) acc(1, dx, dy, dz)
+#define WW(dx, dy, dz) acc(2, dx, dy, dz)
+
+// ---------------------------------------------------------------------------
+// Convection, Harlow-Welch (staggered/func_global.h:345-370)
+// ---------------------------------------------------------------------------
+template <typename T, class A>
+__device__ __forceinline__ void conv_hw(const Mesh<T>& m, const A& acc, int iz, T f[3]) {
+    const int NZ = m.nz;
+    const T dz = m.DZ[iz], dzm = iz == 0 ? T(0) : m.DZ[iz - 1];
+    const T inv_hz = T(1) / dz;
+    const T inv_hz_alt = T(2) / (dz + dzm);
+    const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
+    T f0 = 0, f1 = 0, f2 = 0;
+    f0 -= ((u0 + UU(1, 0, 0)) * (u0 + UU(1, 0, 0)) - (u0 + UU(-1, 0, 0)) * (u0 + UU(-1, 0, 0))) * m.inv_hx;
+    f0 -= ((u0 + UU(0, 1, 0)) * (VV(0, 1, 0) + VV(-1, 1, 0)) - (u0 + UU(0, -1, 0)) * (v0 + VV(-1, 0, 0))) * m.inv_hy;
+    if (iz != NZ - 1) f0 -= (u0 + UU(0, 0, 1)) * (WW(0, 0, 1) + WW(-1, 0, 1)) * inv_hz;
+    if (iz != 0) f0 += (u0 + UU(0, 0, -1)) * (w0 + WW(-1, 0, 0)) * inv_hz;
+
+    f1 -= ((v0 + VV(1, 0, 0)) * (UU(1, 0, 0) + UU(1, -1, 0)) - (v0 + VV(-1, 0, 0)) * (u0 + UU(0, -1, 0))) * m.inv_hx;
+    f1 -= ((v0 + VV(0, 1, 0)) * (v0 + VV(0, 1, 0)) - (v0 + VV(0, -1, 0)) * (v0 + VV(0, -1, 0))) * m.inv_hy;
+    if (iz != NZ - 1) f1 -= (v0 + VV(0, 0, 1)) * (WW(0, 0, 1) + WW(0, -1, 1)) * inv_hz;
+    if (iz != 0) f1 += (v0 + VV(0, 0, -1)) * (w0 + WW(0, -1, 0)) * inv_hz;
+
+    if (iz != 0) {
+        f2 -= ((w0 + WW(1, 0, 0)) * (UU(1, 0, 0) * dz + UU(1, 0, -1) * dzm) - (w0 + WW(-1, 0, 0)) * (u0 * dz + UU(0, 0, -1) * dzm)) *
+              m.inv_hx * inv_hz_alt;
+        f2 -= ((w0 + WW(0, 1, 0)) * (VV(0, 1, 0) * dz + VV(0, 1, -1) * dzm) - (w0 + WW(0, -1, 0)) * (v0 * dz + VV(0, 0, -1) * dzm)) *
+              m.inv_hy * inv_hz_alt;
+        const T wr = iz == NZ - 1 ? T(0) : WW(0, 0, 1);
+        f2 -= (w0 + wr) * (w0 + wr) * inv_hz_alt;
+        f2 += (w0 + WW(0, 0, -1)) * (w0 + WW(0, 0, -1)) * inv_hz_alt;
+    }
+    f[0] = f0 * T(0.25);
+    f[1] = f1 * T(0.25);
+    f[2] = f2 * T(0.25);
+}
+
+// ---------------------------------------------------------------------------
+// Convection, Verstappen-Veldman fourth order (staggered/func_global.h:374-529)
+// Mass fluxes through faces: "bar" (9/-1 interpolation, weight DZ) and
+// "barbar" (3h, weight DZW); near-wall mirror rules at iz in {0,1,2,NZ-3..NZ-1}.
+// ---------------------------------------------------------------------------
+// 4th-order interpolation of a transverse component along x at (dy, dz)
+#define XBAR(A, dy, dz) (T(9) * (A(0, dy, dz) + A(-1, dy, dz)) - (A(-2, dy, dz) + A(1, dy, dz)))
+#define YBAR(A, dx, dz) (T(9) * (A(dx, 0, dz) + A(dx, -1, dz)) - (A(dx, -2, dz) + A(dx, 1, dz)))
+
+template <typename T, class AC>
+__device__ __forceinline__ T zbar_vv(const Mesh<T>& m, const AC& acc, int c, int dx, int dy, int iz,
+                                     const T* __restrict__ A) {
+    // 9*(F(0)*A[iz] + F(-1)*A[iz-1]) - F(-2)*A[iz-2] - F(+1)*A[iz+1] with the wall rules
+    const int NZ = m.nz;
+    const T f0 = F3(vel, c, dx, dy, 0), fm = F3(vel, c, dx, dy, -1);
+    T r = T(9) * (f0 * A[iz] + fm * A[iz - 1]);
+    r -= (iz == 1) ? -fm * A[0] : F3(vel, c, dx, dy, -2) * A[iz - 2];
+    r -= (iz == NZ - 1) ? -f0 * A[NZ - 1] : F3(vel, c, dx, dy, 1) * A[iz + 1];
+    return r;
+}
+
+template <typename T, class A>
+__device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, T f[3]) {
+    const int NZ = m.nz;
+    const T C27 = T(1.0 / 27.0);
+    const T hx = m.hx, hy = m.hy, hxy = hx * hy;
+    const T dz = m.DZ[iz], dzw = m.DZW[iz];
+    const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
+    T f0 = 0, f1 = 0, f2 = 0;
+    // ---- u-equation
+    {
+        const T br = dz * hy * (T(9) * (u0 + UU(1, 0, 0)) - (UU(-1, 0, 0) + UU(2, 0, 0)));
+        const T bl = dz * hy * (T(9) * (UU(-1, 0, 0) + u0) - (UU(-2, 0, 0) + UU(1, 0, 0)));
+        const T bbr = dzw * hy * (T(9) * (UU(1, 0, 0) + UU(2, 0, 0)) - (u0 + UU(3, 0, 0)));
+        const T bbl = dzw * hy * (T(9) * (UU(-2, 0, 0) + UU(-1, 0, 0)) - (UU(-3, 0, 0) + u0));
+        f0 += br * (u0 + UU(1, 0, 0)) - C27 * bbr * (u0 + UU(3, 0, 0)) - bl * (UU(-1, 0, 0) + u0) + C27 * bbl * (UU(-3, 0, 0) + u0);
+    }
+    {
+        const T br = dz * hx * XBAR(VV, 1, 0);
+        const T bl = dz * hx * XBAR(VV, 0, 0);
+        const T bbr = dzw * hx * XBAR(VV, 2, 0);
+        const T bbl = dzw * hx * XBAR(VV, -1, 0);
+        f0 += br * (u0 + UU(0, 1, 0)) - C27 * bbr * (u0 + UU(0, 3, 0)) - bl * (UU(0, -1, 0) + u0) + C27 * bbl * (UU(0, -3, 0) + u0);
+    }
+    {
+        const T bl = iz == 0 ? T(0) : hxy * XBAR(WW, 0, 0);
+        const T br = iz == NZ - 1 ? T(0) : hxy * XBAR(WW, 0, 1);
+        const T bbl = iz == 1 ? T(0) : iz == 0 ? br : hxy * XBAR(WW, 0, -1);
+        const T bbr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? bl : hxy * XBAR(WW, 0, 2);
+        const T q_r = iz == NZ - 1 ? T(0) : u0 + UU(0, 0, 1);
+        const T q_rr = u0 + (iz == NZ - 3 ? -UU(0, 0, 2) : iz == NZ - 2 ? T(0) : iz == NZ - 1 ? UU(0, 0, -2) : UU(0, 0, 3));
+        const T q_l = iz == 0 ? T(0) : UU(0, 0, -1) + u0;
+        const T q_ll = u0 + (iz == 2 ? -UU(0, 0, -2) : iz == 1 ? T(0) : iz == 0 ? UU(0, 0, 2) : UU(0, 0, -3));
+        f0 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
+    }
+    // ---- v-equation
+    {
+        const T br = dz * hy * YBAR(UU, 1, 0);
+        const T bl = dz * hy * YBAR(UU, 0, 0);
+        const T bbr = dzw * hy * YBAR(UU, 2, 0);
+        const T bbl = dzw * hy * YBAR(UU, -1, 0);
+        f1 += br * (v0 + VV(1, 0, 0)) - C27 * bbr * (v0 + VV(3, 0, 0)) - bl * (VV(-1, 0, 0) + v0) + C27 * bbl * (VV(-3, 0, 0) + v0);
+    }
+    {
+        const T br = dz * hx * (T(9) * (v0 + VV(0, 1, 0)) - (VV(0, -1, 0) + VV(0, 2, 0)));
+        const T bl = dz * hx * (T(9) * (VV(0, -1, 0) + v0) - (VV(0, -2, 0) + VV(0, 1, 0)));
+        const T bbr = dzw * hx * (T(9) * (VV(0, 1, 0) + VV(0, 2, 0)) - (v0 + VV(0, 3, 0)));
+        const T bbl = dzw * hx * (T(9) * (VV(0, -2, 0) + VV(0, -1, 0)) - (VV(0, -3, 0) + v0));
+        f1 += br * (v0 + VV(0, 1, 0)) - C27 * bbr * (v0 + VV(0, 3, 0)) - bl * (VV(0, -1, 0) + v0) + C27 * bbl * (VV(0, -3, 0) + v0);
+    }
+    {
+        const T bl = iz == 0 ? T(0) : hxy * YBAR(WW, 0, 0);
+        const T br = iz == NZ - 1 ? T(0) : hxy * YBAR(WW, 0, 1);
+        const T bbl = iz == 1 ? T(0) : iz == 0 ? br : hxy * YBAR(WW, 0, -1);
+        const T bbr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? bl : hxy * YBAR(WW, 0, 2);
+        const T q_r = iz == NZ - 1 ? T(0) : v0 + VV(0, 0, 1);
+        const T q_rr = v0 + (iz == NZ - 3 ? -VV(0, 0, 2) : iz == NZ - 2 ? T(0) : iz == NZ - 1 ? VV(0, 0, -2) : VV(0, 0, 3));
+        const T q_l = iz == 0 ? T(0) : VV(0, 0, -1) + v0;
+        const T q_ll = v0 + (iz == 2 ? -VV(0, 0, -2) : iz == 1 ? T(0) : iz == 0 ? VV(0, 0, 2) : VV(0, 0, -3));
+        f1 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
+    }
+    // ---- w-equation (no unknown at iz = 0)
+    if (iz != 0) {
+        {
+            const T br = hy * zbar_vv(m, acc, 0, 1, 0, iz, m.DZ);
+            const T bl = hy * zbar_vv(m, acc, 0, 0, 0, iz, m.DZ);
+            const T bbr = hy * zbar_vv(m, acc, 0, 2, 0, iz, m.DZW);
+            const T bbl = hy * zbar_vv(m, acc, 0, -1, 0, iz, m.DZW);
+            f2 += br * (w0 + WW(1, 0, 0)) - C27 * bbr * (w0 + WW(3, 0, 0)) - bl * (WW(-1, 0, 0) + w0) + C27 * bbl * (WW(-3, 0, 0) + w0);
+        }
+        {
+            // The reference scales this y-face flux by hy (staggered/func_global.h:475-487); kept.
+            const T br = hy * zbar_vv(m, acc, 1, 0, 1, iz, m.DZ);
+            const T bl = hy * zbar_vv(m, acc, 1, 0, 0, iz, m.DZ);
+            const T bbr = hy * zbar_vv(m, acc, 1, 0, 2, iz, m.DZW);
+            const T bbl = hy * zbar_vv(m, acc, 1, 0, -1, iz, m.DZW);
+            f2 += br * (w0 + WW(0, 1, 0)) - C27 * bbr * (w0 + WW(0, 3, 0)) - bl * (WW(0, -1, 0) + w0) + C27 * bbl * (WW(0, -3, 0) + w0);
+        }
+        {
+            const T wzl = WW(0, 0, -1);
+            const T wzr = iz == NZ - 1 ? T(0) : WW(0, 0, 1);
+            const T wzll = iz == 1 ? w0 : WW(0, 0, -2);
+            const T wzrr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? w0 : WW(0, 0, 2);
+            const T wzlll = iz == 1 ? WW(0, 0, 1) : iz == 2 ? WW(0, 0, -1) : WW(0, 0, -3);
+            const T wzrrr = iz == NZ - 3 ? T(0) : iz == NZ - 2 ? WW(0, 0, 1) : iz == NZ - 1 ? WW(0, 0, -1) : WW(0, 0, 3);
+            const T bl = hxy * (T(9) * (wzl + w0) - (wzll + wzr));
+            const T br = hxy * (T(9) * (w0 + wzr) - (wzl + wzrr));
+            const T bbl = hxy * (T(9) * (wzll + wzl) - (wzlll + w0));
+            const T bbr = hxy * (T(9) * (wzr + wzrr) - (w0 + wzrrr));
+            f2 += br * (w0 + wzr) - bl * (wzl + w0);
+            if (iz < NZ - 2) f2 -= C27 * bbr * (w0 + wzrrr);
+            else if (iz == NZ - 2) f2 -= C27 * bbr * (-WW(0, 0, 1) + w0);
+            else f2 -= C27 * bbr * (WW(0, 0, -1) + w0);
+            if (iz > 2) f2 += C27 * bbl * (wzlll + w0);
+            else if (iz == 2) f2 += C27 * bbl * (-WW(0, 0, -1) + w0);
+            else f2 += C27 * bbl * (WW(0, 0, 1) + w0);
+        }
+    }
+    const T minus_inv_vol = T(-1) / (T(32) * (dz - T(1.0 / 9.0) * dzw) * hx * hy);
+    f[0] = f0 * minus_inv_vol;
+    f[1] = f1 * minus_inv_vol;
+    f[2] = f2 * (-T(1. / 32.) * m.inv_hx * m.inv_hy * m.GCZ[iz]);
+}
+
+// ---------------------------------------------------------------------------
+// Viscous term, constant viscosity (staggered/func_global.h:533-562)
+// ---------------------------------------------------------------------------
+template <typename T, class A>
+__device__ __forceinline__ void visc_cross(const Mesh<T>& m, const A& acc, int iz, T f[3]) {
+    const int NZ = m.nz;
+    const T dz = m.DZ[iz];
+    const T inv_hz = T(1) / dz;
+    const T inv_hz_alt = T(2) / (dz + (iz == 0 ? T(0) : m.DZ[iz - 1]));
+    const T ihx2 = m.inv_hx * m.inv_hx, ihy2 = m.inv_hy * m.inv_hy;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const T a0 = F3(vel, c, 0, 0, 0);
+        f[c] = ihx2 * (F3(vel, c, 1, 0, 0) + F3(vel, c, -1, 0, 0) - T(2) * a0);
+        f[c] += ihy2 * (F3(vel, c, 0, 1, 0) + F3(vel, c, 0, -1, 0) - T(2) * a0);
+    }
+    const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
+    if (iz != NZ - 1) {
+        const T inv_hz_up = T(2) / (dz + m.DZ[iz + 1]);
+        f[0] += (UU(0, 0, 1) - u0) * inv_hz_up * inv_hz;
+        f[1] += (VV(0, 0, 1) - v0) * inv_hz_up * inv_hz;
+        f[2] += (WW(0, 0, 1) - w0) * inv_hz * inv_hz_alt;
+    } else {
+        const T inv_hz_up = T(2) / dz;
+        f[0] += (-u0) * inv_hz_up * inv_hz;
+        f[1] += (-v0) * inv_hz_up * inv_hz;
+        f[2] += (-w0) * inv_hz * inv_hz_alt;
+    }
+    if (iz != 0) {
+        f[0] += (UU(0, 0, -1) - u0) * inv_hz_alt * inv_hz;
+        f[1] += (VV(0, 0, -1) - v0) * inv_hz_alt * inv_hz;
+        f[2] += (WW(0, 0, -1) - w0) * (T(1) / m.DZ[iz - 1]) * inv_hz_alt;
+    } else {
+        f[0] += (-u0) * inv_hz_alt * inv_hz;
+        f[1] += (-v0) * inv_hz_alt * inv_hz;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) f[c] *= m.visc;
+}
+
+// ---------------------------------------------------------------------------
+// Viscous term, div(2 (nu + nu_t) S) (staggered/func_global.h:565-625),
+// including the edge-average of sxy_190 at :598 as the reference has it.
+// ---------------------------------------------------------------------------
+template <typename T, bool TURB, class A, class NU>
+__device__ __forceinline__ void visc_naive(const Mesh<T>& m, const A& acc, const NU& nu, int iz, T f[3]) {
+    const int NZ = m.nz;
+    const T visc = m.visc;
+    auto mu = [&](int dx, int dy, int dz) -> T { return TURB ? visc + nu(dx, dy, dz) : visc; };
+    const T dz = m.DZ[iz];
+    const T inv_hz = T(1) / dz;
+    const T inv_hz_up = T(2) / (dz + (iz == NZ - 1 ? T(0) : m.DZ[iz + 1]));
+    const T inv_hz_down = T(2) / (dz + (iz == 0 ? T(0) : m.DZ[iz - 1]));
+    const T ihx = m.inv_hx, ihy = m.inv_hy;
+    const T q = T(0.25);
+    const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
+    const T mu0 = mu(0, 0, 0);
+    T f0 = 0, f1 = 0, f2 = 0;
+    const T sxx_000 = T(2) * (UU(1, 0, 0) - u0) * ihx * mu0;
+    const T sxx_800 = T(2) * (u0 - UU(-1, 0, 0)) * ihx * mu(-1, 0, 0);
+    f0 += (sxx_000 - sxx_800) * ihx;
+    const T sxy_910 = ((UU(0, 1, 0) - u0) * ihy + (VV(0, 1, 0) - VV(-1, 1, 0)) * ihx) * q * (mu0 + mu(-1, 0, 0) + mu(0, 1, 0) + mu(-1, 1, 0));
+    const T sxy_990 = ((u0 - UU(0, -1, 0)) * ihy + (v0 - VV(-1, 0, 0)) * ihx) * q * (mu(0, -1, 0) + mu(-1, -1, 0) + mu0 + mu(-1, 0, 0));
+    f0 += (sxy_910 - sxy_990) * ihy;
+    const T sxz_901 = (iz == NZ - 1) ? (T(-2) * u0 * inv_hz * visc)
+                                     : ((UU(0, 0, 1) - u0) * inv_hz_up + (WW(0, 0, 1) - WW(-1, 0, 1)) * ihx) * q *
+                                           (mu0 + mu(-1, 0, 0) + mu(0, 0, 1) + mu(-1, 0, 1));
+    const T sxz_909 = (iz == 0) ? (T(2) * u0 * inv_hz * visc)
+                                : ((u0 - UU(0, 0, -1)) * inv_hz_down + (w0 - WW(-1, 0, 0)) * ihx) * q *
+                                      (mu(0, 0, -1) + mu(-1, 0, -1) + mu0 + mu(-1, 0, 0));
+    f0 += (sxz_901 - sxz_909) * inv_hz;
+
+    const T sxy_190 = ((UU(1, 0, 0) - UU(1, -1, 0)) * ihy + (VV(1, 0, 0) - v0) * ihx) * q * (mu0 + mu(1, 0, 0) + mu(0, -1, 0) + mu(1, 1, 0));
+    f1 += (sxy_190 - sxy_990) * ihx;
+    const T syy_000 = T(2) * (VV(0, 1, 0) - v0) * ihy * mu0;
+    const T syy_080 = T(2) * (v0 - VV(0, -1, 0)) * ihy * mu(0, -1, 0);
+    f1 += (syy_000 - syy_080) * ihy;
+    const T syz_091 = (iz == NZ - 1) ? (T(-2) * v0 * inv_hz * visc)
+                                     : ((VV(0, 0, 1) - v0) * inv_hz_up + (WW(0, 0, 1) - WW(0, -1, 1)) * ihy) * q *
+                                           (mu0 + mu(0, -1, 0) + mu(0, 0, 1) + mu(0, -1, 1));
+    const T syz_099 = (iz == 0) ? (T(2) * v0 * inv_hz * visc)
+                                : ((v0 - VV(0, 0, -1)) * inv_hz_down + (w0 - WW(0, -1, 0)) * ihy) * q *
+                                      (mu(0, 0, -1) + mu(0, -1, -1) + mu0 + mu(0, -1, 0));
+    f1 += (syz_091 - syz_099) * inv_hz;
+
+    if (iz != 0) {
+        const T sxz_109 = ((UU(1, 0, 0) - UU(1, 0, -1)) * inv_hz_down + (WW(1, 0, 0) - w0) * ihx) * q *
+                          (mu(0, 0, -1) + mu(1, 0, -1) + mu0 + mu(1, 0, 0));
+        f2 += (sxz_109 - sxz_909) * ihx;
+        const T syz_019 = ((VV(0, 1, 0) - VV(0, 1, -1)) * inv_hz_down + (WW(0, 1, 0) - w0) * ihy) * q *
+                          (mu(0, 0, -1) + mu(0, 1, -1) + mu0 + mu(0, 1, 0));
+        f2 += (syz_019 - syz_099) * ihy;
+        const T szz_000 = T(2) * ((iz == NZ - 1 ? T(0) : WW(0, 0, 1)) - w0) * inv_hz * mu0;
+        const T szz_008 = T(2) * (w0 - WW(0, 0, -1)) * mu(0, 0, -1) / m.DZ[iz - 1];
+        f2 += (szz_000 - szz_008) * inv_hz_down;
+    }
+    f[0] = f0;
+    f[1] = f1;
+    f[2] = f2;
+}
+
+// ---------------------------------------------------------------------------
+// Collocated nodes: skew-symmetric central convection + viscous term
+// (collocated_AB3/func_fluxes.h:21-120). Interior nodes only.
+// ---------------------------------------------------------------------------
+template <typename T, class A>
+__device__ __forceinline__ void conv_ab3(const Mesh<T>& m, const A& acc, int iz, T inv_hz, T f[3]) {
+    const T ih[3] = {m.inv_hx, m.inv_hy, inv_hz};
+    const T u0[3] = {UU(0, 0, 0), VV(0, 0, 0), WW(0, 0, 0)};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int ox = d == 0, oy = d == 1, oz = d == 2;
+        const T uj = acc(d, -ox, -oy, -oz), uk = acc(d, ox, oy, oz);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const T ck = acc(c, ox, oy, oz), cj = acc(c, -ox, -oy, -oz);
+            const T conv_div = T(0.5) * (ck * uk - cj * uj);
+            const T conv_adv = u0[d] * T(0.5) * (ck - cj);
+            f[c] -= T(0.5) * (conv_div + conv_adv) * ih[d];
+        }
+    }
+}
+
+// Q1-Galerkin viscous term with element nu_t (collocated_AB3/func_fluxes.h:21-85).
+template <typename T, bool TURB, class A, class NU>
+__device__ __forceinline__ void visc_galerkin(const Mesh<T>& m, const A& acc, const NU& nu, int iz, T g_out[3]) {
+    const T C9 = T(1.0 / 9.0), C18 = T(1.0 / 18.0), C36 = T(1.0 / 36.0), C3 = T(1.0 / 3.0), C6 = T(1.0 / 6.0);
+    const T hx = m.hx, hy = m.hy;
+    T sum[3] = {0, 0, 0};
+#pragma unroll
+    for (int oz = 0; oz <= 1; oz++) {
+        const int sz = oz ? 1 : -1;
+        const T hz = oz == 0 ? m.DZ[iz - 1] : m.DZ[iz];
+        const T inv_hz = T(1) / hz;
+#pragma unroll
+        for (int oy = 0; oy <= 1; oy++) {
+            const int sy = oy ? 1 : -1;
+#pragma unroll
+            for (int ox = 0; ox <= 1; ox++) {
+                const int sx = ox ? 1 : -1;
+                const T mx = m.inv_hx * hy * hz;
+                const T my = m.inv_hy * hx * hz;
+                const T mz = inv_hz * hx * hy;
+                T U[8][3];  // corners: bit0 x, bit1 y, bit2 z
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int dx = (k & 1) ? sx : 0, dy = (k & 2) ? sy : 0, dz = (k & 4) ? sz : 0;
+                    U[k][0] = acc(0, dx, dy, dz);
+                    U[k][1] = acc(1, dx, dy, dz);
+                    U[k][2] = acc(2, dx, dy, dz);
+                }
+                // names as in the reference: in, in_x, in_y, in_xy, in_z, in_xz, in_yz, in_xyz
+#define Q(name, c) U[name][c]
+                const int I0 = 0, IX_ = 1, IY_ = 2, IXY = 3, IZ_ = 4, IXZ = 5, IYZ = 6, IXYZ = 7;
+                T g[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    g[c] = (C9 * (Q(IX_, c) - Q(I0, c)) + C18 * (Q(IXY, c) - Q(IY_, c)) + C18 * (Q(IXZ, c) - Q(IZ_, c)) + C36 * (Q(IXYZ, c) - Q(IYZ, c))) * mx +
+                           (C9 * (Q(IY_, c) - Q(I0, c)) + C18 * (Q(IXY, c) - Q(IX_, c)) + C18 * (Q(IYZ, c) - Q(IZ_, c)) + C36 * (Q(IXYZ, c) - Q(IXZ, c))) * my +
+                           (C9 * (Q(IZ_, c) - Q(I0, c)) + C18 * (Q(IXZ, c) - Q(IX_, c)) + C18 * (Q(IYZ, c) - Q(IY_, c)) + C36 * (Q(IXYZ, c) - Q(IXY, c))) * mz;
+                T gxx = C9 * (Q(IX_, 0) - Q(I0, 0)) + C18 * (Q(IXY, 0) - Q(IY_, 0)) + C18 * (Q(IXZ, 0) - Q(IZ_, 0)) + C36 * (Q(IXYZ, 0) - Q(IYZ, 0));
+                T gxy = T(0.25) * (C3 * (Q(IX_, 1) - Q(I0, 1) + Q(IXY, 1) - Q(IY_, 1)) + C6 * (Q(IXZ, 1) - Q(IZ_, 1) + Q(IXYZ, 1) - Q(IYZ, 1)));
+                T gxz = T(0.25) * (C3 * (Q(IX_, 2) - Q(I0, 2) + Q(IXZ, 2) - Q(IZ_, 2)) + C6 * (Q(IXY, 2) - Q(IY_, 2) + Q(IXYZ, 2) - Q(IYZ, 2)));
+                T gyy = C9 * (Q(IY_, 1) - Q(I0, 1)) + C18 * (Q(IXY, 1) - Q(IX_, 1)) + C18 * (Q(IYZ, 1) - Q(IZ_, 1)) + C36 * (Q(IXYZ, 1) - Q(IXZ, 1));
+                T gyx = T(0.25) * (C3 * (Q(IY_, 0) - Q(I0, 0) + Q(IXY, 0) - Q(IX_, 0)) + C6 * (Q(IYZ, 0) - Q(IZ_, 0) + Q(IXYZ, 0) - Q(IXZ, 0)));
+                T gyz = T(0.25) * (C3 * (Q(IY_, 2) - Q(I0, 2) + Q(IYZ, 2) - Q(IZ_, 2)) + C6 * (Q(IXY, 2) - Q(IX_, 2) + Q(IXYZ, 2) - Q(IXZ, 2)));
+                T gzz = C9 * (Q(IZ_, 2) - Q(I0, 2)) + C18 * (Q(IYZ, 2) - Q(IY_, 2)) + C18 * (Q(IXZ, 2) - Q(IX_, 2)) + C36 * (Q(IXYZ, 2) - Q(IXY, 2));
+                T gzx = T(0.25) * (C3 * (Q(IZ_, 0) - Q(I0, 0) + Q(IXZ, 0) - Q(IX_, 0)) + C6 * (Q(IYZ, 0) - Q(IY_, 0) + Q(IXYZ, 0) - Q(IXY, 0)));
+                T gzy = T(0.25) * (C3 * (Q(IZ_, 1) - Q(I0, 1) + Q(IYZ, 1) - Q(IY_, 1)) + C6 * (Q(IXZ, 1) - Q(IX_, 1) + Q(IXYZ, 1) - Q(IXY, 1)));
+#undef Q
+                gxx *= mx;
+                gxy *= hz * T(sx * sy);
+                gxz *= hy * T(sx * sz);
+                gyy *= my;
+                gyx *= hz * T(sx * sy);
+                gyz *= hx * T(sy * sz);
+                gzz *= mz;
+                gzx *= hy * T(sx * sz);
+                gzy *= hx * T(sy * sz);
+                g[0] += gxx + gxy + gxz;
+                g[1] += gyy + gyx + gyz;
+                g[2] += gzz + gzx + gzy;
+                T vc = m.visc;
+                if (TURB) vc += nu(ox ? 0 : -1, oy ? 0 : -1, oz ? 0 : -1);
+#pragma unroll
+                for (int c = 0; c < 3; c++) sum[c] += g[c] * vc;
+            }
+        }
+    }
+    g_out[0] = sum[0];
+    g_out[1] = sum[1];
+    g_out[2] = sum[2];
+}
+
+#undef F3
+#undef UU
+#undef VV
+#undef WW
+#undef XBAR
+#undef YBAR
+
+}  // namespace tc
