@@ -31,8 +31,7 @@ PEAK_HBM_GBS = 8000.0
 # element counted once; halos and cache reuse not counted).
 REALS_PER_POINT = {
     "flux": 21,      # reads u_n, u_n-1, u_n-2, F_n-1, F_n-2 (15); writes F_n, r (6)
-    "div": 4,        # reads r (3); writes div (1)
-    "fft_x": 2,      # reads div (1); writes the half spectrum (~1)
+    "fft_x": 4,      # divergence fused in: reads r (3); writes the half spectrum (~1)
     "fft_y": 2,
     "zsolve": 4,     # two passes over the spectrum: read + write each
     "ifft_y": 2,

@@ -38,8 +38,8 @@ enum tc_status { TC_OK = 0, TC_ERR_ARG = 1, TC_ERR_HIP = 2, TC_ERR_STATE = 3, TC
 enum tc_kernel {
     TC_K_TURB_VISC = 0, /* Smagorinsky-van Driest nu_t */
     TC_K_FLUX = 1,      /* explicit term + BDF3/EX3 right-hand side */
-    TC_K_DIV = 2,       /* divergence of the right-hand side */
-    TC_K_FFT_X = 3,     /* forward real FFT along x */
+    TC_K_DIV = 2,       /* divergence of the right-hand side: fused into TC_K_FFT_X, no launch of its own */
+    TC_K_FFT_X = 3,     /* divergence + forward real FFT along x */
     TC_K_FFT_Y = 4,     /* forward complex FFT along y */
     TC_K_ZSOLVE = 5,    /* banded wall-normal solve per (kx, ky) */
     TC_K_IFFT_Y = 6,    /* inverse complex FFT along y */

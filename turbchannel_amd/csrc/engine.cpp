@@ -71,7 +71,6 @@ struct tc_engine {
     void* flux[4] = {};
     void* pres[4] = {};
     void* nut = nullptr;
-    void* divv = nullptr;
     void* dp = nullptr;   // pressure increment (collocated StabType 1) / scratch pressure
     void* spec = nullptr;
     void* dmesh = nullptr;  // mesh arrays in T
@@ -197,8 +196,6 @@ int setup_device(tc_engine* e) {
         if (int r = dalloc(e, &p, n3)) return r;
         e->nut = p;
     }
-    if (int r = dalloc(e, &p, n3)) return r;
-    e->divv = p;
     if (int r = dalloc(e, &p, n3)) return r;
     e->dp = p;
     if (int r = dalloc_bytes(e, &e->spec, (size_t)e->nm * e->nz * sizeof(Cplx<T>))) return r;
@@ -341,45 +338,37 @@ ZFacDev<double> zfac_low_dev(tc_engine* e) {
     return f;
 }
 
-// Pressure solve f -> x (LinearSystemSolveFFT incl. NormalizePressure).
+// UnapplyGradient: p = L^-1 D v (ApplyDivergence + LinearSystemSolveFFT incl.
+// NormalizePressure, staggered/func_global.h:214-248, :290-293).
 template <typename T>
-int solve(tc_engine* e, const T* f, T* x) {
+int unapply(tc_engine* e, const T* v, T* x) {
     hipEvent_t ev;
     Cplx<T>* spec = (Cplx<T>*)e->spec;
     const int nrows = e->ny * e->nz;
     e->tbeg(TC_K_FFT_X, ev);
-    HIPCHK(launch_fft_x_fwd<T>(e->s, e->plx, (const Cplx<T>*)e->twx, f, spec, nrows));
+    HIPCHK(launch_fft_x_fwd_div<T>(e->s, e->plx, (const Cplx<T>*)e->twx, e->mesh<T>(), e->P.scheme, v, spec));
     e->tend(TC_K_FFT_X, ev);
     e->tbeg(TC_K_FFT_Y, ev);
-    HIPCHK(launch_fft_y<T>(e->s, e->ply, (const Cplx<T>*)e->twy, spec, e->nxh, e->nz, 0));
+    HIPCHK(launch_fft_y<T>(e->s, e->ply, (const Cplx<T>*)e->twy, spec, e->nxh, e->nz, 0, nullptr, nullptr));
     e->tend(TC_K_FFT_Y, ev);
     e->tbeg(TC_K_ZSOLVE, ev);
     ZNorm norm;
     norm.Wt = e->d_Wt;
     norm.C = e->d_C;
+    norm.s00 = e->d_s00;
     if (e->fp32) {
         HIPCHK(launch_zsolve<T>(e->s, spec, zfac_dev<T>(e), zfac_low_dev(e), e->d_lowidx, e->nlow, e->ylow, norm));
     } else {
-        norm.s00 = e->d_s00;
         HIPCHK(launch_zsolve<T>(e->s, spec, zfac_dev<T>(e), ZFacDev<double>{}, nullptr, 0, nullptr, norm));
     }
     e->tend(TC_K_ZSOLVE, ev);
     e->tbeg(TC_K_IFFT_Y, ev);
-    HIPCHK(launch_fft_y<T>(e->s, e->ply, (const Cplx<T>*)e->twy, spec, e->nxh, e->nz, 1));
+    HIPCHK(launch_fft_y<T>(e->s, e->ply, (const Cplx<T>*)e->twy, spec, e->nxh, e->nz, 1, e->d_s00, e->d_C));
     e->tend(TC_K_IFFT_Y, ev);
     e->tbeg(TC_K_IFFT_X, ev);
     HIPCHK(launch_fft_x_inv<T>(e->s, e->plx, (const Cplx<T>*)e->twx, spec, x, nrows, (T)(1.0 / ((double)e->nx * e->ny))));
     e->tend(TC_K_IFFT_X, ev);
     return TC_OK;
-}
-
-template <typename T>
-int unapply(tc_engine* e, const T* v, T* p) {
-    hipEvent_t ev;
-    e->tbeg(TC_K_DIV, ev);
-    HIPCHK(launch_divergence<T>(e->s, e->mesh<T>(), e->P.scheme, v, (T*)e->divv));
-    e->tend(TC_K_DIV, ev);
-    return solve<T>(e, (const T*)e->divv, p);
 }
 
 template <typename T>
@@ -412,14 +401,15 @@ int step(tc_engine* e, double tau, int cur, int order) {
     a.pcur = (const T*)e->pres[cur];
     a.inv_tau = (T)(1.0 / tau);
     a.cfr_partial = cfr ? e->d_partial : nullptr;
+    int nblk = 0;
     e->tbeg(TC_K_FLUX, ev);
-    HIPCHK(launch_flux<T>(e->s, m, a));
+    HIPCHK(launch_flux<T>(e->s, m, a, &nblk));
     e->tend(TC_K_FLUX, ev);
     T* ptarget = stab ? (T*)e->dp : (T*)e->pres[next];
     if (int r = unapply<T>(e, (const T*)e->flux[next], ptarget)) return r;
     if (cfr) {
         e->tbeg(TC_K_OTHER, ev);
-        HIPCHK(launch_cfr_shift(e->s, e->d_partial, flux_grid_blocks(e->nx, e->ny, e->nz), tau / bd0,
+        HIPCHK(launch_cfr_shift(e->s, e->d_partial, nblk, tau / bd0,
                                 e->P.cfr_ubulk * e->hm.area, e->hm.vol_u, e->d_shift));
         e->tend(TC_K_OTHER, ev);
     }
@@ -653,7 +643,7 @@ int tc_explicit_term(tc_engine* e, int in_kind, int in_slot, int out_kind, int o
         a.fcur = (T*)out;
         hipEvent_t ev;
         e->tbeg(TC_K_FLUX, ev);
-        HIPCHK(launch_flux<T>(e->s, e->mesh<T>(), a));
+        HIPCHK(launch_flux<T>(e->s, e->mesh<T>(), a, nullptr));
         e->tend(TC_K_FLUX, ev);
         return TC_OK;
     };

@@ -19,14 +19,13 @@ struct FluxArgs {
     T inv_tau = 0;
     double* cfr_partial = nullptr;  // per-block sums of r_u * volume (nullptr: off)
 };
+// nblocks (may be nullptr) receives the number of per-block partial sums written to cfr_partial.
 template <typename T>
-hipError_t launch_flux(hipStream_t s, const Mesh<T>& m, const FluxArgs<T>& a);
-int flux_grid_blocks(int nx, int ny, int nz);  // number of per-block partial sums written by launch_flux
+hipError_t launch_flux(hipStream_t s, const Mesh<T>& m, const FluxArgs<T>& a, int* nblocks);
+int flux_grid_blocks(int nx, int ny, int nz);  // upper bound of that number (buffer sizing)
 
 template <typename T>
 hipError_t launch_turb_visc(hipStream_t s, const Mesh<T>& m, int scheme, const T* vel, T* nut);
-template <typename T>
-hipError_t launch_divergence(hipStream_t s, const Mesh<T>& m, int scheme, const T* v, T* dv);
 template <typename T>
 hipError_t launch_gradient(hipStream_t s, const Mesh<T>& m, int scheme, const T* p, T* g);
 
@@ -56,12 +55,14 @@ hipError_t launch_add_scaled_plane(hipStream_t s, const Mesh<T>& m, T* f, const 
 // ---------------------------------------------------------------------------
 // Spectral pressure solve
 // ---------------------------------------------------------------------------
+// Forward x transform of the discrete divergence of v (computed on the fly; ApplyDivergence).
 template <typename T>
-hipError_t launch_fft_x_fwd(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const T* in, Cplx<T>* spec,
-                            int nrows);
+hipError_t launch_fft_x_fwd_div(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const Mesh<T>& m, int scheme,
+                                const T* v, Cplx<T>* spec);
+// psum/C (inverse only, may be nullptr): X00[iz] -= psum[0] * C[iz] before the transform.
 template <typename T>
 hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cplx<T>* spec, int nxh, int nz,
-                        int inverse);
+                        int inverse, const double* psum, const double* C);
 template <typename T>
 hipError_t launch_fft_x_inv(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const Cplx<T>* spec, T* out,
                             int nrows, T scale);
@@ -74,12 +75,12 @@ struct ZFacDev {
 };
 struct ZNorm {
     const double *Wt = nullptr, *C = nullptr;
-    double* s00 = nullptr;  // nz doubles of scratch for the (0,0) column
+    double* s00 = nullptr;  // receives the (0,0)-mode normalisation sum (one double)
 };
 // Solve every mode. FP32-bulk engines pass lowidx (mode -> index in the FP64
 // factor list, -1 for the FP32 modes), the FP64 factors of those modes and
 // ylow (nz * nlow * 2 doubles of scratch); FP64 engines pass lowidx = nullptr.
-// norm.s00 (FP64 engines) or the low-mode path normalises the (0,0) mode.
+// The lane of the (0,0) mode writes its normalisation sum to norm.s00[0].
 template <typename T>
 hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, const ZFacDev<double>& flow,
                          const int* lowidx, int nlow, double* ylow, ZNorm norm);

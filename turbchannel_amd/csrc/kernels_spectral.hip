@@ -7,6 +7,7 @@
 // transform; the y transforms run on tiles of TK adjacent kx columns so that
 // every global access is a contiguous run of TK complex values.
 #include "kernels.h"
+#include "stencil.h"
 
 namespace tc {
 
@@ -131,22 +132,31 @@ __device__ Cplx<T>* lds_fft(Cplx<T>* src, Cplx<T>* dst, const FFTPlan& pl, int n
 
 constexpr int kFftThreads = 256;
 
-// Forward x transform: rows 2q, 2q+1 of the real input form line q.
-template <typename T>
-__global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd(FFTPlan pl, const Cplx<T>* __restrict__ tw,
-                                                            const T* __restrict__ in, Cplx<T>* __restrict__ spec,
-                                                            int nrows, int lpb) {
+// Forward x transform of the divergence of v: rows 2q, 2q+1 (row = iy + NY*iz)
+// form the real and imaginary parts of line q; the divergence is evaluated
+// while the lines are staged into LDS.
+template <typename T, int SCHEME>
+__global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const Cplx<T>* __restrict__ tw, Mesh<T> m,
+                                                                const T* __restrict__ v, Cplx<T>* __restrict__ spec,
+                                                                int lpb) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
-    const int N = pl.n, nxh = N / 2 + 1;
+    const int N = pl.n, nxh = N / 2 + 1, nrows = m.ny * m.nz;
     Cplx<T>* b1 = b0 + lpb * N;
     const int q0 = blockIdx.x * lpb;
+    constexpr int R = SCHEME == TC_SCHEME_VV ? 2 : 1;
     for (int t = threadIdx.x; t < lpb * N; t += blockDim.x) {
         const int l = t / N, n = t - l * N;
         const int ra = 2 * (q0 + l), rb = ra + 1;
         Cplx<T> z = {T(0), T(0)};
-        if (ra < nrows) z.x = in[(size_t)ra * N + n];
-        if (rb < nrows) z.y = in[(size_t)rb * N + n];
+        if (ra < nrows) {
+            const int iz = ra / m.ny, iy = ra - iz * m.ny;
+            z.x = div_cell<T, SCHEME>(m, make_nbvec<T, R>(v, n, iy, iz, m.nx, m.ny), iz);
+        }
+        if (rb < nrows) {
+            const int iz = rb / m.ny, iy = rb - iz * m.ny;
+            z.y = div_cell<T, SCHEME>(m, make_nbvec<T, R>(v, n, iy, iz, m.nx, m.ny), iz);
+        }
         b0[t] = z;
     }
     __syncthreads();
@@ -201,9 +211,12 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
 }
 
 // y transform of TK adjacent kx columns of one plane, in place.
+// With psum != nullptr (inverse transform of the pressure solve) the (0,0) mode
+// of each plane first gets the normalisation X00[iz] -= psum * C[iz] (FP64).
 template <typename T>
 __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>* __restrict__ tw, Cplx<T>* spec,
-                                                        int nxh, int tk, T s) {
+                                                        int nxh, int tk, T s, const double* __restrict__ psum,
+                                                        const double* __restrict__ C) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n;
@@ -213,7 +226,9 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
     Cplx<T>* base = spec + (size_t)iz * N * nxh + kx0;
     for (int t = threadIdx.x; t < tk * N; t += blockDim.x) {
         const int n = t / tk, k = t - n * tk;
-        b0[t] = k < ncol ? base[(size_t)n * nxh + k] : Cplx<T>{T(0), T(0)};
+        Cplx<T> v = k < ncol ? base[(size_t)n * nxh + k] : Cplx<T>{T(0), T(0)};
+        if (psum && t == 0 && kx0 == 0) v.x = T((double)v.x - psum[0] * C[iz]);
+        b0[t] = v;
     }
     __syncthreads();
     const Cplx<T>* res = lds_fft(b0, b1, pl, tk, 1, tk, tw, s);
@@ -318,18 +333,14 @@ __global__ __launch_bounds__(256) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFa
                 if (W == 3) v = v - u2[k] * z2;
                 if (W == 3) v = v - u3[k] * z3;
                 x[i * xs] = v;
-                if (n00) {
-                    psum += norm.Wt[i] * (double)v;
-                    norm.s00[i] = (double)v;
-                }
+                if (n00) psum += norm.Wt[i] * (double)v;
                 z3 = z2;
                 z2 = z1;
                 z1 = v;
             }
         }
     }
-    if (n00)  // NormalizePressure of the (0,0) mode, in spectral space
-        for (int i = 0; i < nz; i++) x[i * xs] = T(norm.s00[i] - psum * norm.C[i]);
+    if (n00) norm.s00[0] = psum;  // applied by the inverse y transform (NormalizePressure)
 }
 
 // FP64 solve of one listed mode (the FP32-bulk engine's low wavenumbers). The
@@ -401,7 +412,7 @@ __device__ void zsolve_low_lane(Cplx<T>* spec, const ZFacDev<double>& f, int li,
                 v = v - u1[k] * z1;
                 if (W == 3) v = v - u2[k] * z2;
                 if (W == 3) v = v - u3[k] * z3;
-                y[i * ys] = v;
+                x[i * xs] = T(v);
                 if (n00) psum += norm.Wt[i] * v;
                 z3 = z2;
                 z2 = z1;
@@ -409,11 +420,7 @@ __device__ void zsolve_low_lane(Cplx<T>* spec, const ZFacDev<double>& f, int li,
             }
         }
     }
-    for (int i = 0; i < nz; i++) {
-        double v = y[i * ys];
-        if (n00) v -= psum * norm.C[i];  // NormalizePressure of the (0,0) mode
-        x[i * xs] = T(v);
-    }
+    if (n00) norm.s00[0] = psum;  // applied by the inverse y transform (NormalizePressure)
 }
 
 template <typename T>
@@ -427,12 +434,18 @@ int lines_per_block(int n) {
 }  // namespace
 
 template <typename T>
-hipError_t launch_fft_x_fwd(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const T* in, Cplx<T>* spec, int nrows) {
+hipError_t launch_fft_x_fwd_div(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const Mesh<T>& m, int scheme,
+                                const T* v, Cplx<T>* spec) {
     const int lpb = lines_per_block<T>(pl.n);
-    const int nlines = (nrows + 1) / 2;
+    const int nlines = (m.ny * m.nz + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
     const size_t sm = 2 * (size_t)lpb * pl.n * sizeof(Cplx<T>);
-    hipLaunchKernelGGL((k_fft_x_fwd<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, in, spec, nrows, lpb);
+    if (scheme == TC_SCHEME_HW)
+        hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_HW>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb);
+    else if (scheme == TC_SCHEME_VV)
+        hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_VV>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb);
+    else
+        hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_AB3>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb);
     return hipGetLastError();
 }
 
@@ -448,13 +461,14 @@ hipError_t launch_fft_x_inv(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw,
 }
 
 template <typename T>
-hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cplx<T>* spec, int nxh, int nz, int inverse) {
+hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cplx<T>* spec, int nxh, int nz, int inverse,
+                        const double* psum, const double* C) {
     // 128-byte runs (8 complex doubles, 16 complex floats) while the block stays within 48 KiB of LDS
     int tk = 128 / (int)sizeof(Cplx<T>);
     while (tk > 1 && 2 * (size_t)tk * pl.n * sizeof(Cplx<T>) > 48 * 1024) tk >>= 1;
     const dim3 grid((nxh + tk - 1) / tk, nz);
     const size_t sm = 2 * (size_t)tk * pl.n * sizeof(Cplx<T>);
-    hipLaunchKernelGGL((k_fft_y<T>), grid, dim3(kFftThreads), sm, s, pl, tw, spec, nxh, tk, T(inverse ? 1 : -1));
+    hipLaunchKernelGGL((k_fft_y<T>), grid, dim3(kFftThreads), sm, s, pl, tw, spec, nxh, tk, T(inverse ? 1 : -1), psum, C);
     return hipGetLastError();
 }
 
@@ -469,9 +483,11 @@ hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, cons
 }
 
 #define INST(T)                                                                                                      \
-    template hipError_t launch_fft_x_fwd<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const T*, Cplx<T>*, int);   \
+    template hipError_t launch_fft_x_fwd_div<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const Mesh<T>&, int,    \
+                                                const T*, Cplx<T>*);                                                 \
     template hipError_t launch_fft_x_inv<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const Cplx<T>*, T*, int, T); \
-    template hipError_t launch_fft_y<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, Cplx<T>*, int, int, int);       \
+    template hipError_t launch_fft_y<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, Cplx<T>*, int, int, int,       \
+                                        const double*, const double*);                                              \
     template hipError_t launch_zsolve<T>(hipStream_t, Cplx<T>*, const ZFacDev<T>&, const ZFacDev<double>&,         \
                                          const int*, int, double*, ZNorm);
 INST(double)

@@ -96,14 +96,19 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 
 // ---------------------------------------------------------------------------
 // K1: explicit term (+ BDF3/EX3 right-hand side), 2.5D-blocked.
-// A block owns a kTX x kTY tile of (x, y) columns and marches along z over a
+// A block owns a kTX x TY tile of (x, y) columns and marches along z over a
 // chunk of planes. The velocity planes iz-RZ..iz+RZ of the tile plus a halo of
 // RXY cells sit in an LDS ring (one component per sub-array, so a wavefront
 // reads consecutive words); the fourth-order scheme's z+-3 values of the own
 // column come straight from global memory. Blocks are ordered so that each
 // XCD works on neighbouring tiles and shares their halos in its L2.
 // ---------------------------------------------------------------------------
-constexpr int kTX = 32, kTY = 8;
+constexpr int kTX = 32;
+// Tile height (one thread per column of the tile).
+template <typename T, int SCHEME, int VISC>
+constexpr int flux_ty() {
+    return 8;
+}
 
 __device__ __forceinline__ int xcd_swizzle(int b, int nb) {
     const int q = nb / 8, r = nb % 8, x = b % 8, i = b / 8;
@@ -117,31 +122,37 @@ struct GlobalNu {
     __device__ __forceinline__ T operator()(int dx, int dy, int dz) const { return nut[nb->at(dx, dy, dz)]; }
 };
 
-template <typename T, int RXY, int RZ>
+template <typename T, int RXY, int RZ, int TY>
 struct RingAcc {
-    static constexpr int W = kTX + 2 * RXY, H = kTY + 2 * RXY, PS = W * H;
-    const T* pl[2 * RZ + 1];
-    int off;         // position of the cell in the padded tile
-    const T* gcol;   // the cell in global memory (own column beyond the ring)
+    static constexpr int W = kTX + 2 * RXY, H = TY + 2 * RXY, PS = W * H;
+    const T* pl[2 * RZ + 1];  // LDS planes iz-RZ..iz+RZ of the padded tile, real3 AoS
+    int off;                  // position of the cell in the padded tile
+    const T* gcol;            // the cell in global memory (own column beyond the ring)
     int nxy3;
     __device__ __forceinline__ T operator()(int c, int dx, int dy, int dz) const {
         if (dz > RZ || dz < -RZ) return gcol[dz * nxy3 + c];
-        return pl[dz + RZ][c * PS + off + dy * W + dx];
+        return pl[dz + RZ][3 * (off + dy * W + dx) + c];
     }
 };
 
 template <typename T, int SCHEME, int VISC, bool TURB>
-__global__ __launch_bounds__(kTX * kTY) void k_flux(Mesh<T> m, FluxArgs<T> a, int ntx, int ntiles, int zchunk) {
+__global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
+    Mesh<T> m, FluxArgs<T> a, int ntx, int ntiles, int zchunk) {
+    constexpr int TY = flux_ty<T, SCHEME, VISC>();
     constexpr int RXY = SCHEME == TC_SCHEME_VV ? 3 : 1;
     constexpr int RZ = SCHEME == TC_SCHEME_VV ? 2 : 1;
-    constexpr int NS = 2 * RZ + 1;
-    using Acc = RingAcc<T, RXY, RZ>;
+    constexpr int NS = 2 * RZ + 2;  // planes iz-RZ..iz+RZ plus the one in flight
+    using Acc = RingAcc<T, RXY, RZ, TY>;
     constexpr int W = Acc::W, PS = Acc::PS;
-    __shared__ T ring[NS * 3 * PS];
-    __shared__ double sh[kTX * kTY / 64];
+    constexpr int NT = kTX * TY;
+    constexpr int DPC = 3 * (int)sizeof(T) / 4;  // dwords per real3 cell
+    constexpr int DPS = PS * DPC;                // dwords per plane of the padded tile
+    constexpr int ND = (DPS + NT - 1) / NT;
+    __shared__ __align__(16) T ring[NS * 3 * PS];
+    __shared__ double sh[NT / 64];
     const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tile = lb % ntiles, chunk = lb / ntiles;
-    const int x0 = (tile % ntx) * kTX, y0 = (tile / ntx) * kTY;
+    const int x0 = (tile % ntx) * kTX, y0 = (tile / ntx) * TY;
     const int z0 = chunk * zchunk, z1 = min(m.nz, z0 + zchunk);
     const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
     const int ix = x0 + tx, iy = y0 + ty;
@@ -149,27 +160,44 @@ __global__ __launch_bounds__(kTX * kTY) void k_flux(Mesh<T> m, FluxArgs<T> a, in
     Nb nb;  // global neighbours for nu_t and the collocated pressure gradient
     make_nb<1>(nb, active ? ix : 0, active ? iy : 0, 0, m.nx, m.ny);
 
-    auto load_plane = [&](int p) {
-        T* dst = ring + (p % NS) * 3 * PS;
-        const T* __restrict__ src = a.vel + (size_t)3 * p * m.nxy;
-        for (int e = threadIdx.x; e < PS; e += kTX * kTY) {
-            const int lx = e % W, ly = e / W;
-            int gx = (x0 + lx - RXY) % m.nx;
-            if (gx < 0) gx += m.nx;
-            int gy = (y0 + ly - RXY) % m.ny;
-            if (gy < 0) gy += m.ny;
-            const T* sp = src + 3 * (gy * m.nx + gx);
-            dst[e] = sp[0];
-            dst[PS + e] = sp[1];
-            dst[2 * PS + e] = sp[2];
+    // The padded tile of a plane is copied into LDS by direct global->LDS dword
+    // loads: LDS dword d of the slot (cell d / DPC, word d % DPC of the real3 AoS
+    // cell) comes from the cell's periodic image; each lane holds the global dword
+    // offsets of its ND words within a plane, the same for every plane.
+    int gdw[ND];
+#pragma unroll
+    for (int k = 0; k < ND; k++) {
+        const int d = threadIdx.x + k * NT;
+        const int e = d / DPC, wd = d - e * DPC;
+        const int lx = e % W, ly = e / W;
+        int gx = (x0 + lx - RXY) % m.nx;
+        if (gx < 0) gx += m.nx;
+        int gy = (y0 + ly - RXY) % m.ny;
+        if (gy < 0) gy += m.ny;
+        gdw[k] = (gy * m.nx + gx) * DPC + wd;
+    }
+    const int wave = threadIdx.x >> 6;
+    auto fetch_async = [&](int p) {  // plane p -> its ring slot, without registers
+        const unsigned* __restrict__ src = reinterpret_cast<const unsigned*>(a.vel) + (size_t)p * m.nxy * DPC;
+        unsigned* dst = reinterpret_cast<unsigned*>(ring + (p % NS) * 3 * PS);
+#pragma unroll
+        for (int k = 0; k < ND; k++) {
+            if (threadIdx.x + k * NT < DPS)
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + gdw[k]),
+                                                 (__attribute__((address_space(3))) void*)(dst + k * NT + wave * 64), 4, 0,
+                                                 0);
         }
     };
-    for (int p = max(0, z0 - RZ); p < min(m.nz, z0 + RZ); p++) load_plane(p);
+    for (int p = max(0, z0 - RZ); p <= min(m.nz - 1, z0 + RZ); p++) fetch_async(p);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
 
     double cfr = 0.0;
     for (int iz = z0; iz < z1; iz++) {
-        if (iz + RZ < m.nz) load_plane(iz + RZ);
-        __syncthreads();
+        // planes iz-RZ..iz+RZ are in LDS; plane iz+RZ+1 (next iteration) is loaded
+        // into the slot of plane iz-RZ-1, which no thread reads after the last barrier
+        const int pn = iz + RZ + 1;
+        if (pn < m.nz && iz + 1 < z1) fetch_async(pn);
         if (active) {
             Acc acc;
 #pragma unroll
@@ -244,6 +272,7 @@ __global__ __launch_bounds__(kTX * kTY) void k_flux(Mesh<T> m, FluxArgs<T> a, in
                 if (a.cfr_partial) cfr += (double)r[0] * u_volume<T, SCHEME>(m, iz);
             }
         }
+        __builtin_amdgcn_s_waitcnt(0);  // this lane's part of plane iz+RZ+1 has landed
         __syncthreads();
     }
     if (a.cfr_partial) {
@@ -255,12 +284,30 @@ __global__ __launch_bounds__(kTX * kTY) void k_flux(Mesh<T> m, FluxArgs<T> a, in
 struct FluxGrid {
     int ntx, ntiles, zchunk, nblocks;
 };
-inline FluxGrid flux_grid(int nx, int ny, int nz) {
+// Blocks resident at once on the device for one flux kernel instantiation.
+template <typename T, int SCHEME, int VISC, bool TURB>
+int flux_slots() {
+    static int slots = 0;
+    if (slots == 0) {
+        int per_cu = 0, dev = 0, ncu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_flux<T, SCHEME, VISC, TURB>,
+                                                         kTX * flux_ty<T, SCHEME, VISC>(), 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            ncu < 1)
+            ncu = 256;
+        slots = per_cu * ncu;
+    }
+    return slots;
+}
+// z-chunks so that the grid fills whole rounds of resident blocks.
+inline FluxGrid flux_grid(int nx, int ny, int nz, int ty, int slots) {
     FluxGrid g;
     g.ntx = (nx + kTX - 1) / kTX;
-    g.ntiles = g.ntx * ((ny + kTY - 1) / kTY);
-    int nch = (512 + g.ntiles - 1) / g.ntiles;  // about two blocks per CU
-    nch = std::max(1, std::min(nch, std::max(1, nz / 8)));
+    g.ntiles = g.ntx * ((ny + ty - 1) / ty);
+    int nch = std::max(1, slots / g.ntiles);
+    nch = std::min(nch, std::max(1, nz / 4));
     g.zchunk = (nz + nch - 1) / nch;
     g.nblocks = g.ntiles * ((nz + g.zchunk - 1) / g.zchunk);
     return g;
@@ -362,43 +409,6 @@ __global__ __launch_bounds__(kBlock) void k_turb_ab3(Mesh<T> m, const T* __restr
     nut[nb.at(0, 0, 0)] = fvd * delta * delta * S;
 }
 
-// ---------------------------------------------------------------------------
-// Divergence (staggered/func_global.h:306-312, :325-333; collocated_AB3/func_global.h:310-325)
-// ---------------------------------------------------------------------------
-template <typename T, int SCHEME>
-__global__ __launch_bounds__(kBlock) void k_div(Mesh<T> m, const T* __restrict__ v, T* __restrict__ dv) {
-    const int iz = blockIdx.y, NZ = m.nz;
-    int ix, iy;
-    if (!cell_of_thread(m.nx, m.nxy, ix, iy)) return;
-    Nb nb;
-    make_nb<2>(nb, ix, iy, iz, m.nx, m.ny);
-    const int in = nb.at(0, 0, 0);
-    T d;
-    if (SCHEME == TC_SCHEME_HW) {
-        const T hz = m.DZ[iz];
-        d = (v[3 * nb.at(1, 0, 0)] - v[3 * in]) * m.inv_hx * hz + (v[3 * nb.at(0, 1, 0) + 1] - v[3 * in + 1]) * m.inv_hy * hz +
-            ((iz < NZ - 1) ? (v[3 * nb.at(0, 0, 1) + 2] - v[3 * in + 2]) : (-v[3 * in + 2]));
-    } else if (SCHEME == TC_SCHEME_VV) {
-        const T dz = m.DZ[iz], dzw = m.DZW[iz], c27 = T(1.0 / 27.0);
-        const T dudx = m.inv_hx * (dz * (v[3 * nb.at(1, 0, 0)] - v[3 * in]) - c27 * dzw * (v[3 * nb.at(2, 0, 0)] - v[3 * nb.at(-1, 0, 0)]));
-        const T dvdy = m.inv_hy * (dz * (v[3 * nb.at(0, 1, 0) + 1] - v[3 * in + 1]) -
-                                   c27 * dzw * (v[3 * nb.at(0, 2, 0) + 1] - v[3 * nb.at(0, -1, 0) + 1]));
-        const T w0 = v[3 * in + 2];
-        const T wr = (iz == NZ - 1) ? T(0) : v[3 * nb.at(0, 0, 1) + 2];
-        const T wrr = (iz == NZ - 1) ? w0 : (iz == NZ - 2) ? T(0) : v[3 * nb.at(0, 0, 2) + 2];
-        const T wll = (iz == 0) ? v[3 * nb.at(0, 0, 1) + 2] : (iz == 1) ? T(0) : v[3 * nb.at(0, 0, -1) + 2];
-        d = dudx + dvdy + (wr - w0) - c27 * (wrr - wll);
-    } else {
-        const T hz = m.cellh[iz];
-        d = T(0.5) * (v[3 * nb.at(1, 0, 0)] - v[3 * nb.at(-1, 0, 0)]) * m.hy * hz;
-        d += T(0.5) * (v[3 * nb.at(0, 1, 0) + 1] - v[3 * nb.at(0, -1, 0) + 1]) * m.hx * hz;
-        if (iz == 0) d += T(0.5) * (v[3 * nb.at(0, 0, 1) + 2] - v[3 * in + 2]) * m.hx * m.hy;
-        else if (iz == NZ - 1) d += T(0.5) * (v[3 * in + 2] - v[3 * nb.at(0, 0, -1) + 2]) * m.hx * m.hy;
-        else d += T(0.5) * (v[3 * nb.at(0, 0, 1) + 2] - v[3 * nb.at(0, 0, -1) + 2]) * m.hx * m.hy;
-    }
-    dv[in] = d;
-}
-
 template <typename T, int SCHEME>
 __global__ __launch_bounds__(kBlock) void k_grad(Mesh<T> m, const T* __restrict__ p, T* __restrict__ g) {
     const int iz = blockIdx.y;
@@ -463,33 +473,36 @@ dim3 cell_grid(const Mesh<T>& m) {
 
 }  // namespace
 
-int flux_grid_blocks(int nx, int ny, int nz) { return flux_grid(nx, ny, nz).nblocks; }
+// Upper bound of the per-block partial sums written by launch_flux (at least one z-plane per block).
+int flux_grid_blocks(int nx, int ny, int nz) { return ((nx + kTX - 1) / kTX) * ((ny + 7) / 8) * nz; }
 
 template <typename T>
-hipError_t launch_flux(hipStream_t s, const Mesh<T>& m, const FluxArgs<T>& a) {
-    const FluxGrid fg = flux_grid(m.nx, m.ny, m.nz);
-    const dim3 g(fg.nblocks), b(kTX * kTY);
-#define L(SCH, VIS, TRB) hipLaunchKernelGGL((k_flux<T, SCH, VIS, TRB>), g, b, 0, s, m, a, fg.ntx, fg.ntiles, fg.zchunk)
+hipError_t launch_flux(hipStream_t s, const Mesh<T>& m, const FluxArgs<T>& a, int* nblocks) {
+#define L(SCH, VIS, TRB)                                                                                       \
+    {                                                                                                          \
+        constexpr int ty = flux_ty<T, SCH, VIS>();                                                             \
+        const FluxGrid fg = flux_grid(m.nx, m.ny, m.nz, ty, flux_slots<T, SCH, VIS, TRB>());                  \
+        if (nblocks) *nblocks = fg.nblocks;                                                                    \
+        hipLaunchKernelGGL((k_flux<T, SCH, VIS, TRB>), dim3(fg.nblocks), dim3(kTX * ty), 0, s, m, a, fg.ntx,   \
+                           fg.ntiles, fg.zchunk);                                                              \
+    }
     if (a.scheme == TC_SCHEME_HW) {
         if (a.visc_type == 1) {
-            if (a.turb) L(TC_SCHEME_HW, 1, true);
-            else L(TC_SCHEME_HW, 1, false);
+            if (a.turb) L(TC_SCHEME_HW, 1, true) else L(TC_SCHEME_HW, 1, false)
         } else {
-            L(TC_SCHEME_HW, 0, false);
+            L(TC_SCHEME_HW, 0, false)
         }
     } else if (a.scheme == TC_SCHEME_VV) {
         if (a.visc_type == 1) {
-            if (a.turb) L(TC_SCHEME_VV, 1, true);
-            else L(TC_SCHEME_VV, 1, false);
+            if (a.turb) L(TC_SCHEME_VV, 1, true) else L(TC_SCHEME_VV, 1, false)
         } else {
-            L(TC_SCHEME_VV, 0, false);
+            L(TC_SCHEME_VV, 0, false)
         }
     } else {
         if (a.visc_type == 1) {
-            if (a.turb) L(TC_SCHEME_AB3, 1, true);
-            else L(TC_SCHEME_AB3, 1, false);
+            if (a.turb) L(TC_SCHEME_AB3, 1, true) else L(TC_SCHEME_AB3, 1, false)
         } else {
-            L(TC_SCHEME_AB3, 0, false);
+            L(TC_SCHEME_AB3, 0, false)
         }
     }
 #undef L
@@ -501,15 +514,6 @@ hipError_t launch_turb_visc(hipStream_t s, const Mesh<T>& m, int scheme, const T
     const dim3 g = cell_grid(m);
     if (scheme == TC_SCHEME_AB3) hipLaunchKernelGGL((k_turb_ab3<T>), g, dim3(kBlock), 0, s, m, vel, nut);
     else hipLaunchKernelGGL((k_turb_staggered<T>), g, dim3(kBlock), 0, s, m, vel, nut);
-    return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_divergence(hipStream_t s, const Mesh<T>& m, int scheme, const T* v, T* dv) {
-    const dim3 g = cell_grid(m);
-    if (scheme == TC_SCHEME_HW) hipLaunchKernelGGL((k_div<T, TC_SCHEME_HW>), g, dim3(kBlock), 0, s, m, v, dv);
-    else if (scheme == TC_SCHEME_VV) hipLaunchKernelGGL((k_div<T, TC_SCHEME_VV>), g, dim3(kBlock), 0, s, m, v, dv);
-    else hipLaunchKernelGGL((k_div<T, TC_SCHEME_AB3>), g, dim3(kBlock), 0, s, m, v, dv);
     return hipGetLastError();
 }
 
@@ -547,9 +551,8 @@ hipError_t launch_add_scaled_plane(hipStream_t s, const Mesh<T>& m, T* f, const 
 }
 
 #define INST(T)                                                                                             \
-    template hipError_t launch_flux<T>(hipStream_t, const Mesh<T>&, const FluxArgs<T>&);                    \
+    template hipError_t launch_flux<T>(hipStream_t, const Mesh<T>&, const FluxArgs<T>&, int*);              \
     template hipError_t launch_turb_visc<T>(hipStream_t, const Mesh<T>&, int, const T*, T*);                \
-    template hipError_t launch_divergence<T>(hipStream_t, const Mesh<T>&, int, const T*, T*);               \
     template hipError_t launch_gradient<T>(hipStream_t, const Mesh<T>&, int, const T*, T*);                 \
     template hipError_t launch_project<T>(hipStream_t, const Mesh<T>&, const ProjectArgs<T>&);              \
     template hipError_t launch_add_u<T>(hipStream_t, const Mesh<T>&, int, T*, const double*);               \

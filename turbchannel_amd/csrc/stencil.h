@@ -104,6 +104,7 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
         const T q_ll = u0 + (iz == 2 ? -UU(0, 0, -2) : iz == 1 ? T(0) : iz == 0 ? UU(0, 0, 2) : UU(0, 0, -3));
         f0 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
     }
+    __builtin_amdgcn_sched_barrier(0);  // keep the equations' live ranges apart
     // ---- v-equation
     {
         const T br = dz * hy * YBAR(UU, 1, 0);
@@ -130,6 +131,7 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
         const T q_ll = v0 + (iz == 2 ? -VV(0, 0, -2) : iz == 1 ? T(0) : iz == 0 ? VV(0, 0, 2) : VV(0, 0, -3));
         f1 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
     }
+    __builtin_amdgcn_sched_barrier(0);
     // ---- w-equation (no unknown at iz = 0)
     if (iz != 0) {
         {
@@ -373,5 +375,61 @@ __device__ __forceinline__ void visc_galerkin(const Mesh<T>& m, const A& acc, co
 #undef WW
 #undef XBAR
 #undef YBAR
+
+// Periodic-neighbour accessor over a real3 AoS array in global memory for
+// offsets -R..R in x and y (any offset in z).
+template <typename T, int R>
+struct NbVec {
+    const T* __restrict__ v;
+    int x[2 * R + 1], y[2 * R + 1], base, nxy;
+    __device__ __forceinline__ T operator()(int c, int dx, int dy, int dz) const {
+        return v[3 * (base + dz * nxy + y[dy + R] + x[dx + R]) + c];
+    }
+};
+template <typename T, int R>
+__device__ __forceinline__ NbVec<T, R> make_nbvec(const T* v, int ix, int iy, int iz, int nx, int ny) {
+    NbVec<T, R> a;
+    a.v = v;
+#pragma unroll
+    for (int d = -R; d <= R; d++) {
+        int xx = ix + d, yy = iy + d;
+        xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
+        yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
+        a.x[d + R] = xx;
+        a.y[d + R] = yy * nx;
+    }
+    a.nxy = nx * ny;
+    a.base = iz * a.nxy;
+    return a;
+}
+
+// Volume-weighted discrete divergence at one cell (staggered/func_global.h:306-312
+// HW, :325-333 VV; collocated_AB3/func_global.h:310-325).
+template <typename T, int SCHEME, class A>
+__device__ __forceinline__ T div_cell(const Mesh<T>& m, const A& acc, int iz) {
+    const int NZ = m.nz;
+    if (SCHEME == TC_SCHEME_HW) {
+        const T hz = m.DZ[iz];
+        return (acc(0, 1, 0, 0) - acc(0, 0, 0, 0)) * m.inv_hx * hz + (acc(1, 0, 1, 0) - acc(1, 0, 0, 0)) * m.inv_hy * hz +
+               ((iz < NZ - 1) ? (acc(2, 0, 0, 1) - acc(2, 0, 0, 0)) : (-acc(2, 0, 0, 0)));
+    } else if (SCHEME == TC_SCHEME_VV) {
+        const T dz = m.DZ[iz], dzw = m.DZW[iz], c27 = T(1.0 / 27.0);
+        const T dudx = m.inv_hx * (dz * (acc(0, 1, 0, 0) - acc(0, 0, 0, 0)) - c27 * dzw * (acc(0, 2, 0, 0) - acc(0, -1, 0, 0)));
+        const T dvdy = m.inv_hy * (dz * (acc(1, 0, 1, 0) - acc(1, 0, 0, 0)) - c27 * dzw * (acc(1, 0, 2, 0) - acc(1, 0, -1, 0)));
+        const T w0 = acc(2, 0, 0, 0);
+        const T wr = (iz == NZ - 1) ? T(0) : acc(2, 0, 0, 1);
+        const T wrr = (iz == NZ - 1) ? w0 : (iz == NZ - 2) ? T(0) : acc(2, 0, 0, 2);
+        const T wll = (iz == 0) ? acc(2, 0, 0, 1) : (iz == 1) ? T(0) : acc(2, 0, 0, -1);
+        return dudx + dvdy + (wr - w0) - c27 * (wrr - wll);
+    } else {
+        const T hz = m.cellh[iz];
+        T d = T(0.5) * (acc(0, 1, 0, 0) - acc(0, -1, 0, 0)) * m.hy * hz;
+        d += T(0.5) * (acc(1, 0, 1, 0) - acc(1, 0, -1, 0)) * m.hx * hz;
+        if (iz == 0) d += T(0.5) * (acc(2, 0, 0, 1) - acc(2, 0, 0, 0)) * m.hx * m.hy;
+        else if (iz == NZ - 1) d += T(0.5) * (acc(2, 0, 0, 0) - acc(2, 0, 0, -1)) * m.hx * m.hy;
+        else d += T(0.5) * (acc(2, 0, 0, 1) - acc(2, 0, 0, -1)) * m.hx * m.hy;
+        return d;
+    }
+}
 
 }  // namespace tc
