@@ -88,10 +88,34 @@ def cpu_baseline(name, seconds_target=15.0):
         cur = (cur + 1) % 4
         n += 1
         el = time.perf_counter() - t0
-        if el > seconds_target or n >= 50:
+        if el > seconds_target:
             break
     return {"value": nx * ny * nz * n / el, "unit": "grid-pt steps/s", "cores": threads, "kind": "port",
             "sample": f"{n} BDF3 steps of {name} (FP64 oracle, OpenMP {threads} threads, {el:.1f} s)"}
+
+
+# Kernel-class name -> kernel symbol prefix in the rocprofv3 summaries.
+PROFILE_SYMBOL = {"flux": "k_flux", "fft_x": "k_fft_x_fwd_div", "fft_y": "k_fft_y", "zsolve": "k_zsolve",
+                  "ifft_x": "k_fft_x_inv", "project": "k_project"}
+
+
+def measured_traffic(workload, precision, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/step_kernels_<workload>_<precision>.md: FETCH_SIZE and WRITE_SIZE,
+    each scaled by its reading on a plain device copy), or None if absent."""
+    path = os.path.join(ROOT, "profiles", f"step_kernels_{workload}_{precision}.md")
+    sym = PROFILE_SYMBOL.get(kernel)
+    if sym is None or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cols) >= 7 and cols[0].startswith(sym + "<"):
+                try:
+                    return {"bytes": (float(cols[5]) + float(cols[6])) * 1e6, "source": os.path.relpath(path, ROOT)}
+                except ValueError:
+                    return None
+    return None
 
 
 def main():
@@ -184,6 +208,7 @@ def main():
     achieved = REALS_PER_POINT[dom] * rsz * n3d / dom_avg_s / 1e9
 
     value = n3d * args.steps / el
+    traffic = measured_traffic(args.workload, args.precision, dom)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.workload, args.cpu_seconds)
@@ -203,7 +228,10 @@ def main():
         "config": {"workload": args.workload, "scheme": ["HW", "VV", "AB3"][scheme], "grid": [nx, ny, nz],
                    "dt": dt, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "algorithmic_bytes": REALS_PER_POINT[dom] * rsz * n3d,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None},
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "with_diagnostics": {"value": n3d * nd / el_diag, "ms_per_step": 1e3 * el_diag / nd, "steps": nd},
