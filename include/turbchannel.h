@@ -28,15 +28,17 @@ extern "C" {
 #endif
 
 typedef struct tc_engine tc_engine;
+typedef struct tc_mlp tc_mlp;
 
 enum tc_scheme { TC_SCHEME_HW = 0, TC_SCHEME_VV = 1, TC_SCHEME_AB3 = 2 };
 enum tc_precision { TC_FP64 = 0, TC_FP32_BULK = 1 };
-enum tc_kind { TC_VEL = 0, TC_FLUX = 1, TC_PRES = 2, TC_NUT = 3 };
+enum tc_kind { TC_VEL = 0, TC_FLUX = 1, TC_PRES = 2, TC_NUT = 3,
+               TC_TAU = 4 /* learned-closure stress, 6 reals per node: XX YY ZZ XY XZ YZ (slot 0) */ };
 enum tc_status { TC_OK = 0, TC_ERR_ARG = 1, TC_ERR_HIP = 2, TC_ERR_STATE = 3, TC_ERR_NUMERIC = 4 };
 
 /* Kernel classes reported by tc_timing_read(). */
 enum tc_kernel {
-    TC_K_TURB_VISC = 0, /* Smagorinsky-van Driest nu_t */
+    TC_K_TURB_VISC = 0, /* subgrid model: Smagorinsky-van Driest nu_t, or the learned closure's tau */
     TC_K_FLUX = 1,      /* explicit term + BDF3/EX3 right-hand side */
     TC_K_DIV = 2,       /* divergence of the right-hand side: fused into TC_K_FFT_X, no launch of its own */
     TC_K_FFT_X = 3,     /* divergence + forward real FFT along x */
@@ -65,6 +67,9 @@ typedef struct tc_params {
     double hx, hy, xmin, ymin;
     double cfr_ubulk;      /* > 0: hold the bulk velocity at this value (constant flow rate) */
     const double* zz;      /* staggered: nz+1 face coordinates; collocated: nz node coordinates */
+    int sgs_model;         /* collocated only, turb_visc = 0: 1 = learned closure (no reference
+                              counterpart): each step evaluates tau from the velocity with the
+                              network set by tc_set_sgs_params and subtracts its divergence */
 } tc_params;
 
 /* Lifecycle. Replaces data::alloc_all + InitFourier + CopyDataToDevice
@@ -119,6 +124,39 @@ int tc_initial_field(tc_engine* e, int slot, unsigned long long seed, double ubu
  * collocated_AB3/main.cpp:295-300). name: "ZZ","DZ","DZW","GC1","GC2","GCZ",
  * "MAT_A","MAT_D","KERG","KERG_norm2". Returns the length, or -1. No device needed. */
 int tc_mesh_array(const tc_params* p, const char* name, double* out, int cap);
+
+/* ---------------------------------------------------------------------------
+ * Learned subgrid closure (no reference counterpart; the reference's closure is
+ * RecalcTurbVisc, staggered/func_global.h:815-890, collocated_AB3/func_global.h:342-384).
+ * Network 10 -> 64 -> 64 -> 64 -> 6 with GELU (tanh form), bfloat16 MFMA with FP32
+ * accumulation. Inputs: the 9 velocity-gradient components d u_i / d x_j (i-major)
+ * times nu / u_tau^2, and the wall distance times inv_length_scale / 100; outputs:
+ * the deviatoric stress XX YY ZZ XY XZ YZ over u_tau^2, u_tau = visc * inv_length_scale.
+ * Parameters: flat FP32 vector, per layer W (out x in, row-major) then b.
+ * ------------------------------------------------------------------------- */
+int tc_set_sgs_params(tc_engine* e, const float* params, size_t count);
+/* Samples for training from one collocated velocity field: sharp spectral cut-off
+ * by `factor` in x and y, at the coarse grid's interior nodes. dev_x: 10 floats and
+ * dev_t: 6 floats per sample, device memory; tc_sgs_sample_count() samples. */
+long long tc_sgs_sample_count(const tc_engine* e, int factor);
+int tc_sgs_samples(tc_engine* e, int kind, int slot, int factor, float* dev_x, float* dev_t, long long cap,
+                   long long* count);
+
+/* Trainer: FP32 master weights, Adam (beta 0.9, 0.999, eps 1e-8) and bfloat16
+ * device copies on one device. Loss: mean over samples and outputs of the squared
+ * error. All calls return when the work is done. */
+int tc_mlp_param_count(void);
+int tc_mlp_create(int device, unsigned long long seed, tc_mlp** out);
+int tc_mlp_destroy(tc_mlp* m);
+int tc_mlp_set_params(tc_mlp* m, const float* host, size_t count);   /* also resets Adam */
+int tc_mlp_get_params(tc_mlp* m, float* host, size_t count);
+int tc_mlp_forward(tc_mlp* m, const float* dev_x, long long n, float* dev_y);
+/* Loss and its gradient at the current weights (host_grad may be NULL). */
+int tc_mlp_gradient(tc_mlp* m, const float* dev_x, const float* dev_t, long long n, float* host_grad, size_t count,
+                    double* loss);
+/* One Adam step on the batch; loss (may be NULL) is the batch loss before the step. */
+int tc_mlp_train_step(tc_mlp* m, const float* dev_x, const float* dev_t, long long n, float lr, double* loss);
+int tc_mlp_steps(const tc_mlp* m, long long* steps);
 
 /* Per-kernel timing with HIP events on the engine's stream. */
 int tc_timing_enable(tc_engine* e, int on);

@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libtc_oracle.so")
 
 SCHEME_HW, SCHEME_VV, SCHEME_AB3 = 0, 1, 2
-KIND_VEL, KIND_FLUX, KIND_PRES, KIND_NUT = 0, 1, 2, 3
+KIND_VEL, KIND_FLUX, KIND_PRES, KIND_NUT, KIND_TAU = 0, 1, 2, 3, 4
 
 
 class OrParams(ctypes.Structure):
@@ -24,7 +24,7 @@ class OrParams(ctypes.Structure):
         ("fd_mode", ctypes.c_int),
         ("visc", ctypes.c_double), ("sour_dpdx", ctypes.c_double), ("inv_length_scale", ctypes.c_double),
         ("hx", ctypes.c_double), ("hy", ctypes.c_double), ("xmin", ctypes.c_double), ("ymin", ctypes.c_double),
-        ("cfr_ubulk", ctypes.c_double),
+        ("cfr_ubulk", ctypes.c_double), ("sgs_model", ctypes.c_int),
     ]
 
 
@@ -114,11 +114,12 @@ class Solver:
     """Mirror of the reference's ``data`` struct (staggered/data_global_inc.h:102-167)."""
 
     def __init__(self, scheme, nx, ny, nz, zz, *, visc, sour_dpdx=0.0, inv_length_scale=1.0, hx, hy,
-                 turb_visc=0, visc_type=None, stab_type=1, fd_mode=1, xmin=0.0, ymin=0.0, cfr_ubulk=0.0):
+                 turb_visc=0, visc_type=None, stab_type=1, fd_mode=1, xmin=0.0, ymin=0.0, cfr_ubulk=0.0,
+                 sgs_model=0):
         if visc_type is None:
             visc_type = 1 if turb_visc else 0
         self.params = OrParams(scheme, nx, ny, nz, turb_visc, visc_type, stab_type, fd_mode, visc, sour_dpdx,
-                               inv_length_scale, hx, hy, xmin, ymin, cfr_ubulk)
+                               inv_length_scale, hx, hy, xmin, ymin, cfr_ubulk, sgs_model)
         self.scheme, self.nx, self.ny, self.nz = scheme, nx, ny, nz
         self.n3d = nx * ny * nz
         self.zz = np.ascontiguousarray(zz, dtype=np.float64)
@@ -238,7 +239,8 @@ class Solver:
             raise ValueError("bad field")
 
     def get_field(self, kind, slot):
-        out = self.scal() if kind in (KIND_PRES, KIND_NUT) else self.vec()
+        out = self.scal() if kind in (KIND_PRES, KIND_NUT) else np.zeros((self.n3d, 6)) if kind == KIND_TAU \
+            else self.vec()
         if lib().or_get_field(self._h, kind, slot, _p(out)):
             raise ValueError("bad field")
         return out

@@ -130,7 +130,7 @@ struct or_solver {
     int NX, NY, NZ, NXY, N3D;
     double inv_hx, inv_hy, area;
     double *ZZ, *DZ, *DZW, *GC1, *GC2, *GCZ, *MAT_A, MAT_D[6], *KERG, KERG_norm2;
-    double *vel[4], *flux[4], *pres[4], *divv, *nu_t;
+    double *vel[4], *flux[4], *pres[4], *divv, *nu_t, *tau;
     cplx* spec;
     double* cosphi[2];
     fft_plan px, py;
@@ -296,7 +296,8 @@ static void init_ab3_mesh(or_solver* s) {
 }
 
 or_solver* or_create(const or_params* p, const double* zz) {
-    if (!p || !zz || p->nx < 4 || p->ny < 4 || p->nz < 6 || p->scheme < 0 || p->scheme > 2) {
+    if (!p || !zz || p->nx < 4 || p->ny < 4 || p->nz < 6 || p->scheme < 0 || p->scheme > 2 ||
+        (p->sgs_model && p->scheme != OR_SCHEME_AB3)) {
         set_err("bad parameters");
         return NULL;
     }
@@ -332,6 +333,7 @@ or_solver* or_create(const or_params* p, const double* zz) {
     }
     s->divv = (double*)calloc(n3, sizeof(double));
     s->nu_t = p->turb_visc ? (double*)calloc(n3, sizeof(double)) : NULL;
+    s->tau = p->sgs_model ? (double*)calloc(6 * n3, sizeof(double)) : NULL;
     s->spec = (cplx*)calloc(n3, sizeof(cplx));
     s->cosphi[0] = (double*)calloc((size_t)s->NX, sizeof(double));
     s->cosphi[1] = (double*)calloc((size_t)s->NY, sizeof(double));
@@ -348,7 +350,7 @@ void or_destroy(or_solver* s) {
     free(s->ZZ); free(s->DZ); free(s->DZW); free(s->GC1); free(s->GC2); free(s->GCZ);
     free(s->MAT_A); free(s->KERG);
     for (int i = 0; i < 4; i++) { free(s->vel[i]); free(s->flux[i]); free(s->pres[i]); }
-    free(s->divv); free(s->nu_t); free(s->spec); free(s->cosphi[0]); free(s->cosphi[1]);
+    free(s->divv); free(s->nu_t); free(s->tau); free(s->spec); free(s->cosphi[0]); free(s->cosphi[1]);
     plan_free(&s->px);
     plan_free(&s->py);
     free(s);
@@ -887,6 +889,21 @@ static void flux_ab3(const or_solver* s, int in, const double* u, int do_conv, i
                     sum[c] += visc * ((u[3 * ZR_(in) + c] - u[3 * in + c]) / s->DZ[iz] - (u[3 * in + c] - u[3 * ZL_(in) + c]) / s->DZ[iz - 1]) * inv_hz;
             }
         }
+        if (do_visc && s->tau) {
+            /* learned closure (new numerics): -d tau_ij / d x_j, central differences in x, y
+             * and the 3-point nonuniform formula in z; tau is zero on the wall planes */
+            static const int TC[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+            const double* t = s->tau;
+            const double hm = s->DZ[iz - 1], hp = s->DZ[iz];
+            const double cp = hm * hm / (hp * hm * (hp + hm)), cm = hp * hp / (hp * hm * (hp + hm));
+            for (int c = 0; c < 3; c++) {
+                const int ax = TC[c][0], ay = TC[c][1], az = TC[c][2];
+                double d = 0.5 * s->inv_hx * (t[6 * XR_(in) + ax] - t[6 * XL_(in) + ax]);
+                d += 0.5 * s->inv_hy * (t[6 * YR_(in) + ay] - t[6 * YL_(in) + ay]);
+                d += cp * (t[6 * ZR_(in) + az] - t[6 * in + az]) + cm * (t[6 * in + az] - t[6 * ZL_(in) + az]);
+                sum[c] -= d;
+            }
+        }
         sum[0] += s->P.sour_dpdx;
     }
     kuhat[3 * in + 0] = sum[0];
@@ -1321,6 +1338,7 @@ int or_set_field(or_solver* s, int kind, int slot, const double* src) {
     else if (kind == OR_KIND_FLUX) memcpy(s->flux[slot], src, 3 * n3 * sizeof(double));
     else if (kind == OR_KIND_PRES) memcpy(s->pres[slot], src, n3 * sizeof(double));
     else if (kind == OR_KIND_NUT && s->nu_t) memcpy(s->nu_t, src, n3 * sizeof(double));
+    else if (kind == OR_KIND_TAU && s->tau) memcpy(s->tau, src, 6 * n3 * sizeof(double));
     else return 1;
     return 0;
 }
@@ -1332,6 +1350,7 @@ int or_get_field(or_solver* s, int kind, int slot, double* dst) {
     else if (kind == OR_KIND_FLUX) memcpy(dst, s->flux[slot], 3 * n3 * sizeof(double));
     else if (kind == OR_KIND_PRES) memcpy(dst, s->pres[slot], n3 * sizeof(double));
     else if (kind == OR_KIND_NUT && s->nu_t) memcpy(dst, s->nu_t, n3 * sizeof(double));
+    else if (kind == OR_KIND_TAU && s->tau) memcpy(dst, s->tau, 6 * n3 * sizeof(double));
     else return 1;
     return 0;
 }

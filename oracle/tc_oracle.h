@@ -24,7 +24,7 @@ extern "C" {
 #endif
 
 enum { OR_SCHEME_HW = 0, OR_SCHEME_VV = 1, OR_SCHEME_AB3 = 2 };
-enum { OR_KIND_VEL = 0, OR_KIND_FLUX = 1, OR_KIND_PRES = 2, OR_KIND_NUT = 3 };
+enum { OR_KIND_VEL = 0, OR_KIND_FLUX = 1, OR_KIND_PRES = 2, OR_KIND_NUT = 3, OR_KIND_TAU = 4 };
 
 typedef struct or_params {
     int scheme;        /* OR_SCHEME_* */
@@ -36,6 +36,9 @@ typedef struct or_params {
     double visc, sour_dpdx, inv_length_scale;
     double hx, hy, xmin, ymin;
     double cfr_ubulk;  /* >0: hold the bulk velocity at this value (new numerics) */
+    int sgs_model;     /* AB3 only. 1: learned closure; the explicit term subtracts
+                          div tau of the tau field (OR_KIND_TAU, 6 per node, XX YY ZZ XY XZ YZ),
+                          which the caller sets (new numerics) */
 } or_params;
 
 typedef struct or_solver or_solver;
@@ -71,7 +74,7 @@ double or_kinetic_energy(or_solver* s, const double* u);
 int or_profiles(or_solver* s, const double* u, const double* p, double* av);
 double or_cfl(or_solver* s, const double* u, double tau, int mode);
 
-/* Ring-buffer state, as the reference's data::vel/flux/pres[4] and nu_t. */
+/* Ring-buffer state, as the reference's data::vel/flux/pres[4] and nu_t; plus tau. */
 int or_set_field(or_solver* s, int kind, int slot, const double* src);
 int or_get_field(or_solver* s, int kind, int slot, double* dst);
 int or_step(or_solver* s, double t, double tau, int cur, int bdf_order);

@@ -18,7 +18,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "turbchannel.h")
 
 SCHEME_HW, SCHEME_VV, SCHEME_AB3 = 0, 1, 2
 FP64, FP32_BULK = 0, 1
-VEL, FLUX, PRES, NUT = 0, 1, 2, 3
+VEL, FLUX, PRES, NUT, TAU = 0, 1, 2, 3, 4
 KERNELS = ["turb_visc", "flux", "div", "fft_x", "fft_y", "zsolve", "ifft_y", "ifft_x", "project", "other"]
 
 
@@ -36,6 +36,7 @@ class TcParams(ctypes.Structure):
         ("hx", ctypes.c_double), ("hy", ctypes.c_double), ("xmin", ctypes.c_double), ("ymin", ctypes.c_double),
         ("cfr_ubulk", ctypes.c_double),
         ("zz", ctypes.POINTER(ctypes.c_double)),
+        ("sgs_model", ctypes.c_int),
     ]
 
 
@@ -46,7 +47,7 @@ def header_functions():
     """Names of the functions declared in include/turbchannel.h."""
     with open(HEADER_PATH) as f:
         text = f.read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(tc_[a-z_0-9]+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|long long|const char\*)\s+(tc_[a-z_0-9]+)\s*\(", text, re.M)))
 
 
 def lib():
@@ -85,6 +86,21 @@ def lib():
     L.tc_timing_enable.argtypes = [vp, c_int]
     L.tc_timing_read.argtypes = [vp, c_int, dp, P(ctypes.c_longlong)]
     L.tc_timing_reset.argtypes = [vp]
+    fp = P(ctypes.c_float)
+    ll = ctypes.c_longlong
+    L.tc_set_sgs_params.argtypes = [vp, fp, ctypes.c_size_t]
+    L.tc_sgs_sample_count.argtypes = [vp, c_int]
+    L.tc_sgs_sample_count.restype = ll
+    L.tc_sgs_samples.argtypes = [vp, c_int, c_int, c_int, vp, vp, ll, P(ll)]
+    L.tc_mlp_param_count.argtypes = []
+    L.tc_mlp_create.argtypes = [c_int, ctypes.c_ulonglong, P(vp)]
+    L.tc_mlp_destroy.argtypes = [vp]
+    L.tc_mlp_set_params.argtypes = [vp, fp, ctypes.c_size_t]
+    L.tc_mlp_get_params.argtypes = [vp, fp, ctypes.c_size_t]
+    L.tc_mlp_forward.argtypes = [vp, vp, ll, vp]
+    L.tc_mlp_gradient.argtypes = [vp, vp, vp, ll, fp, ctypes.c_size_t, dp]
+    L.tc_mlp_train_step.argtypes = [vp, vp, vp, ll, ctypes.c_float, dp]
+    L.tc_mlp_steps.argtypes = [vp, P(ll)]
     _lib = L
     return L
 
@@ -99,13 +115,13 @@ def _dptr(a):
 
 
 def make_params(scheme, nx, ny, nz, zz, *, precision=FP64, visc, sour_dpdx=0.0, inv_length_scale=1.0, hx, hy,
-                turb_visc=0, visc_type=None, stab_type=1, fd_mode=1, xmin=0.0, ymin=0.0, cfr_ubulk=0.0):
+                turb_visc=0, visc_type=None, stab_type=1, fd_mode=1, xmin=0.0, ymin=0.0, cfr_ubulk=0.0, sgs_model=0):
     """tc_params for a problem; keeps the mesh array alive on the returned object."""
     if visc_type is None:
         visc_type = 1 if turb_visc else 0
     zz = np.ascontiguousarray(zz, dtype=np.float64)
     p = TcParams(scheme, precision, nx, ny, nz, turb_visc, visc_type, stab_type, fd_mode, visc, sour_dpdx,
-                 inv_length_scale, hx, hy, xmin, ymin, cfr_ubulk, _dptr(zz))
+                 inv_length_scale, hx, hy, xmin, ymin, cfr_ubulk, _dptr(zz), sgs_model)
     p._zz = zz
     return p
 
@@ -175,7 +191,7 @@ class Engine:
 
     # --- fields -----------------------------------------------------------
     def _shape(self, kind):
-        return (self.n3d, 3) if kind in (VEL, FLUX) else (self.n3d,)
+        return (self.n3d, 3) if kind in (VEL, FLUX) else (self.n3d, 6) if kind == TAU else (self.n3d,)
 
     def set_field(self, kind, slot, a):
         a = np.ascontiguousarray(np.asarray(a, dtype=self.dtype).reshape(self._shape(kind)))
@@ -246,6 +262,21 @@ class Engine:
     def initial_field(self, slot, seed=42, ubulk=1.0, noise_rms=0.1):
         _check(lib().tc_initial_field(self._h, slot, seed, ubulk, noise_rms))
 
+    # --- learned closure ---------------------------------------------------
+    def set_sgs_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        _check(lib().tc_set_sgs_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size))
+
+    def sgs_sample_count(self, factor):
+        return lib().tc_sgs_sample_count(self._h, factor)
+
+    def sgs_samples(self, vel, factor, dev_x, dev_t, cap):
+        """Writes training samples of field `vel` into device buffers (addresses);
+        returns their number."""
+        n = ctypes.c_longlong()
+        _check(lib().tc_sgs_samples(self._h, vel[0], vel[1], factor, dev_x, dev_t, cap, ctypes.byref(n)))
+        return n.value
+
     # --- timing -------------------------------------------------------------
     def timing(self, on=True):
         _check(lib().tc_timing_enable(self._h, 1 if on else 0))
@@ -261,3 +292,58 @@ class Engine:
             _check(lib().tc_timing_read(self._h, k, ctypes.byref(ms), ctypes.byref(n)))
             res[name] = (ms.value, n.value)
         return res
+
+
+def mlp_param_count():
+    return lib().tc_mlp_param_count()
+
+
+class Mlp:
+    """Trainer of the learned closure on one GPU (tc_mlp_*). Inputs and targets
+    are device buffers given by address: x [n][10], t [n][6] float32."""
+
+    def __init__(self, device=0, seed=0):
+        h = ctypes.c_void_p()
+        _check(lib().tc_mlp_create(device, seed, ctypes.byref(h)))
+        self._h = h
+        self.n_params = mlp_param_count()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tc_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().tc_mlp_set_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size))
+
+    def get_params(self):
+        out = np.zeros(self.n_params, dtype=np.float32)
+        _check(lib().tc_mlp_get_params(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), out.size))
+        return out
+
+    def forward(self, dev_x, n, dev_y):
+        _check(lib().tc_mlp_forward(self._h, dev_x, n, dev_y))
+
+    def gradient(self, dev_x, dev_t, n):
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().tc_mlp_gradient(self._h, dev_x, dev_t, n, g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                     g.size, ctypes.byref(loss)))
+        return loss.value, g
+
+    def train_step(self, dev_x, dev_t, n, lr=1e-3):
+        loss = ctypes.c_double()
+        _check(lib().tc_mlp_train_step(self._h, dev_x, dev_t, n, lr, ctypes.byref(loss)))
+        return loss.value
+
+    def steps(self):
+        n = ctypes.c_longlong()
+        _check(lib().tc_mlp_steps(self._h, ctypes.byref(n)))
+        return n.value

@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libturbchannel_amd.so")
-SOURCES = ["host_setup.cpp", "kernels_stencil.hip", "kernels_spectral.hip", "kernels_reduce.hip", "engine.cpp"]
+SOURCES = ["host_setup.cpp", "kernels_stencil.hip", "kernels_spectral.hip", "kernels_reduce.hip", "kernels_mlp.hip",
+           "engine.cpp", "mlp.cpp"]
 
 ARCH = os.environ.get("TC_OFFLOAD_ARCH", "gfx950")
 FLAGS = [

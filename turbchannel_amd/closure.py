@@ -1,0 +1,66 @@
+"""Host side of the learned subgrid closure: data standardisation, folding the
+standardisation into the network's first and last layers, and the training loop
+over device-resident samples (the arithmetic runs in the HIP kernels behind
+``tc_mlp_*``; this module only orders the calls).
+
+The closure has no counterpart in the reference; it replaces the role of
+RecalcTurbVisc (staggered/func_global.h:815-890, collocated_AB3/func_global.h:342-384)
+in a collocated LES when an engine is created with ``sgs_model=1``.
+"""
+import numpy as np
+
+from . import Mlp, mlp_param_count
+
+N_IN, N_HID, N_OUT = 10, 64, 6
+_OFF_W1 = 0
+_OFF_B1 = _OFF_W1 + N_HID * N_IN
+_OFF_W4 = _OFF_B1 + N_HID + 2 * (N_HID * N_HID + N_HID)
+_OFF_B4 = _OFF_W4 + N_OUT * N_HID
+
+
+class Standardizer:
+    """Per-column mean and scale of inputs and targets."""
+
+    def __init__(self, X, T, eps=1e-12):
+        X = np.asarray(X, dtype=np.float64)
+        T = np.asarray(T, dtype=np.float64)
+        self.mx, self.sx = X.mean(axis=0), np.maximum(X.std(axis=0), eps)
+        self.mt, self.st = T.mean(axis=0), np.maximum(T.std(axis=0), eps)
+
+    def inputs(self, X):
+        return ((np.asarray(X, dtype=np.float64) - self.mx) / self.sx).astype(np.float32)
+
+    def targets(self, T):
+        return ((np.asarray(T, dtype=np.float64) - self.mt) / self.st).astype(np.float32)
+
+    def fold(self, params):
+        """Parameters of the network on standardised data -> the same network on
+        raw inputs and outputs: W1 / sx, b1 - W1 (mx / sx); st W4, st b4 + mt."""
+        p = np.asarray(params, dtype=np.float64).copy()
+        assert p.size == mlp_param_count()
+        W1 = p[_OFF_W1:_OFF_B1].reshape(N_HID, N_IN)
+        b1 = p[_OFF_B1:_OFF_B1 + N_HID]
+        p[_OFF_B1:_OFF_B1 + N_HID] = b1 - W1 @ (self.mx / self.sx)
+        p[_OFF_W1:_OFF_B1] = (W1 / self.sx[None, :]).ravel()
+        W4 = p[_OFF_W4:_OFF_B4].reshape(N_OUT, N_HID)
+        b4 = p[_OFF_B4:_OFF_B4 + N_OUT]
+        p[_OFF_W4:_OFF_B4] = (W4 * self.st[:, None]).ravel()
+        p[_OFF_B4:_OFF_B4 + N_OUT] = b4 * self.st + self.mt
+        return p.astype(np.float32)
+
+
+def train(net, dev_x, dev_t, n, *, batch=65536, iters=1000, lr=1e-3, lr_final=None):
+    """Adam over contiguous mini-batches of device samples (addresses of float32
+    [n][10] inputs and [n][6] targets, already shuffled); returns the losses.
+    The learning rate decays geometrically to lr_final when given."""
+    batch = min(batch, n)
+    nb = max(1, n // batch)
+    losses = []
+    for it in range(iters):
+        off = (it % nb) * batch
+        a = lr if lr_final is None else lr * (lr_final / lr) ** (it / max(1, iters - 1))
+        losses.append(net.train_step(dev_x + 4 * N_IN * off, dev_t + 4 * N_OUT * off, batch, lr=a))
+    return np.array(losses)
+
+
+__all__ = ["Standardizer", "train", "Mlp"]

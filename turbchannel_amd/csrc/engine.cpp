@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "mlp.h"
 
 using namespace tc;
 
@@ -24,6 +25,12 @@ int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
+
+}  // namespace
+
+void tc_set_error(const char* msg) { g_err = msg; }
+
+namespace {
 
 #define HIPCHK(x)                                                                                   \
     do {                                                                                            \
@@ -90,6 +97,15 @@ struct tc_engine {
     double* d_out = nullptr;    // small results (16 doubles)
     double* d_shift = nullptr;  // flow-rate shift
     double* d_prof = nullptr;   // profiles result (nz * 12)
+    // learned closure (collocated, sgs_model == 1)
+    void* tau = nullptr;                // [node][6] closure stress
+    unsigned short* sgs_w = nullptr;    // bfloat16 weight block
+    float* sgs_b = nullptr;             // FP32 biases
+    bool sgs_ready = false;
+    // training-sample filter (allocated on first use)
+    int f_factor = 0;
+    FFTPlan fplx, fply;
+    void *ftwx = nullptr, *ftwy = nullptr, *fspec = nullptr, *fq = nullptr;
     Timing tim;
     std::vector<void*> allocs;
 
@@ -173,11 +189,26 @@ void* field_ptr(tc_engine* e, int kind, int slot) {
         case TC_FLUX: return e->flux[slot];
         case TC_PRES: return e->pres[slot];
         case TC_NUT: return slot == 0 ? e->nut : nullptr;
+        case TC_TAU: return slot == 0 ? e->tau : nullptr;
         default: return nullptr;
     }
 }
 bool is_vec(int kind) { return kind == TC_VEL || kind == TC_FLUX; }
 bool is_scal(int kind) { return kind == TC_PRES || kind == TC_NUT; }
+size_t ncomp(int kind) { return kind == TC_TAU ? 6 : is_vec(kind) ? 3 : 1; }
+
+// FFT twiddles exp(-2 pi i k / n) in T.
+template <typename T>
+int upload_twiddles(tc_engine* e, int n, void** dst) {
+    std::vector<Cplx<T>> t(n);
+    for (int k = 0; k < n; k++) {
+        const double a = 2.0 * 3.14159265358979323846 * (double)k / (double)n;
+        t[k] = {(T)std::cos(a), (T)(-std::sin(a))};
+    }
+    if (int r = dalloc_bytes(e, dst, n * sizeof(Cplx<T>))) return r;
+    HIPCHK(hipMemcpy(*dst, t.data(), n * sizeof(Cplx<T>), hipMemcpyHostToDevice));
+    return TC_OK;
+}
 
 template <typename T>
 int setup_device(tc_engine* e) {
@@ -198,6 +229,12 @@ int setup_device(tc_engine* e) {
     }
     if (int r = dalloc(e, &p, n3)) return r;
     e->dp = p;
+    if (e->P.sgs_model) {
+        if (int r = dalloc(e, &p, 6 * n3)) return r;
+        e->tau = p;
+        if (int r = dalloc(e, &e->sgs_w, (size_t)kWBlock)) return r;
+        if (int r = dalloc(e, &e->sgs_b, (size_t)kBBlock)) return r;
+    }
     if (int r = dalloc_bytes(e, &e->spec, (size_t)e->nm * e->nz * sizeof(Cplx<T>))) return r;
     // mesh arrays: ZZ, DZ, DZW, GC1, GC2, GCZ, cellh, each mstride long
     const HostMesh& hm = e->hm;
@@ -271,19 +308,8 @@ int setup_device(tc_engine* e) {
         HIPCHK(hipMemcpy(e->zfac_low, lf.data(), lf.size() * sizeof(double), hipMemcpyHostToDevice));
         if (int r = dalloc(e, &e->ylow, 2 * lz)) return r;
     }
-    // FFT twiddles exp(-2 pi i k / n)
-    auto tw = [&](int n, void** dst) -> int {
-        std::vector<Cplx<T>> t(n);
-        for (int k = 0; k < n; k++) {
-            const double a = 2.0 * 3.14159265358979323846 * (double)k / (double)n;
-            t[k] = {(T)std::cos(a), (T)(-std::sin(a))};
-        }
-        if (int r = dalloc_bytes(e, dst, n * sizeof(Cplx<T>))) return r;
-        HIPCHK(hipMemcpy(*dst, t.data(), n * sizeof(Cplx<T>), hipMemcpyHostToDevice));
-        return TC_OK;
-    };
-    if (int r = tw(e->nx, &e->twx)) return r;
-    if (int r = tw(e->ny, &e->twy)) return r;
+    if (int r = upload_twiddles<T>(e, e->nx, &e->twx)) return r;
+    if (int r = upload_twiddles<T>(e, e->ny, &e->twy)) return r;
     e->npartial = std::max<size_t>((size_t)kRedBlocks * 3, (size_t)flux_grid_blocks(e->nx, e->ny, e->nz));
     e->npartial = std::max<size_t>(e->npartial, (size_t)profiles_chunks(e->nxy) * e->nz * 12);
     if (int r = dalloc(e, &e->d_partial, e->npartial)) return r;
@@ -371,6 +397,13 @@ int unapply(tc_engine* e, const T* v, T* x) {
     return TC_OK;
 }
 
+MlpDev sgs_view(const tc_engine* e) {
+    MlpDev d;
+    d.w = e->sgs_w;
+    d.b = e->sgs_b;
+    return d;
+}
+
 template <typename T>
 int step(tc_engine* e, double tau, int cur, int order) {
     const int next = (cur + 1) % 4, prev = (cur + 3) % 4, prev2 = (cur + 2) % 4;
@@ -382,6 +415,12 @@ int step(tc_engine* e, double tau, int cur, int order) {
     if (e->P.turb_visc) {
         e->tbeg(TC_K_TURB_VISC, ev);
         HIPCHK(launch_turb_visc<T>(e->s, m, e->P.scheme, (const T*)e->vel[cur], (T*)e->nut));
+        e->tend(TC_K_TURB_VISC, ev);
+    }
+    if (e->P.sgs_model) {
+        if (!e->sgs_ready) return fail(TC_ERR_STATE, "learned closure weights are not set (tc_set_sgs_params)");
+        e->tbeg(TC_K_TURB_VISC, ev);
+        HIPCHK(launch_sgs_tau<T>(e->s, m, sgs_view(e), (const T*)e->vel[cur], (T*)e->tau));
         e->tend(TC_K_TURB_VISC, ev);
     }
     FluxArgs<T> a;
@@ -401,6 +440,7 @@ int step(tc_engine* e, double tau, int cur, int order) {
     a.pcur = (const T*)e->pres[cur];
     a.inv_tau = (T)(1.0 / tau);
     a.cfr_partial = cfr ? e->d_partial : nullptr;
+    a.tau = (const T*)e->tau;
     int nblk = 0;
     e->tbeg(TC_K_FLUX, ev);
     HIPCHK(launch_flux<T>(e->s, m, a, &nblk));
@@ -472,6 +512,41 @@ int project(tc_engine* e, T* v) {
     return TC_OK;
 }
 
+// Training samples of the learned closure from one collocated velocity field:
+// sharp spectral cut-off by `factor` in x and y, sampled on the coarse grid.
+template <typename T>
+int sgs_samples(tc_engine* e, const T* v, int factor, float* x, float* y) {
+    const int nxc = e->nx / factor, nyc = e->ny / factor, nz = e->nz;
+    const int nxhc = nxc / 2 + 1;
+    if (e->f_factor != factor) {
+        if (e->f_factor != 0) return fail(TC_ERR_STATE, "the sample filter factor cannot change on one engine");
+        if (!make_plan(nxc, e->fplx) || !make_plan(nyc, e->fply))
+            return fail(TC_ERR_ARG, "coarse sizes must have only the prime factors 2, 3 and 5");
+        if (int r = upload_twiddles<T>(e, nxc, &e->ftwx)) return r;
+        if (int r = upload_twiddles<T>(e, nyc, &e->ftwy)) return r;
+        if (int r = dalloc_bytes(e, &e->fspec, (size_t)nxhc * nyc * nz * sizeof(Cplx<T>))) return r;
+        if (int r = dalloc_bytes(e, &e->fq, 9 * (size_t)nxc * nyc * nz * sizeof(T))) return r;
+        e->f_factor = factor;
+    }
+    const Mesh<T> m = e->mesh<T>();
+    Cplx<T>* spec = (Cplx<T>*)e->spec;
+    Cplx<T>* cs = (Cplx<T>*)e->fspec;
+    T* q = (T*)e->fq;
+    const size_t fs = (size_t)nxc * nyc * nz;
+    static const int ca[9] = {0, 1, 2, 0, 1, 2, 0, 0, 1}, cb[9] = {-1, -1, -1, 0, 1, 2, 1, 2, 2};
+    for (int f = 0; f < 9; f++) {
+        HIPCHK(launch_fft_x_fwd_q<T>(e->s, e->plx, (const Cplx<T>*)e->twx, v, ca[f], cb[f], e->ny * nz, spec));
+        HIPCHK(launch_fft_y<T>(e->s, e->ply, (const Cplx<T>*)e->twy, spec, e->nxh, nz, 0, nullptr, nullptr));
+        HIPCHK(launch_spec_crop<T>(e->s, spec, e->nxh, e->ny, cs, nxhc, nyc, nz));
+        HIPCHK(launch_fft_y<T>(e->s, e->fply, (const Cplx<T>*)e->ftwy, cs, nxhc, nz, 1, nullptr, nullptr));
+        HIPCHK(launch_fft_x_inv<T>(e->s, e->fplx, (const Cplx<T>*)e->ftwx, cs, q + f * fs, nyc * nz,
+                                   (T)(1.0 / ((double)e->nx * e->ny))));
+    }
+    HIPCHK(launch_sgs_samples<T>(e->s, m, q, nxc, nyc, (T)(e->P.hx * factor), (T)(e->P.hy * factor), x, y));
+    HIPCHK(hipStreamSynchronize(e->s));
+    return TC_OK;
+}
+
 #define DISPATCH(e, call) ((e)->fp32 ? call<float> : call<double>)
 
 int check(tc_engine* e) {
@@ -528,6 +603,9 @@ int tc_create(const tc_params* p, int device, tc_engine** out) {
     if (p->visc_type < 0 || p->visc_type > 1) return fail(TC_ERR_ARG, "visc_type must be 0 or 1");
     if (p->scheme != TC_SCHEME_AB3 && p->turb_visc && p->visc_type == 0)
         return fail(TC_ERR_ARG, "tViscApproximation::CROSS does not work for a turbulence model");
+    if (p->sgs_model < 0 || p->sgs_model > 1) return fail(TC_ERR_ARG, "sgs_model must be 0 or 1");
+    if (p->sgs_model && (p->scheme != TC_SCHEME_AB3 || p->turb_visc))
+        return fail(TC_ERR_ARG, "the learned closure runs on the collocated scheme, without nu_t");
     const double n3 = (double)p->nx * p->ny * p->nz;
     if (n3 * 3 > 2.0e9) return fail(TC_ERR_ARG, "grid too large for 32-bit cell indexing");
     int ndev = 0;
@@ -591,7 +669,7 @@ int tc_set_field(tc_engine* e, int kind, int slot, const void* host, size_t byte
     if (int r = check(e)) return r;
     void* d = field_ptr(e, kind, slot);
     if (!d || !host) return fail(TC_ERR_ARG, "bad field");
-    const size_t need = (is_vec(kind) ? 3 : 1) * e->n3d * e->rsz;
+    const size_t need = ncomp(kind) * e->n3d * e->rsz;
     if (bytes != need) return fail(TC_ERR_ARG, "size mismatch: expected " + std::to_string(need) + " bytes");
     HIPCHK(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, e->s));
     HIPCHK(hipStreamSynchronize(e->s));
@@ -602,7 +680,7 @@ int tc_get_field(tc_engine* e, int kind, int slot, void* host, size_t bytes) {
     if (int r = check(e)) return r;
     void* d = field_ptr(e, kind, slot);
     if (!d || !host) return fail(TC_ERR_ARG, "bad field");
-    const size_t need = (is_vec(kind) ? 3 : 1) * e->n3d * e->rsz;
+    const size_t need = ncomp(kind) * e->n3d * e->rsz;
     if (bytes != need) return fail(TC_ERR_ARG, "size mismatch: expected " + std::to_string(need) + " bytes");
     HIPCHK(hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, e->s));
     HIPCHK(hipStreamSynchronize(e->s));
@@ -640,6 +718,7 @@ int tc_explicit_term(tc_engine* e, int in_kind, int in_slot, int out_kind, int o
         a.order = 0;
         a.vel = (const T*)in;
         a.nut = (const T*)e->nut;
+        a.tau = (const T*)e->tau;
         a.fcur = (T*)out;
         hipEvent_t ev;
         e->tbeg(TC_K_FLUX, ev);
@@ -679,6 +758,12 @@ int tc_recalc_turb_visc(tc_engine* e, int kind, int slot) {
     if (int r = check(e)) return r;
     void* v = field_ptr(e, kind, slot);
     if (!v || !is_vec(kind)) return fail(TC_ERR_ARG, "bad field");
+    if (e->P.sgs_model) {  // the learned closure's counterpart: tau from this field
+        if (!e->sgs_ready) return fail(TC_ERR_STATE, "learned closure weights are not set (tc_set_sgs_params)");
+        if (e->fp32) HIPCHK(launch_sgs_tau<float>(e->s, e->mesh<float>(), sgs_view(e), (const float*)v, (float*)e->tau));
+        else HIPCHK(launch_sgs_tau<double>(e->s, e->mesh<double>(), sgs_view(e), (const double*)v, (double*)e->tau));
+        return TC_OK;
+    }
     if (!e->nut) return TC_OK;  // RecalcTurbVisc returns when nu_t is not allocated
     if (e->fp32) HIPCHK(launch_turb_visc<float>(e->s, e->mesh<float>(), e->P.scheme, (const float*)v, (float*)e->nut));
     else HIPCHK(launch_turb_visc<double>(e->s, e->mesh<double>(), e->P.scheme, (const double*)v, (double*)e->nut));
@@ -791,6 +876,43 @@ int tc_initial_field(tc_engine* e, int slot, unsigned long long seed, double ubu
         HIPCHK(hipMemcpy(v, u.data(), u.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     return tc_project(e, TC_VEL, slot);
+}
+
+int tc_set_sgs_params(tc_engine* e, const float* params, size_t count) {
+    if (int r = check(e)) return r;
+    if (!e->P.sgs_model) return fail(TC_ERR_STATE, "engine created without the learned closure (sgs_model = 0)");
+    if (!params || count != (size_t)kMlpParams)
+        return fail(TC_ERR_ARG, "expected " + std::to_string(kMlpParams) + " parameters");
+    std::vector<unsigned short> w;
+    std::vector<float> b;
+    mlp_pack_device(params, w, b);
+    HIPCHK(hipMemcpyAsync(e->sgs_w, w.data(), w.size() * sizeof(unsigned short), hipMemcpyHostToDevice, e->s));
+    HIPCHK(hipMemcpyAsync(e->sgs_b, b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice, e->s));
+    HIPCHK(hipStreamSynchronize(e->s));
+    e->sgs_ready = true;
+    return TC_OK;
+}
+
+long long tc_sgs_sample_count(const tc_engine* e, int factor) {
+    if (!e || factor < 1 || e->nx % factor || e->ny % factor) return -1;
+    return (long long)(e->nx / factor) * (e->ny / factor) * (e->nz - 2);
+}
+
+int tc_sgs_samples(tc_engine* e, int kind, int slot, int factor, float* dev_x, float* dev_t, long long cap,
+                   long long* count) {
+    if (int r = check(e)) return r;
+    if (e->P.scheme != TC_SCHEME_AB3) return fail(TC_ERR_ARG, "training samples are taken from collocated fields");
+    void* v = field_ptr(e, kind, slot);
+    if (!v || !is_vec(kind) || !dev_x || !dev_t) return fail(TC_ERR_ARG, "bad field or output");
+    if (factor < 1 || e->nx % (2 * factor) || e->ny % (2 * factor))
+        return fail(TC_ERR_ARG, "factor must divide nx/2 and ny/2");
+    const long long n = tc_sgs_sample_count(e, factor);
+    if (cap < n) return fail(TC_ERR_ARG, "output capacity below " + std::to_string(n) + " samples");
+    int r = e->fp32 ? sgs_samples<float>(e, (const float*)v, factor, dev_x, dev_t)
+                    : sgs_samples<double>(e, (const double*)v, factor, dev_x, dev_t);
+    if (r) return r;
+    if (count) *count = n;
+    return TC_OK;
 }
 
 int tc_timing_enable(tc_engine* e, int on) {

@@ -18,6 +18,7 @@ struct FluxArgs {
     const T* pcur = nullptr;  // collocated StabType 1: r -= G p_n
     T inv_tau = 0;
     double* cfr_partial = nullptr;  // per-block sums of r_u * volume (nullptr: off)
+    const T* tau = nullptr;         // collocated learned closure: subtract div tau (nullptr: off)
 };
 // nblocks (may be nullptr) receives the number of per-block partial sums written to cfr_partial.
 template <typename T>
@@ -66,6 +67,16 @@ hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cpl
 template <typename T>
 hipError_t launch_fft_x_inv(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const Cplx<T>* spec, T* out,
                             int nrows, T scale);
+
+// Training samples of the learned closure: forward x transform of component ca
+// (cb < 0) or of the product of components ca, cb of a real3 field, and the
+// coarse half spectrum of the modes a sharp cut-off keeps.
+template <typename T>
+hipError_t launch_fft_x_fwd_q(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const T* v, int ca, int cb,
+                              int nrows, Cplx<T>* spec);
+template <typename T>
+hipError_t launch_spec_crop(hipStream_t s, const Cplx<T>* fine, int nxh, int ny, Cplx<T>* coarse, int nxhc, int nyc,
+                            int nz);
 
 // Factor arrays on the device, [coef][iz][mode] (or [iz][lowmode] for the list).
 template <typename CT>

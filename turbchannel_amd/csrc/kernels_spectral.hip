@@ -423,6 +423,61 @@ __device__ void zsolve_low_lane(Cplx<T>* spec, const ZFacDev<double>& f, int li,
     if (n00) norm.s00[0] = psum;  // applied by the inverse y transform (NormalizePressure)
 }
 
+// Forward x transform of one component (cb < 0) or of the product of two
+// components of a real3 field (training samples of the learned closure).
+template <typename T>
+__global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_q(FFTPlan pl, const Cplx<T>* __restrict__ tw,
+                                                              const T* __restrict__ v, int ca, int cb, int nrows,
+                                                              Cplx<T>* __restrict__ spec, int lpb) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
+    const int N = pl.n, nxh = N / 2 + 1;
+    Cplx<T>* b1 = b0 + lpb * N;
+    const int q0 = blockIdx.x * lpb;
+    auto q = [&](int row, int n) {
+        const T* c = v + 3 * ((size_t)row * N + n);
+        return cb < 0 ? c[ca] : c[ca] * c[cb];
+    };
+    for (int t = threadIdx.x; t < lpb * N; t += blockDim.x) {
+        const int l = t / N, n = t - l * N;
+        const int ra = 2 * (q0 + l), rb = ra + 1;
+        Cplx<T> z = {T(0), T(0)};
+        if (ra < nrows) z.x = q(ra, n);
+        if (rb < nrows) z.y = q(rb, n);
+        b0[t] = z;
+    }
+    __syncthreads();
+    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, N, 1, tw, T(-1));
+    for (int t = threadIdx.x; t < lpb * nxh; t += blockDim.x) {
+        const int l = t / nxh, k = t - l * nxh;
+        const int ra = 2 * (q0 + l), rb = ra + 1;
+        if (ra >= nrows) continue;
+        const Cplx<T> z = res[l * N + k];
+        const Cplx<T> zc = res[l * N + (k == 0 ? 0 : N - k)];
+        spec[(size_t)ra * nxh + k] = {T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
+        if (rb < nrows) spec[(size_t)rb * nxh + k] = {T(0.5) * (z.y + zc.y), T(-0.5) * (z.x - zc.x)};
+    }
+}
+
+// Half spectrum of the coarse grid holding the modes a sharp cut-off keeps:
+// |kx| < nxc/2 and |ky| < nyc/2 (the coarse Nyquist modes are zero).
+template <typename T>
+__global__ void k_spec_crop(const Cplx<T>* __restrict__ fine, int nxh, int ny, Cplx<T>* __restrict__ coarse, int nxhc,
+                            int nyc, int nz) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nxhc * nyc * nz) return;
+    const int kx = (int)(t % nxhc);
+    const long long rest = t / nxhc;
+    const int kyc = (int)(rest % nyc), iz = (int)(rest / nyc);
+    const int nxc = 2 * (nxhc - 1), hy = nyc / 2;
+    Cplx<T> v = {T(0), T(0)};
+    if (kx < nxc / 2) {
+        const int ky = kyc < hy ? kyc : (kyc > hy ? ny - (nyc - kyc) : -1);
+        if (ky >= 0) v = fine[((size_t)iz * ny + ky) * nxh + kx];
+    }
+    coarse[t] = v;
+}
+
 template <typename T>
 int lines_per_block(int n) {
     // two ping-pong buffers per line; keep a block near 24 KiB of LDS
@@ -482,6 +537,27 @@ hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, cons
     return hipGetLastError();
 }
 
+
+template <typename T>
+hipError_t launch_fft_x_fwd_q(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const T* v, int ca, int cb,
+                              int nrows, Cplx<T>* spec) {
+    const int lpb = lines_per_block<T>(pl.n);
+    const int nlines = (nrows + 1) / 2;
+    const int grid = (nlines + lpb - 1) / lpb;
+    const size_t sm = 2 * (size_t)lpb * pl.n * sizeof(Cplx<T>);
+    hipLaunchKernelGGL((k_fft_x_fwd_q<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, v, ca, cb, nrows, spec, lpb);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_spec_crop(hipStream_t s, const Cplx<T>* fine, int nxh, int ny, Cplx<T>* coarse, int nxhc, int nyc,
+                            int nz) {
+    const long long n = (long long)nxhc * nyc * nz;
+    hipLaunchKernelGGL((k_spec_crop<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fine, nxh, ny, coarse, nxhc,
+                       nyc, nz);
+    return hipGetLastError();
+}
+
 #define INST(T)                                                                                                      \
     template hipError_t launch_fft_x_fwd_div<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const Mesh<T>&, int,    \
                                                 const T*, Cplx<T>*);                                                 \
@@ -489,7 +565,10 @@ hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, cons
     template hipError_t launch_fft_y<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, Cplx<T>*, int, int, int,       \
                                         const double*, const double*);                                              \
     template hipError_t launch_zsolve<T>(hipStream_t, Cplx<T>*, const ZFacDev<T>&, const ZFacDev<double>&,         \
-                                         const int*, int, double*, ZNorm);
+                                         const int*, int, double*, ZNorm);                                   \
+    template hipError_t launch_fft_x_fwd_q<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const T*, int, int, int,  \
+                                              Cplx<T>*);                                                             \
+    template hipError_t launch_spec_crop<T>(hipStream_t, const Cplx<T>*, int, int, Cplx<T>*, int, int, int);
 INST(double)
 INST(float)
 #undef INST

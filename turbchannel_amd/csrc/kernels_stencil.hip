@@ -135,6 +135,28 @@ struct RingAcc {
     }
 };
 
+// Learned closure (new numerics, no reference counterpart): k -= d tau_ij / d x_j at
+// an interior collocated node, central differences in x, y and the 3-point
+// formula on the nonuniform z nodes; tau is [node][XX YY ZZ XY XZ YZ].
+template <typename T>
+__device__ __forceinline__ void sgs_divergence(const Mesh<T>& m, const T* __restrict__ t, const Nb& nb, int iz, T k[3]) {
+    const T hm = m.DZ[iz - 1], hp = m.DZ[iz];
+    const T den = hp * hm * (hp + hm);
+    const T cp = hm * hm / den, cm = hp * hp / den;
+    const T hx2 = T(0.5) * m.inv_hx, hy2 = T(0.5) * m.inv_hy;
+    const int c0 = 6 * nb.at(0, 0, 0), xr = 6 * nb.at(1, 0, 0), xl = 6 * nb.at(-1, 0, 0);
+    const int yr = 6 * nb.at(0, 1, 0), yl = 6 * nb.at(0, -1, 0), zr = 6 * nb.at(0, 0, 1), zl = 6 * nb.at(0, 0, -1);
+    constexpr int ax[3] = {0, 3, 4}, ay[3] = {3, 1, 5}, az[3] = {4, 5, 2};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        T d = hx2 * (t[xr + ax[c]] - t[xl + ax[c]]);
+        d += hy2 * (t[yr + ay[c]] - t[yl + ay[c]]);
+        const T t0 = t[c0 + az[c]];
+        d += cp * (t[zr + az[c]] - t0) + cm * (t0 - t[zl + az[c]]);
+        k[c] -= d;
+    }
+}
+
 template <typename T, int SCHEME, int VISC, bool TURB>
 __global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
     Mesh<T> m, FluxArgs<T> a, int ntx, int ntiles, int zchunk) {
@@ -229,6 +251,7 @@ __global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
                             k[c] += m.visc * ((acc(c, 0, 0, 1) - a0) / m.DZ[iz] - (a0 - acc(c, 0, 0, -1)) / m.DZ[iz - 1]) * inv_hz;
                         }
                     }
+                    if (a.tau) sgs_divergence(m, a.tau, nb, iz, k);
                     k[0] += m.sour;
                 }
             } else {  // ExplicitTerm (staggered/func_global.h:642-649)
