@@ -39,17 +39,28 @@ REALS_PER_POINT = {
     "project": 7,    # reads p (1), r (3); writes u_n+1 (3)
 }
 
+PEAK_BF16_TFLOPS = 2500.0  # dense bfloat16 MFMA
+# flops per sample of the closure network (10 -> 64 -> 64 -> 64 -> 6): forward, and
+# forward + backward (weight gradients of every layer, data gradients of layers 2-4)
+MLP_FWD_FLOP = 2 * (10 * 64 + 64 * 64 + 64 * 64 + 64 * 6)
+MLP_TRAIN_FLOP = 2 * MLP_FWD_FLOP + 2 * (64 * 6 + 64 * 64 + 64 * 64)
+
 WORKLOADS = {
-    # name: scheme, nx, ny, nz, gamma, Re_b, Re_tau, dt
-    "vv_192x160x128": (tca.SCHEME_VV, 192, 160, 128, 1.8, 5600.0, 180.0, 1e-3),
-    "hw_64x64x64": (tca.SCHEME_HW, 64, 64, 64, 1.8, 5600.0, 180.0, 2e-3),
-    "ab3_384x384x192": (tca.SCHEME_AB3, 384, 384, 192, 2.0, 13750.0, 395.0, 5e-4),
-    "vv_384x384x192": (tca.SCHEME_VV, 384, 384, 192, 2.0, 13750.0, 395.0, 5e-4),
+    # name: scheme, nx, ny, nz, gamma, Re_b, Re_tau, dt, extra engine parameters
+    "vv_192x160x128": (tca.SCHEME_VV, 192, 160, 128, 1.8, 5600.0, 180.0, 1e-3, {}),
+    "hw_64x64x64": (tca.SCHEME_HW, 64, 64, 64, 1.8, 5600.0, 180.0, 2e-3, {}),
+    "ab3_384x384x192": (tca.SCHEME_AB3, 384, 384, 192, 2.0, 13750.0, 395.0, 5e-4, {}),
+    "vv_384x384x192": (tca.SCHEME_VV, 384, 384, 192, 2.0, 13750.0, 395.0, 5e-4, {}),
+    # collocated LES at Re_tau 590, 64 x 64 x 96 nodes: learned closure, and the
+    # reference's Smagorinsky-van Driest model on the same grid
+    "les_learned_64x64x96": (tca.SCHEME_AB3, 64, 64, 96, 2.0, 21900.0, 590.0, 2e-3, dict(visc_type=1, sgs_model=1)),
+    "les_smag_64x64x96": (tca.SCHEME_AB3, 64, 64, 96, 2.0, 21900.0, 590.0, 2e-3, dict(visc_type=1, turb_visc=1)),
 }
+TRAIN_WORKLOAD = "closure_train_b65536"
 
 
 def problem(name, precision):
-    scheme, nx, ny, nz, gamma, re_b, re_tau, dt = WORKLOADS[name]
+    scheme, nx, ny, nz, gamma, re_b, re_tau, dt, extra = WORKLOADS[name]
     if scheme == tca.SCHEME_AB3:
         k = np.arange(nz)
         zz = 1.0 + np.tanh(gamma * (2.0 * k / (nz - 1) - 1.0)) / math.tanh(gamma)
@@ -63,7 +74,58 @@ def problem(name, precision):
               hy=math.pi / ny, cfr_ubulk=1.0, precision=precision)
     if scheme == tca.SCHEME_AB3:
         kw.update(stab_type=1, fd_mode=1, visc_type=0)
+    kw.update(extra)
     return scheme, nx, ny, nz, zz, kw, dt
+
+
+def closure_weights(seed=7):
+    """Closure parameters for timing runs: the trainer's seeded initial network
+    with its output layer scaled by 1e-2 (same cost as a trained one)."""
+    net = tca.Mlp(seed=seed)
+    p = net.get_params()
+    net.close()
+    n_out = 6 * 64 + 6
+    p[-n_out:] *= 1e-2
+    return p
+
+
+def bench_training(args):
+    """Closure training throughput: Adam steps on batches of 65536 samples."""
+    import torch  # device memory only
+
+    batch, nbatch = 65536, 16
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(batch * nbatch, 10)).astype(np.float32)
+    T = rng.normal(size=(batch * nbatch, 6)).astype(np.float32)
+    x = torch.from_numpy(X).to("cuda")
+    t = torch.from_numpy(T).to("cuda")
+    torch.cuda.synchronize()
+    net = tca.Mlp(seed=0)
+    step = lambda k: net.train_step(x.data_ptr() + 40 * batch * (k % nbatch),  # noqa: E731
+                                    t.data_ptr() + 24 * batch * (k % nbatch), batch, lr=1e-3)
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(k)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    sps = batch * args.steps / el
+    tflops = sps * MLP_TRAIN_FLOP / 1e12
+    rec = {
+        "metric": "closure training samples/s", "value": sps, "unit": "samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (fp32 accumulation, fp32 Adam)", "data": "synthetic (normal inputs and targets)",
+        "config": {"workload": TRAIN_WORKLOAD, "network": "10-64-64-64-6 GELU", "batch": batch,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "kernel": "k_mlp_train + reduce/Adam (one step)", "achieved": round(tflops, 2),
+                     "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / PEAK_BF16_TFLOPS, 5),
+                     "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(rec))
 
 
 def cpu_baseline(name, seconds_target=15.0):
@@ -77,14 +139,28 @@ def cpu_baseline(name, seconds_target=15.0):
     sol = oracle.Solver(scheme, nx, ny, nz, zz, **kw)
     u0 = sol.project(sol.initial_field(seed=42))
     sol.set_field(oracle.KIND_VEL, 0, u0)
+    closure = None
+    if kw.get("sgs_model"):  # learned closure: the network forward (numpy) feeds each step's tau
+        from oracle import mlp
+
+        p = closure_weights()
+        u_tau = kw["visc"] * kw["inv_length_scale"]
+        closure = lambda u: mlp.node_tau(p, u, nx, ny, nz, zz, kw["hx"], kw["hy"], u_tau,  # noqa: E731
+                                         kw["inv_length_scale"], emulate=False)
+
+    def step(it, cur, order):
+        if closure is not None:
+            sol.set_field(oracle.KIND_TAU, 0, closure(sol.get_field(oracle.KIND_VEL, cur)))
+        sol.step(it * dt, dt, cur, order)
+
     cur = 0
     for it in range(3):  # order ramp, untimed
-        sol.step(it * dt, dt, cur, it + 1)
+        step(it, cur, it + 1)
         cur = (cur + 1) % 4
     t0 = time.perf_counter()
     n = 0
     while True:
-        sol.step(n * dt, dt, cur, 3)
+        step(n, cur, 3)
         cur = (cur + 1) % 4
         n += 1
         el = time.perf_counter() - t0
@@ -123,7 +199,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="vv_192x160x128", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="vv_192x160x128", choices=sorted(WORKLOADS) + [TRAIN_WORKLOAD])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -144,10 +220,15 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
+    if args.workload == TRAIN_WORKLOAD:
+        bench_training(args)
+        return
     precision = tca.FP64 if args.precision == "fp64" else tca.FP32_BULK
     scheme, nx, ny, nz, zz, kw, dt = problem(args.workload, precision)
     n3d = nx * ny * nz
     eng = tca.Engine(scheme, nx, ny, nz, zz, device=0, **kw)
+    if kw.get("sgs_model"):
+        eng.set_sgs_params(closure_weights())
     eng.initial_field(0, seed=42, ubulk=1.0, noise_rms=0.1)
     cur = 0
     for it in range(max(args.warmup, 3)):
@@ -209,6 +290,13 @@ def main():
 
     value = n3d * args.steps / el
     traffic = measured_traffic(args.workload, args.precision, dom)
+    closure = None
+    if kw.get("sgs_model") and kt["turb_visc"][1]:
+        avg_s = kt["turb_visc"][0] / kt["turb_visc"][1] * 1e-3
+        tf = MLP_FWD_FLOP * nx * ny * (nz - 2) / avg_s / 1e12
+        closure = {"kernel": "k_sgs_tau", "avg_us": round(avg_s * 1e6, 2), "bound": "mfma",
+                   "achieved": round(tf, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                   "frac": round(tf / PEAK_BF16_TFLOPS, 5)}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.workload, args.cpu_seconds)
@@ -233,6 +321,7 @@ def main():
                      "traffic": traffic["bytes"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None},
         "cpu_baseline": cpu,
+        "closure": closure,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "with_diagnostics": {"value": n3d * nd / el_diag, "ms_per_step": 1e3 * el_diag / nd, "steps": nd},
         "kernels": per_kernel,

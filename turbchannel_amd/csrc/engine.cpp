@@ -87,6 +87,7 @@ struct tc_engine {
     void* zfac = nullptr;        // factors in T
     double* zfac_low = nullptr;  // factors of the low modes in double
     int* d_lowidx = nullptr;  // mode -> index in the FP64 low-mode list, or -1
+    int* d_lowmode = nullptr; // index in the low-mode list -> mode
     int nlow = 0;
     double* ylow = nullptr;
     double *d_Wt = nullptr, *d_C = nullptr, *d_s00 = nullptr;
@@ -266,17 +267,20 @@ int setup_device(tc_engine* e) {
     // wall-normal factors
     const ZFactors& zf = e->zf;
     const size_t fz = (size_t)e->nz * e->nm;
-    const int ncoef = 2 * zf.W + 1;
-    if (int r = dalloc_bytes(e, &e->zfac, ncoef * fz * sizeof(T))) return r;
+    const int FW = zfac_fw(zf.W), UW = zfac_uw(zf.W);
+    // packed per row: F[i][FW] = (M1..M_W, D), U[i][UW] = (U1..U_W), i = iz * ncols + col
+    auto pack = [&](auto* F, auto* U, size_t n, auto idx) {
+        for (size_t i = 0; i < n; i++) {
+            const size_t j = idx(i);
+            for (int c = 0; c < zf.W; c++) F[i * FW + c] = zf.M[c][j];
+            F[i * FW + FW - 1] = zf.D[j];
+            for (int c = 0; c < zf.W; c++) U[i * UW + c] = zf.U[c][j];
+        }
+    };
+    if (int r = dalloc_bytes(e, &e->zfac, (FW + UW) * fz * sizeof(T))) return r;
     {
-        std::vector<T> tmp(ncoef * fz);
-        int k = 0;
-        for (int c = 0; c < zf.W; c++, k++)
-            for (size_t i = 0; i < fz; i++) tmp[k * fz + i] = (T)zf.M[c][i];
-        for (size_t i = 0; i < fz; i++) tmp[k * fz + i] = (T)zf.D[i];
-        k++;
-        for (int c = 0; c < zf.W; c++, k++)
-            for (size_t i = 0; i < fz; i++) tmp[k * fz + i] = (T)zf.U[c][i];
+        std::vector<T> tmp((FW + UW) * fz);
+        pack(tmp.data(), tmp.data() + FW * fz, fz, [](size_t i) { return i; });
         HIPCHK(hipMemcpy(e->zfac, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
     }
     if (int r = dalloc(e, &e->d_Wt, e->nz)) return r;
@@ -294,16 +298,13 @@ int setup_device(tc_engine* e) {
         for (int li = 0; li < e->nlow; li++) lowidx[low[li]] = li;
         if (int r = dalloc(e, &e->d_lowidx, lowidx.size())) return r;
         HIPCHK(hipMemcpy(e->d_lowidx, lowidx.data(), lowidx.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (int r = dalloc(e, &e->d_lowmode, low.size())) return r;
+        HIPCHK(hipMemcpy(e->d_lowmode, low.data(), low.size() * sizeof(int), hipMemcpyHostToDevice));
         const size_t lz = (size_t)e->nz * e->nlow;
-        std::vector<double> lf(ncoef * lz);
-        auto col = [&](const std::vector<double>& a, int k) {
-            for (int iz = 0; iz < e->nz; iz++)
-                for (int li = 0; li < e->nlow; li++) lf[k * lz + (size_t)iz * e->nlow + li] = a[(size_t)iz * e->nm + low[li]];
-        };
-        int k = 0;
-        for (int c = 0; c < zf.W; c++) col(zf.M[c], k++);
-        col(zf.D, k++);
-        for (int c = 0; c < zf.W; c++) col(zf.U[c], k++);
+        std::vector<double> lf((FW + UW) * lz);
+        const int nlow = e->nlow, nm = e->nm;
+        pack(lf.data(), lf.data() + FW * lz, lz,
+             [&](size_t i) { return (i / nlow) * nm + low[i % nlow]; });
         if (int r = dalloc(e, &e->zfac_low, lf.size())) return r;
         HIPCHK(hipMemcpy(e->zfac_low, lf.data(), lf.size() * sizeof(double), hipMemcpyHostToDevice));
         if (int r = dalloc(e, &e->ylow, 2 * lz)) return r;
@@ -326,19 +327,8 @@ ZFacDev<T> zfac_dev(tc_engine* e) {
     f.nz = e->nz;
     f.ncols = e->nm;
     const size_t fz = (size_t)e->nz * e->nm;
-    const T* b = (const T*)e->zfac;
-    int k = 0;
-    f.M1 = b + (k++) * fz;
-    if (f.W == 3) {
-        f.M2 = b + (k++) * fz;
-        f.M3 = b + (k++) * fz;
-    }
-    f.D = b + (k++) * fz;
-    f.U1 = b + (k++) * fz;
-    if (f.W == 3) {
-        f.U2 = b + (k++) * fz;
-        f.U3 = b + (k++) * fz;
-    }
+    f.F = (const T*)e->zfac;
+    f.U = f.F + zfac_fw(f.W) * fz;
     return f;
 }
 
@@ -348,19 +338,8 @@ ZFacDev<double> zfac_low_dev(tc_engine* e) {
     f.nz = e->nz;
     f.ncols = e->nlow;
     const size_t lz = (size_t)e->nz * e->nlow;
-    const double* b = e->zfac_low;
-    int k = 0;
-    f.M1 = b + (k++) * lz;
-    if (f.W == 3) {
-        f.M2 = b + (k++) * lz;
-        f.M3 = b + (k++) * lz;
-    }
-    f.D = b + (k++) * lz;
-    f.U1 = b + (k++) * lz;
-    if (f.W == 3) {
-        f.U2 = b + (k++) * lz;
-        f.U3 = b + (k++) * lz;
-    }
+    f.F = e->zfac_low;
+    f.U = f.F + zfac_fw(f.W) * lz;
     return f;
 }
 
@@ -383,9 +362,10 @@ int unapply(tc_engine* e, const T* v, T* x) {
     norm.C = e->d_C;
     norm.s00 = e->d_s00;
     if (e->fp32) {
-        HIPCHK(launch_zsolve<T>(e->s, spec, zfac_dev<T>(e), zfac_low_dev(e), e->d_lowidx, e->nlow, e->ylow, norm));
+        HIPCHK(launch_zsolve<T>(e->s, spec, zfac_dev<T>(e), zfac_low_dev(e), e->d_lowidx, e->d_lowmode, e->nlow, e->ylow,
+                                norm));
     } else {
-        HIPCHK(launch_zsolve<T>(e->s, spec, zfac_dev<T>(e), ZFacDev<double>{}, nullptr, 0, nullptr, norm));
+        HIPCHK(launch_zsolve<T>(e->s, spec, zfac_dev<T>(e), ZFacDev<double>{}, nullptr, nullptr, 0, nullptr, norm));
     }
     e->tend(TC_K_ZSOLVE, ev);
     e->tbeg(TC_K_IFFT_Y, ev);

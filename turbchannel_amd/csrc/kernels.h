@@ -78,23 +78,30 @@ template <typename T>
 hipError_t launch_spec_crop(hipStream_t s, const Cplx<T>* fine, int nxh, int ny, Cplx<T>* coarse, int nxhc, int nyc,
                             int nz);
 
-// Factor arrays on the device, [coef][iz][mode] (or [iz][lowmode] for the list).
+// Factor arrays on the device, packed per row so that a row costs one or two
+// load instructions: F[iz][col][FW] = (M1, M2, M3, D) for W = 3, (M1, D) for
+// W = 1; U[iz][col][UW] = (U1, U2, U3) or (U1). col = mode (or index in the
+// low-mode list).
 template <typename CT>
 struct ZFacDev {
     int W = 1, nz = 0, ncols = 0;
-    const CT *M1 = nullptr, *M2 = nullptr, *M3 = nullptr, *D = nullptr, *U1 = nullptr, *U2 = nullptr, *U3 = nullptr;
+    const CT* F = nullptr;
+    const CT* U = nullptr;
 };
+constexpr int zfac_fw(int W) { return W == 3 ? 4 : 2; }
+constexpr int zfac_uw(int W) { return W == 3 ? 3 : 1; }
 struct ZNorm {
     const double *Wt = nullptr, *C = nullptr;
     double* s00 = nullptr;  // receives the (0,0)-mode normalisation sum (one double)
 };
 // Solve every mode. FP32-bulk engines pass lowidx (mode -> index in the FP64
-// factor list, -1 for the FP32 modes), the FP64 factors of those modes and
-// ylow (nz * nlow * 2 doubles of scratch); FP64 engines pass lowidx = nullptr.
-// The lane of the (0,0) mode writes its normalisation sum to norm.s00[0].
+// factor list, -1 for the FP32 modes), lowmode (index -> mode), the FP64 factors
+// of those modes and ylow (nz * nlow * 2 doubles of scratch); FP64 engines pass
+// lowidx = lowmode = nullptr. The lane of the (0,0) mode writes its
+// normalisation sum to norm.s00[0].
 template <typename T>
 hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, const ZFacDev<double>& flow,
-                         const int* lowidx, int nlow, double* ylow, ZNorm norm);
+                         const int* lowidx, const int* lowmode, int nlow, double* ylow, ZNorm norm);
 
 // ---------------------------------------------------------------------------
 // Reductions (FP64 accumulation, deterministic two-stage)

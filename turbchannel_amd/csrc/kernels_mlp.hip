@@ -13,6 +13,7 @@
 // GELU derivatives and back-propagated gradients are bfloat16 (nearest even);
 // weights are bfloat16 copies of FP32 master weights; products accumulate in
 // FP32; biases, outputs and the loss are FP32.
+#include <algorithm>
 #include <cstring>
 
 #include "kernels.h"
@@ -44,10 +45,15 @@ __device__ __forceinline__ f32x4 mfma(u16x8 a, u16x8 b, f32x4 c) {
 
 constexpr float kGc = 0.7978845608028654f;  // sqrt(2/pi)
 constexpr float kGa = 0.044715f;
+constexpr float kLog2e = 1.4426950408889634f;
+// GELU, tanh form: 0.5 z (1 + tanh u) = z s with s = 1 / (1 + exp(-2u)),
+// u = sqrt(2/pi) (z + 0.044715 z^3), evaluated with the hardware exp2 and
+// reciprocal (no libm call); derivative s + 2 z s (1 - s) u'(z).
 __device__ __forceinline__ float gelu(float z, float& der) {
-    const float t = tanhf(kGc * (z + kGa * z * z * z));
-    der = 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * kGc * (1.f + 3.f * kGa * z * z);
-    return 0.5f * z * (1.f + t);
+    const float u = kGc * (z + kGa * z * z * z);
+    const float s = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-2.f * kLog2e * u));
+    der = s + 2.f * z * s * (1.f - s) * kGc * (1.f + 3.f * kGa * z * z);
+    return z * s;
 }
 
 __device__ __forceinline__ int lane() { return threadIdx.x & 63; }
@@ -132,57 +138,98 @@ __device__ __forceinline__ void stage_inputs(u16* xs, const float (&x)[kMlpIn]) 
     }
 }
 
-// Forward pass of one wave's 64 points held in tile `buf` (inputs staged);
-// hidden layers overwrite the tile. Returns the output accumulators:
-// y[nt][i] = output 4 g + i of point 16 nt + r, bias included.
-__device__ __forceinline__ void forward_wave(const MlpDev& w, u16* buf, f32x4 (&y)[4]) {
+// Inference keeps the forward weights (A operands, 88 VGPRs) in registers for
+// the life of a persistent wave, which walks over 64-point chunks: per chunk only
+// the inputs and activations move.
+struct FwdRegs {
+    u16x8 a1[4][1], a2[4][2], a3[4][2], a4[1][2];
+};
+
+__device__ __forceinline__ void load_fwd_regs(const MlpDev& w, FwdRegs& R) {
+    const int r = lane() & 15, g = lane() >> 4;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        R.a1[mt][0] = *(const u16x8*)(w.w + kW1p + (16 * mt + r) * 32 + 8 * g);
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            R.a2[mt][ks] = *(const u16x8*)(w.w + kW2p + (16 * mt + r) * 64 + 32 * ks + 8 * g);
+            R.a3[mt][ks] = *(const u16x8*)(w.w + kW3p + (16 * mt + r) * 64 + 32 * ks + 8 * g);
+        }
+    }
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) R.a4[0][ks] = *(const u16x8*)(w.w + kW4p + r * 64 + 32 * ks + 8 * g);
+}
+
+template <int MT, int KS>
+__device__ __forceinline__ void mfma_layer_r(const u16x8 (&a)[MT][KS], const u16* act, f32x4 (&acc)[MT][4]) {
+    const int r = lane() & 15, g = lane() >> 4;
+    u16x8 b[4][KS];
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) b[nt][ks] = *(const u16x8*)(act + (16 * nt + r) * kLda + 32 * ks + 8 * g);
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) c = mfma(a[mt][ks], b[nt][ks], c);
+            acc[mt][nt] = c;
+        }
+}
+
+// Forward pass of one wave (a block of one wave) on tile `buf`; see forward_wave.
+__device__ __forceinline__ void forward_wave_r(const FwdRegs& R, const float* __restrict__ bias, u16* buf,
+                                               f32x4 (&y)[4]) {
     f32x4 acc[4][4];
-    mfma_layer<4, 1>(w.w + kW1p, buf, acc);
+    mfma_layer_r<4, 1>(R.a1, buf, acc);
     __syncthreads();
-    hidden_out<false>(acc, w.b, buf, nullptr);
+    hidden_out<false>(acc, bias, buf, nullptr);
     __syncthreads();
-    mfma_layer<4, 2>(w.w + kW2p, buf, acc);
+    mfma_layer_r<4, 2>(R.a2, buf, acc);
     __syncthreads();
-    hidden_out<false>(acc, w.b + 64, buf, nullptr);
+    hidden_out<false>(acc, bias + 64, buf, nullptr);
     __syncthreads();
-    mfma_layer<4, 2>(w.w + kW3p, buf, acc);
+    mfma_layer_r<4, 2>(R.a3, buf, acc);
     __syncthreads();
-    hidden_out<false>(acc, w.b + 128, buf, nullptr);
+    hidden_out<false>(acc, bias + 128, buf, nullptr);
     __syncthreads();
     f32x4 o[1][4];
-    mfma_layer<1, 2>(w.w + kW4p, buf, o);
+    mfma_layer_r<1, 2>(R.a4, buf, o);
     const int g = lane() >> 4;
 #pragma unroll
     for (int nt = 0; nt < 4; nt++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) y[nt][i] = o[0][nt][i] + w.b[192 + 4 * g + i];
+        for (int i = 0; i < 4; i++) y[nt][i] = o[0][nt][i] + bias[192 + 4 * g + i];
 }
 
-constexpr int kFwdWaves = 4;
-
-__global__ __launch_bounds__(64 * kFwdWaves) void k_mlp_forward(MlpDev w, const float* __restrict__ x, long long n,
-                                                                 float* __restrict__ y) {
-    __shared__ __align__(16) u16 sm[kFwdWaves * kTile];
-    const int wv = threadIdx.x >> 6;
-    u16* buf = sm + wv * kTile;
-    const long long p0 = ((long long)blockIdx.x * kFwdWaves + wv) * 64;
-    const long long pi = p0 + lane();
-    float xv[kMlpIn];
-#pragma unroll
-    for (int k = 0; k < kMlpIn; k++) xv[k] = pi < n ? x[pi * kMlpIn + k] : 0.f;
-    stage_inputs(buf, xv);
-    __syncthreads();
-    f32x4 o[4];
-    forward_wave(w, buf, o);
+__global__ __launch_bounds__(64) void k_mlp_forward(MlpDev w, const float* __restrict__ x, long long n,
+                                                    float* __restrict__ y) {
+    __shared__ __align__(16) u16 buf[kTile];
+    FwdRegs R;
+    load_fwd_regs(w, R);
+    const long long nch = (n + 63) / 64;
     const int r = lane() & 15, g = lane() >> 4;
+    for (long long c = blockIdx.x; c < nch; c += gridDim.x) {
+        const long long p0 = c * 64, pi = p0 + lane();
+        float xv[kMlpIn];
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
-        const long long p = p0 + 16 * nt + r;
+        for (int k = 0; k < kMlpIn; k++) xv[k] = pi < n ? x[pi * kMlpIn + k] : 0.f;
+        stage_inputs(buf, xv);
+        __syncthreads();
+        f32x4 o[4];
+        forward_wave_r(R, w.b, buf, o);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int f = 4 * g + i;
-            if (f < kMlpOut && p < n) y[p * kMlpOut + f] = o[nt][i];
+        for (int nt = 0; nt < 4; nt++) {
+            const long long p = p0 + 16 * nt + r;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int f = 4 * g + i;
+                if (f < kMlpOut && p < n) y[p * kMlpOut + f] = o[nt][i];
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -418,59 +465,63 @@ __global__ void k_mlp_reduce_adam(const double* __restrict__ tmp, float* __restr
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 template <typename T>
-__global__ __launch_bounds__(64 * kFwdWaves) void k_sgs_tau(Mesh<T> m, MlpDev w, const T* __restrict__ vel,
-                                                            T* __restrict__ tau, T u_tau2, T gscale, T yscale) {
-    __shared__ __align__(16) u16 sm[kFwdWaves * kTile];
-    const int wv = threadIdx.x >> 6;
-    u16* buf = sm + wv * kTile;
-    const long long p0 = ((long long)blockIdx.x * kFwdWaves + wv) * 64;
-    const long long node = p0 + lane();
+__global__ __launch_bounds__(64) void k_sgs_tau(Mesh<T> m, MlpDev w, const T* __restrict__ vel, T* __restrict__ tau,
+                                                T u_tau2, T gscale, T yscale) {
+    __shared__ __align__(16) u16 buf[kTile];
+    FwdRegs R;
+    load_fwd_regs(w, R);
     const int nxy = m.nxy;
-    const int iz = (int)(node / nxy);
-    const bool interior = node < m.n3d && iz > 0 && iz < m.nz - 1;
-    float xv[kMlpIn];
-#pragma unroll
-    for (int k = 0; k < kMlpIn; k++) xv[k] = 0.f;
-    if (interior) {
-        const int rem = (int)(node - (long long)iz * nxy), iy = rem / m.nx, ix = rem - iy * m.nx;
-        const size_t b = (size_t)iz * nxy;
-        const size_t xr = b + iy * m.nx + wrapi(ix + 1, m.nx), xl = b + iy * m.nx + wrapi(ix - 1, m.nx);
-        const size_t yr = b + wrapi(iy + 1, m.ny) * m.nx + ix, yl = b + wrapi(iy - 1, m.ny) * m.nx + ix;
-        const size_t c0 = (size_t)node, zr = c0 + nxy, zl = c0 - nxy;
-        const T hm = m.DZ[iz - 1], hp = m.DZ[iz];
-        const T den = hp * hm * (hp + hm);
-        const T cp = hm * hm / den, cm = hp * hp / den;
-        const T hx2 = T(0.5) * m.inv_hx, hy2 = T(0.5) * m.inv_hy;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const T u0 = vel[3 * c0 + c];
-            const T gx = hx2 * (vel[3 * xr + c] - vel[3 * xl + c]);
-            const T gy = hy2 * (vel[3 * yr + c] - vel[3 * yl + c]);
-            const T gz = cp * (vel[3 * zr + c] - u0) + cm * (u0 - vel[3 * zl + c]);
-            xv[3 * c + 0] = (float)(gx * gscale);
-            xv[3 * c + 1] = (float)(gy * gscale);
-            xv[3 * c + 2] = (float)(gz * gscale);
-        }
-        const T z = m.ZZ[iz];
-        const T d = (z - m.zlo) < (m.zhi - z) ? (z - m.zlo) : (m.zhi - z);
-        xv[9] = (float)(d * yscale);
-    }
-    stage_inputs(buf, xv);
-    __syncthreads();
-    f32x4 o[4];
-    forward_wave(w, buf, o);
+    const long long nch = ((long long)m.n3d + 63) / 64;
     const int r = lane() & 15, g = lane() >> 4;
+    const T hx2 = T(0.5) * m.inv_hx, hy2 = T(0.5) * m.inv_hy;
+    for (long long c = blockIdx.x; c < nch; c += gridDim.x) {
+        const long long p0 = c * 64;
+        const long long node = p0 + lane();
+        const int iz = (int)(node / nxy);
+        const bool interior = node < m.n3d && iz > 0 && iz < m.nz - 1;
+        float xv[kMlpIn];
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
-        const long long p = p0 + 16 * nt + r;
-        if (p >= m.n3d) continue;
-        const int pz = (int)(p / nxy);
-        const bool in = pz > 0 && pz < m.nz - 1;
+        for (int k = 0; k < kMlpIn; k++) xv[k] = 0.f;
+        if (interior) {
+            const int rem = (int)(node - (long long)iz * nxy), iy = rem / m.nx, ix = rem - iy * m.nx;
+            const size_t b = (size_t)iz * nxy;
+            const size_t xr = b + iy * m.nx + wrapi(ix + 1, m.nx), xl = b + iy * m.nx + wrapi(ix - 1, m.nx);
+            const size_t yr = b + wrapi(iy + 1, m.ny) * m.nx + ix, yl = b + wrapi(iy - 1, m.ny) * m.nx + ix;
+            const size_t c0 = (size_t)node, zr = c0 + nxy, zl = c0 - nxy;
+            const T hm = m.DZ[iz - 1], hp = m.DZ[iz];
+            const T den = hp * hm * (hp + hm);
+            const T cp = hm * hm / den, cm = hp * hp / den;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int f = 4 * g + i;
-            if (f < kMlpOut) tau[p * kMlpOut + f] = in ? T(o[nt][i]) * u_tau2 : T(0);
+            for (int cc = 0; cc < 3; cc++) {
+                const T u0 = vel[3 * c0 + cc];
+                const T gx = hx2 * (vel[3 * xr + cc] - vel[3 * xl + cc]);
+                const T gy = hy2 * (vel[3 * yr + cc] - vel[3 * yl + cc]);
+                const T gz = cp * (vel[3 * zr + cc] - u0) + cm * (u0 - vel[3 * zl + cc]);
+                xv[3 * cc + 0] = (float)(gx * gscale);
+                xv[3 * cc + 1] = (float)(gy * gscale);
+                xv[3 * cc + 2] = (float)(gz * gscale);
+            }
+            const T z = m.ZZ[iz];
+            const T d = (z - m.zlo) < (m.zhi - z) ? (z - m.zlo) : (m.zhi - z);
+            xv[9] = (float)(d * yscale);
         }
+        stage_inputs(buf, xv);
+        __syncthreads();
+        f32x4 o[4];
+        forward_wave_r(R, w.b, buf, o);
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            const long long p = p0 + 16 * nt + r;
+            if (p >= m.n3d) continue;
+            const int pz = (int)(p / nxy);
+            const bool in = pz > 0 && pz < m.nz - 1;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int f = 4 * g + i;
+                if (f < kMlpOut) tau[p * kMlpOut + f] = in ? T(o[nt][i]) * u_tau2 : T(0);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -547,16 +598,30 @@ void mlp_pack_device(const float* P, std::vector<unsigned short>& w, std::vector
     for (int o = 0; o < kMlpOut; o++) b[192 + o] = P[kOffB4 + o];
 }
 
+namespace {
+// Persistent grid: as many one-wave blocks as are resident on the device at once.
+template <typename K>
+long long resident_blocks(K kernel) {
+    int per_cu = 0, dev = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu < 1)
+        ncu = 256;
+    return (long long)per_cu * ncu;
+}
+}  // namespace
+
 hipError_t launch_mlp_forward(hipStream_t s, MlpDev w, const float* x, long long n, float* y) {
     if (n <= 0) return hipSuccess;
-    const long long nb = (n + 64 * kFwdWaves - 1) / (64 * kFwdWaves);
-    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)nb), dim3(64 * kFwdWaves), 0, s, w, x, n, y);
+    static const long long resident = resident_blocks(k_mlp_forward);
+    const long long nb = std::min((n + 63) / 64, resident);
+    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)nb), dim3(64), 0, s, w, x, n, y);
     return hipGetLastError();
 }
 
 int mlp_train_blocks(long long n) {
     const long long nch = (n + 63) / 64;
-    return (int)(nch < 1024 ? (nch < 1 ? 1 : nch) : 1024);
+    return (int)(nch < 512 ? (nch < 1 ? 1 : nch) : 512);
 }
 
 hipError_t launch_mlp_partials(hipStream_t s, MlpDev w, const float* x, const float* t, long long n, float* partial) {
@@ -576,8 +641,9 @@ hipError_t launch_mlp_reduce(hipStream_t s, const float* partial, int nrec, doub
 template <typename T>
 hipError_t launch_sgs_tau(hipStream_t s, const Mesh<T>& m, MlpDev w, const T* vel, T* tau) {
     const double u_tau = (double)m.visc * (double)m.inv_len;
-    const long long nb = ((long long)m.n3d + 64 * kFwdWaves - 1) / (64 * kFwdWaves);
-    hipLaunchKernelGGL((k_sgs_tau<T>), dim3((unsigned)nb), dim3(64 * kFwdWaves), 0, s, m, w, vel, tau,
+    static const long long resident = resident_blocks(k_sgs_tau<T>);
+    const long long nb = std::min(((long long)m.n3d + 63) / 64, resident);
+    hipLaunchKernelGGL((k_sgs_tau<T>), dim3((unsigned)nb), dim3(64), 0, s, m, w, vel, tau,
                        T(u_tau * u_tau), T(1.0 / (u_tau * (double)m.inv_len)), T((double)m.inv_len * kYPlusScale));
     return hipGetLastError();
 }

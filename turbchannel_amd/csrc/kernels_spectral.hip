@@ -243,184 +243,168 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
 // collocated_AB3/func_global.h:8-61) with factors precomputed on the host.
 // One thread per (mode, real/imaginary part): the matrix is real.
 // ---------------------------------------------------------------------------
-template <typename T, int W>
-__device__ void zsolve_low_lane(Cplx<T>* spec, const ZFacDev<double>& f, int li, int part, int nm, int mode,
-                                double* ylow, int nlow, const ZNorm& norm);
+// One row of packed factors (see ZFacDev).
+template <typename CT, int W>
+__device__ __forceinline__ void load_fwd_row(const CT* __restrict__ F, CT& m1, CT& m2, CT& m3, CT& d) {
+    if (W == 3) {
+        struct alignas(4 * sizeof(CT)) V4 { CT a, b, c, d; };
+        const V4 q = *reinterpret_cast<const V4*>(F);
+        m1 = q.a;
+        m2 = q.b;
+        m3 = q.c;
+        d = q.d;
+    } else {
+        struct alignas(2 * sizeof(CT)) V2 { CT a, b; };
+        const V2 q = *reinterpret_cast<const V2*>(F);
+        m1 = q.a;
+        d = q.b;
+    }
+}
+template <typename CT, int W>
+__device__ __forceinline__ void load_bwd_row(const CT* __restrict__ U, CT& u1, CT& u2, CT& u3) {
+    if (W == 3) {
+        struct V3 { CT a, b, c; };
+        const V3 q = *reinterpret_cast<const V3*>(U);
+        u1 = q.a;
+        u2 = q.b;
+        u3 = q.c;
+    } else {
+        u1 = *U;
+    }
+}
 
-template <typename T, int W>
-__global__ __launch_bounds__(256) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFacDev<double> flow, const int* lowidx,
-                                                 int nlow, double* ylow, ZNorm norm) {
-    constexpr int B = 8;  // rows whose loads are issued together, ahead of the recurrence
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nm = f.ncols, nz = f.nz;
-    const int mode = t >> 1, part = t & 1;
-    if (mode >= nm) return;
-    if (lowidx) {  // FP32-bulk: the lowest wavenumbers take the FP64 path
-        const int li = lowidx[mode];
-        if (li >= 0) {
-            zsolve_low_lane<T, W>(spec, flow, li, part, nm, mode, ylow, nlow, norm);
-            return;
-        }
-    }
-    T* __restrict__ x = reinterpret_cast<T*>(spec) + 2 * mode + part;
-    const size_t xs = 2 * (size_t)nm;  // stride between planes, in reals
-    const T* __restrict__ M1 = f.M1 + mode;
-    const T* __restrict__ M2 = f.M2 + mode;
-    const T* __restrict__ M3 = f.M3 + mode;
-    const T* __restrict__ D = f.D + mode;
-    const T* __restrict__ U1 = f.U1 + mode;
-    const T* __restrict__ U2 = f.U2 + mode;
-    const T* __restrict__ U3 = f.U3 + mode;
-    T y1 = 0, y2 = 0, y3 = 0;
-    // forward substitution
-    for (int i0 = 0; i0 < nz; i0 += B) {
-        T b[B], m1[B], m2[B], m3[B], d[B];
+// One lane's solve: forward sweep x -> y, backward sweep y -> x, in CT
+// arithmetic. Rows move in full batches of B with no per-row conditions (a
+// condition per row splits the code into blocks at whose joins the compiler
+// waits for every load in flight, one memory round trip per row). The loads of
+// a batch are issued NB-1 batches ahead of the recurrence and before the stores
+// of the batch being finished, since the vector-memory counter counts loads and
+// stores together. The nz % B remaining rows follow one at a time.
+template <typename T, typename CT, int W, int B, int NB>
+__device__ __forceinline__ void lane_solve(T* __restrict__ x, size_t xs, const CT* __restrict__ F, size_t fs,
+                                           const CT* __restrict__ U, size_t us, CT* __restrict__ y, size_t ys, int nz,
+                                           bool n00, const ZNorm& norm) {
+    const int nfull = nz / B;
+    CT y1 = 0, y2 = 0, y3 = 0;
+    auto fwd_row = [&](CT b, CT m1, CT m2, CT m3, CT d, int i) {
+        CT v = b;
+        if (W == 3) v = v + m3 * y3;
+        if (W == 3) v = v + m2 * y2;
+        v = v + m1 * y1;
+        v = v * d;
+        y[i * ys] = v;
+        y3 = y2;
+        y2 = y1;
+        y1 = v;
+    };
+    // forward substitution: y[i] = (((x[i] + M3 y[i-3]) + M2 y[i-2]) + M1 y[i-1]) * D
+    if (nfull > 0) {
+        CT b[NB][B], m1[NB][B], m2[NB][B], m3[NB][B], d[NB][B];
+        auto load = [&](int kb, CT(&bb)[B], CT(&q1)[B], CT(&q2)[B], CT(&q3)[B], CT(&qd)[B]) {
+            const int i0 = (kb < nfull ? kb : nfull - 1) * B;
 #pragma unroll
-        for (int k = 0; k < B; k++) {
-            const int i = i0 + k;
-            if (i < nz) {
-                const size_t fi = (size_t)i * nm;
-                b[k] = x[i * xs];
-                m1[k] = M1[fi];
-                d[k] = D[fi];
-                if (W == 3) {
-                    m2[k] = M2[fi];
-                    m3[k] = M3[fi];
-                }
+            for (int k = 0; k < B; k++) {
+                bb[k] = (CT)x[(i0 + k) * xs];
+                load_fwd_row<CT, W>(F + (i0 + k) * fs, q1[k], q2[k], q3[k], qd[k]);
             }
-        }
+        };
 #pragma unroll
-        for (int k = 0; k < B; k++) {
-            const int i = i0 + k;
-            if (i < nz) {
-                T v = b[k];
-                if (W == 3) v = v + m3[k] * y3;
-                if (W == 3) v = v + m2[k] * y2;
-                v = v + m1[k] * y1;
-                v = v * d[k];
-                x[i * xs] = v;
-                y3 = y2;
-                y2 = y1;
-                y1 = v;
+        for (int sb = 0; sb < NB - 1; sb++) load(sb, b[sb], m1[sb], m2[sb], m3[sb], d[sb]);
+        for (int kb0 = 0; kb0 < nfull; kb0 += NB) {
+#pragma unroll
+            for (int sb = 0; sb < NB; sb++) {
+                const int kb = kb0 + sb;
+                if (kb >= nfull) break;
+                const int nx = (sb + NB - 1) % NB;
+                load(kb + NB - 1, b[nx], m1[nx], m2[nx], m3[nx], d[nx]);
+#pragma unroll
+                for (int k = 0; k < B; k++) fwd_row(b[sb][k], m1[sb][k], m2[sb][k], m3[sb][k], d[sb][k], kb * B + k);
             }
         }
     }
-    // backward substitution
-    const bool n00 = (mode == 0 && part == 0 && norm.s00 != nullptr);
+    for (int i = nfull * B; i < nz; i++) {
+        CT m1, m2 = 0, m3 = 0, d;
+        load_fwd_row<CT, W>(F + i * fs, m1, m2, m3, d);
+        fwd_row((CT)x[i * xs], m1, m2, m3, d, i);
+    }
+    // backward substitution: x[i] = y[i] - U1 x[i+1] - U2 x[i+2] - U3 x[i+3]
     double psum = 0.0;
-    T z1 = 0, z2 = 0, z3 = 0;
-    for (int i0 = nz - 1; i0 >= 0; i0 -= B) {
-        T b[B], u1[B], u2[B], u3[B];
+    CT z1 = 0, z2 = 0, z3 = 0;
+    auto bwd_row = [&](CT b, CT u1, CT u2, CT u3, double wt, int i) {
+        CT v = b;
+        v = v - u1 * z1;
+        if (W == 3) v = v - u2 * z2;
+        if (W == 3) v = v - u3 * z3;
+        x[i * xs] = T(v);
+        psum += wt * (double)v;
+        z3 = z2;
+        z2 = z1;
+        z1 = v;
+    };
+    if (nfull > 0) {
+        CT b[NB][B], u1[NB][B], u2[NB][B], u3[NB][B];
+        double wt[NB][B];  // normalisation weights (used by the (0,0) lane; a broadcast load)
+        auto load = [&](int kb, CT(&bb)[B], CT(&q1)[B], CT(&q2)[B], CT(&q3)[B], double(&qw)[B]) {
+            const int i0 = nz - 1 - (kb < nfull ? kb : nfull - 1) * B;
 #pragma unroll
-        for (int k = 0; k < B; k++) {
-            const int i = i0 - k;
-            if (i >= 0) {
-                const size_t fi = (size_t)i * nm;
-                b[k] = x[i * xs];
-                u1[k] = U1[fi];
-                if (W == 3) {
-                    u2[k] = U2[fi];
-                    u3[k] = U3[fi];
-                }
+            for (int k = 0; k < B; k++) {
+                bb[k] = y[(i0 - k) * ys];
+                load_bwd_row<CT, W>(U + (i0 - k) * us, q1[k], q2[k], q3[k]);
+                qw[k] = norm.Wt[i0 - k];
+            }
+        };
+#pragma unroll
+        for (int sb = 0; sb < NB - 1; sb++) load(sb, b[sb], u1[sb], u2[sb], u3[sb], wt[sb]);
+        for (int kb0 = 0; kb0 < nfull; kb0 += NB) {
+#pragma unroll
+            for (int sb = 0; sb < NB; sb++) {
+                const int kb = kb0 + sb;
+                if (kb >= nfull) break;
+                const int nx = (sb + NB - 1) % NB;
+                load(kb + NB - 1, b[nx], u1[nx], u2[nx], u3[nx], wt[nx]);
+#pragma unroll
+                for (int k = 0; k < B; k++)
+                    bwd_row(b[sb][k], u1[sb][k], u2[sb][k], u3[sb][k], wt[sb][k], nz - 1 - (kb * B + k));
             }
         }
-#pragma unroll
-        for (int k = 0; k < B; k++) {
-            const int i = i0 - k;
-            if (i >= 0) {
-                T v = b[k];
-                v = v - u1[k] * z1;
-                if (W == 3) v = v - u2[k] * z2;
-                if (W == 3) v = v - u3[k] * z3;
-                x[i * xs] = v;
-                if (n00) psum += norm.Wt[i] * (double)v;
-                z3 = z2;
-                z2 = z1;
-                z1 = v;
-            }
-        }
+    }
+    for (int i = nz - 1 - nfull * B; i >= 0; i--) {
+        CT u1, u2 = 0, u3 = 0;
+        load_bwd_row<CT, W>(U + i * us, u1, u2, u3);
+        bwd_row(y[i * ys], u1, u2, u3, norm.Wt[i], i);
     }
     if (n00) norm.s00[0] = psum;  // applied by the inverse y transform (NormalizePressure)
 }
 
-// FP64 solve of one listed mode (the FP32-bulk engine's low wavenumbers). The
-// intermediate of the forward pass stays in double (ylow: nz * nlow * 2).
+// Few independent systems (2 per mode) for the latency of every row: one wave
+// per block so the waves spread over all CUs, and pipelined row batches. The
+// FP32-bulk engine's FP64 low modes run in blocks of their own, after the bulk
+// blocks, so that no FP32 wave carries a divergent FP64 lane; their forward
+// intermediate stays in double (ylow: nz * nlow * 2).
 template <typename T, int W>
-__device__ void zsolve_low_lane(Cplx<T>* spec, const ZFacDev<double>& f, int li, int part, int nm, int mode,
-                                double* ylow, int nlow, const ZNorm& norm) {
-    constexpr int B = 8;
-    const int nz = f.nz;
-    T* __restrict__ x = reinterpret_cast<T*>(spec) + 2 * mode + part;
-    const size_t xs = 2 * (size_t)nm;
-    double* __restrict__ y = ylow + 2 * li + part;
-    const size_t ys = 2 * (size_t)nlow;
-    double y1 = 0, y2 = 0, y3 = 0;
-    for (int i0 = 0; i0 < nz; i0 += B) {
-        double b[B], m1[B], m2[B], m3[B], d[B];
-#pragma unroll
-        for (int k = 0; k < B; k++) {
-            const int i = i0 + k;
-            if (i < nz) {
-                const size_t fi = (size_t)i * nlow + li;
-                b[k] = (double)x[i * xs];
-                m1[k] = f.M1[fi];
-                d[k] = f.D[fi];
-                if (W == 3) {
-                    m2[k] = f.M2[fi];
-                    m3[k] = f.M3[fi];
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < B; k++) {
-            const int i = i0 + k;
-            if (i < nz) {
-                double v = b[k];
-                if (W == 3) v = v + m3[k] * y3;
-                if (W == 3) v = v + m2[k] * y2;
-                v = v + m1[k] * y1;
-                v = v * d[k];
-                y[i * ys] = v;
-                y3 = y2;
-                y2 = y1;
-                y1 = v;
-            }
-        }
+__global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFacDev<double> flow, const int* lowidx,
+                                                const int* lowmode, int nlow, double* ylow, ZNorm norm, int nbulk) {
+    constexpr int FW = zfac_fw(W), UW = zfac_uw(W);
+    const int nm = f.ncols, nz = f.nz;
+    const size_t xs = 2 * (size_t)nm;  // stride between planes, in reals
+    if ((int)blockIdx.x >= nbulk) {    // FP64 low-mode blocks
+        const int t = (blockIdx.x - nbulk) * blockDim.x + threadIdx.x;
+        const int li = t >> 1, part = t & 1;
+        if (li >= nlow) return;
+        const int mode = lowmode[li];
+        lane_solve<T, double, W, 8, 2>(reinterpret_cast<T*>(spec) + 2 * mode + part, xs, flow.F + (size_t)li * FW,
+                                        (size_t)nlow * FW, flow.U + (size_t)li * UW, (size_t)nlow * UW,
+                                        ylow + 2 * li + part, 2 * (size_t)nlow, nz, mode == 0 && part == 0, norm);
+        return;
     }
-    const bool n00 = (mode == 0 && part == 0);
-    double psum = 0.0, z1 = 0, z2 = 0, z3 = 0;
-    for (int i0 = nz - 1; i0 >= 0; i0 -= B) {
-        double b[B], u1[B], u2[B], u3[B];
-#pragma unroll
-        for (int k = 0; k < B; k++) {
-            const int i = i0 - k;
-            if (i >= 0) {
-                const size_t fi = (size_t)i * nlow + li;
-                b[k] = y[i * ys];
-                u1[k] = f.U1[fi];
-                if (W == 3) {
-                    u2[k] = f.U2[fi];
-                    u3[k] = f.U3[fi];
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < B; k++) {
-            const int i = i0 - k;
-            if (i >= 0) {
-                double v = b[k];
-                v = v - u1[k] * z1;
-                if (W == 3) v = v - u2[k] * z2;
-                if (W == 3) v = v - u3[k] * z3;
-                x[i * xs] = T(v);
-                if (n00) psum += norm.Wt[i] * v;
-                z3 = z2;
-                z2 = z1;
-                z1 = v;
-            }
-        }
-    }
-    if (n00) norm.s00[0] = psum;  // applied by the inverse y transform (NormalizePressure)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mode = t >> 1, part = t & 1;
+    if (mode >= nm) return;
+    if (lowidx && lowidx[mode] >= 0) return;  // solved in FP64 by the low-mode blocks
+    T* x = reinterpret_cast<T*>(spec) + 2 * mode + part;
+    constexpr int NB = sizeof(T) == 8 ? 2 : 3;
+    lane_solve<T, T, W, 8, NB>(x, xs, f.F + (size_t)mode * FW, (size_t)nm * FW, f.U + (size_t)mode * UW,
+                               (size_t)nm * UW, x, xs, nz, mode == 0 && part == 0 && norm.s00 != nullptr, norm);
 }
 
 // Forward x transform of one component (cb < 0) or of the product of two
@@ -529,11 +513,14 @@ hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cpl
 
 template <typename T>
 hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, const ZFacDev<double>& flow,
-                         const int* lowidx, int nlow, double* ylow, ZNorm norm) {
-    const int nthreads = 2 * f.ncols;
-    const int grid = (nthreads + 255) / 256;
-    if (f.W == 3) hipLaunchKernelGGL((k_zsolve<T, 3>), dim3(grid), dim3(256), 0, s, spec, f, flow, lowidx, nlow, ylow, norm);
-    else hipLaunchKernelGGL((k_zsolve<T, 1>), dim3(grid), dim3(256), 0, s, spec, f, flow, lowidx, nlow, ylow, norm);
+                         const int* lowidx, const int* lowmode, int nlow, double* ylow, ZNorm norm) {
+    const int nbulk = (2 * f.ncols + 63) / 64;
+    const int nlowb = lowidx ? (2 * nlow + 63) / 64 : 0;
+    const dim3 grid(nbulk + nlowb);
+    if (f.W == 3)
+        hipLaunchKernelGGL((k_zsolve<T, 3>), grid, dim3(64), 0, s, spec, f, flow, lowidx, lowmode, nlow, ylow, norm, nbulk);
+    else
+        hipLaunchKernelGGL((k_zsolve<T, 1>), grid, dim3(64), 0, s, spec, f, flow, lowidx, lowmode, nlow, ylow, norm, nbulk);
     return hipGetLastError();
 }
 
@@ -565,7 +552,7 @@ hipError_t launch_spec_crop(hipStream_t s, const Cplx<T>* fine, int nxh, int ny,
     template hipError_t launch_fft_y<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, Cplx<T>*, int, int, int,       \
                                         const double*, const double*);                                              \
     template hipError_t launch_zsolve<T>(hipStream_t, Cplx<T>*, const ZFacDev<T>&, const ZFacDev<double>&,         \
-                                         const int*, int, double*, ZNorm);                                   \
+                                         const int*, const int*, int, double*, ZNorm);                                   \
     template hipError_t launch_fft_x_fwd_q<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const T*, int, int, int,  \
                                               Cplx<T>*);                                                             \
     template hipError_t launch_spec_crop<T>(hipStream_t, const Cplx<T>*, int, int, Cplx<T>*, int, int, int);
