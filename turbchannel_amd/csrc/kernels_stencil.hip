@@ -122,15 +122,17 @@ struct GlobalNu {
     __device__ __forceinline__ T operator()(int dx, int dy, int dz) const { return nut[nb->at(dx, dy, dz)]; }
 };
 
+// The stencils reach beyond the ring in z only on the cell's own column
+// (the fourth-order scheme's z+-3 values), held in registers.
 template <typename T, int RXY, int RZ, int TY>
 struct RingAcc {
     static constexpr int W = kTX + 2 * RXY, H = TY + 2 * RXY, PS = W * H;
     const T* pl[2 * RZ + 1];  // LDS planes iz-RZ..iz+RZ of the padded tile, real3 AoS
     int off;                  // position of the cell in the padded tile
-    const T* gcol;            // the cell in global memory (own column beyond the ring)
-    int nxy3;
+    T cp[3], cm[3];           // own column at dz = +(RZ+1) and -(RZ+1)
     __device__ __forceinline__ T operator()(int c, int dx, int dy, int dz) const {
-        if (dz > RZ || dz < -RZ) return gcol[dz * nxy3 + c];
+        if (dz == RZ + 1) return cp[c];
+        if (dz == -RZ - 1) return cm[c];
         return pl[dz + RZ][3 * (off + dy * W + dx) + c];
     }
 };
@@ -138,21 +140,37 @@ struct RingAcc {
 // Learned closure (new numerics, no reference counterpart): k -= d tau_ij / d x_j at
 // an interior collocated node, central differences in x, y and the 3-point
 // formula on the nonuniform z nodes; tau is [node][XX YY ZZ XY XZ YZ].
+// The 21 stress values it needs are loaded ahead (sgs_load) into q[c][7] =
+// (x+, x-, y+, y-, z+, own, z-) of component c's x-, y- and z-stresses.
 template <typename T>
-__device__ __forceinline__ void sgs_divergence(const Mesh<T>& m, const T* __restrict__ t, const Nb& nb, int iz, T k[3]) {
+__device__ __forceinline__ void sgs_load(const T* __restrict__ t, const Nb& nb, int iz, int nz, T q[3][7]) {
+    constexpr int ax[3] = {0, 3, 4}, ay[3] = {3, 1, 5}, az[3] = {4, 5, 2};
+    const int dzp = iz < nz - 1 ? 1 : 0, dzm = iz > 0 ? -1 : 0;  // the wall planes never use these
+    const int c0 = 6 * nb.at(0, 0, 0), xr = 6 * nb.at(1, 0, 0), xl = 6 * nb.at(-1, 0, 0);
+    const int yr = 6 * nb.at(0, 1, 0), yl = 6 * nb.at(0, -1, 0), zr = 6 * nb.at(0, 0, dzp), zl = 6 * nb.at(0, 0, dzm);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        q[c][0] = t[xr + ax[c]];
+        q[c][1] = t[xl + ax[c]];
+        q[c][2] = t[yr + ay[c]];
+        q[c][3] = t[yl + ay[c]];
+        q[c][4] = t[zr + az[c]];
+        q[c][5] = t[c0 + az[c]];
+        q[c][6] = t[zl + az[c]];
+    }
+}
+template <typename T>
+__device__ __forceinline__ void sgs_divergence(const Mesh<T>& m, const T q[3][7], int iz, T k[3]) {
     const T hm = m.DZ[iz - 1], hp = m.DZ[iz];
     const T den = hp * hm * (hp + hm);
     const T cp = hm * hm / den, cm = hp * hp / den;
     const T hx2 = T(0.5) * m.inv_hx, hy2 = T(0.5) * m.inv_hy;
-    const int c0 = 6 * nb.at(0, 0, 0), xr = 6 * nb.at(1, 0, 0), xl = 6 * nb.at(-1, 0, 0);
-    const int yr = 6 * nb.at(0, 1, 0), yl = 6 * nb.at(0, -1, 0), zr = 6 * nb.at(0, 0, 1), zl = 6 * nb.at(0, 0, -1);
-    constexpr int ax[3] = {0, 3, 4}, ay[3] = {3, 1, 5}, az[3] = {4, 5, 2};
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        T d = hx2 * (t[xr + ax[c]] - t[xl + ax[c]]);
-        d += hy2 * (t[yr + ay[c]] - t[yl + ay[c]]);
-        const T t0 = t[c0 + az[c]];
-        d += cp * (t[zr + az[c]] - t0) + cm * (t0 - t[zl + az[c]]);
+        T d = hx2 * (q[c][0] - q[c][1]);
+        d += hy2 * (q[c][2] - q[c][3]);
+        const T t0 = q[c][5];
+        d += cp * (q[c][4] - t0) + cm * (t0 - q[c][6]);
         k[c] -= d;
     }
 }
@@ -170,8 +188,27 @@ __global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
     constexpr int DPC = 3 * (int)sizeof(T) / 4;  // dwords per real3 cell
     constexpr int DPS = PS * DPC;                // dwords per plane of the padded tile
     constexpr int ND = (DPS + NT - 1) / NT;
-    __shared__ __align__(16) T ring[NS * 3 * PS];
+    // slot stride: every lane writes ND dwords per plane fill (the surplus into
+    // padding), so the fill is the same instruction stream for every plane
+    constexpr int SLOT = ND * NT * 4 / (int)sizeof(T);  // reals per ring slot
+    __shared__ __align__(16) T ring[NS * SLOT];
     __shared__ double sh[NT / 64];
+    // The mesh arrays the stencils read, mirrored in LDS (dynamic, 4 nz reals):
+    // LDS reads wait on their own counter, never on the global loads in flight.
+    extern __shared__ __align__(16) unsigned char mesh_raw[];
+    {
+        T* ml = reinterpret_cast<T*>(mesh_raw);
+        for (int i = threadIdx.x; i < m.nz; i += NT) {
+            ml[i] = m.DZ[i];
+            ml[m.nz + i] = m.DZW[i];
+            ml[2 * m.nz + i] = m.GCZ[i];
+            ml[3 * m.nz + i] = m.cellh[i];
+        }
+        m.DZ = ml;
+        m.DZW = ml + m.nz;
+        m.GCZ = ml + 2 * m.nz;
+        m.cellh = ml + 3 * m.nz;
+    }
     const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tile = lb % ntiles, chunk = lb / ntiles;
     const int x0 = (tile % ntx) * kTX, y0 = (tile / ntx) * TY;
@@ -189,7 +226,7 @@ __global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
     int gdw[ND];
 #pragma unroll
     for (int k = 0; k < ND; k++) {
-        const int d = threadIdx.x + k * NT;
+        const int d = min(threadIdx.x + k * NT, DPS - 1);  // surplus lanes re-read a valid word
         const int e = d / DPC, wd = d - e * DPC;
         const int lx = e % W, ly = e / W;
         int gx = (x0 + lx - RXY) % m.nx;
@@ -198,105 +235,171 @@ __global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
         if (gy < 0) gy += m.ny;
         gdw[k] = (gy * m.nx + gx) * DPC + wd;
     }
-    const int wave = threadIdx.x >> 6;
-    auto fetch_async = [&](int p) {  // plane p -> its ring slot, without registers
-        const unsigned* __restrict__ src = reinterpret_cast<const unsigned*>(a.vel) + (size_t)p * m.nxy * DPC;
-        unsigned* dst = reinterpret_cast<unsigned*>(ring + (p % NS) * 3 * PS);
+    // A plane fill is ND dword loads per lane into registers, written to the
+    // ring slot after the plane in between has been computed: the loads in
+    // flight target registers, so no LDS read of the computation waits for them.
+    // Lanes past the plane re-read a valid word into the slot's padding.
+    unsigned pf[ND];
+    auto fetch = [&](int p) {  // plane min(p, nz-1) -> registers
+        const unsigned* __restrict__ src =
+            reinterpret_cast<const unsigned*>(a.vel) + (size_t)min(p, m.nz - 1) * m.nxy * DPC;
 #pragma unroll
-        for (int k = 0; k < ND; k++) {
-            if (threadIdx.x + k * NT < DPS)
-                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + gdw[k]),
-                                                 (__attribute__((address_space(3))) void*)(dst + k * NT + wave * 64), 4, 0,
-                                                 0);
-        }
+        for (int k = 0; k < ND; k++) pf[k] = src[gdw[k]];
     };
-    for (int p = max(0, z0 - RZ); p <= min(m.nz - 1, z0 + RZ); p++) fetch_async(p);
-    __builtin_amdgcn_s_waitcnt(0);
+    auto commit = [&](int p) {  // registers -> the ring slot of plane p
+        unsigned* dst = reinterpret_cast<unsigned*>(ring + (p % NS) * SLOT);
+#pragma unroll
+        for (int k = 0; k < ND; k++) dst[threadIdx.x + k * NT] = pf[k];
+    };
+    for (int p = max(0, z0 - RZ); p <= min(m.nz - 1, z0 + RZ); p++) {
+        fetch(p);
+        commit(p);
+    }
     __syncthreads();
 
+    // Plane loop. Per plane: (1) store the previous plane's results, (2) issue
+    // this plane's own global loads, (3) issue the next plane's LDS fill, (4)
+    // compute from LDS and registers only. The vector-memory counter is in
+    // order, so waiting for (2) leaves (3) in flight; no global load sits inside
+    // the computation, and every thread computes (outside cells on clamped
+    // indices, results not stored) so that the mesh arrays are read uniformly.
     double cfr = 0.0;
+    const int cx = active ? ix : 0, cy = active ? iy : 0;
+    T kq[3] = {T(0), T(0), T(0)}, rq[3] = {T(0), T(0), T(0)};
+    int in_q = 0;
     for (int iz = z0; iz < z1; iz++) {
-        // planes iz-RZ..iz+RZ are in LDS; plane iz+RZ+1 (next iteration) is loaded
-        // into the slot of plane iz-RZ-1, which no thread reads after the last barrier
-        const int pn = iz + RZ + 1;
-        if (pn < m.nz && iz + 1 < z1) fetch_async(pn);
-        if (active) {
-            Acc acc;
-#pragma unroll
-            for (int d = -RZ; d <= RZ; d++) acc.pl[d + RZ] = ring + (((iz + d) % NS + NS) % NS) * 3 * PS;
-            acc.off = (ty + RXY) * W + tx + RXY;
-            const int in = iz * m.nxy + iy * m.nx + ix;
-            acc.gcol = a.vel + 3 * in;
-            acc.nxy3 = 3 * m.nxy;
-            nb.base = iz * m.nxy;
-            GlobalNu<T> nu{a.nut, &nb};
-            T k[3] = {0, 0, 0};
-            if (SCHEME == TC_SCHEME_AB3) {
-                if (iz != 0 && iz != m.nz - 1) {
-                    const T inv_hz = T(2) / (m.DZ[iz - 1] + m.DZ[iz]);
-                    conv_ab3(m, acc, iz, inv_hz, k);
-                    if (VISC == 1) {
-                        T g[3];
-                        visc_galerkin<T, TURB>(m, acc, nu, iz, g);
-                        const T s = m.inv_hx * m.inv_hy * inv_hz;
-#pragma unroll
-                        for (int c = 0; c < 3; c++) k[c] += g[c] * s;
-                    } else {  // naive Laplacian (collocated_AB3/func_fluxes.h:110-113)
-                        const T ihx2 = m.visc * m.inv_hx * m.inv_hx, ihy2 = m.visc * m.inv_hy * m.inv_hy;
-#pragma unroll
-                        for (int c = 0; c < 3; c++) {
-                            const T a0 = acc(c, 0, 0, 0);
-                            k[c] += ihx2 * (acc(c, 1, 0, 0) + acc(c, -1, 0, 0) - T(2) * a0);
-                            k[c] += ihy2 * (acc(c, 0, 1, 0) + acc(c, 0, -1, 0) - T(2) * a0);
-                            k[c] += m.visc * ((acc(c, 0, 0, 1) - a0) / m.DZ[iz] - (a0 - acc(c, 0, 0, -1)) / m.DZ[iz - 1]) * inv_hz;
-                        }
-                    }
-                    if (a.tau) sgs_divergence(m, a.tau, nb, iz, k);
-                    k[0] += m.sour;
-                }
-            } else {  // ExplicitTerm (staggered/func_global.h:642-649)
-                T c[3], v[3];
-                if (SCHEME == TC_SCHEME_HW) conv_hw(m, acc, iz, c);
-                else conv_vv(m, acc, iz, c);
-                if (VISC == 1) visc_naive<T, TURB>(m, acc, nu, iz, v);
-                else visc_cross(m, acc, iz, v);
-                k[0] = (m.sour + c[0]) + v[0];
-                k[1] = (T(0) + c[1]) + v[1];
-                k[2] = iz == 0 ? T(0) : (T(0) + c[2]) + v[2];
-            }
-            T* fc = a.fcur + 3 * in;
-            fc[0] = k[0];
-            fc[1] = k[1];
-            fc[2] = k[2];
-            if (a.order > 0) {  // BDF/EX combine (staggered/func_global.h:909-932)
-                T r[3];
-                const T it = a.inv_tau;
-#pragma unroll
-                for (int cc = 0; cc < 3; cc++) {
-                    const int j = 3 * in + cc;
-                    const T u0 = acc(cc, 0, 0, 0);
-                    if (a.order == 1) r[cc] = k[cc] + it * u0;
-                    else if (a.order == 2) r[cc] = T(2) * k[cc] - a.fprev[j] + it * (T(2) * u0 - T(0.5) * a.vprev[j]);
-                    else
-                        r[cc] = T(3) * k[cc] - T(3) * a.fprev[j] + a.fprev2[j] +
-                                it * (T(3) * u0 - T(1.5) * a.vprev[j] + T(1.0 / 3.0) * a.vprev2[j]);
-                }
-                if (SCHEME == TC_SCHEME_AB3 && a.stab) {  // collocated_AB3/func_global.h:435-437
-                    T g[3];
-                    grad_at<T, TC_SCHEME_AB3>(m, a.pcur, nb, iz, g);
-                    r[0] -= g[0];
-                    r[1] -= g[1];
-                    r[2] -= g[2];
-                }
-                T* fn = a.fnext + 3 * in;
-                fn[0] = r[0];
-                fn[1] = r[1];
-                fn[2] = r[2];
-                if (a.cfr_partial) cfr += (double)r[0] * u_volume<T, SCHEME>(m, iz);
+        if (iz > z0 && active) {
+            T* fc = a.fcur + 3 * in_q;
+            fc[0] = kq[0];
+            fc[1] = kq[1];
+            fc[2] = kq[2];
+            if (a.order > 0) {
+                T* fn = a.fnext + 3 * in_q;
+                fn[0] = rq[0];
+                fn[1] = rq[1];
+                fn[2] = rq[2];
             }
         }
-        __builtin_amdgcn_s_waitcnt(0);  // this lane's part of plane iz+RZ+1 has landed
+        const int in = iz * m.nxy + cy * m.nx + cx;
+        T vp[3] = {T(0), T(0), T(0)}, vp2[3] = {T(0), T(0), T(0)}, fp[3] = {T(0), T(0), T(0)}, fp2[3] = {T(0), T(0), T(0)};
+        if (a.order >= 2) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                vp[c] = a.vprev[3 * in + c];
+                fp[c] = a.fprev[3 * in + c];
+            }
+        }
+        if (a.order == 3) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                vp2[c] = a.vprev2[3 * in + c];
+                fp2[c] = a.fprev2[3 * in + c];
+            }
+        }
+        Acc acc;
+        {
+            const int zp = min(iz + RZ + 1, m.nz - 1), zm = max(iz - RZ - 1, 0);
+            const T* cpp = a.vel + 3 * ((size_t)zp * m.nxy + cy * m.nx + cx);
+            const T* cmp = a.vel + 3 * ((size_t)zm * m.nxy + cy * m.nx + cx);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                acc.cp[c] = cpp[c];
+                acc.cm[c] = cmp[c];
+            }
+        }
+        nb.base = iz * m.nxy;
+        T pg[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};  // collocated StabType 1: p at x+-, y+-, z+-
+        const bool stab = SCHEME == TC_SCHEME_AB3 && a.order > 0 && a.stab && iz != 0 && iz != m.nz - 1;
+        if (stab) {
+            pg[0] = a.pcur[nb.at(1, 0, 0)];
+            pg[1] = a.pcur[nb.at(-1, 0, 0)];
+            pg[2] = a.pcur[nb.at(0, 1, 0)];
+            pg[3] = a.pcur[nb.at(0, -1, 0)];
+            pg[4] = a.pcur[nb.at(0, 0, 1)];
+            pg[5] = a.pcur[nb.at(0, 0, -1)];
+        }
+        T tq[3][7];
+        const bool sgs = SCHEME == TC_SCHEME_AB3 && a.tau != nullptr && iz != 0 && iz != m.nz - 1;
+        if (sgs) sgs_load(a.tau, nb, iz, m.nz, tq);
+        __builtin_amdgcn_sched_barrier(0);
+        // planes iz-RZ..iz+RZ are in LDS; plane iz+RZ+1 (next iteration) goes into
+        // the slot of plane iz-RZ-1, which no thread reads after the last barrier
+        fetch(iz + RZ + 1);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int d = -RZ; d <= RZ; d++) acc.pl[d + RZ] = ring + (((iz + d) % NS + NS) % NS) * SLOT;
+        acc.off = (ty + RXY) * W + tx + RXY;
+        GlobalNu<T> nu{a.nut, &nb};
+        T k[3] = {0, 0, 0};
+        if (SCHEME == TC_SCHEME_AB3) {
+            if (iz != 0 && iz != m.nz - 1) {
+                const T inv_hz = T(2) / (m.DZ[iz - 1] + m.DZ[iz]);
+                conv_ab3(m, acc, iz, inv_hz, k);
+                if (VISC == 1) {
+                    T g[3];
+                    visc_galerkin<T, TURB>(m, acc, nu, iz, g);
+                    const T s = m.inv_hx * m.inv_hy * inv_hz;
+#pragma unroll
+                    for (int c = 0; c < 3; c++) k[c] += g[c] * s;
+                } else {  // naive Laplacian (collocated_AB3/func_fluxes.h:110-113)
+                    const T ihx2 = m.visc * m.inv_hx * m.inv_hx, ihy2 = m.visc * m.inv_hy * m.inv_hy;
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        const T a0 = acc(c, 0, 0, 0);
+                        k[c] += ihx2 * (acc(c, 1, 0, 0) + acc(c, -1, 0, 0) - T(2) * a0);
+                        k[c] += ihy2 * (acc(c, 0, 1, 0) + acc(c, 0, -1, 0) - T(2) * a0);
+                        k[c] += m.visc * ((acc(c, 0, 0, 1) - a0) / m.DZ[iz] - (a0 - acc(c, 0, 0, -1)) / m.DZ[iz - 1]) * inv_hz;
+                    }
+                }
+                if (sgs) sgs_divergence(m, tq, iz, k);
+                k[0] += m.sour;
+            }
+        } else {  // ExplicitTerm (staggered/func_global.h:642-649)
+            T c[3], v[3];
+            if (SCHEME == TC_SCHEME_HW) conv_hw(m, acc, iz, c);
+            else conv_vv(m, acc, iz, c);
+            if (VISC == 1) visc_naive<T, TURB>(m, acc, nu, iz, v);
+            else visc_cross(m, acc, iz, v);
+            k[0] = (m.sour + c[0]) + v[0];
+            k[1] = (T(0) + c[1]) + v[1];
+            k[2] = iz == 0 ? T(0) : (T(0) + c[2]) + v[2];
+        }
+        if (a.order > 0) {  // BDF/EX combine (staggered/func_global.h:909-932)
+            const T it = a.inv_tau;
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) {
+                const T u0 = acc(cc, 0, 0, 0);
+                if (a.order == 1) rq[cc] = k[cc] + it * u0;
+                else if (a.order == 2) rq[cc] = T(2) * k[cc] - fp[cc] + it * (T(2) * u0 - T(0.5) * vp[cc]);
+                else
+                    rq[cc] = T(3) * k[cc] - T(3) * fp[cc] + fp2[cc] +
+                             it * (T(3) * u0 - T(1.5) * vp[cc] + T(1.0 / 3.0) * vp2[cc]);
+            }
+            if (stab) {  // r -= G p_n (collocated_AB3/func_global.h:435-437; ApplyGradient :298-303)
+                rq[0] -= T(0.5) * m.inv_hx * (pg[0] - pg[1]);
+                rq[1] -= T(0.5) * m.inv_hy * (pg[2] - pg[3]);
+                rq[2] -= T(0.5) * (pg[4] - pg[5]) / m.cellh[iz];
+            }
+            if (a.cfr_partial && active) cfr += (double)rq[0] * u_volume<T, SCHEME>(m, iz);
+        }
+        kq[0] = k[0];
+        kq[1] = k[1];
+        kq[2] = k[2];
+        in_q = in;
+        commit(iz + RZ + 1);  // the slot of plane iz-RZ-1, which no thread reads after the last barrier
         __syncthreads();
+    }
+    if (z1 > z0 && active) {
+        T* fc = a.fcur + 3 * in_q;
+        fc[0] = kq[0];
+        fc[1] = kq[1];
+        fc[2] = kq[2];
+        if (a.order > 0) {
+            T* fn = a.fnext + 3 * in_q;
+            fn[0] = rq[0];
+            fn[1] = rq[1];
+            fn[2] = rq[2];
+        }
     }
     if (a.cfr_partial) {
         const double s = block_sum(cfr, sh);
@@ -309,18 +412,20 @@ struct FluxGrid {
 };
 // Blocks resident at once on the device for one flux kernel instantiation.
 template <typename T, int SCHEME, int VISC, bool TURB>
-int flux_slots() {
+int flux_slots(size_t dyn_lds) {
     static int slots = 0;
-    if (slots == 0) {
+    static size_t slots_lds = 0;
+    if (slots == 0 || slots_lds != dyn_lds) {
         int per_cu = 0, dev = 0, ncu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_flux<T, SCHEME, VISC, TURB>,
-                                                         kTX * flux_ty<T, SCHEME, VISC>(), 0) != hipSuccess ||
+                                                         kTX * flux_ty<T, SCHEME, VISC>(), dyn_lds) != hipSuccess ||
             per_cu < 1)
             per_cu = 1;
         if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             ncu < 1)
             ncu = 256;
         slots = per_cu * ncu;
+        slots_lds = dyn_lds;
     }
     return slots;
 }
@@ -504,9 +609,10 @@ hipError_t launch_flux(hipStream_t s, const Mesh<T>& m, const FluxArgs<T>& a, in
 #define L(SCH, VIS, TRB)                                                                                       \
     {                                                                                                          \
         constexpr int ty = flux_ty<T, SCH, VIS>();                                                             \
-        const FluxGrid fg = flux_grid(m.nx, m.ny, m.nz, ty, flux_slots<T, SCH, VIS, TRB>());                  \
+        const size_t lds = 4 * (size_t)m.nz * sizeof(T);                                                     \
+        const FluxGrid fg = flux_grid(m.nx, m.ny, m.nz, ty, flux_slots<T, SCH, VIS, TRB>(lds));               \
         if (nblocks) *nblocks = fg.nblocks;                                                                    \
-        hipLaunchKernelGGL((k_flux<T, SCH, VIS, TRB>), dim3(fg.nblocks), dim3(kTX * ty), 0, s, m, a, fg.ntx,   \
+        hipLaunchKernelGGL((k_flux<T, SCH, VIS, TRB>), dim3(fg.nblocks), dim3(kTX * ty), lds, s, m, a, fg.ntx, \
                            fg.ntiles, fg.zchunk);                                                              \
     }
     if (a.scheme == TC_SCHEME_HW) {
