@@ -131,6 +131,28 @@ __device__ Cplx<T>* lds_fft(Cplx<T>* src, Cplx<T>* dst, const FFTPlan& pl, int n
 }
 
 constexpr int kFftThreads = 256;
+constexpr int kTwRegs = 4;  // twiddle loads per thread issued with the data loads (N <= 1024 in one round)
+
+// Staging into LDS: every global load of a thread's share is issued before any
+// of the values is used (unconditional, clamped addresses; a conditional load
+// per element would cost a full memory round trip per element), and the
+// twiddle table goes to LDS with them, so the Stockham passes read no global
+// memory.
+template <typename T>
+struct TwStage {
+    Cplx<T> r[kTwRegs];
+    __device__ __forceinline__ void load(const Cplx<T>* __restrict__ tw, int N) {
+#pragma unroll
+        for (int u = 0; u < kTwRegs; u++) r[u] = tw[min((int)threadIdx.x + u * kFftThreads, N - 1)];
+    }
+    __device__ __forceinline__ void store(Cplx<T>* twl, const Cplx<T>* __restrict__ tw, int N) const {
+        // unconditional (surplus lanes store the same last entry), so that the
+        // loads are not sunk into a branch
+#pragma unroll
+        for (int u = 0; u < kTwRegs; u++) twl[min((int)threadIdx.x + u * kFftThreads, N - 1)] = r[u];
+        for (int i = threadIdx.x + kTwRegs * kFftThreads; i < N; i += kFftThreads) twl[i] = tw[i];
+    }
+};
 
 // Forward x transform of the divergence of v: rows 2q, 2q+1 (row = iy + NY*iz)
 // form the real and imaginary parts of line q; the divergence is evaluated
@@ -143,24 +165,36 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n, nxh = N / 2 + 1, nrows = m.ny * m.nz;
     Cplx<T>* b1 = b0 + lpb * N;
+    Cplx<T>* twl = b1 + lpb * N;
     const int q0 = blockIdx.x * lpb;
     constexpr int R = SCHEME == TC_SCHEME_VV ? 2 : 1;
-    for (int t = threadIdx.x; t < lpb * N; t += blockDim.x) {
-        const int l = t / N, n = t - l * N;
-        const int ra = 2 * (q0 + l), rb = ra + 1;
-        Cplx<T> z = {T(0), T(0)};
-        if (ra < nrows) {
-            const int iz = ra / m.ny, iy = ra - iz * m.ny;
-            z.x = div_cell<T, SCHEME>(m, make_nbvec<T, R>(v, n, iy, iz, m.nx, m.ny), iz);
+    constexpr int U = SCHEME == TC_SCHEME_VV ? 2 : 4;
+    TwStage<T> tws;
+    tws.load(tw, N);
+    const int tot = lpb * N;
+    for (int t0 = threadIdx.x; t0 < tot; t0 += U * kFftThreads) {
+        T zx[U], zy[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = min(t0 + u * kFftThreads, tot - 1);
+            const int l = t / N, n = t - l * N;
+            const int ra = min(2 * (q0 + l), nrows - 1), rb = min(2 * (q0 + l) + 1, nrows - 1);
+            const int iza = ra / m.ny, iya = ra - iza * m.ny, izb = rb / m.ny, iyb = rb - izb * m.ny;
+            zx[u] = div_cell<T, SCHEME>(m, make_nbvec<T, R>(v, n, iya, iza, m.nx, m.ny, m.nz), iza);
+            zy[u] = div_cell<T, SCHEME>(m, make_nbvec<T, R>(v, n, iyb, izb, m.nx, m.ny, m.nz), izb);
         }
-        if (rb < nrows) {
-            const int iz = rb / m.ny, iy = rb - iz * m.ny;
-            z.y = div_cell<T, SCHEME>(m, make_nbvec<T, R>(v, n, iy, iz, m.nx, m.ny), iz);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u * kFftThreads;
+            if (t < tot) {
+                const int ra = 2 * (q0 + t / N);
+                b0[t] = {ra < nrows ? zx[u] : T(0), ra + 1 < nrows ? zy[u] : T(0)};
+            }
         }
-        b0[t] = z;
     }
+    tws.store(twl, tw, N);
     __syncthreads();
-    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, N, 1, tw, T(-1));
+    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, N, 1, twl, T(-1));
     for (int t = threadIdx.x; t < lpb * nxh; t += blockDim.x) {
         const int l = t / nxh, k = t - l * nxh;
         const int ra = 2 * (q0 + l), rb = ra + 1;
@@ -182,26 +216,46 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n, nxh = N / 2 + 1, half = N / 2;
     Cplx<T>* b1 = b0 + lpb * N;
+    Cplx<T>* twl = b1 + lpb * N;
     const int q0 = blockIdx.x * lpb;
-    for (int t = threadIdx.x; t < lpb * N; t += blockDim.x) {
-        const int l = t / N, k = t - l * N;
-        const int ra = 2 * (q0 + l), rb = ra + 1;
-        const int kk = k <= half ? k : N - k;
-        Cplx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
-        if (ra < nrows) A = spec[(size_t)ra * nxh + kk];
-        if (rb < nrows) B = spec[(size_t)rb * nxh + kk];
-        if (kk == 0 || kk == half) {  // real-to-complex symmetry: these bins are real
-            A.y = T(0);
-            B.y = T(0);
+    constexpr int U = 4;
+    TwStage<T> tws;
+    tws.load(tw, N);
+    const int tot = lpb * N;
+    for (int t0 = threadIdx.x; t0 < tot; t0 += U * kFftThreads) {
+        Cplx<T> A[U], B[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = min(t0 + u * kFftThreads, tot - 1);
+            const int l = t / N, k = t - l * N;
+            const int kk = k <= half ? k : N - k;
+            const int ra = min(2 * (q0 + l), nrows - 1), rb = min(2 * (q0 + l) + 1, nrows - 1);
+            A[u] = spec[(size_t)ra * nxh + kk];
+            B[u] = spec[(size_t)rb * nxh + kk];
         }
-        if (k > half) {
-            A.y = -A.y;
-            B.y = -B.y;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u * kFftThreads;
+            if (t >= tot) continue;
+            const int l = t / N, k = t - l * N;
+            const int kk = k <= half ? k : N - k;
+            const int ra = 2 * (q0 + l);
+            Cplx<T> a = ra < nrows ? A[u] : Cplx<T>{T(0), T(0)};
+            Cplx<T> b = ra + 1 < nrows ? B[u] : Cplx<T>{T(0), T(0)};
+            if (kk == 0 || kk == half) {  // real-to-complex symmetry: these bins are real
+                a.y = T(0);
+                b.y = T(0);
+            }
+            if (k > half) {
+                a.y = -a.y;
+                b.y = -b.y;
+            }
+            b0[t] = {a.x - b.y, a.y + b.x};  // A + iB
         }
-        b0[t] = {A.x - B.y, A.y + B.x};  // A + iB
     }
+    tws.store(twl, tw, N);
     __syncthreads();
-    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, N, 1, tw, T(1));
+    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, N, 1, twl, T(1));
     for (int t = threadIdx.x; t < lpb * N; t += blockDim.x) {
         const int l = t / N, n = t - l * N;
         const int ra = 2 * (q0 + l), rb = ra + 1;
@@ -221,17 +275,35 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n;
     Cplx<T>* b1 = b0 + tk * N;
+    Cplx<T>* twl = b1 + tk * N;
     const int kx0 = blockIdx.x * tk, iz = blockIdx.y;
     const int ncol = min(tk, nxh - kx0);
     Cplx<T>* base = spec + (size_t)iz * N * nxh + kx0;
-    for (int t = threadIdx.x; t < tk * N; t += blockDim.x) {
-        const int n = t / tk, k = t - n * tk;
-        Cplx<T> v = k < ncol ? base[(size_t)n * nxh + k] : Cplx<T>{T(0), T(0)};
-        if (psum && t == 0 && kx0 == 0) v.x = T((double)v.x - psum[0] * C[iz]);
-        b0[t] = v;
+    constexpr int U = 4;
+    TwStage<T> tws;
+    tws.load(tw, N);
+    const int tot = tk * N;
+    for (int t0 = threadIdx.x; t0 < tot; t0 += U * kFftThreads) {
+        Cplx<T> v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = min(t0 + u * kFftThreads, tot - 1);
+            const int n = t / tk, k = t - n * tk;
+            v[u] = base[(size_t)n * nxh + min(k, ncol - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u * kFftThreads;
+            if (t >= tot) continue;
+            const int n = t / tk, k = t - n * tk;
+            Cplx<T> w = k < ncol ? v[u] : Cplx<T>{T(0), T(0)};
+            if (psum && t == 0 && kx0 == 0) w.x = T((double)w.x - psum[0] * C[iz]);
+            b0[t] = w;
+        }
     }
+    tws.store(twl, tw, N);
     __syncthreads();
-    const Cplx<T>* res = lds_fft(b0, b1, pl, tk, 1, tk, tw, s);
+    const Cplx<T>* res = lds_fft(b0, b1, pl, tk, 1, tk, twl, s);
     for (int t = threadIdx.x; t < tk * N; t += blockDim.x) {
         const int n = t / tk, k = t - n * tk;
         if (k < ncol) base[(size_t)n * nxh + k] = res[t];
@@ -417,21 +489,38 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_q(FFTPlan pl, const C
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n, nxh = N / 2 + 1;
     Cplx<T>* b1 = b0 + lpb * N;
+    Cplx<T>* twl = b1 + lpb * N;
     const int q0 = blockIdx.x * lpb;
     auto q = [&](int row, int n) {
         const T* c = v + 3 * ((size_t)row * N + n);
-        return cb < 0 ? c[ca] : c[ca] * c[cb];
+        const T x = c[ca], y = c[cb < 0 ? ca : cb];
+        return cb < 0 ? x : x * y;
     };
-    for (int t = threadIdx.x; t < lpb * N; t += blockDim.x) {
-        const int l = t / N, n = t - l * N;
-        const int ra = 2 * (q0 + l), rb = ra + 1;
-        Cplx<T> z = {T(0), T(0)};
-        if (ra < nrows) z.x = q(ra, n);
-        if (rb < nrows) z.y = q(rb, n);
-        b0[t] = z;
+    constexpr int U = 4;
+    TwStage<T> tws;
+    tws.load(tw, N);
+    const int tot = lpb * N;
+    for (int t0 = threadIdx.x; t0 < tot; t0 += U * kFftThreads) {
+        T zx[U], zy[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = min(t0 + u * kFftThreads, tot - 1);
+            const int l = t / N, n = t - l * N;
+            zx[u] = q(min(2 * (q0 + l), nrows - 1), n);
+            zy[u] = q(min(2 * (q0 + l) + 1, nrows - 1), n);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u * kFftThreads;
+            if (t < tot) {
+                const int ra = 2 * (q0 + t / N);
+                b0[t] = {ra < nrows ? zx[u] : T(0), ra + 1 < nrows ? zy[u] : T(0)};
+            }
+        }
     }
+    tws.store(twl, tw, N);
     __syncthreads();
-    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, N, 1, tw, T(-1));
+    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, N, 1, twl, T(-1));
     for (int t = threadIdx.x; t < lpb * nxh; t += blockDim.x) {
         const int l = t / nxh, k = t - l * nxh;
         const int ra = 2 * (q0 + l), rb = ra + 1;
@@ -478,7 +567,7 @@ hipError_t launch_fft_x_fwd_div(hipStream_t s, const FFTPlan& pl, const Cplx<T>*
     const int lpb = lines_per_block<T>(pl.n);
     const int nlines = (m.ny * m.nz + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
-    const size_t sm = 2 * (size_t)lpb * pl.n * sizeof(Cplx<T>);
+    const size_t sm = (2 * (size_t)lpb + 1) * pl.n * sizeof(Cplx<T>);
     if (scheme == TC_SCHEME_HW)
         hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_HW>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb);
     else if (scheme == TC_SCHEME_VV)
@@ -494,7 +583,7 @@ hipError_t launch_fft_x_inv(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw,
     const int lpb = lines_per_block<T>(pl.n);
     const int nlines = (nrows + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
-    const size_t sm = 2 * (size_t)lpb * pl.n * sizeof(Cplx<T>);
+    const size_t sm = (2 * (size_t)lpb + 1) * pl.n * sizeof(Cplx<T>);
     hipLaunchKernelGGL((k_fft_x_inv<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, spec, out, nrows, lpb, scale);
     return hipGetLastError();
 }
@@ -506,7 +595,7 @@ hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cpl
     int tk = 128 / (int)sizeof(Cplx<T>);
     while (tk > 1 && 2 * (size_t)tk * pl.n * sizeof(Cplx<T>) > 48 * 1024) tk >>= 1;
     const dim3 grid((nxh + tk - 1) / tk, nz);
-    const size_t sm = 2 * (size_t)tk * pl.n * sizeof(Cplx<T>);
+    const size_t sm = (2 * (size_t)tk + 1) * pl.n * sizeof(Cplx<T>);
     hipLaunchKernelGGL((k_fft_y<T>), grid, dim3(kFftThreads), sm, s, pl, tw, spec, nxh, tk, T(inverse ? 1 : -1), psum, C);
     return hipGetLastError();
 }
@@ -531,7 +620,7 @@ hipError_t launch_fft_x_fwd_q(hipStream_t s, const FFTPlan& pl, const Cplx<T>* t
     const int lpb = lines_per_block<T>(pl.n);
     const int nlines = (nrows + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
-    const size_t sm = 2 * (size_t)lpb * pl.n * sizeof(Cplx<T>);
+    const size_t sm = (2 * (size_t)lpb + 1) * pl.n * sizeof(Cplx<T>);
     hipLaunchKernelGGL((k_fft_x_fwd_q<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, v, ca, cb, nrows, spec, lpb);
     return hipGetLastError();
 }

@@ -109,6 +109,20 @@ template <typename T, int SCHEME, int VISC>
 constexpr int flux_ty() {
     return 8;
 }
+// Threads per cell: in FP64 the fourth-order scheme gives each momentum
+// component its own wave (the component is uniform across a wave), so that a
+// thread holds one equation's registers and three waves instead of one share a
+// SIMD to hide LDS latency. In FP32 the whole cell fits twice per SIMD already,
+// and the split only adds waiting at the per-plane barrier for the costlier
+// w-equation.
+template <typename T, int SCHEME>
+constexpr int flux_nc() {
+    return SCHEME == TC_SCHEME_VV && sizeof(T) == 8 ? 3 : 1;
+}
+template <typename T, int SCHEME, int VISC>
+constexpr int flux_threads() {
+    return kTX * flux_ty<T, SCHEME, VISC>() * flux_nc<T, SCHEME>();
+}
 
 __device__ __forceinline__ int xcd_swizzle(int b, int nb) {
     const int q = nb / 8, r = nb % 8, x = b % 8, i = b / 8;
@@ -176,7 +190,7 @@ __device__ __forceinline__ void sgs_divergence(const Mesh<T>& m, const T q[3][7]
 }
 
 template <typename T, int SCHEME, int VISC, bool TURB>
-__global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
+__global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>())) void k_flux(
     Mesh<T> m, FluxArgs<T> a, int ntx, int ntiles, int zchunk) {
     constexpr int TY = flux_ty<T, SCHEME, VISC>();
     constexpr int RXY = SCHEME == TC_SCHEME_VV ? 3 : 1;
@@ -184,7 +198,8 @@ __global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
     constexpr int NS = 2 * RZ + 2;  // planes iz-RZ..iz+RZ plus the one in flight
     using Acc = RingAcc<T, RXY, RZ, TY>;
     constexpr int W = Acc::W, PS = Acc::PS;
-    constexpr int NT = kTX * TY;
+    constexpr int NC = flux_nc<T, SCHEME>();
+    constexpr int NT = kTX * TY * NC;
     constexpr int DPC = 3 * (int)sizeof(T) / 4;  // dwords per real3 cell
     constexpr int DPS = PS * DPC;                // dwords per plane of the padded tile
     constexpr int ND = (DPS + NT - 1) / NT;
@@ -213,7 +228,9 @@ __global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
     const int tile = lb % ntiles, chunk = lb / ntiles;
     const int x0 = (tile % ntx) * kTX, y0 = (tile / ntx) * TY;
     const int z0 = chunk * zchunk, z1 = min(m.nz, z0 + zchunk);
-    const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+    const int tcell = threadIdx.x % (kTX * TY);
+    const int comp = NC == 3 ? (int)(threadIdx.x / (kTX * TY)) : -1;  // wave-uniform
+    const int tx = tcell % kTX, ty = tcell / kTX;
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix < m.nx && iy < m.ny;
     Nb nb;  // global neighbours for nu_t and the collocated pressure gradient
@@ -265,10 +282,14 @@ __global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
     // indices, results not stored) so that the mesh arrays are read uniformly.
     double cfr = 0.0;
     const int cx = active ? ix : 0, cy = active ? iy : 0;
+    // NC == 3: a thread keeps component `comp` only (kc, rc; history vp[0] etc.)
     T kq[3] = {T(0), T(0), T(0)}, rq[3] = {T(0), T(0), T(0)};
     int in_q = 0;
-    for (int iz = z0; iz < z1; iz++) {
-        if (iz > z0 && active) {
+    auto store_results = [&]() {
+        if (NC == 3) {
+            a.fcur[3 * in_q + comp] = kq[0];
+            if (a.order > 0) a.fnext[3 * in_q + comp] = rq[0];
+        } else {
             T* fc = a.fcur + 3 * in_q;
             fc[0] = kq[0];
             fc[1] = kq[1];
@@ -280,20 +301,25 @@ __global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
                 fn[2] = rq[2];
             }
         }
+    };
+    constexpr int NH = NC == 3 ? 1 : 3;  // history components a thread loads
+    for (int iz = z0; iz < z1; iz++) {
+        if (iz > z0 && active) store_results();
         const int in = iz * m.nxy + cy * m.nx + cx;
+        const int c0 = NC == 3 ? comp : 0;
         T vp[3] = {T(0), T(0), T(0)}, vp2[3] = {T(0), T(0), T(0)}, fp[3] = {T(0), T(0), T(0)}, fp2[3] = {T(0), T(0), T(0)};
         if (a.order >= 2) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                vp[c] = a.vprev[3 * in + c];
-                fp[c] = a.fprev[3 * in + c];
+            for (int c = 0; c < NH; c++) {
+                vp[c] = a.vprev[3 * in + c0 + c];
+                fp[c] = a.fprev[3 * in + c0 + c];
             }
         }
         if (a.order == 3) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                vp2[c] = a.vprev2[3 * in + c];
-                fp2[c] = a.fprev2[3 * in + c];
+            for (int c = 0; c < NH; c++) {
+                vp2[c] = a.vprev2[3 * in + c0 + c];
+                fp2[c] = a.fprev2[3 * in + c0 + c];
             }
         }
         Acc acc;
@@ -357,18 +383,21 @@ __global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
         } else {  // ExplicitTerm (staggered/func_global.h:642-649)
             T c[3], v[3];
             if (SCHEME == TC_SCHEME_HW) conv_hw(m, acc, iz, c);
-            else conv_vv(m, acc, iz, c);
-            if (VISC == 1) visc_naive<T, TURB>(m, acc, nu, iz, v);
-            else visc_cross(m, acc, iz, v);
+            else conv_vv(m, acc, iz, c, comp);
+            if (VISC == 1) visc_naive<T, TURB>(m, acc, nu, iz, v, comp);
+            else visc_cross(m, acc, iz, v, comp);
             k[0] = (m.sour + c[0]) + v[0];
             k[1] = (T(0) + c[1]) + v[1];
             k[2] = iz == 0 ? T(0) : (T(0) + c[2]) + v[2];
         }
+        if (NC == 3) {  // the kept component moves to slot 0
+            k[0] = comp == 0 ? k[0] : comp == 1 ? k[1] : k[2];
+        }
         if (a.order > 0) {  // BDF/EX combine (staggered/func_global.h:909-932)
             const T it = a.inv_tau;
 #pragma unroll
-            for (int cc = 0; cc < 3; cc++) {
-                const T u0 = acc(cc, 0, 0, 0);
+            for (int cc = 0; cc < NH; cc++) {
+                const T u0 = acc(c0 + cc, 0, 0, 0);
                 if (a.order == 1) rq[cc] = k[cc] + it * u0;
                 else if (a.order == 2) rq[cc] = T(2) * k[cc] - fp[cc] + it * (T(2) * u0 - T(0.5) * vp[cc]);
                 else
@@ -380,7 +409,7 @@ __global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
                 rq[1] -= T(0.5) * m.inv_hy * (pg[2] - pg[3]);
                 rq[2] -= T(0.5) * (pg[4] - pg[5]) / m.cellh[iz];
             }
-            if (a.cfr_partial && active) cfr += (double)rq[0] * u_volume<T, SCHEME>(m, iz);
+            if (a.cfr_partial && active && c0 == 0) cfr += (double)rq[0] * u_volume<T, SCHEME>(m, iz);
         }
         kq[0] = k[0];
         kq[1] = k[1];
@@ -389,18 +418,7 @@ __global__ __launch_bounds__((kTX * flux_ty<T, SCHEME, VISC>())) void k_flux(
         commit(iz + RZ + 1);  // the slot of plane iz-RZ-1, which no thread reads after the last barrier
         __syncthreads();
     }
-    if (z1 > z0 && active) {
-        T* fc = a.fcur + 3 * in_q;
-        fc[0] = kq[0];
-        fc[1] = kq[1];
-        fc[2] = kq[2];
-        if (a.order > 0) {
-            T* fn = a.fnext + 3 * in_q;
-            fn[0] = rq[0];
-            fn[1] = rq[1];
-            fn[2] = rq[2];
-        }
-    }
+    if (z1 > z0 && active) store_results();
     if (a.cfr_partial) {
         const double s = block_sum(cfr, sh);
         if (threadIdx.x == 0) a.cfr_partial[blockIdx.x] = s;
@@ -418,7 +436,7 @@ int flux_slots(size_t dyn_lds) {
     if (slots == 0 || slots_lds != dyn_lds) {
         int per_cu = 0, dev = 0, ncu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_flux<T, SCHEME, VISC, TURB>,
-                                                         kTX * flux_ty<T, SCHEME, VISC>(), dyn_lds) != hipSuccess ||
+                                                         flux_threads<T, SCHEME, VISC>(), dyn_lds) != hipSuccess ||
             per_cu < 1)
             per_cu = 1;
         if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -612,7 +630,7 @@ hipError_t launch_flux(hipStream_t s, const Mesh<T>& m, const FluxArgs<T>& a, in
         const size_t lds = 4 * (size_t)m.nz * sizeof(T);                                                     \
         const FluxGrid fg = flux_grid(m.nx, m.ny, m.nz, ty, flux_slots<T, SCH, VIS, TRB>(lds));               \
         if (nblocks) *nblocks = fg.nblocks;                                                                    \
-        hipLaunchKernelGGL((k_flux<T, SCH, VIS, TRB>), dim3(fg.nblocks), dim3(kTX * ty), lds, s, m, a, fg.ntx, \
+        hipLaunchKernelGGL((k_flux<T, SCH, VIS, TRB>), dim3(fg.nblocks), dim3(flux_threads<T, SCH, VIS>()), lds, s, m, a, fg.ntx, \
                            fg.ntiles, fg.zchunk);                                                              \
     }
     if (a.scheme == TC_SCHEME_HW) {

@@ -70,8 +70,9 @@ __device__ __forceinline__ T zbar_vv(const Mesh<T>& m, const AC& acc, int c, int
     return r;
 }
 
+// only: -1 all components, else the one component (wave-uniform) the caller keeps.
 template <typename T, class A>
-__device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, T f[3]) {
+__device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, T f[3], int only = -1) {
     const int NZ = m.nz;
     const T C27 = T(1.0 / 27.0);
     const T hx = m.hx, hy = m.hy, hxy = hx * hy;
@@ -79,6 +80,7 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
     const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
     T f0 = 0, f1 = 0, f2 = 0;
     // ---- u-equation
+    if (only < 0 || only == 0) {
     {
         const T br = dz * hy * (T(9) * (u0 + UU(1, 0, 0)) - (UU(-1, 0, 0) + UU(2, 0, 0)));
         const T bl = dz * hy * (T(9) * (UU(-1, 0, 0) + u0) - (UU(-2, 0, 0) + UU(1, 0, 0)));
@@ -104,8 +106,10 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
         const T q_ll = u0 + (iz == 2 ? -UU(0, 0, -2) : iz == 1 ? T(0) : iz == 0 ? UU(0, 0, 2) : UU(0, 0, -3));
         f0 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
     }
+    }
     __builtin_amdgcn_sched_barrier(0);  // keep the equations' live ranges apart
     // ---- v-equation
+    if (only < 0 || only == 1) {
     {
         const T br = dz * hy * YBAR(UU, 1, 0);
         const T bl = dz * hy * YBAR(UU, 0, 0);
@@ -131,9 +135,10 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
         const T q_ll = v0 + (iz == 2 ? -VV(0, 0, -2) : iz == 1 ? T(0) : iz == 0 ? VV(0, 0, 2) : VV(0, 0, -3));
         f1 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
     }
+    }
     __builtin_amdgcn_sched_barrier(0);
     // ---- w-equation (no unknown at iz = 0)
-    if (iz != 0) {
+    if (iz != 0 && (only < 0 || only == 2)) {
         {
             const T br = hy * zbar_vv(m, acc, 0, 1, 0, iz, m.DZ);
             const T bl = hy * zbar_vv(m, acc, 0, 0, 0, iz, m.DZ);
@@ -179,7 +184,7 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
 // Viscous term, constant viscosity (staggered/func_global.h:533-562)
 // ---------------------------------------------------------------------------
 template <typename T, class A>
-__device__ __forceinline__ void visc_cross(const Mesh<T>& m, const A& acc, int iz, T f[3]) {
+__device__ __forceinline__ void visc_cross(const Mesh<T>& m, const A& acc, int iz, T f[3], int only = -1) {
     const int NZ = m.nz;
     const T dz = m.DZ[iz];
     const T inv_hz = T(1) / dz;
@@ -187,6 +192,10 @@ __device__ __forceinline__ void visc_cross(const Mesh<T>& m, const A& acc, int i
     const T ihx2 = m.inv_hx * m.inv_hx, ihy2 = m.inv_hy * m.inv_hy;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
+        if (only >= 0 && c != only) {
+            f[c] = T(0);
+            continue;
+        }
         const T a0 = F3(vel, c, 0, 0, 0);
         f[c] = ihx2 * (F3(vel, c, 1, 0, 0) + F3(vel, c, -1, 0, 0) - T(2) * a0);
         f[c] += ihy2 * (F3(vel, c, 0, 1, 0) + F3(vel, c, 0, -1, 0) - T(2) * a0);
@@ -220,7 +229,7 @@ __device__ __forceinline__ void visc_cross(const Mesh<T>& m, const A& acc, int i
 // including the edge-average of sxy_190 at :598 as the reference has it.
 // ---------------------------------------------------------------------------
 template <typename T, bool TURB, class A, class NU>
-__device__ __forceinline__ void visc_naive(const Mesh<T>& m, const A& acc, const NU& nu, int iz, T f[3]) {
+__device__ __forceinline__ void visc_naive(const Mesh<T>& m, const A& acc, const NU& nu, int iz, T f[3], int only = -1) {
     const int NZ = m.nz;
     const T visc = m.visc;
     auto mu = [&](int dx, int dy, int dz) -> T { return TURB ? visc + nu(dx, dy, dz) : visc; };
@@ -232,41 +241,50 @@ __device__ __forceinline__ void visc_naive(const Mesh<T>& m, const A& acc, const
     const T q = T(0.25);
     const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
     const T mu0 = mu(0, 0, 0);
+    // stresses shared by two equations (recomputed where a thread keeps one component)
+    auto sxy_990 = [&]() {
+        return ((u0 - UU(0, -1, 0)) * ihy + (v0 - VV(-1, 0, 0)) * ihx) * q * (mu(0, -1, 0) + mu(-1, -1, 0) + mu0 + mu(-1, 0, 0));
+    };
+    auto sxz_909 = [&]() {
+        return (iz == 0) ? (T(2) * u0 * inv_hz * visc)
+                         : ((u0 - UU(0, 0, -1)) * inv_hz_down + (w0 - WW(-1, 0, 0)) * ihx) * q *
+                               (mu(0, 0, -1) + mu(-1, 0, -1) + mu0 + mu(-1, 0, 0));
+    };
+    auto syz_099 = [&]() {
+        return (iz == 0) ? (T(2) * v0 * inv_hz * visc)
+                         : ((v0 - VV(0, 0, -1)) * inv_hz_down + (w0 - WW(0, -1, 0)) * ihy) * q *
+                               (mu(0, 0, -1) + mu(0, -1, -1) + mu0 + mu(0, -1, 0));
+    };
     T f0 = 0, f1 = 0, f2 = 0;
-    const T sxx_000 = T(2) * (UU(1, 0, 0) - u0) * ihx * mu0;
-    const T sxx_800 = T(2) * (u0 - UU(-1, 0, 0)) * ihx * mu(-1, 0, 0);
-    f0 += (sxx_000 - sxx_800) * ihx;
-    const T sxy_910 = ((UU(0, 1, 0) - u0) * ihy + (VV(0, 1, 0) - VV(-1, 1, 0)) * ihx) * q * (mu0 + mu(-1, 0, 0) + mu(0, 1, 0) + mu(-1, 1, 0));
-    const T sxy_990 = ((u0 - UU(0, -1, 0)) * ihy + (v0 - VV(-1, 0, 0)) * ihx) * q * (mu(0, -1, 0) + mu(-1, -1, 0) + mu0 + mu(-1, 0, 0));
-    f0 += (sxy_910 - sxy_990) * ihy;
-    const T sxz_901 = (iz == NZ - 1) ? (T(-2) * u0 * inv_hz * visc)
-                                     : ((UU(0, 0, 1) - u0) * inv_hz_up + (WW(0, 0, 1) - WW(-1, 0, 1)) * ihx) * q *
-                                           (mu0 + mu(-1, 0, 0) + mu(0, 0, 1) + mu(-1, 0, 1));
-    const T sxz_909 = (iz == 0) ? (T(2) * u0 * inv_hz * visc)
-                                : ((u0 - UU(0, 0, -1)) * inv_hz_down + (w0 - WW(-1, 0, 0)) * ihx) * q *
-                                      (mu(0, 0, -1) + mu(-1, 0, -1) + mu0 + mu(-1, 0, 0));
-    f0 += (sxz_901 - sxz_909) * inv_hz;
-
-    const T sxy_190 = ((UU(1, 0, 0) - UU(1, -1, 0)) * ihy + (VV(1, 0, 0) - v0) * ihx) * q * (mu0 + mu(1, 0, 0) + mu(0, -1, 0) + mu(1, 1, 0));
-    f1 += (sxy_190 - sxy_990) * ihx;
-    const T syy_000 = T(2) * (VV(0, 1, 0) - v0) * ihy * mu0;
-    const T syy_080 = T(2) * (v0 - VV(0, -1, 0)) * ihy * mu(0, -1, 0);
-    f1 += (syy_000 - syy_080) * ihy;
-    const T syz_091 = (iz == NZ - 1) ? (T(-2) * v0 * inv_hz * visc)
-                                     : ((VV(0, 0, 1) - v0) * inv_hz_up + (WW(0, 0, 1) - WW(0, -1, 1)) * ihy) * q *
-                                           (mu0 + mu(0, -1, 0) + mu(0, 0, 1) + mu(0, -1, 1));
-    const T syz_099 = (iz == 0) ? (T(2) * v0 * inv_hz * visc)
-                                : ((v0 - VV(0, 0, -1)) * inv_hz_down + (w0 - WW(0, -1, 0)) * ihy) * q *
-                                      (mu(0, 0, -1) + mu(0, -1, -1) + mu0 + mu(0, -1, 0));
-    f1 += (syz_091 - syz_099) * inv_hz;
-
-    if (iz != 0) {
+    if (only < 0 || only == 0) {
+        const T sxx_000 = T(2) * (UU(1, 0, 0) - u0) * ihx * mu0;
+        const T sxx_800 = T(2) * (u0 - UU(-1, 0, 0)) * ihx * mu(-1, 0, 0);
+        f0 += (sxx_000 - sxx_800) * ihx;
+        const T sxy_910 = ((UU(0, 1, 0) - u0) * ihy + (VV(0, 1, 0) - VV(-1, 1, 0)) * ihx) * q * (mu0 + mu(-1, 0, 0) + mu(0, 1, 0) + mu(-1, 1, 0));
+        f0 += (sxy_910 - sxy_990()) * ihy;
+        const T sxz_901 = (iz == NZ - 1) ? (T(-2) * u0 * inv_hz * visc)
+                                         : ((UU(0, 0, 1) - u0) * inv_hz_up + (WW(0, 0, 1) - WW(-1, 0, 1)) * ihx) * q *
+                                               (mu0 + mu(-1, 0, 0) + mu(0, 0, 1) + mu(-1, 0, 1));
+        f0 += (sxz_901 - sxz_909()) * inv_hz;
+    }
+    if (only < 0 || only == 1) {
+        const T sxy_190 = ((UU(1, 0, 0) - UU(1, -1, 0)) * ihy + (VV(1, 0, 0) - v0) * ihx) * q * (mu0 + mu(1, 0, 0) + mu(0, -1, 0) + mu(1, 1, 0));
+        f1 += (sxy_190 - sxy_990()) * ihx;
+        const T syy_000 = T(2) * (VV(0, 1, 0) - v0) * ihy * mu0;
+        const T syy_080 = T(2) * (v0 - VV(0, -1, 0)) * ihy * mu(0, -1, 0);
+        f1 += (syy_000 - syy_080) * ihy;
+        const T syz_091 = (iz == NZ - 1) ? (T(-2) * v0 * inv_hz * visc)
+                                         : ((VV(0, 0, 1) - v0) * inv_hz_up + (WW(0, 0, 1) - WW(0, -1, 1)) * ihy) * q *
+                                               (mu0 + mu(0, -1, 0) + mu(0, 0, 1) + mu(0, -1, 1));
+        f1 += (syz_091 - syz_099()) * inv_hz;
+    }
+    if (iz != 0 && (only < 0 || only == 2)) {
         const T sxz_109 = ((UU(1, 0, 0) - UU(1, 0, -1)) * inv_hz_down + (WW(1, 0, 0) - w0) * ihx) * q *
                           (mu(0, 0, -1) + mu(1, 0, -1) + mu0 + mu(1, 0, 0));
-        f2 += (sxz_109 - sxz_909) * ihx;
+        f2 += (sxz_109 - sxz_909()) * ihx;
         const T syz_019 = ((VV(0, 1, 0) - VV(0, 1, -1)) * inv_hz_down + (WW(0, 1, 0) - w0) * ihy) * q *
                           (mu(0, 0, -1) + mu(0, 1, -1) + mu0 + mu(0, 1, 0));
-        f2 += (syz_019 - syz_099) * ihy;
+        f2 += (syz_019 - syz_099()) * ihy;
         const T szz_000 = T(2) * ((iz == NZ - 1 ? T(0) : WW(0, 0, 1)) - w0) * inv_hz * mu0;
         const T szz_008 = T(2) * (w0 - WW(0, 0, -1)) * mu(0, 0, -1) / m.DZ[iz - 1];
         f2 += (szz_000 - szz_008) * inv_hz_down;
@@ -381,13 +399,16 @@ __device__ __forceinline__ void visc_galerkin(const Mesh<T>& m, const A& acc, co
 template <typename T, int R>
 struct NbVec {
     const T* __restrict__ v;
-    int x[2 * R + 1], y[2 * R + 1], base, nxy;
+    int x[2 * R + 1], y[2 * R + 1], iz, nz, nxy;
+    // z neighbours clamped to the grid: every load is valid, so the wall rules
+    // select among values loaded unconditionally (no branch around a load)
     __device__ __forceinline__ T operator()(int c, int dx, int dy, int dz) const {
-        return v[3 * (base + dz * nxy + y[dy + R] + x[dx + R]) + c];
+        const int zz = min(max(iz + dz, 0), nz - 1);
+        return v[3 * (zz * nxy + y[dy + R] + x[dx + R]) + c];
     }
 };
 template <typename T, int R>
-__device__ __forceinline__ NbVec<T, R> make_nbvec(const T* v, int ix, int iy, int iz, int nx, int ny) {
+__device__ __forceinline__ NbVec<T, R> make_nbvec(const T* v, int ix, int iy, int iz, int nx, int ny, int nz) {
     NbVec<T, R> a;
     a.v = v;
 #pragma unroll
@@ -399,7 +420,8 @@ __device__ __forceinline__ NbVec<T, R> make_nbvec(const T* v, int ix, int iy, in
         a.y[d + R] = yy * nx;
     }
     a.nxy = nx * ny;
-    a.base = iz * a.nxy;
+    a.iz = iz;
+    a.nz = nz;
     return a;
 }
 
@@ -410,24 +432,25 @@ __device__ __forceinline__ T div_cell(const Mesh<T>& m, const A& acc, int iz) {
     const int NZ = m.nz;
     if (SCHEME == TC_SCHEME_HW) {
         const T hz = m.DZ[iz];
+        const T w0 = acc(2, 0, 0, 0), wp = acc(2, 0, 0, 1);
         return (acc(0, 1, 0, 0) - acc(0, 0, 0, 0)) * m.inv_hx * hz + (acc(1, 0, 1, 0) - acc(1, 0, 0, 0)) * m.inv_hy * hz +
-               ((iz < NZ - 1) ? (acc(2, 0, 0, 1) - acc(2, 0, 0, 0)) : (-acc(2, 0, 0, 0)));
+               ((iz < NZ - 1) ? (wp - w0) : (-w0));
     } else if (SCHEME == TC_SCHEME_VV) {
         const T dz = m.DZ[iz], dzw = m.DZW[iz], c27 = T(1.0 / 27.0);
         const T dudx = m.inv_hx * (dz * (acc(0, 1, 0, 0) - acc(0, 0, 0, 0)) - c27 * dzw * (acc(0, 2, 0, 0) - acc(0, -1, 0, 0)));
         const T dvdy = m.inv_hy * (dz * (acc(1, 0, 1, 0) - acc(1, 0, 0, 0)) - c27 * dzw * (acc(1, 0, 2, 0) - acc(1, 0, -1, 0)));
-        const T w0 = acc(2, 0, 0, 0);
-        const T wr = (iz == NZ - 1) ? T(0) : acc(2, 0, 0, 1);
-        const T wrr = (iz == NZ - 1) ? w0 : (iz == NZ - 2) ? T(0) : acc(2, 0, 0, 2);
-        const T wll = (iz == 0) ? acc(2, 0, 0, 1) : (iz == 1) ? T(0) : acc(2, 0, 0, -1);
+        const T w0 = acc(2, 0, 0, 0), wp1 = acc(2, 0, 0, 1), wp2 = acc(2, 0, 0, 2), wm1 = acc(2, 0, 0, -1);
+        const T wr = (iz == NZ - 1) ? T(0) : wp1;
+        const T wrr = (iz == NZ - 1) ? w0 : (iz == NZ - 2) ? T(0) : wp2;
+        const T wll = (iz == 0) ? wp1 : (iz == 1) ? T(0) : wm1;
         return dudx + dvdy + (wr - w0) - c27 * (wrr - wll);
     } else {
         const T hz = m.cellh[iz];
         T d = T(0.5) * (acc(0, 1, 0, 0) - acc(0, -1, 0, 0)) * m.hy * hz;
         d += T(0.5) * (acc(1, 0, 1, 0) - acc(1, 0, -1, 0)) * m.hx * hz;
-        if (iz == 0) d += T(0.5) * (acc(2, 0, 0, 1) - acc(2, 0, 0, 0)) * m.hx * m.hy;
-        else if (iz == NZ - 1) d += T(0.5) * (acc(2, 0, 0, 0) - acc(2, 0, 0, -1)) * m.hx * m.hy;
-        else d += T(0.5) * (acc(2, 0, 0, 1) - acc(2, 0, 0, -1)) * m.hx * m.hy;
+        const T w0 = acc(2, 0, 0, 0), wp = acc(2, 0, 0, 1), wm = acc(2, 0, 0, -1);
+        const T dw = iz == 0 ? (wp - w0) : iz == NZ - 1 ? (w0 - wm) : (wp - wm);
+        d += T(0.5) * dw * m.hx * m.hy;
         return d;
     }
 }
