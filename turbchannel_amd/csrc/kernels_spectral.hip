@@ -66,6 +66,11 @@ __device__ __forceinline__ void bfly(Cplx<T>* v, T s) {
     }
 }
 
+// LDS element index with one pad element every 16: the early Stockham passes
+// write with a stride of R elements, which would otherwise fall on few banks.
+__device__ __forceinline__ int ldsi(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr size_t lds_elems(size_t n) { return n + (n >> 4) + 1; }
+
 // One Stockham pass of radix R over nlines lines of length N (Ns = product of
 // the radices already applied). Element e of line l lives at
 // buf[l*lstride + e*estride]; tw[k] = exp(-2 pi i k/N). Butterflies are
@@ -87,23 +92,22 @@ __device__ __forceinline__ void stockham_pass(const Cplx<T>* __restrict__ src, C
             l = t - j * nlines;
         }
         Cplx<T> v[R];
-        const Cplx<T>* sp = src + l * lstride;
+        const int l0 = l * lstride;
 #pragma unroll
-        for (int r = 0; r < R; r++) v[r] = sp[(j + r * NR) * estride];
+        for (int r = 0; r < R; r++) v[r] = src[ldsi(l0 + (j + r * NR) * estride)];
         const int jm = j % Ns;
         if (jm != 0) {
 #pragma unroll
             for (int r = 1; r < R; r++) {
-                Cplx<T> w = tw[r * jm * tstep];
+                Cplx<T> w = tw[ldsi(r * jm * tstep)];
                 w.y *= -s;  // forward table; conjugate for the inverse
                 v[r] = cmul(v[r], w);
             }
         }
         bfly<T, R>(v, s);
         const int o = (j - jm) * R + jm;
-        Cplx<T>* dp = dst + l * lstride;
 #pragma unroll
-        for (int r = 0; r < R; r++) dp[(o + r * Ns) * estride] = v[r];
+        for (int r = 0; r < R; r++) dst[ldsi(l0 + (o + r * Ns) * estride)] = v[r];
     }
 }
 
@@ -149,8 +153,8 @@ struct TwStage {
         // unconditional (surplus lanes store the same last entry), so that the
         // loads are not sunk into a branch
 #pragma unroll
-        for (int u = 0; u < kTwRegs; u++) twl[min((int)threadIdx.x + u * kFftThreads, N - 1)] = r[u];
-        for (int i = threadIdx.x + kTwRegs * kFftThreads; i < N; i += kFftThreads) twl[i] = tw[i];
+        for (int u = 0; u < kTwRegs; u++) twl[ldsi(min((int)threadIdx.x + u * kFftThreads, N - 1))] = r[u];
+        for (int i = threadIdx.x + kTwRegs * kFftThreads; i < N; i += kFftThreads) twl[ldsi(i)] = tw[i];
     }
 };
 
@@ -164,8 +168,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n, nxh = N / 2 + 1, nrows = m.ny * m.nz;
-    Cplx<T>* b1 = b0 + lpb * N;
-    Cplx<T>* twl = b1 + lpb * N;
+    Cplx<T>* b1 = b0 + lds_elems(lpb * N);
+    Cplx<T>* twl = b1 + lds_elems(lpb * N);
     const int q0 = blockIdx.x * lpb;
     constexpr int R = SCHEME == TC_SCHEME_VV ? 2 : 1;
     constexpr int U = SCHEME == TC_SCHEME_VV ? 2 : 4;
@@ -188,7 +192,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
             const int t = t0 + u * kFftThreads;
             if (t < tot) {
                 const int ra = 2 * (q0 + t / N);
-                b0[t] = {ra < nrows ? zx[u] : T(0), ra + 1 < nrows ? zy[u] : T(0)};
+                b0[ldsi(t)] = {ra < nrows ? zx[u] : T(0), ra + 1 < nrows ? zy[u] : T(0)};
             }
         }
     }
@@ -199,8 +203,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
         const int l = t / nxh, k = t - l * nxh;
         const int ra = 2 * (q0 + l), rb = ra + 1;
         if (ra >= nrows) continue;
-        const Cplx<T> z = res[l * N + k];
-        const Cplx<T> zc = res[l * N + (k == 0 ? 0 : N - k)];
+        const Cplx<T> z = res[ldsi(l * N + k)];
+        const Cplx<T> zc = res[ldsi(l * N + (k == 0 ? 0 : N - k))];
         // A = (Z + conj Zc)/2, B = -i (Z - conj Zc)/2
         spec[(size_t)ra * nxh + k] = {T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
         if (rb < nrows) spec[(size_t)rb * nxh + k] = {T(0.5) * (z.y + zc.y), T(-0.5) * (z.x - zc.x)};
@@ -215,8 +219,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n, nxh = N / 2 + 1, half = N / 2;
-    Cplx<T>* b1 = b0 + lpb * N;
-    Cplx<T>* twl = b1 + lpb * N;
+    Cplx<T>* b1 = b0 + lds_elems(lpb * N);
+    Cplx<T>* twl = b1 + lds_elems(lpb * N);
     const int q0 = blockIdx.x * lpb;
     constexpr int U = 4;
     TwStage<T> tws;
@@ -250,7 +254,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
                 a.y = -a.y;
                 b.y = -b.y;
             }
-            b0[t] = {a.x - b.y, a.y + b.x};  // A + iB
+            b0[ldsi(t)] = {a.x - b.y, a.y + b.x};  // A + iB
         }
     }
     tws.store(twl, tw, N);
@@ -259,8 +263,9 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
     for (int t = threadIdx.x; t < lpb * N; t += blockDim.x) {
         const int l = t / N, n = t - l * N;
         const int ra = 2 * (q0 + l), rb = ra + 1;
-        if (ra < nrows) out[(size_t)ra * N + n] = res[t].x * scale;
-        if (rb < nrows) out[(size_t)rb * N + n] = res[t].y * scale;
+        const Cplx<T> z = res[ldsi(t)];
+        if (ra < nrows) out[(size_t)ra * N + n] = z.x * scale;
+        if (rb < nrows) out[(size_t)rb * N + n] = z.y * scale;
     }
 }
 
@@ -274,8 +279,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n;
-    Cplx<T>* b1 = b0 + tk * N;
-    Cplx<T>* twl = b1 + tk * N;
+    Cplx<T>* b1 = b0 + lds_elems(tk * N);
+    Cplx<T>* twl = b1 + lds_elems(tk * N);
     const int kx0 = blockIdx.x * tk, iz = blockIdx.y;
     const int ncol = min(tk, nxh - kx0);
     Cplx<T>* base = spec + (size_t)iz * N * nxh + kx0;
@@ -298,7 +303,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
             const int n = t / tk, k = t - n * tk;
             Cplx<T> w = k < ncol ? v[u] : Cplx<T>{T(0), T(0)};
             if (psum && t == 0 && kx0 == 0) w.x = T((double)w.x - psum[0] * C[iz]);
-            b0[t] = w;
+            b0[ldsi(t)] = w;
         }
     }
     tws.store(twl, tw, N);
@@ -306,7 +311,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
     const Cplx<T>* res = lds_fft(b0, b1, pl, tk, 1, tk, twl, s);
     for (int t = threadIdx.x; t < tk * N; t += blockDim.x) {
         const int n = t / tk, k = t - n * tk;
-        if (k < ncol) base[(size_t)n * nxh + k] = res[t];
+        if (k < ncol) base[(size_t)n * nxh + k] = res[ldsi(t)];
     }
 }
 
@@ -488,8 +493,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_q(FFTPlan pl, const C
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n, nxh = N / 2 + 1;
-    Cplx<T>* b1 = b0 + lpb * N;
-    Cplx<T>* twl = b1 + lpb * N;
+    Cplx<T>* b1 = b0 + lds_elems(lpb * N);
+    Cplx<T>* twl = b1 + lds_elems(lpb * N);
     const int q0 = blockIdx.x * lpb;
     auto q = [&](int row, int n) {
         const T* c = v + 3 * ((size_t)row * N + n);
@@ -514,7 +519,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_q(FFTPlan pl, const C
             const int t = t0 + u * kFftThreads;
             if (t < tot) {
                 const int ra = 2 * (q0 + t / N);
-                b0[t] = {ra < nrows ? zx[u] : T(0), ra + 1 < nrows ? zy[u] : T(0)};
+                b0[ldsi(t)] = {ra < nrows ? zx[u] : T(0), ra + 1 < nrows ? zy[u] : T(0)};
             }
         }
     }
@@ -525,8 +530,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_q(FFTPlan pl, const C
         const int l = t / nxh, k = t - l * nxh;
         const int ra = 2 * (q0 + l), rb = ra + 1;
         if (ra >= nrows) continue;
-        const Cplx<T> z = res[l * N + k];
-        const Cplx<T> zc = res[l * N + (k == 0 ? 0 : N - k)];
+        const Cplx<T> z = res[ldsi(l * N + k)];
+        const Cplx<T> zc = res[ldsi(l * N + (k == 0 ? 0 : N - k))];
         spec[(size_t)ra * nxh + k] = {T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
         if (rb < nrows) spec[(size_t)rb * nxh + k] = {T(0.5) * (z.y + zc.y), T(-0.5) * (z.x - zc.x)};
     }
@@ -567,7 +572,7 @@ hipError_t launch_fft_x_fwd_div(hipStream_t s, const FFTPlan& pl, const Cplx<T>*
     const int lpb = lines_per_block<T>(pl.n);
     const int nlines = (m.ny * m.nz + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
-    const size_t sm = (2 * (size_t)lpb + 1) * pl.n * sizeof(Cplx<T>);
+    const size_t sm = (2 * lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
     if (scheme == TC_SCHEME_HW)
         hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_HW>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb);
     else if (scheme == TC_SCHEME_VV)
@@ -583,7 +588,7 @@ hipError_t launch_fft_x_inv(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw,
     const int lpb = lines_per_block<T>(pl.n);
     const int nlines = (nrows + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
-    const size_t sm = (2 * (size_t)lpb + 1) * pl.n * sizeof(Cplx<T>);
+    const size_t sm = (2 * lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
     hipLaunchKernelGGL((k_fft_x_inv<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, spec, out, nrows, lpb, scale);
     return hipGetLastError();
 }
@@ -595,7 +600,7 @@ hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cpl
     int tk = 128 / (int)sizeof(Cplx<T>);
     while (tk > 1 && 2 * (size_t)tk * pl.n * sizeof(Cplx<T>) > 48 * 1024) tk >>= 1;
     const dim3 grid((nxh + tk - 1) / tk, nz);
-    const size_t sm = (2 * (size_t)tk + 1) * pl.n * sizeof(Cplx<T>);
+    const size_t sm = (2 * lds_elems((size_t)tk * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
     hipLaunchKernelGGL((k_fft_y<T>), grid, dim3(kFftThreads), sm, s, pl, tw, spec, nxh, tk, T(inverse ? 1 : -1), psum, C);
     return hipGetLastError();
 }
@@ -620,7 +625,7 @@ hipError_t launch_fft_x_fwd_q(hipStream_t s, const FFTPlan& pl, const Cplx<T>* t
     const int lpb = lines_per_block<T>(pl.n);
     const int nlines = (nrows + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
-    const size_t sm = (2 * (size_t)lpb + 1) * pl.n * sizeof(Cplx<T>);
+    const size_t sm = (2 * lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
     hipLaunchKernelGGL((k_fft_x_fwd_q<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, v, ca, cb, nrows, spec, lpb);
     return hipGetLastError();
 }

@@ -447,14 +447,27 @@ int flux_slots(size_t dyn_lds) {
     }
     return slots;
 }
-// z-chunks so that the grid fills whole rounds of resident blocks.
-inline FluxGrid flux_grid(int nx, int ny, int nz, int ty, int slots) {
+// z-chunks: the count whose grid, in whole rounds of resident blocks, takes the
+// fewest plane-steps, each block paying its chunk plus the ring's start-up
+// planes (rounds x (zchunk + halo)); ties go to fewer chunks.
+inline FluxGrid flux_grid(int nx, int ny, int nz, int ty, int slots, int halo) {
     FluxGrid g;
     g.ntx = (nx + kTX - 1) / kTX;
     g.ntiles = g.ntx * ((ny + ty - 1) / ty);
-    int nch = std::max(1, slots / g.ntiles);
-    nch = std::min(nch, std::max(1, nz / 4));
-    g.zchunk = (nz + nch - 1) / nch;
+    const int max_nch = std::max(1, nz / 4);
+    long best = -1;
+    int best_nch = 1;
+    for (int nch = 1; nch <= max_nch; nch++) {
+        const int zc = (nz + nch - 1) / nch;
+        const long blocks = (long)g.ntiles * ((nz + zc - 1) / zc);
+        const long rounds = (blocks + slots - 1) / slots;
+        const long cost = rounds * (zc + halo);
+        if (best < 0 || cost < best) {
+            best = cost;
+            best_nch = nch;
+        }
+    }
+    g.zchunk = (nz + best_nch - 1) / best_nch;
     g.nblocks = g.ntiles * ((nz + g.zchunk - 1) / g.zchunk);
     return g;
 }
@@ -579,16 +592,22 @@ __global__ __launch_bounds__(kBlock) void k_project(Mesh<T> m, ProjectArgs<T> a)
     Nb nb;
     make_nb<2>(nb, ix, iy, iz, m.nx, m.ny);
     const int in = nb.at(0, 0, 0);
+    // every read before the first write (the output may alias an input, so a
+    // read after a write would wait for that write)
+    const T* r = a.r + 3 * in;
+    const T r0 = r[0], r1 = r[1], r2 = r[2];
+    const T sh = a.shift ? T(*a.shift) : T(0);
+    T pn = T(0);
+    if (STAB) pn = a.p[in] + a.pcur[in];
     T g[3];
     grad_at<T, SCHEME>(m, a.p, nb, iz, g);
-    const T* r = a.r + 3 * in;
     T* o = a.vnext + 3 * in;
-    T u = (r[0] - g[0]) * a.tau_over_bd0;
-    if (a.shift && !(SCHEME == TC_SCHEME_AB3 && (iz == 0 || iz == m.nz - 1))) u += T(*a.shift);
+    T u = (r0 - g[0]) * a.tau_over_bd0;
+    if (a.shift && !(SCHEME == TC_SCHEME_AB3 && (iz == 0 || iz == m.nz - 1))) u += sh;
     o[0] = u;
-    o[1] = (r[1] - g[1]) * a.tau_over_bd0;
-    o[2] = (r[2] - g[2]) * a.tau_over_bd0;
-    if (STAB) a.pnext[in] = a.p[in] + a.pcur[in];
+    o[1] = (r1 - g[1]) * a.tau_over_bd0;
+    o[2] = (r2 - g[2]) * a.tau_over_bd0;
+    if (STAB) a.pnext[in] = pn;
 }
 
 template <typename T, int SCHEME>
@@ -628,7 +647,7 @@ hipError_t launch_flux(hipStream_t s, const Mesh<T>& m, const FluxArgs<T>& a, in
     {                                                                                                          \
         constexpr int ty = flux_ty<T, SCH, VIS>();                                                             \
         const size_t lds = 4 * (size_t)m.nz * sizeof(T);                                                     \
-        const FluxGrid fg = flux_grid(m.nx, m.ny, m.nz, ty, flux_slots<T, SCH, VIS, TRB>(lds));               \
+        const FluxGrid fg = flux_grid(m.nx, m.ny, m.nz, ty, flux_slots<T, SCH, VIS, TRB>(lds), SCH == TC_SCHEME_VV ? 5 : 3);               \
         if (nblocks) *nblocks = fg.nblocks;                                                                    \
         hipLaunchKernelGGL((k_flux<T, SCH, VIS, TRB>), dim3(fg.nblocks), dim3(flux_threads<T, SCH, VIS>()), lds, s, m, a, fg.ntx, \
                            fg.ntiles, fg.zchunk);                                                              \
