@@ -1,8 +1,10 @@
-"""Build the HIP engine in-tree: turbchannel_amd/libturbchannel_amd.so (gfx950).
+"""Build the HIP engine in-tree: turbchannel_amd/libturbchannel_amd.so (gfx950),
+and the command-line solver turbchannel_amd/turbch that drives it.
 
 Every source is compiled by hipcc for gfx950 only, with packed FP32
 instructions disabled, then linked into one shared library whose exported
-symbols are the C ABI of include/turbchannel.h.
+symbols are the C ABI of include/turbchannel.h. The driver is host C++ that
+uses only that C ABI; it finds the library next to itself.
 """
 import concurrent.futures
 import os
@@ -15,6 +17,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libturbchannel_amd.so")
+DRIVER = os.path.join(HERE, "turbch")
+DRIVER_SRC = os.path.join(HERE, "driver", "turbch.cpp")
 SOURCES = ["host_setup.cpp", "kernels_stencil.hip", "kernels_spectral.hip", "kernels_reduce.hip", "kernels_mlp.hip",
            "engine.cpp", "mlp.cpp"]
 
@@ -53,9 +57,25 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"linking failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_driver()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", DRIVER)
     return LIB
+
+
+def build_driver():
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if not cxx:
+        raise RuntimeError("no host C++ compiler for the driver")
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    tmp = DRIVER + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), DRIVER_SRC,
+           "-o", tmp, "-L", HERE, "-lturbchannel_amd", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath-link,{rocm_lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"building the driver failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":
