@@ -129,11 +129,16 @@ __device__ __forceinline__ int xcd_swizzle(int b, int nb) {
     return x * q + min(x, r) + i;
 }
 
+// nu_t at an offset; an offset past a wall reads the wall plane (the stencils
+// read such values unconditionally and discard them).
 template <typename T>
 struct GlobalNu {
     const T* nut;
     const Nb* nb;
-    __device__ __forceinline__ T operator()(int dx, int dy, int dz) const { return nut[nb->at(dx, dy, dz)]; }
+    int iz, nz;
+    __device__ __forceinline__ T operator()(int dx, int dy, int dz) const {
+        return nut[nb->at(dx, dy, min(max(iz + dz, 0), nz - 1) - iz)];
+    }
 };
 
 // The stencils reach beyond the ring in z only on the cell's own column
@@ -355,7 +360,7 @@ __global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>())) void k_flux(
 #pragma unroll
         for (int d = -RZ; d <= RZ; d++) acc.pl[d + RZ] = ring + (((iz + d) % NS + NS) % NS) * SLOT;
         acc.off = (ty + RXY) * W + tx + RXY;
-        GlobalNu<T> nu{a.nut, &nb};
+        GlobalNu<T> nu{a.nut, &nb, iz, m.nz};
         T k[3] = {0, 0, 0};
         if (SCHEME == TC_SCHEME_AB3) {
             if (iz != 0 && iz != m.nz - 1) {

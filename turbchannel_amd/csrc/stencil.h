@@ -14,35 +14,49 @@ namespace tc {
 #define VV(dx, dy, dz) acc(1, dx, dy, dz)
 #define WW(dx, dy, dz) acc(2, dx, dy, dz)
 
+// The near-wall rules below read every operand unconditionally and then select:
+// the plane index is uniform across a block, so a conditional read would become
+// a branch with its own wait for the LDS or global load, once per rule and plane.
+// Offsets past a wall read a stale or clamped value that the select discards;
+// mesh array indices are clamped into the array.
+__device__ __forceinline__ int zcl(int i, int nz) { return min(max(i, 0), nz - 1); }
+
 // ---------------------------------------------------------------------------
 // Convection, Harlow-Welch (staggered/func_global.h:345-370)
 // ---------------------------------------------------------------------------
 template <typename T, class A>
 __device__ __forceinline__ void conv_hw(const Mesh<T>& m, const A& acc, int iz, T f[3]) {
     const int NZ = m.nz;
-    const T dz = m.DZ[iz], dzm = iz == 0 ? T(0) : m.DZ[iz - 1];
+    const bool bot = iz == 0, top = iz == NZ - 1;
+    const T dz = m.DZ[iz], dzm = bot ? T(0) : m.DZ[zcl(iz - 1, NZ)];
     const T inv_hz = T(1) / dz;
     const T inv_hz_alt = T(2) / (dz + dzm);
     const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
+    const T up = UU(0, 0, 1), um = UU(0, 0, -1), vp = VV(0, 0, 1), vm = VV(0, 0, -1);
+    const T wp = WW(0, 0, 1), wm = WW(0, 0, -1), wxp = WW(-1, 0, 1), wyp = WW(0, -1, 1);
     T f0 = 0, f1 = 0, f2 = 0;
     f0 -= ((u0 + UU(1, 0, 0)) * (u0 + UU(1, 0, 0)) - (u0 + UU(-1, 0, 0)) * (u0 + UU(-1, 0, 0))) * m.inv_hx;
     f0 -= ((u0 + UU(0, 1, 0)) * (VV(0, 1, 0) + VV(-1, 1, 0)) - (u0 + UU(0, -1, 0)) * (v0 + VV(-1, 0, 0))) * m.inv_hy;
-    if (iz != NZ - 1) f0 -= (u0 + UU(0, 0, 1)) * (WW(0, 0, 1) + WW(-1, 0, 1)) * inv_hz;
-    if (iz != 0) f0 += (u0 + UU(0, 0, -1)) * (w0 + WW(-1, 0, 0)) * inv_hz;
+    const T f0t = f0 - (u0 + up) * (wp + wxp) * inv_hz;
+    f0 = top ? f0 : f0t;
+    const T f0b = f0 + (u0 + um) * (w0 + WW(-1, 0, 0)) * inv_hz;
+    f0 = bot ? f0 : f0b;
 
     f1 -= ((v0 + VV(1, 0, 0)) * (UU(1, 0, 0) + UU(1, -1, 0)) - (v0 + VV(-1, 0, 0)) * (u0 + UU(0, -1, 0))) * m.inv_hx;
     f1 -= ((v0 + VV(0, 1, 0)) * (v0 + VV(0, 1, 0)) - (v0 + VV(0, -1, 0)) * (v0 + VV(0, -1, 0))) * m.inv_hy;
-    if (iz != NZ - 1) f1 -= (v0 + VV(0, 0, 1)) * (WW(0, 0, 1) + WW(0, -1, 1)) * inv_hz;
-    if (iz != 0) f1 += (v0 + VV(0, 0, -1)) * (w0 + WW(0, -1, 0)) * inv_hz;
+    const T f1t = f1 - (v0 + vp) * (wp + wyp) * inv_hz;
+    f1 = top ? f1 : f1t;
+    const T f1b = f1 + (v0 + vm) * (w0 + WW(0, -1, 0)) * inv_hz;
+    f1 = bot ? f1 : f1b;
 
-    if (iz != 0) {
-        f2 -= ((w0 + WW(1, 0, 0)) * (UU(1, 0, 0) * dz + UU(1, 0, -1) * dzm) - (w0 + WW(-1, 0, 0)) * (u0 * dz + UU(0, 0, -1) * dzm)) *
+    if (!bot) {
+        f2 -= ((w0 + WW(1, 0, 0)) * (UU(1, 0, 0) * dz + UU(1, 0, -1) * dzm) - (w0 + WW(-1, 0, 0)) * (u0 * dz + um * dzm)) *
               m.inv_hx * inv_hz_alt;
-        f2 -= ((w0 + WW(0, 1, 0)) * (VV(0, 1, 0) * dz + VV(0, 1, -1) * dzm) - (w0 + WW(0, -1, 0)) * (v0 * dz + VV(0, 0, -1) * dzm)) *
+        f2 -= ((w0 + WW(0, 1, 0)) * (VV(0, 1, 0) * dz + VV(0, 1, -1) * dzm) - (w0 + WW(0, -1, 0)) * (v0 * dz + vm * dzm)) *
               m.inv_hy * inv_hz_alt;
-        const T wr = iz == NZ - 1 ? T(0) : WW(0, 0, 1);
+        const T wr = top ? T(0) : wp;
         f2 -= (w0 + wr) * (w0 + wr) * inv_hz_alt;
-        f2 += (w0 + WW(0, 0, -1)) * (w0 + WW(0, 0, -1)) * inv_hz_alt;
+        f2 += (w0 + wm) * (w0 + wm) * inv_hz_alt;
     }
     f[0] = f0 * T(0.25);
     f[1] = f1 * T(0.25);
@@ -64,9 +78,13 @@ __device__ __forceinline__ T zbar_vv(const Mesh<T>& m, const AC& acc, int c, int
     // 9*(F(0)*A[iz] + F(-1)*A[iz-1]) - F(-2)*A[iz-2] - F(+1)*A[iz+1] with the wall rules
     const int NZ = m.nz;
     const T f0 = F3(vel, c, dx, dy, 0), fm = F3(vel, c, dx, dy, -1);
+    const T fmm = F3(vel, c, dx, dy, -2), fp = F3(vel, c, dx, dy, 1);
+    const T amm = A[zcl(iz - 2, NZ)], ap = A[zcl(iz + 1, NZ)], a_first = A[0], a_last = A[NZ - 1];
     T r = T(9) * (f0 * A[iz] + fm * A[iz - 1]);
-    r -= (iz == 1) ? -fm * A[0] : F3(vel, c, dx, dy, -2) * A[iz - 2];
-    r -= (iz == NZ - 1) ? -f0 * A[NZ - 1] : F3(vel, c, dx, dy, 1) * A[iz + 1];
+    const T lo = fmm * amm, lo_wall = -fm * a_first;
+    r -= (iz == 1) ? lo_wall : lo;
+    const T hi = fp * ap, hi_wall = -f0 * a_last;
+    r -= (iz == NZ - 1) ? hi_wall : hi;
     return r;
 }
 
@@ -96,14 +114,18 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
         f0 += br * (u0 + UU(0, 1, 0)) - C27 * bbr * (u0 + UU(0, 3, 0)) - bl * (UU(0, -1, 0) + u0) + C27 * bbl * (UU(0, -3, 0) + u0);
     }
     {
-        const T bl = iz == 0 ? T(0) : hxy * XBAR(WW, 0, 0);
-        const T br = iz == NZ - 1 ? T(0) : hxy * XBAR(WW, 0, 1);
-        const T bbl = iz == 1 ? T(0) : iz == 0 ? br : hxy * XBAR(WW, 0, -1);
-        const T bbr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? bl : hxy * XBAR(WW, 0, 2);
-        const T q_r = iz == NZ - 1 ? T(0) : u0 + UU(0, 0, 1);
-        const T q_rr = u0 + (iz == NZ - 3 ? -UU(0, 0, 2) : iz == NZ - 2 ? T(0) : iz == NZ - 1 ? UU(0, 0, -2) : UU(0, 0, 3));
-        const T q_l = iz == 0 ? T(0) : UU(0, 0, -1) + u0;
-        const T q_ll = u0 + (iz == 2 ? -UU(0, 0, -2) : iz == 1 ? T(0) : iz == 0 ? UU(0, 0, 2) : UU(0, 0, -3));
+        const T x0 = hxy * XBAR(WW, 0, 0), x1 = hxy * XBAR(WW, 0, 1);
+        const T xm1 = hxy * XBAR(WW, 0, -1), x2 = hxy * XBAR(WW, 0, 2);
+        const T p1 = UU(0, 0, 1), p2 = UU(0, 0, 2), p3 = UU(0, 0, 3);
+        const T m1 = UU(0, 0, -1), m2 = UU(0, 0, -2), m3 = UU(0, 0, -3);
+        const T bl = iz == 0 ? T(0) : x0;
+        const T br = iz == NZ - 1 ? T(0) : x1;
+        const T bbl = iz == 1 ? T(0) : iz == 0 ? br : xm1;
+        const T bbr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? bl : x2;
+        const T q_r = iz == NZ - 1 ? T(0) : u0 + p1;
+        const T q_rr = u0 + (iz == NZ - 3 ? -p2 : iz == NZ - 2 ? T(0) : iz == NZ - 1 ? m2 : p3);
+        const T q_l = iz == 0 ? T(0) : m1 + u0;
+        const T q_ll = u0 + (iz == 2 ? -m2 : iz == 1 ? T(0) : iz == 0 ? p2 : m3);
         f0 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
     }
     }
@@ -125,14 +147,18 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
         f1 += br * (v0 + VV(0, 1, 0)) - C27 * bbr * (v0 + VV(0, 3, 0)) - bl * (VV(0, -1, 0) + v0) + C27 * bbl * (VV(0, -3, 0) + v0);
     }
     {
-        const T bl = iz == 0 ? T(0) : hxy * YBAR(WW, 0, 0);
-        const T br = iz == NZ - 1 ? T(0) : hxy * YBAR(WW, 0, 1);
-        const T bbl = iz == 1 ? T(0) : iz == 0 ? br : hxy * YBAR(WW, 0, -1);
-        const T bbr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? bl : hxy * YBAR(WW, 0, 2);
-        const T q_r = iz == NZ - 1 ? T(0) : v0 + VV(0, 0, 1);
-        const T q_rr = v0 + (iz == NZ - 3 ? -VV(0, 0, 2) : iz == NZ - 2 ? T(0) : iz == NZ - 1 ? VV(0, 0, -2) : VV(0, 0, 3));
-        const T q_l = iz == 0 ? T(0) : VV(0, 0, -1) + v0;
-        const T q_ll = v0 + (iz == 2 ? -VV(0, 0, -2) : iz == 1 ? T(0) : iz == 0 ? VV(0, 0, 2) : VV(0, 0, -3));
+        const T y0 = hxy * YBAR(WW, 0, 0), y1 = hxy * YBAR(WW, 0, 1);
+        const T ym1 = hxy * YBAR(WW, 0, -1), y2 = hxy * YBAR(WW, 0, 2);
+        const T p1 = VV(0, 0, 1), p2 = VV(0, 0, 2), p3 = VV(0, 0, 3);
+        const T m1 = VV(0, 0, -1), m2 = VV(0, 0, -2), m3 = VV(0, 0, -3);
+        const T bl = iz == 0 ? T(0) : y0;
+        const T br = iz == NZ - 1 ? T(0) : y1;
+        const T bbl = iz == 1 ? T(0) : iz == 0 ? br : ym1;
+        const T bbr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? bl : y2;
+        const T q_r = iz == NZ - 1 ? T(0) : v0 + p1;
+        const T q_rr = v0 + (iz == NZ - 3 ? -p2 : iz == NZ - 2 ? T(0) : iz == NZ - 1 ? m2 : p3);
+        const T q_l = iz == 0 ? T(0) : m1 + v0;
+        const T q_ll = v0 + (iz == 2 ? -m2 : iz == 1 ? T(0) : iz == 0 ? p2 : m3);
         f1 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
     }
     }
@@ -156,22 +182,22 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
         }
         {
             const T wzl = WW(0, 0, -1);
-            const T wzr = iz == NZ - 1 ? T(0) : WW(0, 0, 1);
-            const T wzll = iz == 1 ? w0 : WW(0, 0, -2);
-            const T wzrr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? w0 : WW(0, 0, 2);
-            const T wzlll = iz == 1 ? WW(0, 0, 1) : iz == 2 ? WW(0, 0, -1) : WW(0, 0, -3);
-            const T wzrrr = iz == NZ - 3 ? T(0) : iz == NZ - 2 ? WW(0, 0, 1) : iz == NZ - 1 ? WW(0, 0, -1) : WW(0, 0, 3);
+            const T p1 = WW(0, 0, 1), p2 = WW(0, 0, 2), p3 = WW(0, 0, 3);
+            const T m2 = WW(0, 0, -2), m3 = WW(0, 0, -3);
+            const T wzr = iz == NZ - 1 ? T(0) : p1;
+            const T wzll = iz == 1 ? w0 : m2;
+            const T wzrr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? w0 : p2;
+            const T wzlll = iz == 1 ? p1 : iz == 2 ? wzl : m3;
+            const T wzrrr = iz == NZ - 3 ? T(0) : iz == NZ - 2 ? p1 : iz == NZ - 1 ? wzl : p3;
             const T bl = hxy * (T(9) * (wzl + w0) - (wzll + wzr));
             const T br = hxy * (T(9) * (w0 + wzr) - (wzl + wzrr));
             const T bbl = hxy * (T(9) * (wzll + wzl) - (wzlll + w0));
             const T bbr = hxy * (T(9) * (wzr + wzrr) - (w0 + wzrrr));
             f2 += br * (w0 + wzr) - bl * (wzl + w0);
-            if (iz < NZ - 2) f2 -= C27 * bbr * (w0 + wzrrr);
-            else if (iz == NZ - 2) f2 -= C27 * bbr * (-WW(0, 0, 1) + w0);
-            else f2 -= C27 * bbr * (WW(0, 0, -1) + w0);
-            if (iz > 2) f2 += C27 * bbl * (wzlll + w0);
-            else if (iz == 2) f2 += C27 * bbl * (-WW(0, 0, -1) + w0);
-            else f2 += C27 * bbl * (WW(0, 0, 1) + w0);
+            const T qr = iz < NZ - 2 ? (w0 + wzrrr) : iz == NZ - 2 ? (-p1 + w0) : (wzl + w0);
+            f2 -= C27 * bbr * qr;
+            const T ql = iz > 2 ? (wzlll + w0) : iz == 2 ? (-wzl + w0) : (p1 + w0);
+            f2 += C27 * bbl * ql;
         }
     }
     const T minus_inv_vol = T(-1) / (T(32) * (dz - T(1.0 / 9.0) * dzw) * hx * hy);
@@ -186,9 +212,10 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
 template <typename T, class A>
 __device__ __forceinline__ void visc_cross(const Mesh<T>& m, const A& acc, int iz, T f[3], int only = -1) {
     const int NZ = m.nz;
-    const T dz = m.DZ[iz];
+    const bool bot = iz == 0, top = iz == NZ - 1;
+    const T dz = m.DZ[iz], dzm = m.DZ[zcl(iz - 1, NZ)], dzp = m.DZ[zcl(iz + 1, NZ)];
     const T inv_hz = T(1) / dz;
-    const T inv_hz_alt = T(2) / (dz + (iz == 0 ? T(0) : m.DZ[iz - 1]));
+    const T inv_hz_alt = T(2) / (dz + (bot ? T(0) : dzm));
     const T ihx2 = m.inv_hx * m.inv_hx, ihy2 = m.inv_hy * m.inv_hy;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -201,24 +228,19 @@ __device__ __forceinline__ void visc_cross(const Mesh<T>& m, const A& acc, int i
         f[c] += ihy2 * (F3(vel, c, 0, 1, 0) + F3(vel, c, 0, -1, 0) - T(2) * a0);
     }
     const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
-    if (iz != NZ - 1) {
-        const T inv_hz_up = T(2) / (dz + m.DZ[iz + 1]);
-        f[0] += (UU(0, 0, 1) - u0) * inv_hz_up * inv_hz;
-        f[1] += (VV(0, 0, 1) - v0) * inv_hz_up * inv_hz;
-        f[2] += (WW(0, 0, 1) - w0) * inv_hz * inv_hz_alt;
-    } else {
-        const T inv_hz_up = T(2) / dz;
-        f[0] += (-u0) * inv_hz_up * inv_hz;
-        f[1] += (-v0) * inv_hz_up * inv_hz;
-        f[2] += (-w0) * inv_hz * inv_hz_alt;
+    const T up = UU(0, 0, 1), vp = VV(0, 0, 1), wp = WW(0, 0, 1);
+    const T um = UU(0, 0, -1), vm = VV(0, 0, -1), wm = WW(0, 0, -1);
+    {  // upper neighbour; past the top wall the no-slip ghost (-u) enters as 0 - u
+        const T inv_hz_up = T(2) / (dz + (top ? T(0) : dzp));
+        f[0] += ((top ? T(0) : up) - u0) * inv_hz_up * inv_hz;
+        f[1] += ((top ? T(0) : vp) - v0) * inv_hz_up * inv_hz;
+        f[2] += ((top ? T(0) : wp) - w0) * inv_hz * inv_hz_alt;
     }
-    if (iz != 0) {
-        f[0] += (UU(0, 0, -1) - u0) * inv_hz_alt * inv_hz;
-        f[1] += (VV(0, 0, -1) - v0) * inv_hz_alt * inv_hz;
-        f[2] += (WW(0, 0, -1) - w0) * (T(1) / m.DZ[iz - 1]) * inv_hz_alt;
-    } else {
-        f[0] += (-u0) * inv_hz_alt * inv_hz;
-        f[1] += (-v0) * inv_hz_alt * inv_hz;
+    {  // lower neighbour; at the bottom wall there is no w-equation
+        f[0] += ((bot ? T(0) : um) - u0) * inv_hz_alt * inv_hz;
+        f[1] += ((bot ? T(0) : vm) - v0) * inv_hz_alt * inv_hz;
+        const T f2b = f[2] + (wm - w0) * (T(1) / dzm) * inv_hz_alt;
+        f[2] = bot ? f[2] : f2b;
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) f[c] *= m.visc;
@@ -233,10 +255,11 @@ __device__ __forceinline__ void visc_naive(const Mesh<T>& m, const A& acc, const
     const int NZ = m.nz;
     const T visc = m.visc;
     auto mu = [&](int dx, int dy, int dz) -> T { return TURB ? visc + nu(dx, dy, dz) : visc; };
-    const T dz = m.DZ[iz];
+    const bool bot = iz == 0, top = iz == NZ - 1;
+    const T dz = m.DZ[iz], dzm = m.DZ[zcl(iz - 1, NZ)], dzp = m.DZ[zcl(iz + 1, NZ)];
     const T inv_hz = T(1) / dz;
-    const T inv_hz_up = T(2) / (dz + (iz == NZ - 1 ? T(0) : m.DZ[iz + 1]));
-    const T inv_hz_down = T(2) / (dz + (iz == 0 ? T(0) : m.DZ[iz - 1]));
+    const T inv_hz_up = T(2) / (dz + (top ? T(0) : dzp));
+    const T inv_hz_down = T(2) / (dz + (bot ? T(0) : dzm));
     const T ihx = m.inv_hx, ihy = m.inv_hy;
     const T q = T(0.25);
     const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
@@ -246,14 +269,16 @@ __device__ __forceinline__ void visc_naive(const Mesh<T>& m, const A& acc, const
         return ((u0 - UU(0, -1, 0)) * ihy + (v0 - VV(-1, 0, 0)) * ihx) * q * (mu(0, -1, 0) + mu(-1, -1, 0) + mu0 + mu(-1, 0, 0));
     };
     auto sxz_909 = [&]() {
-        return (iz == 0) ? (T(2) * u0 * inv_hz * visc)
-                         : ((u0 - UU(0, 0, -1)) * inv_hz_down + (w0 - WW(-1, 0, 0)) * ihx) * q *
-                               (mu(0, 0, -1) + mu(-1, 0, -1) + mu0 + mu(-1, 0, 0));
+        const T wall = T(2) * u0 * inv_hz * visc;
+        const T in = ((u0 - UU(0, 0, -1)) * inv_hz_down + (w0 - WW(-1, 0, 0)) * ihx) * q *
+                     (mu(0, 0, -1) + mu(-1, 0, -1) + mu0 + mu(-1, 0, 0));
+        return bot ? wall : in;
     };
     auto syz_099 = [&]() {
-        return (iz == 0) ? (T(2) * v0 * inv_hz * visc)
-                         : ((v0 - VV(0, 0, -1)) * inv_hz_down + (w0 - WW(0, -1, 0)) * ihy) * q *
-                               (mu(0, 0, -1) + mu(0, -1, -1) + mu0 + mu(0, -1, 0));
+        const T wall = T(2) * v0 * inv_hz * visc;
+        const T in = ((v0 - VV(0, 0, -1)) * inv_hz_down + (w0 - WW(0, -1, 0)) * ihy) * q *
+                     (mu(0, 0, -1) + mu(0, -1, -1) + mu0 + mu(0, -1, 0));
+        return bot ? wall : in;
     };
     T f0 = 0, f1 = 0, f2 = 0;
     if (only < 0 || only == 0) {
@@ -262,9 +287,10 @@ __device__ __forceinline__ void visc_naive(const Mesh<T>& m, const A& acc, const
         f0 += (sxx_000 - sxx_800) * ihx;
         const T sxy_910 = ((UU(0, 1, 0) - u0) * ihy + (VV(0, 1, 0) - VV(-1, 1, 0)) * ihx) * q * (mu0 + mu(-1, 0, 0) + mu(0, 1, 0) + mu(-1, 1, 0));
         f0 += (sxy_910 - sxy_990()) * ihy;
-        const T sxz_901 = (iz == NZ - 1) ? (T(-2) * u0 * inv_hz * visc)
-                                         : ((UU(0, 0, 1) - u0) * inv_hz_up + (WW(0, 0, 1) - WW(-1, 0, 1)) * ihx) * q *
-                                               (mu0 + mu(-1, 0, 0) + mu(0, 0, 1) + mu(-1, 0, 1));
+        const T wall = T(-2) * u0 * inv_hz * visc;
+        const T in = ((UU(0, 0, 1) - u0) * inv_hz_up + (WW(0, 0, 1) - WW(-1, 0, 1)) * ihx) * q *
+                     (mu0 + mu(-1, 0, 0) + mu(0, 0, 1) + mu(-1, 0, 1));
+        const T sxz_901 = top ? wall : in;
         f0 += (sxz_901 - sxz_909()) * inv_hz;
     }
     if (only < 0 || only == 1) {
@@ -273,9 +299,10 @@ __device__ __forceinline__ void visc_naive(const Mesh<T>& m, const A& acc, const
         const T syy_000 = T(2) * (VV(0, 1, 0) - v0) * ihy * mu0;
         const T syy_080 = T(2) * (v0 - VV(0, -1, 0)) * ihy * mu(0, -1, 0);
         f1 += (syy_000 - syy_080) * ihy;
-        const T syz_091 = (iz == NZ - 1) ? (T(-2) * v0 * inv_hz * visc)
-                                         : ((VV(0, 0, 1) - v0) * inv_hz_up + (WW(0, 0, 1) - WW(0, -1, 1)) * ihy) * q *
-                                               (mu0 + mu(0, -1, 0) + mu(0, 0, 1) + mu(0, -1, 1));
+        const T wall = T(-2) * v0 * inv_hz * visc;
+        const T in = ((VV(0, 0, 1) - v0) * inv_hz_up + (WW(0, 0, 1) - WW(0, -1, 1)) * ihy) * q *
+                     (mu0 + mu(0, -1, 0) + mu(0, 0, 1) + mu(0, -1, 1));
+        const T syz_091 = top ? wall : in;
         f1 += (syz_091 - syz_099()) * inv_hz;
     }
     if (iz != 0 && (only < 0 || only == 2)) {
@@ -285,7 +312,8 @@ __device__ __forceinline__ void visc_naive(const Mesh<T>& m, const A& acc, const
         const T syz_019 = ((VV(0, 1, 0) - VV(0, 1, -1)) * inv_hz_down + (WW(0, 1, 0) - w0) * ihy) * q *
                           (mu(0, 0, -1) + mu(0, 1, -1) + mu0 + mu(0, 1, 0));
         f2 += (syz_019 - syz_099()) * ihy;
-        const T szz_000 = T(2) * ((iz == NZ - 1 ? T(0) : WW(0, 0, 1)) - w0) * inv_hz * mu0;
+        const T wp = WW(0, 0, 1);
+        const T szz_000 = T(2) * ((top ? T(0) : wp) - w0) * inv_hz * mu0;
         const T szz_008 = T(2) * (w0 - WW(0, 0, -1)) * mu(0, 0, -1) / m.DZ[iz - 1];
         f2 += (szz_000 - szz_008) * inv_hz_down;
     }
