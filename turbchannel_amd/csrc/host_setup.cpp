@@ -395,15 +395,36 @@ void make_initial_field(const tc_params& p, const HostMesh& m, unsigned long lon
     }
 }
 
-// Radix sequence of a mixed-radix transform: 4s first, then 2, 3, 5.
+// Radix sequence of a mixed-radix transform, few stages: 8s first, then one 4,
+// a 6 where a 2 and a 3 remain, then 2, 3, 5 (384 = 8 8 6, 160 = 8 4 5).
 bool make_plan(int n, FFTPlan& pl) {
+    pl = FFTPlan();
     pl.n = n;
-    pl.nst = 0;
     int r = n;
-    while (r % 4 == 0 && pl.nst < kMaxStages) { pl.radix[pl.nst++] = 4; r /= 4; }
+    auto push = [&](int f) {
+        if (pl.nst >= kMaxStages) return false;
+        pl.radix[pl.nst++] = f;
+        r /= f;
+        return true;
+    };
+    while (r % 8 == 0)
+        if (!push(8)) return false;
+    if (r % 4 == 0 && !push(4)) return false;
+    if (r % 6 == 0 && !push(6)) return false;
     for (int f : {2, 3, 5})
-        while (r % f == 0 && pl.nst < kMaxStages) { pl.radix[pl.nst++] = f; r /= f; }
-    return r == 1;
+        while (r % f == 0)
+            if (!push(f)) return false;
+    if (r != 1 || n < 1) return false;
+    int ns = 1;
+    for (int st = 0; st < pl.nst; st++) {
+        pl.ns[st] = ns;
+        pl.tstep[st] = n / (ns * pl.radix[st]);
+        pl.dnr[st] = FastDiv((unsigned)(n / pl.radix[st]));
+        pl.dns[st] = FastDiv((unsigned)ns);
+        ns *= pl.radix[st];
+    }
+    pl.dn = FastDiv((unsigned)n);
+    return true;
 }
 
 }  // namespace tc

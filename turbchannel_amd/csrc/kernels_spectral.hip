@@ -25,10 +25,41 @@ __device__ __forceinline__ Cplx<T> csub(Cplx<T> a, Cplx<T> b) { return {a.x - b.
 template <typename T>
 __device__ __forceinline__ Cplx<T> cmul_si(Cplx<T> a, T s) { return {-s * a.y, s * a.x}; }
 
-// In-register DFT of length R (2, 3, 4, 5); s = -1 forward, +1 inverse.
+// In-register DFT of length R (2, 3, 4, 5, 6, 8); s = -1 forward, +1 inverse.
+template <typename T, int R>
+__device__ __forceinline__ void bfly(Cplx<T>* v, T s);
+
+// Length 2P from two length-P DFTs of the even and odd inputs (decimation in
+// time): X[k] = A[k] + w^k B[k], X[k+P] = A[k] - w^k B[k], w = exp(s 2 pi i / 2P).
+template <typename T, int P>
+__device__ __forceinline__ void bfly_split(Cplx<T>* v, T s, const Cplx<T>* w) {
+    Cplx<T> a[P], b[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+        a[k] = v[2 * k];
+        b[k] = v[2 * k + 1];
+    }
+    bfly<T, P>(a, s);
+    bfly<T, P>(b, s);
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+        const Cplx<T> t = k == 0 ? b[0] : cmul(b[k], w[k]);
+        v[k] = cadd(a[k], t);
+        v[k + P] = csub(a[k], t);
+    }
+}
+
 template <typename T, int R>
 __device__ __forceinline__ void bfly(Cplx<T>* v, T s) {
-    if (R == 2) {
+    if (R == 8) {
+        const T c = T(0.70710678118654752440);  // cos(pi/4)
+        const Cplx<T> w[4] = {{T(1), T(0)}, {c, s * c}, {T(0), s}, {-c, s * c}};
+        bfly_split<T, 4>(v, s, w);
+    } else if (R == 6) {
+        const T sn = T(0.86602540378443864676);  // sin(pi/3)
+        const Cplx<T> w[3] = {{T(1), T(0)}, {T(0.5), s * sn}, {T(-0.5), s * sn}};
+        bfly_split<T, 3>(v, s, w);
+    } else if (R == 2) {
         const Cplx<T> a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
@@ -77,25 +108,25 @@ __host__ __device__ constexpr size_t lds_elems(size_t n) { return n + (n >> 4) +
 // numbered so that consecutive threads touch consecutive LDS words:
 // line-fastest when the lines are interleaved (estride > 1).
 template <typename T, int R>
-__device__ __forceinline__ void stockham_pass(const Cplx<T>* __restrict__ src, Cplx<T>* __restrict__ dst, int N, int Ns,
-                                              int nlines, int lstride, int estride, const Cplx<T>* __restrict__ tw, T s) {
-    const int NR = N / R;
+__device__ __forceinline__ void stockham_pass(const Cplx<T>* __restrict__ src, Cplx<T>* __restrict__ dst,
+                                              const FFTPlan& pl, int st, int nlines, FastDiv dl, int lstride,
+                                              int estride, const Cplx<T>* __restrict__ tw, T s) {
+    const int NR = (int)pl.dnr[st].d, Ns = pl.ns[st], tstep = pl.tstep[st];
     const int total = NR * nlines;
-    const int tstep = N / (Ns * R);
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
         int l, j;
         if (estride == 1) {
-            l = t / NR;
+            l = pl.dnr[st].div(t);
             j = t - l * NR;
         } else {
-            j = t / nlines;
+            j = dl.div(t);
             l = t - j * nlines;
         }
         Cplx<T> v[R];
         const int l0 = l * lstride;
 #pragma unroll
         for (int r = 0; r < R; r++) v[r] = src[ldsi(l0 + (j + r * NR) * estride)];
-        const int jm = j % Ns;
+        const int jq = pl.dns[st].div(j), jm = j - jq * Ns;
         if (jm != 0) {
 #pragma unroll
             for (int r = 1; r < R; r++) {
@@ -105,31 +136,30 @@ __device__ __forceinline__ void stockham_pass(const Cplx<T>* __restrict__ src, C
             }
         }
         bfly<T, R>(v, s);
-        const int o = (j - jm) * R + jm;
+        const int o = jq * Ns * R + jm;
 #pragma unroll
         for (int r = 0; r < R; r++) dst[ldsi(l0 + (o + r * Ns) * estride)] = v[r];
     }
 }
 
 // Stockham autosort transform held in LDS; returns the buffer with the result.
+// dl divides by nlines (used when the lines are interleaved, estride > 1).
 template <typename T>
-__device__ Cplx<T>* lds_fft(Cplx<T>* src, Cplx<T>* dst, const FFTPlan& pl, int nlines, int lstride, int estride,
-                            const Cplx<T>* __restrict__ tw, T s) {
-    const int N = pl.n;
-    int Ns = 1;
+__device__ Cplx<T>* lds_fft(Cplx<T>* src, Cplx<T>* dst, const FFTPlan& pl, int nlines, FastDiv dl, int lstride,
+                            int estride, const Cplx<T>* __restrict__ tw, T s) {
     for (int st = 0; st < pl.nst; st++) {
-        const int R = pl.radix[st];
-        switch (R) {
-            case 2: stockham_pass<T, 2>(src, dst, N, Ns, nlines, lstride, estride, tw, s); break;
-            case 3: stockham_pass<T, 3>(src, dst, N, Ns, nlines, lstride, estride, tw, s); break;
-            case 4: stockham_pass<T, 4>(src, dst, N, Ns, nlines, lstride, estride, tw, s); break;
-            default: stockham_pass<T, 5>(src, dst, N, Ns, nlines, lstride, estride, tw, s); break;
+        switch (pl.radix[st]) {
+            case 8: stockham_pass<T, 8>(src, dst, pl, st, nlines, dl, lstride, estride, tw, s); break;
+            case 6: stockham_pass<T, 6>(src, dst, pl, st, nlines, dl, lstride, estride, tw, s); break;
+            case 4: stockham_pass<T, 4>(src, dst, pl, st, nlines, dl, lstride, estride, tw, s); break;
+            case 2: stockham_pass<T, 2>(src, dst, pl, st, nlines, dl, lstride, estride, tw, s); break;
+            case 3: stockham_pass<T, 3>(src, dst, pl, st, nlines, dl, lstride, estride, tw, s); break;
+            default: stockham_pass<T, 5>(src, dst, pl, st, nlines, dl, lstride, estride, tw, s); break;
         }
         __syncthreads();
         Cplx<T>* tmp = src;
         src = dst;
         dst = tmp;
-        Ns *= R;
     }
     return src;
 }
@@ -164,7 +194,7 @@ struct TwStage {
 template <typename T, int SCHEME>
 __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const Cplx<T>* __restrict__ tw, Mesh<T> m,
                                                                 const T* __restrict__ v, Cplx<T>* __restrict__ spec,
-                                                                int lpb) {
+                                                                int lpb, FastDiv dny, FastDiv dnxh) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n, nxh = N / 2 + 1, nrows = m.ny * m.nz;
@@ -181,9 +211,9 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = min(t0 + u * kFftThreads, tot - 1);
-            const int l = t / N, n = t - l * N;
+            const int l = pl.dn.div(t), n = t - l * N;
             const int ra = min(2 * (q0 + l), nrows - 1), rb = min(2 * (q0 + l) + 1, nrows - 1);
-            const int iza = ra / m.ny, iya = ra - iza * m.ny, izb = rb / m.ny, iyb = rb - izb * m.ny;
+            const int iza = dny.div(ra), iya = ra - iza * m.ny, izb = dny.div(rb), iyb = rb - izb * m.ny;
             zx[u] = div_cell<T, SCHEME>(m, make_nbvec<T, R>(v, n, iya, iza, m.nx, m.ny, m.nz), iza);
             zy[u] = div_cell<T, SCHEME>(m, make_nbvec<T, R>(v, n, iyb, izb, m.nx, m.ny, m.nz), izb);
         }
@@ -191,16 +221,16 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
         for (int u = 0; u < U; u++) {
             const int t = t0 + u * kFftThreads;
             if (t < tot) {
-                const int ra = 2 * (q0 + t / N);
+                const int ra = 2 * (q0 + pl.dn.div(t));
                 b0[ldsi(t)] = {ra < nrows ? zx[u] : T(0), ra + 1 < nrows ? zy[u] : T(0)};
             }
         }
     }
     tws.store(twl, tw, N);
     __syncthreads();
-    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, N, 1, twl, T(-1));
+    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, FastDiv(), N, 1, twl, T(-1));
     for (int t = threadIdx.x; t < lpb * nxh; t += blockDim.x) {
-        const int l = t / nxh, k = t - l * nxh;
+        const int l = dnxh.div(t), k = t - l * nxh;
         const int ra = 2 * (q0 + l), rb = ra + 1;
         if (ra >= nrows) continue;
         const Cplx<T> z = res[ldsi(l * N + k)];
@@ -231,7 +261,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = min(t0 + u * kFftThreads, tot - 1);
-            const int l = t / N, k = t - l * N;
+            const int l = pl.dn.div(t), k = t - l * N;
             const int kk = k <= half ? k : N - k;
             const int ra = min(2 * (q0 + l), nrows - 1), rb = min(2 * (q0 + l) + 1, nrows - 1);
             A[u] = spec[(size_t)ra * nxh + kk];
@@ -241,7 +271,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
         for (int u = 0; u < U; u++) {
             const int t = t0 + u * kFftThreads;
             if (t >= tot) continue;
-            const int l = t / N, k = t - l * N;
+            const int l = pl.dn.div(t), k = t - l * N;
             const int kk = k <= half ? k : N - k;
             const int ra = 2 * (q0 + l);
             Cplx<T> a = ra < nrows ? A[u] : Cplx<T>{T(0), T(0)};
@@ -259,9 +289,9 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
     }
     tws.store(twl, tw, N);
     __syncthreads();
-    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, N, 1, twl, T(1));
+    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, FastDiv(), N, 1, twl, T(1));
     for (int t = threadIdx.x; t < lpb * N; t += blockDim.x) {
-        const int l = t / N, n = t - l * N;
+        const int l = pl.dn.div(t), n = t - l * N;
         const int ra = 2 * (q0 + l), rb = ra + 1;
         const Cplx<T> z = res[ldsi(t)];
         if (ra < nrows) out[(size_t)ra * N + n] = z.x * scale;
@@ -274,7 +304,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
 // of each plane first gets the normalisation X00[iz] -= psum * C[iz] (FP64).
 template <typename T>
 __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>* __restrict__ tw, Cplx<T>* spec,
-                                                        int nxh, int tk, T s, const double* __restrict__ psum,
+                                                        int nxh, int tk, FastDiv dtk, T s, const double* __restrict__ psum,
                                                         const double* __restrict__ C) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
@@ -293,14 +323,14 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = min(t0 + u * kFftThreads, tot - 1);
-            const int n = t / tk, k = t - n * tk;
+            const int n = dtk.div(t), k = t - n * tk;
             v[u] = base[(size_t)n * nxh + min(k, ncol - 1)];
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = t0 + u * kFftThreads;
             if (t >= tot) continue;
-            const int n = t / tk, k = t - n * tk;
+            const int n = dtk.div(t), k = t - n * tk;
             Cplx<T> w = k < ncol ? v[u] : Cplx<T>{T(0), T(0)};
             if (psum && t == 0 && kx0 == 0) w.x = T((double)w.x - psum[0] * C[iz]);
             b0[ldsi(t)] = w;
@@ -308,9 +338,9 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
     }
     tws.store(twl, tw, N);
     __syncthreads();
-    const Cplx<T>* res = lds_fft(b0, b1, pl, tk, 1, tk, twl, s);
+    const Cplx<T>* res = lds_fft(b0, b1, pl, tk, dtk, 1, tk, twl, s);
     for (int t = threadIdx.x; t < tk * N; t += blockDim.x) {
-        const int n = t / tk, k = t - n * tk;
+        const int n = dtk.div(t), k = t - n * tk;
         if (k < ncol) base[(size_t)n * nxh + k] = res[ldsi(t)];
     }
 }
@@ -489,7 +519,7 @@ __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFac
 template <typename T>
 __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_q(FFTPlan pl, const Cplx<T>* __restrict__ tw,
                                                               const T* __restrict__ v, int ca, int cb, int nrows,
-                                                              Cplx<T>* __restrict__ spec, int lpb) {
+                                                              Cplx<T>* __restrict__ spec, int lpb, FastDiv dnxh) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n, nxh = N / 2 + 1;
@@ -510,7 +540,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_q(FFTPlan pl, const C
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = min(t0 + u * kFftThreads, tot - 1);
-            const int l = t / N, n = t - l * N;
+            const int l = pl.dn.div(t), n = t - l * N;
             zx[u] = q(min(2 * (q0 + l), nrows - 1), n);
             zy[u] = q(min(2 * (q0 + l) + 1, nrows - 1), n);
         }
@@ -518,16 +548,16 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_q(FFTPlan pl, const C
         for (int u = 0; u < U; u++) {
             const int t = t0 + u * kFftThreads;
             if (t < tot) {
-                const int ra = 2 * (q0 + t / N);
+                const int ra = 2 * (q0 + pl.dn.div(t));
                 b0[ldsi(t)] = {ra < nrows ? zx[u] : T(0), ra + 1 < nrows ? zy[u] : T(0)};
             }
         }
     }
     tws.store(twl, tw, N);
     __syncthreads();
-    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, N, 1, twl, T(-1));
+    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, FastDiv(), N, 1, twl, T(-1));
     for (int t = threadIdx.x; t < lpb * nxh; t += blockDim.x) {
-        const int l = t / nxh, k = t - l * nxh;
+        const int l = dnxh.div(t), k = t - l * nxh;
         const int ra = 2 * (q0 + l), rb = ra + 1;
         if (ra >= nrows) continue;
         const Cplx<T> z = res[ldsi(l * N + k)];
@@ -573,12 +603,16 @@ hipError_t launch_fft_x_fwd_div(hipStream_t s, const FFTPlan& pl, const Cplx<T>*
     const int nlines = (m.ny * m.nz + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
     const size_t sm = (2 * lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
+    const FastDiv dny((unsigned)m.ny), dnxh((unsigned)(pl.n / 2 + 1));
     if (scheme == TC_SCHEME_HW)
-        hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_HW>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb);
+        hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_HW>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb,
+                           dny, dnxh);
     else if (scheme == TC_SCHEME_VV)
-        hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_VV>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb);
+        hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_VV>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb,
+                           dny, dnxh);
     else
-        hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_AB3>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb);
+        hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_AB3>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb,
+                           dny, dnxh);
     return hipGetLastError();
 }
 
@@ -601,7 +635,8 @@ hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cpl
     while (tk > 1 && 2 * (size_t)tk * pl.n * sizeof(Cplx<T>) > 48 * 1024) tk >>= 1;
     const dim3 grid((nxh + tk - 1) / tk, nz);
     const size_t sm = (2 * lds_elems((size_t)tk * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
-    hipLaunchKernelGGL((k_fft_y<T>), grid, dim3(kFftThreads), sm, s, pl, tw, spec, nxh, tk, T(inverse ? 1 : -1), psum, C);
+    hipLaunchKernelGGL((k_fft_y<T>), grid, dim3(kFftThreads), sm, s, pl, tw, spec, nxh, tk, FastDiv((unsigned)tk),
+                       T(inverse ? 1 : -1), psum, C);
     return hipGetLastError();
 }
 
@@ -626,7 +661,8 @@ hipError_t launch_fft_x_fwd_q(hipStream_t s, const FFTPlan& pl, const Cplx<T>* t
     const int nlines = (nrows + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
     const size_t sm = (2 * lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
-    hipLaunchKernelGGL((k_fft_x_fwd_q<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, v, ca, cb, nrows, spec, lpb);
+    hipLaunchKernelGGL((k_fft_x_fwd_q<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, v, ca, cb, nrows, spec, lpb,
+                       FastDiv((unsigned)(pl.n / 2 + 1)));
     return hipGetLastError();
 }
 

@@ -123,6 +123,11 @@ template <typename T, int SCHEME, int VISC>
 constexpr int flux_threads() {
     return kTX * flux_ty<T, SCHEME, VISC>() * flux_nc<T, SCHEME>();
 }
+// Waves per SIMD the register allocation must allow.
+template <typename T, int SCHEME, int VISC>
+constexpr int flux_min_waves() {
+    return 1;
+}
 
 __device__ __forceinline__ int xcd_swizzle(int b, int nb) {
     const int q = nb / 8, r = nb % 8, x = b % 8, i = b / 8;
@@ -195,7 +200,7 @@ __device__ __forceinline__ void sgs_divergence(const Mesh<T>& m, const T q[3][7]
 }
 
 template <typename T, int SCHEME, int VISC, bool TURB>
-__global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>())) void k_flux(
+__global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>()), (flux_min_waves<T, SCHEME, VISC>())) void k_flux(
     Mesh<T> m, FluxArgs<T> a, int ntx, int ntiles, int zchunk) {
     constexpr int TY = flux_ty<T, SCHEME, VISC>();
     constexpr int RXY = SCHEME == TC_SCHEME_VV ? 3 : 1;

@@ -53,10 +53,32 @@ struct Cplx {
     T x, y;
 };
 
-constexpr int kMaxStages = 16;
+// Division by a constant d >= 1 for 0 <= n < 2^31: q = (mulhi(n, m) + n) >> s
+// with s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1 (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", 1994).
+// The kernels' index arithmetic uses it in place of integer division.
+struct FastDiv {
+    unsigned d = 1, m = 1;
+    int s = 0;
+    FastDiv() = default;
+    explicit FastDiv(unsigned dd) : d(dd) {
+        while ((1ull << s) < dd) s++;
+        m = (unsigned)(((1ull << 32) * ((1ull << s) - dd)) / dd + 1);
+    }
+    __host__ __device__ __forceinline__ int div(int n) const {
+        const unsigned hi = (unsigned)(((unsigned long long)(unsigned)n * m) >> 32);
+        return (int)((hi + (unsigned)n) >> s);
+    }
+};
+
+constexpr int kMaxStages = 12;
+// Mixed-radix Stockham plan. Stage st has radix radix[st], ns[st] = product of
+// the radices before it, tstep[st] = n / (ns[st] * radix[st]) (twiddle stride),
+// and divisors by n / radix[st] and by ns[st]; dn divides by n.
 struct FFTPlan {
     int n = 0, nst = 0;
-    int radix[kMaxStages] = {0};
+    int radix[kMaxStages] = {0}, ns[kMaxStages] = {0}, tstep[kMaxStages] = {0};
+    FastDiv dnr[kMaxStages], dns[kMaxStages], dn;
 };
 bool make_plan(int n, FFTPlan& pl);
 
