@@ -290,9 +290,9 @@ int setup_device(tc_engine* e) {
     HIPCHK(hipMemcpy(e->d_C, zf.C.data(), e->nz * sizeof(double), hipMemcpyHostToDevice));
     if (e->fp32) {  // low-mode list and its double factors
         std::vector<int> low;
-        for (int ky = 0; ky < e->ny; ky++)
-            for (int kx = 0; kx <= kLowK && kx < e->nxh; kx++)
-                if (ky <= kLowK || ky >= e->ny - kLowK) low.push_back(ky * e->nxh + kx);
+        for (int kx = 0; kx <= kLowK && kx < e->nxh; kx++)
+            for (int ky = 0; ky < e->ny; ky++)
+                if (ky <= kLowK || ky >= e->ny - kLowK) low.push_back(kx * e->ny + ky);
         e->nlow = (int)low.size();
         std::vector<int> lowidx(e->nm, -1);
         for (int li = 0; li < e->nlow; li++) lowidx[low[li]] = li;
@@ -372,7 +372,8 @@ int unapply(tc_engine* e, const T* v, T* x) {
     HIPCHK(launch_fft_y<T>(e->s, e->ply, (const Cplx<T>*)e->twy, spec, e->nxh, e->nz, 1, e->d_s00, e->d_C));
     e->tend(TC_K_IFFT_Y, ev);
     e->tbeg(TC_K_IFFT_X, ev);
-    HIPCHK(launch_fft_x_inv<T>(e->s, e->plx, (const Cplx<T>*)e->twx, spec, x, nrows, (T)(1.0 / ((double)e->nx * e->ny))));
+    HIPCHK(launch_fft_x_inv<T>(e->s, e->plx, (const Cplx<T>*)e->twx, spec, x, e->ny, nrows,
+                               (T)(1.0 / ((double)e->nx * e->ny))));
     e->tend(TC_K_IFFT_X, ev);
     return TC_OK;
 }
@@ -515,11 +516,11 @@ int sgs_samples(tc_engine* e, const T* v, int factor, float* x, float* y) {
     const size_t fs = (size_t)nxc * nyc * nz;
     static const int ca[9] = {0, 1, 2, 0, 1, 2, 0, 0, 1}, cb[9] = {-1, -1, -1, 0, 1, 2, 1, 2, 2};
     for (int f = 0; f < 9; f++) {
-        HIPCHK(launch_fft_x_fwd_q<T>(e->s, e->plx, (const Cplx<T>*)e->twx, v, ca[f], cb[f], e->ny * nz, spec));
+        HIPCHK(launch_fft_x_fwd_q<T>(e->s, e->plx, (const Cplx<T>*)e->twx, v, ca[f], cb[f], e->ny, e->ny * nz, spec));
         HIPCHK(launch_fft_y<T>(e->s, e->ply, (const Cplx<T>*)e->twy, spec, e->nxh, nz, 0, nullptr, nullptr));
         HIPCHK(launch_spec_crop<T>(e->s, spec, e->nxh, e->ny, cs, nxhc, nyc, nz));
         HIPCHK(launch_fft_y<T>(e->s, e->fply, (const Cplx<T>*)e->ftwy, cs, nxhc, nz, 1, nullptr, nullptr));
-        HIPCHK(launch_fft_x_inv<T>(e->s, e->fplx, (const Cplx<T>*)e->ftwx, cs, q + f * fs, nyc * nz,
+        HIPCHK(launch_fft_x_inv<T>(e->s, e->fplx, (const Cplx<T>*)e->ftwx, cs, q + f * fs, nyc, nyc * nz,
                                    (T)(1.0 / ((double)e->nx * e->ny))));
     }
     HIPCHK(launch_sgs_samples<T>(e->s, m, q, nxc, nyc, (T)(e->P.hx * factor), (T)(e->P.hy * factor), x, y));

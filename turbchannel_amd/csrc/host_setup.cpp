@@ -177,7 +177,7 @@ std::string build_zfactors(const tc_params& p, const HostMesh& m, ZFactors& f) {
     const double* A = m.MAT_A.data();
     std::string err;
     for (int mode = 0; mode < nm; mode++) {
-        const int ix = mode % nxh, iy = mode / nxh;
+        const int ix = mode / ny, iy = mode % ny;  // spectrum order: ky fastest
         auto at = [&](int iz) { return (size_t)iz * nm + mode; };
         if (p.scheme == TC_SCHEME_HW || p.scheme == TC_SCHEME_AB3) {
             const double cphi = std::cos(2. * kPi * ix / nx), cpsi = std::cos(2. * kPi * iy / ny);

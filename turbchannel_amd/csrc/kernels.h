@@ -66,14 +66,14 @@ hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cpl
                         int inverse, const double* psum, const double* C);
 template <typename T>
 hipError_t launch_fft_x_inv(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const Cplx<T>* spec, T* out,
-                            int nrows, T scale);
+                            int ny, int nrows, T scale);
 
 // Training samples of the learned closure: forward x transform of component ca
 // (cb < 0) or of the product of components ca, cb of a real3 field, and the
 // coarse half spectrum of the modes a sharp cut-off keeps.
 template <typename T>
 hipError_t launch_fft_x_fwd_q(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const T* v, int ca, int cb,
-                              int nrows, Cplx<T>* spec);
+                              int ny, int nrows, Cplx<T>* spec);
 template <typename T>
 hipError_t launch_spec_crop(hipStream_t s, const Cplx<T>* fine, int nxh, int ny, Cplx<T>* coarse, int nxhc, int nyc,
                             int nz);

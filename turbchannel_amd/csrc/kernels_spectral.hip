@@ -1,11 +1,12 @@
 // Spectral pressure solve: batched mixed-radix FFTs held in LDS and the
 // wall-normal banded solves per horizontal mode.
 //
-// Spectrum layout (complex, half spectrum in x, as the reference's cuFFT R2C
-// output, staggered/func_global.h:197-206): spec[(iz*NY + ky)*NXH + kx],
-// NXH = NX/2 + 1. The x transforms pack two real rows into one complex
-// transform; the y transforms run on tiles of TK adjacent kx columns so that
-// every global access is a contiguous run of TK complex values.
+// Spectrum layout (complex, half spectrum in x as the reference's cuFFT R2C
+// output, staggered/func_global.h:197-206, stored ky-fastest):
+// spec[(iz*NXH + kx)*NY + ky], NXH = NX/2 + 1, mode = kx*NY + ky. The x
+// transforms pack two real rows into one complex transform and exchange the
+// spectra with runs of consecutive ky; the y transforms then run on contiguous
+// lines like the x transforms.
 #include "kernels.h"
 #include "stencil.h"
 
@@ -188,18 +189,40 @@ struct TwStage {
     }
 };
 
+// The half spectra of the block's 2*lpb rows (row = iy + NY*iz; line l holds
+// rows 2l and 2l+1 as real and imaginary parts) to spec[(iz*NXH + kx)*NY + iy]:
+// consecutive threads take consecutive rows of one kx, so each store
+// instruction writes runs of 2*lpb consecutive values.
+template <typename T>
+__device__ __forceinline__ void store_half_spectra(const Cplx<T>* res, int N, int ny, FastDiv dny, FastDiv d2l, int q0,
+                                                   int nrows, Cplx<T>* __restrict__ spec) {
+    const int nr = (int)d2l.d, nxh = N / 2 + 1;
+    for (int t = threadIdx.x; t < nxh * nr; t += blockDim.x) {
+        const int k = d2l.div(t), rr = t - k * nr;
+        const int l = rr >> 1, row = 2 * q0 + rr;
+        if (row >= nrows) continue;
+        const Cplx<T> z = res[ldsi(l * N + k)];
+        const Cplx<T> zc = res[ldsi(l * N + (k == 0 ? 0 : N - k))];
+        // row 2l: A = (Z + conj Zc)/2; row 2l+1: B = -i (Z - conj Zc)/2
+        const Cplx<T> out = (rr & 1) ? Cplx<T>{T(0.5) * (z.y + zc.y), T(-0.5) * (z.x - zc.x)}
+                                     : Cplx<T>{T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
+        const int iz = dny.div(row), iy = row - iz * ny;
+        spec[((size_t)iz * nxh + k) * ny + iy] = out;
+    }
+}
+
 // Forward x transform of the divergence of v: rows 2q, 2q+1 (row = iy + NY*iz)
 // form the real and imaginary parts of line q; the divergence is evaluated
 // while the lines are staged into LDS.
 template <typename T, int SCHEME>
 __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const Cplx<T>* __restrict__ tw, Mesh<T> m,
                                                                 const T* __restrict__ v, Cplx<T>* __restrict__ spec,
-                                                                int lpb, FastDiv dny, FastDiv dnxh) {
+                                                                int lpb, FastDiv dny, FastDiv d2l) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
-    const int N = pl.n, nxh = N / 2 + 1, nrows = m.ny * m.nz;
+    const int N = pl.n, nrows = m.ny * m.nz;
     Cplx<T>* b1 = b0 + lds_elems(lpb * N);
-    Cplx<T>* twl = b1 + lds_elems(lpb * N);
+    Cplx<T>* twl = b1 + lds_elems(lpb * N + 2 * lpb);
     const int q0 = blockIdx.x * lpb;
     constexpr int R = SCHEME == TC_SCHEME_VV ? 2 : 1;
     constexpr int U = SCHEME == TC_SCHEME_VV ? 2 : 4;
@@ -229,65 +252,63 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
     tws.store(twl, tw, N);
     __syncthreads();
     const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, FastDiv(), N, 1, twl, T(-1));
-    for (int t = threadIdx.x; t < lpb * nxh; t += blockDim.x) {
-        const int l = dnxh.div(t), k = t - l * nxh;
-        const int ra = 2 * (q0 + l), rb = ra + 1;
-        if (ra >= nrows) continue;
-        const Cplx<T> z = res[ldsi(l * N + k)];
-        const Cplx<T> zc = res[ldsi(l * N + (k == 0 ? 0 : N - k))];
-        // A = (Z + conj Zc)/2, B = -i (Z - conj Zc)/2
-        spec[(size_t)ra * nxh + k] = {T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
-        if (rb < nrows) spec[(size_t)rb * nxh + k] = {T(0.5) * (z.y + zc.y), T(-0.5) * (z.x - zc.x)};
-    }
+    store_half_spectra(res, N, m.ny, dny, d2l, q0, nrows, spec);
 }
 
 // Inverse x transform: half spectra of rows 2q, 2q+1 -> real rows (times scale).
+// The block's spectra are first read with runs of consecutive rows (ky) per kx
+// into a staging buffer, then combined into the complex lines A + iB.
 template <typename T>
 __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cplx<T>* __restrict__ tw,
                                                             const Cplx<T>* __restrict__ spec, T* __restrict__ out,
-                                                            int nrows, int lpb, T scale) {
+                                                            int nrows, int ny, FastDiv dny, int lpb, FastDiv d2l,
+                                                            T scale) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n, nxh = N / 2 + 1, half = N / 2;
+    const int nr = 2 * lpb;
     Cplx<T>* b1 = b0 + lds_elems(lpb * N);
-    Cplx<T>* twl = b1 + lds_elems(lpb * N);
+    Cplx<T>* twl = b1 + lds_elems(lpb * N + 2 * lpb);
     const int q0 = blockIdx.x * lpb;
     constexpr int U = 4;
     TwStage<T> tws;
     tws.load(tw, N);
-    const int tot = lpb * N;
-    for (int t0 = threadIdx.x; t0 < tot; t0 += U * kFftThreads) {
-        Cplx<T> A[U], B[U];
+    const int tst = nxh * nr;  // staging: b1[kk * nr + rr]
+    for (int t0 = threadIdx.x; t0 < tst; t0 += U * kFftThreads) {
+        Cplx<T> a[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int t = min(t0 + u * kFftThreads, tot - 1);
-            const int l = pl.dn.div(t), k = t - l * N;
-            const int kk = k <= half ? k : N - k;
-            const int ra = min(2 * (q0 + l), nrows - 1), rb = min(2 * (q0 + l) + 1, nrows - 1);
-            A[u] = spec[(size_t)ra * nxh + kk];
-            B[u] = spec[(size_t)rb * nxh + kk];
+            const int t = min(t0 + u * kFftThreads, tst - 1);
+            const int kk = d2l.div(t), rr = t - kk * nr;
+            const int row = min(2 * q0 + rr, nrows - 1);
+            const int iz = dny.div(row), iy = row - iz * ny;
+            a[u] = spec[((size_t)iz * nxh + kk) * ny + iy];
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = t0 + u * kFftThreads;
-            if (t >= tot) continue;
-            const int l = pl.dn.div(t), k = t - l * N;
-            const int kk = k <= half ? k : N - k;
-            const int ra = 2 * (q0 + l);
-            Cplx<T> a = ra < nrows ? A[u] : Cplx<T>{T(0), T(0)};
-            Cplx<T> b = ra + 1 < nrows ? B[u] : Cplx<T>{T(0), T(0)};
-            if (kk == 0 || kk == half) {  // real-to-complex symmetry: these bins are real
-                a.y = T(0);
-                b.y = T(0);
-            }
-            if (k > half) {
-                a.y = -a.y;
-                b.y = -b.y;
-            }
-            b0[ldsi(t)] = {a.x - b.y, a.y + b.x};  // A + iB
+            if (t < tst) b1[ldsi(t)] = a[u];
         }
     }
     tws.store(twl, tw, N);
+    __syncthreads();
+    for (int t = threadIdx.x; t < lpb * N; t += blockDim.x) {
+        const int l = pl.dn.div(t), k = t - l * N;
+        const int kk = k <= half ? k : N - k;
+        const int ra = 2 * (q0 + l);
+        Cplx<T> a = b1[ldsi(kk * nr + 2 * l)], b = b1[ldsi(kk * nr + 2 * l + 1)];
+        if (ra >= nrows) a = {T(0), T(0)};
+        if (ra + 1 >= nrows) b = {T(0), T(0)};
+        if (kk == 0 || kk == half) {  // real-to-complex symmetry: these bins are real
+            a.y = T(0);
+            b.y = T(0);
+        }
+        if (k > half) {
+            a.y = -a.y;
+            b.y = -b.y;
+        }
+        b0[ldsi(t)] = {a.x - b.y, a.y + b.x};  // A + iB
+    }
     __syncthreads();
     const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, FastDiv(), N, 1, twl, T(1));
     for (int t = threadIdx.x; t < lpb * N; t += blockDim.x) {
@@ -299,50 +320,46 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
     }
 }
 
-// y transform of TK adjacent kx columns of one plane, in place.
+// y transform of lpb contiguous lines (line L = iz*NXH + kx, NY values), in place.
 // With psum != nullptr (inverse transform of the pressure solve) the (0,0) mode
 // of each plane first gets the normalisation X00[iz] -= psum * C[iz] (FP64).
 template <typename T>
 __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>* __restrict__ tw, Cplx<T>* spec,
-                                                        int nxh, int tk, FastDiv dtk, T s, const double* __restrict__ psum,
-                                                        const double* __restrict__ C) {
+                                                        int nlines, FastDiv dnxh, int lpb, T s,
+                                                        const double* __restrict__ psum, const double* __restrict__ C) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n;
-    Cplx<T>* b1 = b0 + lds_elems(tk * N);
-    Cplx<T>* twl = b1 + lds_elems(tk * N);
-    const int kx0 = blockIdx.x * tk, iz = blockIdx.y;
-    const int ncol = min(tk, nxh - kx0);
-    Cplx<T>* base = spec + (size_t)iz * N * nxh + kx0;
+    Cplx<T>* b1 = b0 + lds_elems(lpb * N);
+    Cplx<T>* twl = b1 + lds_elems(lpb * N);
+    const int L0 = blockIdx.x * lpb;
+    const int nl = min(lpb, nlines - L0);
+    Cplx<T>* base = spec + (size_t)L0 * N;
     constexpr int U = 4;
     TwStage<T> tws;
     tws.load(tw, N);
-    const int tot = tk * N;
+    const int tot = lpb * N, tin = nl * N;
     for (int t0 = threadIdx.x; t0 < tot; t0 += U * kFftThreads) {
         Cplx<T> v[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int t = min(t0 + u * kFftThreads, tot - 1);
-            const int n = dtk.div(t), k = t - n * tk;
-            v[u] = base[(size_t)n * nxh + min(k, ncol - 1)];
-        }
+        for (int u = 0; u < U; u++) v[u] = base[min(t0 + u * kFftThreads, tin - 1)];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = t0 + u * kFftThreads;
             if (t >= tot) continue;
-            const int n = dtk.div(t), k = t - n * tk;
-            Cplx<T> w = k < ncol ? v[u] : Cplx<T>{T(0), T(0)};
-            if (psum && t == 0 && kx0 == 0) w.x = T((double)w.x - psum[0] * C[iz]);
+            Cplx<T> w = t < tin ? v[u] : Cplx<T>{T(0), T(0)};
+            if (psum) {
+                const int l = pl.dn.div(t), n = t - l * N, L = L0 + l;
+                const int iz = dnxh.div(L);
+                if (n == 0 && L == iz * (int)dnxh.d && l < nl) w.x = T((double)w.x - psum[0] * C[iz]);
+            }
             b0[ldsi(t)] = w;
         }
     }
     tws.store(twl, tw, N);
     __syncthreads();
-    const Cplx<T>* res = lds_fft(b0, b1, pl, tk, dtk, 1, tk, twl, s);
-    for (int t = threadIdx.x; t < tk * N; t += blockDim.x) {
-        const int n = dtk.div(t), k = t - n * tk;
-        if (k < ncol) base[(size_t)n * nxh + k] = res[ldsi(t)];
-    }
+    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, FastDiv(), N, 1, twl, s);
+    for (int t = threadIdx.x; t < tin; t += blockDim.x) base[t] = res[ldsi(t)];
 }
 
 // ---------------------------------------------------------------------------
@@ -519,12 +536,13 @@ __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFac
 template <typename T>
 __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_q(FFTPlan pl, const Cplx<T>* __restrict__ tw,
                                                               const T* __restrict__ v, int ca, int cb, int nrows,
-                                                              Cplx<T>* __restrict__ spec, int lpb, FastDiv dnxh) {
+                                                              Cplx<T>* __restrict__ spec, int lpb, int ny, FastDiv dny,
+                                                              FastDiv d2l) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
-    const int N = pl.n, nxh = N / 2 + 1;
+    const int N = pl.n;
     Cplx<T>* b1 = b0 + lds_elems(lpb * N);
-    Cplx<T>* twl = b1 + lds_elems(lpb * N);
+    Cplx<T>* twl = b1 + lds_elems(lpb * N + 2 * lpb);
     const int q0 = blockIdx.x * lpb;
     auto q = [&](int row, int n) {
         const T* c = v + 3 * ((size_t)row * N + n);
@@ -556,15 +574,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_q(FFTPlan pl, const C
     tws.store(twl, tw, N);
     __syncthreads();
     const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, FastDiv(), N, 1, twl, T(-1));
-    for (int t = threadIdx.x; t < lpb * nxh; t += blockDim.x) {
-        const int l = dnxh.div(t), k = t - l * nxh;
-        const int ra = 2 * (q0 + l), rb = ra + 1;
-        if (ra >= nrows) continue;
-        const Cplx<T> z = res[ldsi(l * N + k)];
-        const Cplx<T> zc = res[ldsi(l * N + (k == 0 ? 0 : N - k))];
-        spec[(size_t)ra * nxh + k] = {T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
-        if (rb < nrows) spec[(size_t)rb * nxh + k] = {T(0.5) * (z.y + zc.y), T(-0.5) * (z.x - zc.x)};
-    }
+    store_half_spectra(res, N, ny, dny, d2l, q0, nrows, spec);
 }
 
 // Half spectrum of the coarse grid holding the modes a sharp cut-off keeps:
@@ -574,14 +584,14 @@ __global__ void k_spec_crop(const Cplx<T>* __restrict__ fine, int nxh, int ny, C
                             int nyc, int nz) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)nxhc * nyc * nz) return;
-    const int kx = (int)(t % nxhc);
-    const long long rest = t / nxhc;
-    const int kyc = (int)(rest % nyc), iz = (int)(rest / nyc);
+    const int kyc = (int)(t % nyc);
+    const long long rest = t / nyc;
+    const int kx = (int)(rest % nxhc), iz = (int)(rest / nxhc);
     const int nxc = 2 * (nxhc - 1), hy = nyc / 2;
     Cplx<T> v = {T(0), T(0)};
     if (kx < nxc / 2) {
         const int ky = kyc < hy ? kyc : (kyc > hy ? ny - (nyc - kyc) : -1);
-        if (ky >= 0) v = fine[((size_t)iz * ny + ky) * nxh + kx];
+        if (ky >= 0) v = fine[((size_t)iz * nxh + kx) * ny + ky];
     }
     coarse[t] = v;
 }
@@ -602,41 +612,43 @@ hipError_t launch_fft_x_fwd_div(hipStream_t s, const FFTPlan& pl, const Cplx<T>*
     const int lpb = lines_per_block<T>(pl.n);
     const int nlines = (m.ny * m.nz + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
-    const size_t sm = (2 * lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
-    const FastDiv dny((unsigned)m.ny), dnxh((unsigned)(pl.n / 2 + 1));
+    const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems((size_t)lpb * pl.n + 2 * lpb) + lds_elems(pl.n)) *
+                      sizeof(Cplx<T>);
+    const FastDiv dny((unsigned)m.ny), d2l((unsigned)(2 * lpb));
     if (scheme == TC_SCHEME_HW)
         hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_HW>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb,
-                           dny, dnxh);
+                           dny, d2l);
     else if (scheme == TC_SCHEME_VV)
         hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_VV>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb,
-                           dny, dnxh);
+                           dny, d2l);
     else
         hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_AB3>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb,
-                           dny, dnxh);
+                           dny, d2l);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_fft_x_inv(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const Cplx<T>* spec, T* out, int nrows,
-                            T scale) {
+hipError_t launch_fft_x_inv(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const Cplx<T>* spec, T* out, int ny,
+                            int nrows, T scale) {
     const int lpb = lines_per_block<T>(pl.n);
     const int nlines = (nrows + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
-    const size_t sm = (2 * lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
-    hipLaunchKernelGGL((k_fft_x_inv<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, spec, out, nrows, lpb, scale);
+    const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems((size_t)lpb * pl.n + 2 * lpb) + lds_elems(pl.n)) *
+                      sizeof(Cplx<T>);
+    hipLaunchKernelGGL((k_fft_x_inv<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, spec, out, nrows, ny,
+                       FastDiv((unsigned)ny), lpb, FastDiv((unsigned)(2 * lpb)), scale);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cplx<T>* spec, int nxh, int nz, int inverse,
                         const double* psum, const double* C) {
-    // 128-byte runs (8 complex doubles, 16 complex floats) while the block stays within 48 KiB of LDS
-    int tk = 128 / (int)sizeof(Cplx<T>);
-    while (tk > 1 && 2 * (size_t)tk * pl.n * sizeof(Cplx<T>) > 48 * 1024) tk >>= 1;
-    const dim3 grid((nxh + tk - 1) / tk, nz);
-    const size_t sm = (2 * lds_elems((size_t)tk * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
-    hipLaunchKernelGGL((k_fft_y<T>), grid, dim3(kFftThreads), sm, s, pl, tw, spec, nxh, tk, FastDiv((unsigned)tk),
-                       T(inverse ? 1 : -1), psum, C);
+    const int lpb = lines_per_block<T>(pl.n);
+    const int nlines = nxh * nz;
+    const int grid = (nlines + lpb - 1) / lpb;
+    const size_t sm = (2 * lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
+    hipLaunchKernelGGL((k_fft_y<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, spec, nlines, FastDiv((unsigned)nxh),
+                       lpb, T(inverse ? 1 : -1), psum, C);
     return hipGetLastError();
 }
 
@@ -655,14 +667,15 @@ hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, cons
 
 
 template <typename T>
-hipError_t launch_fft_x_fwd_q(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const T* v, int ca, int cb,
+hipError_t launch_fft_x_fwd_q(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const T* v, int ca, int cb, int ny,
                               int nrows, Cplx<T>* spec) {
     const int lpb = lines_per_block<T>(pl.n);
     const int nlines = (nrows + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
-    const size_t sm = (2 * lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
-    hipLaunchKernelGGL((k_fft_x_fwd_q<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, v, ca, cb, nrows, spec, lpb,
-                       FastDiv((unsigned)(pl.n / 2 + 1)));
+    const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems((size_t)lpb * pl.n + 2 * lpb) + lds_elems(pl.n)) *
+                      sizeof(Cplx<T>);
+    hipLaunchKernelGGL((k_fft_x_fwd_q<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, v, ca, cb, nrows, spec, lpb, ny,
+                       FastDiv((unsigned)ny), FastDiv((unsigned)(2 * lpb)));
     return hipGetLastError();
 }
 
@@ -678,12 +691,12 @@ hipError_t launch_spec_crop(hipStream_t s, const Cplx<T>* fine, int nxh, int ny,
 #define INST(T)                                                                                                      \
     template hipError_t launch_fft_x_fwd_div<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const Mesh<T>&, int,    \
                                                 const T*, Cplx<T>*);                                                 \
-    template hipError_t launch_fft_x_inv<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const Cplx<T>*, T*, int, T); \
+    template hipError_t launch_fft_x_inv<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const Cplx<T>*, T*, int, int, T); \
     template hipError_t launch_fft_y<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, Cplx<T>*, int, int, int,       \
                                         const double*, const double*);                                              \
     template hipError_t launch_zsolve<T>(hipStream_t, Cplx<T>*, const ZFacDev<T>&, const ZFacDev<double>&,         \
                                          const int*, const int*, int, double*, ZNorm);                                   \
-    template hipError_t launch_fft_x_fwd_q<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const T*, int, int, int,  \
+    template hipError_t launch_fft_x_fwd_q<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const T*, int, int, int, int, \
                                               Cplx<T>*);                                                             \
     template hipError_t launch_spec_crop<T>(hipStream_t, const Cplx<T>*, int, int, Cplx<T>*, int, int, int);
 INST(double)
