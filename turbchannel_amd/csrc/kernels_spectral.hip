@@ -224,32 +224,130 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
     Cplx<T>* b1 = b0 + lds_elems(lpb * N);
     Cplx<T>* twl = b1 + lds_elems(lpb * N + 2 * lpb);
     const int q0 = blockIdx.x * lpb;
-    constexpr int R = SCHEME == TC_SCHEME_VV ? 2 : 1;
-    constexpr int U = SCHEME == TC_SCHEME_VV ? 2 : 4;
     TwStage<T> tws;
     tws.load(tw, N);
     const int tot = lpb * N;
-    for (int t0 = threadIdx.x; t0 < tot; t0 += U * kFftThreads) {
-        T zx[U], zy[U];
+    // Rows 2l, 2l+1 of line l are neighbours in y within one plane (NY is even).
+    // (1) u of the block's rows -> LDS (ub[row * N + x]), so the x taps of the
+    // divergence come from LDS; (2) per element the own v and w, the v of the
+    // rows y-1, y+2, y+3 around the pair (the pair shares its own v as each
+    // other's y taps) and the w of the z taps, all loaded before any is used.
+    T* ub = reinterpret_cast<T*>(b1);
+    {
+        constexpr int U = 4;
+        for (int t0 = threadIdx.x; t0 < tot; t0 += U * kFftThreads) {
+            T a[U], b[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int t = min(t0 + u * kFftThreads, tot - 1);
-            const int l = pl.dn.div(t), n = t - l * N;
-            const int ra = min(2 * (q0 + l), nrows - 1), rb = min(2 * (q0 + l) + 1, nrows - 1);
-            const int iza = dny.div(ra), iya = ra - iza * m.ny, izb = dny.div(rb), iyb = rb - izb * m.ny;
-            zx[u] = div_cell<T, SCHEME>(m, make_nbvec<T, R>(v, n, iya, iza, m.nx, m.ny, m.nz), iza);
-            zy[u] = div_cell<T, SCHEME>(m, make_nbvec<T, R>(v, n, iyb, izb, m.nx, m.ny, m.nz), izb);
-        }
+            for (int u = 0; u < U; u++) {
+                const int t = min(t0 + u * kFftThreads, tot - 1);
+                const int l = pl.dn.div(t), n = t - l * N;
+                const int ra = min(2 * (q0 + l), nrows - 1), rb = min(2 * (q0 + l) + 1, nrows - 1);
+                a[u] = v[3 * ((size_t)ra * N + n)];
+                b[u] = v[3 * ((size_t)rb * N + n)];
+            }
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int t = t0 + u * kFftThreads;
-            if (t < tot) {
-                const int ra = 2 * (q0 + pl.dn.div(t));
-                b0[ldsi(t)] = {ra < nrows ? zx[u] : T(0), ra + 1 < nrows ? zy[u] : T(0)};
+            for (int u = 0; u < U; u++) {
+                const int t = t0 + u * kFftThreads;
+                if (t < tot) {
+                    const int l = pl.dn.div(t), n = t - l * N;
+                    ub[2 * l * N + n] = a[u];
+                    ub[(2 * l + 1) * N + n] = b[u];
+                }
             }
         }
     }
     tws.store(twl, tw, N);
+    __syncthreads();
+    constexpr int U = SCHEME == TC_SCHEME_VV ? 2 : 4;
+    const int ny = m.ny, nz = m.nz;
+    for (int t0 = threadIdx.x; t0 < tot; t0 += U * kFftThreads) {
+        T va[U], vb[U], wa[U], wb[U], vm1[U] = {}, vp2[U], vp3[U] = {};
+        T wa1[U], wb1[U], wa2[U] = {}, wb2[U] = {}, wam[U] = {}, wbm[U] = {};
+        T dzv[U] = {}, dzwv[U] = {}, chv[U] = {};
+        int iz[U], l_[U], n_[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = min(t0 + u * kFftThreads, tot - 1);
+            const int l = pl.dn.div(t), n = t - l * N;
+            const int ra = min(2 * (q0 + l), nrows - 2);  // rows ra, ra + 1 of one plane
+            const int z = dny.div(ra), iy = ra - z * ny;
+            iz[u] = z;
+            l_[u] = l;
+            n_[u] = n;
+            auto at = [&](int zz, int yy) { return 3 * (((size_t)zz * ny + yy) * N + n); };
+            auto wrapy = [&](int yy) { return yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy); };
+            va[u] = v[at(z, iy) + 1];
+            wa[u] = v[at(z, iy) + 2];
+            vb[u] = v[at(z, iy + 1) + 1];
+            wb[u] = v[at(z, iy + 1) + 2];
+            vp2[u] = v[at(z, wrapy(iy + 2)) + 1];
+            if (SCHEME != TC_SCHEME_HW) vm1[u] = v[at(z, wrapy(iy - 1)) + 1];
+            if (SCHEME == TC_SCHEME_VV) vp3[u] = v[at(z, wrapy(iy + 3)) + 1];
+            const int zp = min(z + 1, nz - 1), zm = max(z - 1, 0), zpp = min(z + 2, nz - 1);
+            wa1[u] = v[at(zp, iy) + 2];
+            wb1[u] = v[at(zp, iy + 1) + 2];
+            if (SCHEME != TC_SCHEME_HW) {
+                wam[u] = v[at(zm, iy) + 2];
+                wbm[u] = v[at(zm, iy + 1) + 2];
+            }
+            if (SCHEME == TC_SCHEME_VV) {
+                wa2[u] = v[at(zpp, iy) + 2];
+                wb2[u] = v[at(zpp, iy + 1) + 2];
+            }
+            if (SCHEME == TC_SCHEME_AB3) {
+                chv[u] = m.cellh[z];
+            } else {
+                dzv[u] = m.DZ[z];
+                if (SCHEME == TC_SCHEME_VV) dzwv[u] = m.DZW[z];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int l = l_[u], n = n_[u];
+            const int xm = n == 0 ? N - 1 : n - 1, xp = n + 1 >= N ? n + 1 - N : n + 1, xpp = n + 2 >= N ? n + 2 - N : n + 2;
+            const T* ua = ub + 2 * l * N;
+            const T* ubb = ua + N;
+            TapVals<T> qa, qb;
+#pragma unroll
+            for (int k = 0; k < 12; k++) qa.v[k] = qb.v[k] = T(0);
+            qa.v[0] = ua[xp];
+            qb.v[0] = ubb[xp];
+            qa.v[1] = ua[n];
+            qb.v[1] = ubb[n];
+            qa.v[3] = ua[xm];
+            qb.v[3] = ubb[xm];
+            if (SCHEME == TC_SCHEME_VV) {
+                qa.v[2] = ua[xpp];
+                qb.v[2] = ubb[xpp];
+            }
+            // v taps y+1, y, y+2, y-1
+            qa.v[4] = vb[u];
+            qa.v[5] = va[u];
+            qa.v[6] = vp2[u];
+            qa.v[7] = vm1[u];
+            qb.v[4] = vp2[u];
+            qb.v[5] = vb[u];
+            qb.v[6] = vp3[u];
+            qb.v[7] = va[u];
+            // w taps z, z+1, z+2, z-1
+            qa.v[8] = wa[u];
+            qa.v[9] = wa1[u];
+            qa.v[10] = wa2[u];
+            qa.v[11] = wam[u];
+            qb.v[8] = wb[u];
+            qb.v[9] = wb1[u];
+            qb.v[10] = wb2[u];
+            qb.v[11] = wbm[u];
+            const T zx = div_cell<T, SCHEME>(m, qa, iz[u], dzv[u], dzwv[u], chv[u]);
+            const T zy = div_cell<T, SCHEME>(m, qb, iz[u], dzv[u], dzwv[u], chv[u]);
+            const int t = t0 + u * kFftThreads;
+            if (t < tot) {
+                const int ra = 2 * (q0 + l);
+                b0[ldsi(t)] = {ra < nrows ? zx : T(0), ra + 1 < nrows ? zy : T(0)};
+            }
+        }
+    }
     __syncthreads();
     const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, FastDiv(), N, 1, twl, T(-1));
     store_half_spectra(res, N, m.ny, dny, d2l, q0, nrows, spec);

@@ -453,18 +453,70 @@ __device__ __forceinline__ NbVec<T, R> make_nbvec(const T* v, int ix, int iy, in
     return a;
 }
 
-// Volume-weighted discrete divergence at one cell (staggered/func_global.h:306-312
-// HW, :325-333 VV; collocated_AB3/func_global.h:310-325).
+// The values the divergence at a cell reads: one set of 12 offsets covers the
+// three schemes. A kernel loads them all first (load_taps) and evaluates
+// div_cell on the loaded copy, so every load of a batch of cells is in flight
+// before any is used. Offsets are literal at every use, so the lookup folds to
+// a register.
+__host__ __device__ constexpr int tap_key(int c, int dx, int dy, int dz) {
+    return ((c * 8 + dx + 3) * 8 + dy + 3) * 8 + dz + 3;
+}
+template <typename T>
+struct TapVals {
+    T v[12];
+    __device__ __forceinline__ T operator()(int c, int dx, int dy, int dz) const {
+        switch (tap_key(c, dx, dy, dz)) {
+            case tap_key(0, 1, 0, 0): return v[0];
+            case tap_key(0, 0, 0, 0): return v[1];
+            case tap_key(0, 2, 0, 0): return v[2];
+            case tap_key(0, -1, 0, 0): return v[3];
+            case tap_key(1, 0, 1, 0): return v[4];
+            case tap_key(1, 0, 0, 0): return v[5];
+            case tap_key(1, 0, 2, 0): return v[6];
+            case tap_key(1, 0, -1, 0): return v[7];
+            case tap_key(2, 0, 0, 0): return v[8];
+            case tap_key(2, 0, 0, 1): return v[9];
+            case tap_key(2, 0, 0, 2): return v[10];
+            case tap_key(2, 0, 0, -1): return v[11];
+            default: return T(0);
+        }
+    }
+};
 template <typename T, int SCHEME, class A>
-__device__ __forceinline__ T div_cell(const Mesh<T>& m, const A& acc, int iz) {
+__device__ __forceinline__ void load_taps(const A& acc, TapVals<T>& r) {
+    r.v[0] = acc(0, 1, 0, 0);
+    r.v[4] = acc(1, 0, 1, 0);
+    r.v[8] = acc(2, 0, 0, 0);
+    r.v[9] = acc(2, 0, 0, 1);
+    if (SCHEME != TC_SCHEME_AB3) {
+        r.v[1] = acc(0, 0, 0, 0);
+        r.v[5] = acc(1, 0, 0, 0);
+    }
+    if (SCHEME != TC_SCHEME_HW) {
+        r.v[3] = acc(0, -1, 0, 0);
+        r.v[7] = acc(1, 0, -1, 0);
+        r.v[11] = acc(2, 0, 0, -1);
+    }
+    if (SCHEME == TC_SCHEME_VV) {
+        r.v[2] = acc(0, 2, 0, 0);
+        r.v[6] = acc(1, 0, 2, 0);
+        r.v[10] = acc(2, 0, 0, 2);
+    }
+}
+
+// Volume-weighted discrete divergence at one cell (staggered/func_global.h:306-312
+// HW, :325-333 VV; collocated_AB3/func_global.h:310-325). dz = DZ[iz], dzw = DZW[iz]
+// (staggered), ch = the collocated node's cell height.
+template <typename T, int SCHEME, class A>
+__device__ __forceinline__ T div_cell(const Mesh<T>& m, const A& acc, int iz, T dz, T dzw, T ch) {
     const int NZ = m.nz;
     if (SCHEME == TC_SCHEME_HW) {
-        const T hz = m.DZ[iz];
+        const T hz = dz;
         const T w0 = acc(2, 0, 0, 0), wp = acc(2, 0, 0, 1);
         return (acc(0, 1, 0, 0) - acc(0, 0, 0, 0)) * m.inv_hx * hz + (acc(1, 0, 1, 0) - acc(1, 0, 0, 0)) * m.inv_hy * hz +
                ((iz < NZ - 1) ? (wp - w0) : (-w0));
     } else if (SCHEME == TC_SCHEME_VV) {
-        const T dz = m.DZ[iz], dzw = m.DZW[iz], c27 = T(1.0 / 27.0);
+        const T c27 = T(1.0 / 27.0);
         const T dudx = m.inv_hx * (dz * (acc(0, 1, 0, 0) - acc(0, 0, 0, 0)) - c27 * dzw * (acc(0, 2, 0, 0) - acc(0, -1, 0, 0)));
         const T dvdy = m.inv_hy * (dz * (acc(1, 0, 1, 0) - acc(1, 0, 0, 0)) - c27 * dzw * (acc(1, 0, 2, 0) - acc(1, 0, -1, 0)));
         const T w0 = acc(2, 0, 0, 0), wp1 = acc(2, 0, 0, 1), wp2 = acc(2, 0, 0, 2), wm1 = acc(2, 0, 0, -1);
@@ -473,7 +525,7 @@ __device__ __forceinline__ T div_cell(const Mesh<T>& m, const A& acc, int iz) {
         const T wll = (iz == 0) ? wp1 : (iz == 1) ? T(0) : wm1;
         return dudx + dvdy + (wr - w0) - c27 * (wrr - wll);
     } else {
-        const T hz = m.cellh[iz];
+        const T hz = ch;
         T d = T(0.5) * (acc(0, 1, 0, 0) - acc(0, -1, 0, 0)) * m.hy * hz;
         d += T(0.5) * (acc(1, 0, 1, 0) - acc(1, 0, -1, 0)) * m.hx * hz;
         const T w0 = acc(2, 0, 0, 0), wp = acc(2, 0, 0, 1), wm = acc(2, 0, 0, -1);
