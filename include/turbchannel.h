@@ -107,6 +107,16 @@ int tc_cfl(tc_engine* e, int kind, int slot, double tau, int mode, double* out);
  * (u[3], uu[6], nu_t, p, pp) and collocated 10 (u[3], uu[6], nu_t). */
 int tc_profiles(tc_engine* e, int vel_kind, int vel_slot, int p_slot, double* host_av, size_t count);
 
+/* The per-step diagnostics of the reference's time loop (staggered/main.cpp:312-338:
+ * CalcIntegral, CalcProfiles, CalcCFL; collocated_AB3/main.cpp:414-441) from one
+ * pass over velocity slot vel_slot (and pressure slot p_slot, nu_t): one kernel
+ * and one copy to the host instead of three of each. Each output may be NULL.
+ * integral: as tc_integral; kinetic_energy: as tc_kinetic_energy; cfl: as
+ * tc_cfl(tau, cfl_mode); host_av/count: as tc_profiles. No reference counterpart
+ * as a single call. */
+int tc_diagnostics(tc_engine* e, int vel_slot, int p_slot, double tau, int cfl_mode, double integral[3],
+                   double* kinetic_energy, double* cfl, double* host_av, size_t count);
+
 /* Constant-flow-rate correction of one velocity field (no reference counterpart;
  * tc_step applies it itself when cfr_ubulk > 0). shift may be NULL. */
 int tc_cfr_correct(tc_engine* e, int kind, int slot, double* shift);

@@ -97,6 +97,40 @@ def test_reductions(name):
     assert np.abs(got - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
 
 
+@pytest.mark.parametrize("name", ALL)
+def test_fused_diagnostics(name):
+    """tc_diagnostics (one pass) against the oracle's CalcIntegral,
+    CalcKineticEnergy, CalcCFL and CalcProfiles on the same fields."""
+    sol = parity.make_oracle(name)
+    eng = parity.make_engine(name)
+    u, p = fields(sol)
+    eng.set_field(tca.VEL, 1, u)
+    eng.set_field(tca.PRES, 0, p)
+    if sol.params.turb_visc:
+        sol.recalc_turb_visc(u)
+        eng.set_field(tca.NUT, 0, sol.get_field(tca.NUT, 0))
+    for mode in (1, 2, 3):
+        d = eng.diagnostics(1, 0, 1e-3, mode)
+        np.testing.assert_allclose(d["cfl"], sol.cfl(u, 1e-3, mode), rtol=1e-12)
+    np.testing.assert_allclose(d["integral"], sol.integral(u), rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(d["kinetic_energy"], sol.kinetic_energy(u), rtol=1e-12)
+    ref = sol.profiles(u, p)
+    assert np.abs(d["profiles"] - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def test_fused_diagnostics_fp32_matches_separate_calls():
+    eng = parity.make_engine("vv_les", precision=tca.FP32_BULK)
+    sol = parity.make_oracle("vv_les")
+    u, p = fields(sol)
+    eng.set_field(tca.VEL, 2, u)
+    eng.set_field(tca.PRES, 2, p)
+    eng.recalc_turb_visc((tca.VEL, 2))
+    d = eng.diagnostics(2, 2, 2e-3, 3)
+    np.testing.assert_allclose(d["integral"], eng.integral((tca.VEL, 2)), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(d["cfl"], eng.cfl((tca.VEL, 2), 2e-3, 3), rtol=1e-12)
+    np.testing.assert_allclose(d["profiles"], eng.profiles((tca.VEL, 2), 2), rtol=1e-12, atol=1e-12)
+
+
 def load_history(sol, eng, cur, seed=11):
     """Fill vel/flux/pres slots cur, cur-1, cur-2 with consistent seeded data."""
     for k, slot in enumerate((cur, (cur + 3) % 4, (cur + 2) % 4)):

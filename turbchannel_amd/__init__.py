@@ -78,6 +78,7 @@ def lib():
     L.tc_kinetic_energy.argtypes = [vp, c_int, c_int, dp]
     L.tc_cfl.argtypes = [vp, c_int, c_int, c_double, c_int, dp]
     L.tc_profiles.argtypes = [vp, c_int, c_int, c_int, dp, ctypes.c_size_t]
+    L.tc_diagnostics.argtypes = [vp, c_int, c_int, c_double, c_int, dp, dp, dp, dp, ctypes.c_size_t]
     L.tc_cfr_correct.argtypes = [vp, c_int, c_int, dp]
     L.tc_project.argtypes = [vp, c_int, c_int]
     L.tc_make_initial_field.argtypes = [P(TcParams), ctypes.c_ulonglong, c_double, c_double, dp]
@@ -250,6 +251,17 @@ class Engine:
         out = np.zeros((self.nz, nv))
         _check(lib().tc_profiles(self._h, vel[0], vel[1], p_slot, _dptr(out), out.size))
         return out
+
+    def diagnostics(self, vel_slot, p_slot=None, tau=1.0, cfl_mode=3):
+        """Integral, kinetic energy, CFL and profiles of velocity slot vel_slot in
+        one pass (tc_diagnostics); returns a dict of the four."""
+        nv = 10 if self.scheme == SCHEME_AB3 else 12
+        prof = np.zeros((self.nz, nv))
+        q = np.zeros(3)
+        ke, cfl = ctypes.c_double(), ctypes.c_double()
+        _check(lib().tc_diagnostics(self._h, vel_slot, vel_slot if p_slot is None else p_slot, tau, cfl_mode, _dptr(q),
+                                    ctypes.byref(ke), ctypes.byref(cfl), _dptr(prof), prof.size))
+        return dict(integral=q, kinetic_energy=ke.value, cfl=cfl.value, profiles=prof)
 
     def cfr_correct(self, f):
         s = ctypes.c_double()

@@ -98,6 +98,8 @@ struct tc_engine {
     double* d_out = nullptr;    // small results (16 doubles)
     double* d_shift = nullptr;  // flow-rate shift
     double* d_prof = nullptr;   // profiles result (nz * 12)
+    double* d_diag = nullptr;   // fused diagnostics, nz records
+    std::vector<double> h_diag;
     // learned closure (collocated, sgs_model == 1)
     void* tau = nullptr;                // [node][6] closure stress
     unsigned short* sgs_w = nullptr;    // bfloat16 weight block
@@ -312,11 +314,12 @@ int setup_device(tc_engine* e) {
     if (int r = upload_twiddles<T>(e, e->nx, &e->twx)) return r;
     if (int r = upload_twiddles<T>(e, e->ny, &e->twy)) return r;
     e->npartial = std::max<size_t>((size_t)kRedBlocks * 3, (size_t)flux_grid_blocks(e->nx, e->ny, e->nz));
-    e->npartial = std::max<size_t>(e->npartial, (size_t)profiles_chunks(e->nxy) * e->nz * 12);
+    e->npartial = std::max<size_t>(e->npartial, (size_t)profiles_chunks(e->nxy) * e->nz * diag_record(e->P.scheme));
     if (int r = dalloc(e, &e->d_partial, e->npartial)) return r;
     if (int r = dalloc(e, &e->d_out, 16)) return r;
     if (int r = dalloc(e, &e->d_shift, 4)) return r;
     if (int r = dalloc(e, &e->d_prof, (size_t)e->nz * 12)) return r;
+    if (int r = dalloc(e, &e->d_diag, (size_t)e->nz * diag_record(e->P.scheme))) return r;
     return TC_OK;
 }
 
@@ -819,6 +822,59 @@ int tc_profiles(tc_engine* e, int vel_kind, int vel_slot, int p_slot, double* ho
                                        (const double*)e->nut, e->d_partial, e->d_prof));
     HIPCHK(hipMemcpyAsync(host_av, e->d_prof, count * sizeof(double), hipMemcpyDeviceToHost, e->s));
     HIPCHK(hipStreamSynchronize(e->s));
+    return TC_OK;
+}
+
+int tc_diagnostics(tc_engine* e, int vel_slot, int p_slot, double tau, int cfl_mode, double integral[3],
+                   double* kinetic_energy, double* cfl, double* host_av, size_t count) {
+    if (int r = check(e)) return r;
+    const bool ab3 = e->P.scheme == TC_SCHEME_AB3;
+    const int nv = ab3 ? 10 : 12, nrec = diag_record(e->P.scheme);
+    void* v = field_ptr(e, TC_VEL, vel_slot);
+    void* p = ab3 ? nullptr : field_ptr(e, TC_PRES, p_slot);
+    if (!v) return fail(TC_ERR_ARG, "bad velocity slot");
+    if (!ab3 && !p) return fail(TC_ERR_ARG, "bad pressure slot");
+    if (host_av && count != (size_t)nv * e->nz) return fail(TC_ERR_ARG, "count must be nz*" + std::to_string(nv));
+    if (cfl && (cfl_mode < 1 || cfl_mode > 3)) return fail(TC_ERR_ARG, "cfl_mode must be 1, 2 or 3");
+    const double visc_fac = ab3 ? 4.0 * (e->P.visc_type + 1) : 4.0 * ((e->P.visc_type != 0) + 1);
+    if (e->fp32)
+        HIPCHK(launch_diag<float>(e->s, e->mesh<float>(), e->P.scheme, (const float*)v, (const float*)p,
+                                  (const float*)e->nut, visc_fac, e->d_partial, e->d_diag));
+    else
+        HIPCHK(launch_diag<double>(e->s, e->mesh<double>(), e->P.scheme, (const double*)v, (const double*)p,
+                                   (const double*)e->nut, visc_fac, e->d_partial, e->d_diag));
+    e->h_diag.resize((size_t)e->nz * nrec);
+    HIPCHK(hipMemcpyAsync(e->h_diag.data(), e->d_diag, e->h_diag.size() * sizeof(double), hipMemcpyDeviceToHost, e->s));
+    HIPCHK(hipStreamSynchronize(e->s));
+    // volumes as the device kernels see them: the mesh in the engine's precision
+    auto r = [&](double x) { return e->fp32 ? (double)(float)x : x; };
+    const double hxy = r(e->P.hx) * r(e->P.hy), inv_nxy = 1.0 / (double)e->nxy;
+    double q[3] = {0, 0, 0}, ke = 0, cmax = 0;
+    for (int iz = 0; iz < e->nz; iz++) {
+        const double* d = e->h_diag.data() + (size_t)iz * nrec;
+        const double sw = d[nv], sww = d[nv + 1];
+        if (ab3) {
+            const double vol = r(e->hm.cellh[iz]) * hxy;
+            q[0] += d[0] * vol;
+            q[1] += d[1] * vol;
+            q[2] += sw * vol;
+            ke += (d[3] + d[4] + sww) * vol;
+        } else {
+            const double dz = r(e->hm.DZ[iz]);
+            const double dzw = iz == 0 ? 0.0 : 0.5 * (dz + r(e->hm.DZ[iz - 1]));
+            q[0] += d[0] * dz * hxy;
+            q[1] += d[1] * dz * hxy;
+            q[2] += sw * dzw * hxy;
+            ke += ((d[3] + d[4]) * dz + sww * dzw) * hxy;
+        }
+        cmax = std::max(cmax, d[nv + 1 + cfl_mode]);
+        if (host_av)
+            for (int k = 0; k < nv; k++) host_av[(size_t)iz * nv + k] = d[k] * inv_nxy;
+    }
+    if (integral)
+        for (int k = 0; k < 3; k++) integral[k] = q[k];
+    if (kinetic_energy) *kinetic_energy = 0.5 * ke;
+    if (cfl) *cfl = tau * cmax;
     return TC_OK;
 }
 

@@ -129,5 +129,14 @@ template <typename T>
 hipError_t launch_profiles(hipStream_t s, const Mesh<T>& m, int scheme, const T* v, const T* p, const T* nut,
                            double* partial, double* out);
 int profiles_chunks(int nxy);
+// Fused diagnostics: per plane a record of diag_record(scheme) doubles: the plane
+// sums of the profile statistics (NV values as launch_profiles, not divided by
+// the plane size), sum w, sum w^2 (w at its own points), and the maxima of the
+// convective, viscous and total CFL numbers over tau. partial: profiles_chunks *
+// nz * diag_record doubles.
+int diag_record(int scheme);
+template <typename T>
+hipError_t launch_diag(hipStream_t s, const Mesh<T>& m, int scheme, const T* v, const T* p, const T* nut,
+                       double visc_fac, double* partial, double* out);
 
 }  // namespace tc

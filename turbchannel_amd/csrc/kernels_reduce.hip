@@ -166,7 +166,108 @@ __global__ void k_profiles_final(const double* partial, int nch, int nv, double 
     out[iz * nv + k] = r * inv_nxy;
 }
 
+// Fused per-step diagnostics: one pass over a velocity field (and p, nu_t) per
+// plane chunk accumulates the plane statistics of k_profiles, the raw sums of w
+// and w^2 the volume integrals need, and the plane maxima of the CFL numbers
+// divided by tau (convective, viscous, both). Column layout of a plane record:
+// [0, NV) as k_profiles, NV: sum w, NV+1: sum w^2, NV+2..NV+4: maxima.
+template <typename T, int SCHEME>
+__global__ __launch_bounds__(kThreads) void k_diag(Mesh<T> m, const T* __restrict__ vel, const T* __restrict__ p,
+                                                   const T* __restrict__ nut, double visc_fac, double* partial) {
+    constexpr int NV = SCHEME == TC_SCHEME_AB3 ? 10 : 12;
+    constexpr int NS = NV + 2, NX = NS + 3;
+    __shared__ double sh[NS][kThreads / 64];
+    __shared__ double shm[3][kThreads / 64];
+    const int iz = blockIdx.y, nch = gridDim.x;
+    double s[NS], mx[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < NS; k++) s[k] = 0.0;
+    const int base = iz * m.nxy;
+    const double ihx = m.inv_hx, ihy = m.inv_hy;
+    const double inv_hz = SCHEME == TC_SCHEME_AB3 ? 1.0 / (double)m.cellh[iz] : 1.0 / (double)m.DZ[iz];
+    const double kv = visc_fac * (ihx * ihx + ihy * ihy + inv_hz * inv_hz);
+    const bool top = iz == m.nz - 1;
+    const bool nut_node = SCHEME == TC_SCHEME_AB3 && nut && iz != 0 && !top;
+    for (int j = blockIdx.x * kThreads + threadIdx.x; j < m.nxy; j += nch * kThreads) {
+        const int in = base + j;
+        const double u = vel[3 * in], v = vel[3 * in + 1], wr = vel[3 * in + 2];
+        double w = wr;
+        if (SCHEME != TC_SCHEME_AB3) w = 0.5 * (wr + (top ? 0.0 : (double)vel[3 * (in + m.nxy) + 2]));
+        s[0] += u;
+        s[1] += v;
+        s[2] += w;
+        s[3] += u * u;
+        s[4] += v * v;
+        s[5] += w * w;
+        s[6] += u * v;
+        s[7] += u * w;
+        s[8] += v * w;
+        if (nut) s[9] += nut[in];
+        if (SCHEME != TC_SCHEME_AB3) {
+            const double pp = p[in];
+            s[10] += pp;
+            s[11] += pp * pp;
+        }
+        s[NV] += wr;
+        s[NV + 1] += wr * wr;
+        double vc = m.visc;
+        if (SCHEME == TC_SCHEME_AB3) {
+            if (nut_node) {  // GetNuTAtNode: mean of the 8 elements around the node
+                const int iy = j / m.nx, ix = j - iy * m.nx;
+                const int xl = wrapi<T>(ix - 1, m.nx), yl = wrapi<T>(iy - 1, m.ny);
+                const int p0 = iz * m.nxy, p1 = (iz - 1) * m.nxy;
+                const T* n = nut;
+                vc += 0.125 * ((double)n[p0 + iy * m.nx + ix] + n[p0 + iy * m.nx + xl] + n[p0 + yl * m.nx + ix] +
+                               n[p0 + yl * m.nx + xl] + n[p1 + iy * m.nx + ix] + n[p1 + iy * m.nx + xl] +
+                               n[p1 + yl * m.nx + ix] + n[p1 + yl * m.nx + xl]);
+            }
+        } else if (nut) {
+            vc = (double)m.visc + (double)nut[in];
+        }
+        const double cc = fabs(u) * ihx + fabs(v) * ihy + fabs(wr) * inv_hz;
+        const double cv = kv * vc;
+        mx[0] = fmax(mx[0], cc);
+        mx[1] = fmax(mx[1], cv);
+        mx[2] = fmax(mx[2], cc + cv);
+    }
+    block_reduce<NS>(s, sh, false);
+    block_reduce<3>(mx, shm, true);
+    if (threadIdx.x == 0) {
+        double* o = partial + ((size_t)iz * nch + blockIdx.x) * NX;
+#pragma unroll
+        for (int k = 0; k < NS; k++) o[k] = s[k];
+        for (int k = 0; k < 3; k++) o[NS + k] = mx[k];
+    }
+}
+
+__global__ void k_diag_final(const double* partial, int nch, int nx_rec, int nsum, double* out) {
+    const int iz = blockIdx.x, k = threadIdx.x;
+    if (k >= nx_rec) return;
+    double r = 0.0;
+    for (int c = 0; c < nch; c++) {
+        const double x = partial[((size_t)iz * nch + c) * nx_rec + k];
+        r = k < nsum ? r + x : fmax(r, x);
+    }
+    out[iz * nx_rec + k] = r;
+}
+
 }  // namespace
+
+int diag_record(int scheme) { return (scheme == TC_SCHEME_AB3 ? 10 : 12) + 5; }
+
+template <typename T>
+hipError_t launch_diag(hipStream_t s, const Mesh<T>& m, int scheme, const T* v, const T* p, const T* nut,
+                       double visc_fac, double* partial, double* out) {
+    const int nch = profiles_chunks(m.nxy);
+    const dim3 g(nch, m.nz);
+    const int nx_rec = diag_record(scheme);
+    if (scheme == TC_SCHEME_AB3)
+        hipLaunchKernelGGL((k_diag<T, TC_SCHEME_AB3>), g, dim3(kThreads), 0, s, m, v, p, nut, visc_fac, partial);
+    else
+        hipLaunchKernelGGL((k_diag<T, TC_SCHEME_HW>), g, dim3(kThreads), 0, s, m, v, p, nut, visc_fac, partial);
+    hipLaunchKernelGGL(k_diag_final, dim3(m.nz), dim3(64), 0, s, partial, nch, nx_rec, nx_rec - 3, out);
+    return hipGetLastError();
+}
 
 int profiles_chunks(int nxy) {
     int c = (nxy + 4 * kThreads - 1) / (4 * kThreads);
@@ -215,6 +316,8 @@ hipError_t launch_profiles(hipStream_t s, const Mesh<T>& m, int scheme, const T*
 }
 
 #define INST(T)                                                                                                 \
+    template hipError_t launch_diag<T>(hipStream_t, const Mesh<T>&, int, const T*, const T*, const T*, double,   \
+                                       double*, double*);                                                         \
     template hipError_t launch_reduce<T>(hipStream_t, const Mesh<T>&, const RedArgs<T>&, double*, double*);     \
     template hipError_t launch_profiles<T>(hipStream_t, const Mesh<T>&, int, const T*, const T*, const T*, double*, \
                                            double*);

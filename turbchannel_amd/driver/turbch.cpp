@@ -384,7 +384,7 @@ struct Run {
     double t = 0.0, tau;
     long long it = 0, it_start = 0;
     int cur = 0;
-    double sec_step = 0, sec_red = 0, sec_prof = 0, sec_out = 0;
+    double sec_step = 0, sec_red = 0, sec_out = 0;
 
     Run(const Options& o_, Problem& P_) : o(o_), P(P_), tau(o_.tau) {
         n3d = (size_t)P.nx * P.ny * P.nz;
@@ -728,16 +728,14 @@ struct Run {
             double t1 = now();
             sec_step += t1 - t0;
 
-            double q[3];
-            check(tc_integral(e, TC_VEL, cur, q), "tc_integral");
+            // CalcIntegral, CalcProfiles and CalcCFL of the new field in one pass
+            double q[3], cfl;
+            check(tc_diagnostics(e, cur, cur, tau, 3, q, nullptr, &cfl, prof.data(), prof.size()), "tc_diagnostics");
             // CalcIntegral(...) / area: a real3 divided by a real
             const T flow_rate = T(T(q[0]) * T(1.0 / (double)T(P.area)));
-            double t2 = now();
-            sec_red += t2 - t1;
-            check(tc_profiles(e, TC_VEL, cur, cur, prof.data(), prof.size()), "tc_profiles");
             for (size_t i = 0; i < prof.size(); i++) av[i] = T(prof[i]);
             double t3 = now();
-            sec_prof += t3 - t2;
+            sec_red += t3 - t1;
 
             if (t >= o.time_start_averaging) {
                 for (size_t i = 0; i < av.size(); i++) {
@@ -759,8 +757,6 @@ struct Run {
             }
             const double ufr_cur = dudy_cur < 0. ? 0. : std::sqrt(dudy_cur * P.visc);
             const double ufr_int = dudy_int < 0. ? 0. : std::sqrt(dudy_int * P.visc);
-            double cfl;
-            check(tc_cfl(e, TC_VEL, cur, tau, 3, &cfl), "tc_cfl");
             if (it % sprog == 0 || final_step) {
                 printf("T=%f Ubulk=%.04f tau=%.1e CFL=%.04f Ufric=%.04f UfricAv=%.04f\n", t, (double)flow_rate, tau,
                        cfl, ufr_cur, ufr_int);
@@ -796,11 +792,10 @@ struct Run {
         const double total = now() - t_total0;
         if (!P.collocated) {
             printf("Total time: %.04f\n", total);
-            printf("Channels: step %.04f, velprof %.04f, reduction %.04f, output %.04f\n", sec_step, sec_prof, sec_red,
-                   sec_out);
+            printf("Channels: step %.04f, diagnostics %.04f, output %.04f\n", sec_step, sec_red, sec_out);
         } else {
-            printf("Time: total %.04f, step %.04f, velprof %.04f, reduction %.04f, output %.04f\n", total, sec_step,
-                   sec_prof, sec_red, sec_out);
+            printf("Time: total %.04f, step %.04f, diagnostics %.04f, output %.04f\n", total, sec_step, sec_red,
+                   sec_out);
         }
         if (o.timing) {
             static const char* names[TC_K_COUNT] = {"turb_visc", "flux", "div", "fft_x", "fft_y",
