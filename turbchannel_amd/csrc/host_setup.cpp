@@ -415,13 +415,14 @@ bool make_plan(int n, FFTPlan& pl) {
         while (r % f == 0)
             if (!push(f)) return false;
     if (r != 1 || n < 1) return false;
-    int ns = 1;
+    int L = n;
     for (int st = 0; st < pl.nst; st++) {
-        pl.ns[st] = ns;
-        pl.tstep[st] = n / (ns * pl.radix[st]);
-        pl.dnr[st] = FastDiv((unsigned)(n / pl.radix[st]));
-        pl.dns[st] = FastDiv((unsigned)ns);
-        ns *= pl.radix[st];
+        const int R = pl.radix[st];
+        pl.twstride[st] = n / L;
+        pl.dnr[st] = FastDiv((unsigned)(n / R));
+        pl.dspan[st] = FastDiv((unsigned)(L / R));
+        pl.drad[st] = FastDiv((unsigned)R);
+        L /= R;
     }
     pl.dn = FastDiv((unsigned)n);
     return true;

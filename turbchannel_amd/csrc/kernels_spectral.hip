@@ -103,66 +103,76 @@ __device__ __forceinline__ void bfly(Cplx<T>* v, T s) {
 __device__ __forceinline__ int ldsi(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr size_t lds_elems(size_t n) { return n + (n >> 4) + 1; }
 
-// One Stockham pass of radix R over nlines lines of length N (Ns = product of
-// the radices already applied). Element e of line l lives at
-// buf[l*lstride + e*estride]; tw[k] = exp(-2 pi i k/N). Butterflies are
-// numbered so that consecutive threads touch consecutive LDS words:
-// line-fastest when the lines are interleaved (estride > 1).
+// One in-place decimation-in-frequency pass of radix R over nlines lines of
+// length N. Element e of line l lives at buf[ldsi(l*lstride + e*estride)];
+// tw[ldsi(k)] = exp(-2 pi i k/N). A butterfly reads the R elements j + r*S of
+// its sub-transform (length L = R*S), applies the length-R DFT, multiplies
+// output r by exp(s 2 pi i j r / L) and writes back to the same places, so
+// one buffer serves the whole transform. Butterflies are numbered so that
+// consecutive threads touch consecutive LDS words: line-fastest when the lines
+// are interleaved (estride > 1).
 template <typename T, int R>
-__device__ __forceinline__ void stockham_pass(const Cplx<T>* __restrict__ src, Cplx<T>* __restrict__ dst,
-                                              const FFTPlan& pl, int st, int nlines, FastDiv dl, int lstride,
-                                              int estride, const Cplx<T>* __restrict__ tw, T s) {
-    const int NR = (int)pl.dnr[st].d, Ns = pl.ns[st], tstep = pl.tstep[st];
+__device__ __forceinline__ void dif_pass(Cplx<T>* buf, const FFTPlan& pl, int st, int nlines, FastDiv dl, int lstride,
+                                         int estride, const Cplx<T>* __restrict__ tw, T s) {
+    const int NR = (int)pl.dnr[st].d, S = (int)pl.dspan[st].d, L = S * R, tws = pl.twstride[st];
     const int total = NR * nlines;
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
-        int l, j;
+        int l, b;
         if (estride == 1) {
             l = pl.dnr[st].div(t);
-            j = t - l * NR;
+            b = t - l * NR;
         } else {
-            j = dl.div(t);
-            l = t - j * nlines;
+            b = dl.div(t);
+            l = t - b * nlines;
         }
+        const int blk = pl.dspan[st].div(b), j = b - blk * S;
+        const int e0 = l * lstride + (blk * L + j) * estride, es = S * estride;
         Cplx<T> v[R];
-        const int l0 = l * lstride;
 #pragma unroll
-        for (int r = 0; r < R; r++) v[r] = src[ldsi(l0 + (j + r * NR) * estride)];
-        const int jq = pl.dns[st].div(j), jm = j - jq * Ns;
-        if (jm != 0) {
+        for (int r = 0; r < R; r++) v[r] = buf[ldsi(e0 + r * es)];
+        bfly<T, R>(v, s);
+        if (j != 0) {
 #pragma unroll
             for (int r = 1; r < R; r++) {
-                Cplx<T> w = tw[ldsi(r * jm * tstep)];
+                Cplx<T> w = tw[ldsi(r * j * tws)];
                 w.y *= -s;  // forward table; conjugate for the inverse
                 v[r] = cmul(v[r], w);
             }
         }
-        bfly<T, R>(v, s);
-        const int o = jq * Ns * R + jm;
 #pragma unroll
-        for (int r = 0; r < R; r++) dst[ldsi(l0 + (o + r * Ns) * estride)] = v[r];
+        for (int r = 0; r < R; r++) buf[ldsi(e0 + r * es)] = v[r];
     }
 }
 
-// Stockham autosort transform held in LDS; returns the buffer with the result.
-// dl divides by nlines (used when the lines are interleaved, estride > 1).
+// Transform held in one LDS buffer, in place; the result of frequency k sits
+// at position dif_pos(pl, k) (mixed-radix digit reversal). Call after the
+// barrier that ends the staging; ends with a barrier.
 template <typename T>
-__device__ Cplx<T>* lds_fft(Cplx<T>* src, Cplx<T>* dst, const FFTPlan& pl, int nlines, FastDiv dl, int lstride,
-                            int estride, const Cplx<T>* __restrict__ tw, T s) {
+__device__ void lds_fft(Cplx<T>* buf, const FFTPlan& pl, int nlines, FastDiv dl, int lstride, int estride,
+                        const Cplx<T>* __restrict__ tw, T s) {
     for (int st = 0; st < pl.nst; st++) {
         switch (pl.radix[st]) {
-            case 8: stockham_pass<T, 8>(src, dst, pl, st, nlines, dl, lstride, estride, tw, s); break;
-            case 6: stockham_pass<T, 6>(src, dst, pl, st, nlines, dl, lstride, estride, tw, s); break;
-            case 4: stockham_pass<T, 4>(src, dst, pl, st, nlines, dl, lstride, estride, tw, s); break;
-            case 2: stockham_pass<T, 2>(src, dst, pl, st, nlines, dl, lstride, estride, tw, s); break;
-            case 3: stockham_pass<T, 3>(src, dst, pl, st, nlines, dl, lstride, estride, tw, s); break;
-            default: stockham_pass<T, 5>(src, dst, pl, st, nlines, dl, lstride, estride, tw, s); break;
+            case 8: dif_pass<T, 8>(buf, pl, st, nlines, dl, lstride, estride, tw, s); break;
+            case 6: dif_pass<T, 6>(buf, pl, st, nlines, dl, lstride, estride, tw, s); break;
+            case 4: dif_pass<T, 4>(buf, pl, st, nlines, dl, lstride, estride, tw, s); break;
+            case 2: dif_pass<T, 2>(buf, pl, st, nlines, dl, lstride, estride, tw, s); break;
+            case 3: dif_pass<T, 3>(buf, pl, st, nlines, dl, lstride, estride, tw, s); break;
+            default: dif_pass<T, 5>(buf, pl, st, nlines, dl, lstride, estride, tw, s); break;
         }
         __syncthreads();
-        Cplx<T>* tmp = src;
-        src = dst;
-        dst = tmp;
     }
-    return src;
+}
+
+// Position of frequency k after lds_fft: its digits in the radices of the
+// stages, least significant first, weight each by that stage's span.
+__device__ __forceinline__ int dif_pos(const FFTPlan& pl, int k) {
+    int p = 0;
+    for (int st = 0; st < pl.nst; st++) {
+        const int q = pl.drad[st].div(k);
+        p += (k - q * pl.radix[st]) * (int)pl.dspan[st].d;
+        k = q;
+    }
+    return p;
 }
 
 constexpr int kFftThreads = 256;
@@ -194,15 +204,15 @@ struct TwStage {
 // consecutive threads take consecutive rows of one kx, so each store
 // instruction writes runs of 2*lpb consecutive values.
 template <typename T>
-__device__ __forceinline__ void store_half_spectra(const Cplx<T>* res, int N, int ny, FastDiv dny, FastDiv d2l, int q0,
-                                                   int nrows, Cplx<T>* __restrict__ spec) {
-    const int nr = (int)d2l.d, nxh = N / 2 + 1;
+__device__ __forceinline__ void store_half_spectra(const Cplx<T>* res, const FFTPlan& pl, int ny, FastDiv dny,
+                                                   FastDiv d2l, int q0, int nrows, Cplx<T>* __restrict__ spec) {
+    const int N = pl.n, nr = (int)d2l.d, nxh = N / 2 + 1;
     for (int t = threadIdx.x; t < nxh * nr; t += blockDim.x) {
         const int k = d2l.div(t), rr = t - k * nr;
         const int l = rr >> 1, row = 2 * q0 + rr;
         if (row >= nrows) continue;
-        const Cplx<T> z = res[ldsi(l * N + k)];
-        const Cplx<T> zc = res[ldsi(l * N + (k == 0 ? 0 : N - k))];
+        const Cplx<T> z = res[ldsi(l * N + dif_pos(pl, k))];
+        const Cplx<T> zc = res[ldsi(l * N + dif_pos(pl, k == 0 ? 0 : N - k))];
         // row 2l: A = (Z + conj Zc)/2; row 2l+1: B = -i (Z - conj Zc)/2
         const Cplx<T> out = (rr & 1) ? Cplx<T>{T(0.5) * (z.y + zc.y), T(-0.5) * (z.x - zc.x)}
                                      : Cplx<T>{T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
@@ -349,70 +359,62 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
         }
     }
     __syncthreads();
-    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, FastDiv(), N, 1, twl, T(-1));
-    store_half_spectra(res, N, m.ny, dny, d2l, q0, nrows, spec);
+    lds_fft(b0, pl, lpb, FastDiv(), N, 1, twl, T(-1));
+    store_half_spectra(b0, pl, m.ny, dny, d2l, q0, nrows, spec);
 }
 
 // Inverse x transform: half spectra of rows 2q, 2q+1 -> real rows (times scale).
-// The block's spectra are first read with runs of consecutive rows (ky) per kx
-// into a staging buffer, then combined into the complex lines A + iB.
+// A thread reads the two rows' values of one kx together (consecutive ky), so
+// each load instruction covers runs of lpb value pairs, and writes the
+// complex line A + iB at k and, by the conjugate symmetry, at N - k.
 template <typename T>
 __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cplx<T>* __restrict__ tw,
                                                             const Cplx<T>* __restrict__ spec, T* __restrict__ out,
-                                                            int nrows, int ny, FastDiv dny, int lpb, FastDiv d2l,
+                                                            int nrows, int ny, FastDiv dny, int lpb, FastDiv dlpb,
                                                             T scale) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n, nxh = N / 2 + 1, half = N / 2;
-    const int nr = 2 * lpb;
-    Cplx<T>* b1 = b0 + lds_elems(lpb * N);
-    Cplx<T>* twl = b1 + lds_elems(lpb * N + 2 * lpb);
+    Cplx<T>* twl = b0 + lds_elems(lpb * N);
     const int q0 = blockIdx.x * lpb;
+    struct alignas(2 * sizeof(Cplx<T>)) Pair { Cplx<T> a, b; };
     constexpr int U = 4;
     TwStage<T> tws;
     tws.load(tw, N);
-    const int tst = nxh * nr;  // staging: b1[kk * nr + rr]
+    const int tst = nxh * lpb;  // items (kk, l), l fastest
     for (int t0 = threadIdx.x; t0 < tst; t0 += U * kFftThreads) {
-        Cplx<T> a[U];
+        Pair q[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = min(t0 + u * kFftThreads, tst - 1);
-            const int kk = d2l.div(t), rr = t - kk * nr;
-            const int row = min(2 * q0 + rr, nrows - 1);
-            const int iz = dny.div(row), iy = row - iz * ny;
-            a[u] = spec[((size_t)iz * nxh + kk) * ny + iy];
+            const int kk = dlpb.div(t), l = t - kk * lpb;
+            const int ra = min(2 * (q0 + l), nrows - 2);  // rows ra, ra + 1: one plane, consecutive ky
+            const int iz = dny.div(ra), iy = ra - iz * ny;
+            q[u] = *reinterpret_cast<const Pair*>(spec + ((size_t)iz * nxh + kk) * ny + iy);
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = t0 + u * kFftThreads;
-            if (t < tst) b1[ldsi(t)] = a[u];
+            if (t >= tst) continue;
+            const int kk = dlpb.div(t), l = t - kk * lpb;
+            const int ra = 2 * (q0 + l);
+            Cplx<T> a = ra < nrows ? q[u].a : Cplx<T>{T(0), T(0)};
+            Cplx<T> b = ra + 1 < nrows ? q[u].b : Cplx<T>{T(0), T(0)};
+            if (kk == 0 || kk == half) {  // real-to-complex symmetry: these bins are real
+                a.y = T(0);
+                b.y = T(0);
+            }
+            b0[ldsi(l * N + kk)] = {a.x - b.y, a.y + b.x};  // A + iB
+            if (kk != 0 && kk != half) b0[ldsi(l * N + N - kk)] = {a.x + b.y, b.x - a.y};  // conj A + i conj B
         }
     }
     tws.store(twl, tw, N);
     __syncthreads();
-    for (int t = threadIdx.x; t < lpb * N; t += blockDim.x) {
-        const int l = pl.dn.div(t), k = t - l * N;
-        const int kk = k <= half ? k : N - k;
-        const int ra = 2 * (q0 + l);
-        Cplx<T> a = b1[ldsi(kk * nr + 2 * l)], b = b1[ldsi(kk * nr + 2 * l + 1)];
-        if (ra >= nrows) a = {T(0), T(0)};
-        if (ra + 1 >= nrows) b = {T(0), T(0)};
-        if (kk == 0 || kk == half) {  // real-to-complex symmetry: these bins are real
-            a.y = T(0);
-            b.y = T(0);
-        }
-        if (k > half) {
-            a.y = -a.y;
-            b.y = -b.y;
-        }
-        b0[ldsi(t)] = {a.x - b.y, a.y + b.x};  // A + iB
-    }
-    __syncthreads();
-    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, FastDiv(), N, 1, twl, T(1));
+    lds_fft(b0, pl, lpb, FastDiv(), N, 1, twl, T(1));
     for (int t = threadIdx.x; t < lpb * N; t += blockDim.x) {
         const int l = pl.dn.div(t), n = t - l * N;
         const int ra = 2 * (q0 + l), rb = ra + 1;
-        const Cplx<T> z = res[ldsi(t)];
+        const Cplx<T> z = b0[ldsi(l * N + dif_pos(pl, n))];
         if (ra < nrows) out[(size_t)ra * N + n] = z.x * scale;
         if (rb < nrows) out[(size_t)rb * N + n] = z.y * scale;
     }
@@ -428,8 +430,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n;
-    Cplx<T>* b1 = b0 + lds_elems(lpb * N);
-    Cplx<T>* twl = b1 + lds_elems(lpb * N);
+    Cplx<T>* twl = b0 + lds_elems(lpb * N);
     const int L0 = blockIdx.x * lpb;
     const int nl = min(lpb, nlines - L0);
     Cplx<T>* base = spec + (size_t)L0 * N;
@@ -456,8 +457,11 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
     }
     tws.store(twl, tw, N);
     __syncthreads();
-    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, FastDiv(), N, 1, twl, s);
-    for (int t = threadIdx.x; t < tin; t += blockDim.x) base[t] = res[ldsi(t)];
+    lds_fft(b0, pl, lpb, FastDiv(), N, 1, twl, s);
+    for (int t = threadIdx.x; t < tin; t += blockDim.x) {
+        const int l = pl.dn.div(t), n = t - l * N;
+        base[t] = b0[ldsi(l * N + dif_pos(pl, n))];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -639,8 +643,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_q(FFTPlan pl, const C
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n;
-    Cplx<T>* b1 = b0 + lds_elems(lpb * N);
-    Cplx<T>* twl = b1 + lds_elems(lpb * N + 2 * lpb);
+    Cplx<T>* twl = b0 + lds_elems(lpb * N);
     const int q0 = blockIdx.x * lpb;
     auto q = [&](int row, int n) {
         const T* c = v + 3 * ((size_t)row * N + n);
@@ -671,8 +674,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_q(FFTPlan pl, const C
     }
     tws.store(twl, tw, N);
     __syncthreads();
-    const Cplx<T>* res = lds_fft(b0, b1, pl, lpb, FastDiv(), N, 1, twl, T(-1));
-    store_half_spectra(res, N, ny, dny, d2l, q0, nrows, spec);
+    lds_fft(b0, pl, lpb, FastDiv(), N, 1, twl, T(-1));
+    store_half_spectra(b0, pl, ny, dny, d2l, q0, nrows, spec);
 }
 
 // Half spectrum of the coarse grid holding the modes a sharp cut-off keeps:
@@ -696,7 +699,7 @@ __global__ void k_spec_crop(const Cplx<T>* __restrict__ fine, int nxh, int ny, C
 
 template <typename T>
 int lines_per_block(int n) {
-    // two ping-pong buffers per line; keep a block near 24 KiB of LDS
+    // 12 KiB of LDS per block for the lines (fwd_div adds as much for its u rows)
     const int per_line = 2 * n * (int)sizeof(Cplx<T>);
     int l = (24 * 1024) / per_line;
     return l < 1 ? 1 : (l > 16 ? 16 : l);
@@ -731,10 +734,9 @@ hipError_t launch_fft_x_inv(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw,
     const int lpb = lines_per_block<T>(pl.n);
     const int nlines = (nrows + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
-    const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems((size_t)lpb * pl.n + 2 * lpb) + lds_elems(pl.n)) *
-                      sizeof(Cplx<T>);
+    const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
     hipLaunchKernelGGL((k_fft_x_inv<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, spec, out, nrows, ny,
-                       FastDiv((unsigned)ny), lpb, FastDiv((unsigned)(2 * lpb)), scale);
+                       FastDiv((unsigned)ny), lpb, FastDiv((unsigned)lpb), scale);
     return hipGetLastError();
 }
 
@@ -744,7 +746,7 @@ hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cpl
     const int lpb = lines_per_block<T>(pl.n);
     const int nlines = nxh * nz;
     const int grid = (nlines + lpb - 1) / lpb;
-    const size_t sm = (2 * lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
+    const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
     hipLaunchKernelGGL((k_fft_y<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, spec, nlines, FastDiv((unsigned)nxh),
                        lpb, T(inverse ? 1 : -1), psum, C);
     return hipGetLastError();
@@ -770,8 +772,7 @@ hipError_t launch_fft_x_fwd_q(hipStream_t s, const FFTPlan& pl, const Cplx<T>* t
     const int lpb = lines_per_block<T>(pl.n);
     const int nlines = (nrows + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
-    const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems((size_t)lpb * pl.n + 2 * lpb) + lds_elems(pl.n)) *
-                      sizeof(Cplx<T>);
+    const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
     hipLaunchKernelGGL((k_fft_x_fwd_q<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, v, ca, cb, nrows, spec, lpb, ny,
                        FastDiv((unsigned)ny), FastDiv((unsigned)(2 * lpb)));
     return hipGetLastError();

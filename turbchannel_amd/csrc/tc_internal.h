@@ -72,13 +72,15 @@ struct FastDiv {
 };
 
 constexpr int kMaxStages = 12;
-// Mixed-radix Stockham plan. Stage st has radix radix[st], ns[st] = product of
-// the radices before it, tstep[st] = n / (ns[st] * radix[st]) (twiddle stride),
-// and divisors by n / radix[st] and by ns[st]; dn divides by n.
+// Mixed-radix in-place decimation-in-frequency plan. Stage st has radix
+// radix[st] and works on sub-transforms of length L = n / (product of the
+// radices before it) with butterfly span S = L / radix[st]; twstride[st] =
+// n / L. Divisors: by n / radix[st] (butterflies per line), by S, by the radix;
+// dn divides by n.
 struct FFTPlan {
     int n = 0, nst = 0;
-    int radix[kMaxStages] = {0}, ns[kMaxStages] = {0}, tstep[kMaxStages] = {0};
-    FastDiv dnr[kMaxStages], dns[kMaxStages], dn;
+    int radix[kMaxStages] = {0}, twstride[kMaxStages] = {0};
+    FastDiv dnr[kMaxStages], dspan[kMaxStages], drad[kMaxStages], dn;
 };
 bool make_plan(int n, FFTPlan& pl);
 
