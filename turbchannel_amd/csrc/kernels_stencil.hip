@@ -151,13 +151,15 @@ struct GlobalNu {
 template <typename T, int RXY, int RZ, int TY>
 struct RingAcc {
     static constexpr int W = kTX + 2 * RXY, H = TY + 2 * RXY, PS = W * H;
-    const T* pl[2 * RZ + 1];  // LDS planes iz-RZ..iz+RZ of the padded tile, real3 AoS
-    int off;                  // position of the cell in the padded tile
-    T cp[3], cm[3];           // own column at dz = +(RZ+1) and -(RZ+1)
+    // LDS planes iz-RZ..iz+RZ of the padded tile (real3 AoS), each pointing at the
+    // corner (dx, dy) = (-RXY, -RXY) of the cell's window, so that every offset
+    // is non-negative and becomes an immediate of the LDS read
+    const T* pl[2 * RZ + 1];
+    T cp[3], cm[3];  // own column at dz = +(RZ+1) and -(RZ+1)
     __device__ __forceinline__ T operator()(int c, int dx, int dy, int dz) const {
         if (dz == RZ + 1) return cp[c];
         if (dz == -RZ - 1) return cm[c];
-        return pl[dz + RZ][3 * (off + dy * W + dx) + c];
+        return pl[dz + RZ][3 * ((dy + RXY) * W + dx + RXY) + c];
     }
 };
 
@@ -363,8 +365,7 @@ __global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>()), (flux_min_waves<
         fetch(iz + RZ + 1);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int d = -RZ; d <= RZ; d++) acc.pl[d + RZ] = ring + (((iz + d) % NS + NS) % NS) * SLOT;
-        acc.off = (ty + RXY) * W + tx + RXY;
+        for (int d = -RZ; d <= RZ; d++) acc.pl[d + RZ] = ring + (((iz + d) % NS + NS) % NS) * SLOT + 3 * (ty * W + tx);
         GlobalNu<T> nu{a.nut, &nb, iz, m.nz};
         T k[3] = {0, 0, 0};
         if (SCHEME == TC_SCHEME_AB3) {
