@@ -203,6 +203,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel of a step (no HIP graph replay)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 256 MiB device copy (torch) whose counter readings scale the traffic figures")
     args = ap.parse_args()
@@ -230,8 +231,10 @@ def main():
     if kw.get("sgs_model"):
         eng.set_sgs_params(closure_weights())
     eng.initial_field(0, seed=42, ubulk=1.0, noise_rms=0.1)
+    eng.graphs(not args.no_graphs)
     cur = 0
-    for it in range(max(args.warmup, 3)):
+    # warm-up, long enough to capture the four order-3 steps (one per ring slot)
+    for it in range(max(args.warmup, 7)):
         eng.step(it * dt, dt, cur, min(it + 1, 3))
         cur = (cur + 1) % 4
     eng.sync()
@@ -243,9 +246,7 @@ def main():
     except Exception:  # torch is plumbing only
         sync_torch = lambda: None  # noqa: E731
 
-    # timed region 1: steps only, with per-kernel HIP events on the engine's stream
-    eng.timing_reset()
-    eng.timing(True)
+    # timed region: the steps only (replayed HIP graphs unless --no-graphs)
     eng.sync()
     sync_torch()
     t0 = time.perf_counter()
@@ -255,20 +256,27 @@ def main():
     eng.sync()
     sync_torch()
     el = time.perf_counter() - t0
+
+    # per-kernel durations: the same steps launched kernel by kernel with HIP
+    # events on the engine's stream (events bypass the graphs)
+    eng.timing_reset()
+    eng.timing(True)
+    for it in range(min(args.steps, 50)):
+        eng.step(it * dt, dt, cur, 3)
+        cur = (cur + 1) % 4
+    eng.sync()
     eng.timing(False)
     kt = eng.timing_read()
 
-    # timed region 2: steps plus the reference loop's per-step diagnostics
-    # (flow rate, plane statistics, CFL: staggered/main.cpp:312-338)
+    # steps plus the reference loop's per-step diagnostics (flow rate, plane
+    # statistics, CFL: staggered/main.cpp:312-338) in one fused pass each
     nd = max(1, args.steps // 4)
     eng.sync()
     t1 = time.perf_counter()
     for it in range(nd):
         eng.step(it * dt, dt, cur, 3)
         cur = (cur + 1) % 4
-        eng.integral((tca.VEL, cur))
-        eng.profiles((tca.VEL, cur), cur)
-        eng.cfl((tca.VEL, cur), dt, 3)
+        eng.diagnostics(cur, cur, dt, 3)
     eng.sync()
     el_diag = time.perf_counter() - t1
 
@@ -314,7 +322,7 @@ def main():
         "dtype": "f64" if precision == tca.FP64 else "f32 (f64 low-mode solves, normalisation, flow rate, statistics)",
         "data": "synthetic (seeded divergence-free initial field)",
         "config": {"workload": args.workload, "scheme": ["HW", "VV", "AB3"][scheme], "grid": [nx, ny, nz],
-                   "dt": dt, "parallelism": "single GPU"},
+                   "dt": dt, "parallelism": "single GPU", "graphs": not args.no_graphs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "algorithmic_bytes": REALS_PER_POINT[dom] * rsz * n3d,

@@ -91,6 +91,11 @@ int tc_field_device_ptr(tc_engine* e, int kind, int slot, void** dev_ptr);
  * (staggered/func_global.h:897-942; collocated_AB3/func_global.h:387-450).
  * Reads slots cur, cur-1, cur-2 (mod 4) and writes slot (cur+1)%4. */
 int tc_step(tc_engine* e, double t, double tau, int cur, int bdf_order);
+/* on != 0: tc_step captures each distinct (cur, bdf_order, tau) step into a HIP
+ * graph on first use and replays it afterwards, so a step is one graph launch
+ * instead of about ten kernel launches (the results are identical). Off by
+ * default; ignored while kernel timing is on. No reference counterpart. */
+int tc_set_graphs(tc_engine* e, int on);
 
 /* Operators used by the reference's self-tests (staggered/selftest.cpp:360-431). */
 int tc_explicit_term(tc_engine* e, int in_kind, int in_slot, int out_kind, int out_slot);     /* data::ExplicitTerm */

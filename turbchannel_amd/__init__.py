@@ -70,6 +70,7 @@ def lib():
     L.tc_get_field.argtypes = [vp, c_int, c_int, vp, ctypes.c_size_t]
     L.tc_field_device_ptr.argtypes = [vp, c_int, c_int, P(vp)]
     L.tc_step.argtypes = [vp, c_double, c_double, c_int, c_int]
+    L.tc_set_graphs.argtypes = [vp, c_int]
     for n in ("tc_explicit_term", "tc_apply_gradient", "tc_unapply_gradient"):
         getattr(L, n).argtypes = [vp, c_int, c_int, c_int, c_int]
     L.tc_normalize_pressure.argtypes = [vp, c_int, c_int]
@@ -215,6 +216,10 @@ class Engine:
     def step(self, t, tau, cur, bdf_order):
         """data::Step (staggered/func_global.h:897-942)."""
         _check(lib().tc_step(self._h, t, tau, cur, bdf_order))
+
+    def graphs(self, on=True):
+        """Replay repeated steps from captured HIP graphs (tc_set_graphs)."""
+        _check(lib().tc_set_graphs(self._h, 1 if on else 0))
 
     def explicit_term(self, src, dst):
         _check(lib().tc_explicit_term(self._h, src[0], src[1], dst[0], dst[1]))

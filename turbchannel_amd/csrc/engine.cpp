@@ -9,7 +9,9 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "kernels.h"
@@ -111,6 +113,10 @@ struct tc_engine {
     void *ftwx = nullptr, *ftwy = nullptr, *fspec = nullptr, *fq = nullptr;
     Timing tim;
     std::vector<void*> allocs;
+    // captured steps, replayed when tc_step is called again with the same
+    // (current slot, order, tau): one graph launch instead of ~10 kernel launches
+    bool use_graphs = false;
+    std::map<std::tuple<int, int, unsigned long long>, hipGraphExec_t> graphs;
 
     template <typename T>
     Mesh<T> mesh() const {
@@ -635,6 +641,7 @@ int tc_destroy(tc_engine* e) {
         hipEventDestroy(rc.b);
     }
     for (auto ev : e->tim.pool) hipEventDestroy(ev);
+    for (auto& g : e->graphs) hipGraphExecDestroy(g.second);
     for (void* p : e->allocs) hipFree(p);
     if (e->s) hipStreamDestroy(e->s);
     delete e;
@@ -685,7 +692,41 @@ int tc_step(tc_engine* e, double t, double tau, int cur, int bdf_order) {
     if (bdf_order < 1 || bdf_order > 3) return fail(TC_ERR_ARG, "Wrong bdf_order");
     if (cur < 0 || cur > 3) return fail(TC_ERR_ARG, "bad current_field");
     if (!(tau > 0)) return fail(TC_ERR_ARG, "tau must be positive");
-    return e->fp32 ? step<float>(e, tau, cur, bdf_order) : step<double>(e, tau, cur, bdf_order);
+    auto run = [&]() { return e->fp32 ? step<float>(e, tau, cur, bdf_order) : step<double>(e, tau, cur, bdf_order); };
+    if (!e->use_graphs || e->tim.on) return run();
+    if (e->P.sgs_model && !e->sgs_ready) return fail(TC_ERR_STATE, "learned closure weights are not set (tc_set_sgs_params)");
+    unsigned long long tb;
+    std::memcpy(&tb, &tau, sizeof tb);
+    const auto key = std::make_tuple(cur, bdf_order, tb);
+    auto it = e->graphs.find(key);
+    if (it == e->graphs.end()) {
+        if (e->graphs.size() >= 48) {  // step sizes changed many times: start over
+            for (auto& g : e->graphs) hipGraphExecDestroy(g.second);
+            e->graphs.clear();
+        }
+        HIPCHK(hipStreamBeginCapture(e->s, hipStreamCaptureModeRelaxed));
+        const int r = run();
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(e->s, &g);
+        if (r) {
+            if (g) hipGraphDestroy(g);
+            return r;
+        }
+        HIPCHK(ec);
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        HIPCHK(ei);
+        it = e->graphs.emplace(key, x).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second, e->s));
+    return TC_OK;
+}
+
+int tc_set_graphs(tc_engine* e, int on) {
+    if (int r = check(e)) return r;
+    e->use_graphs = on != 0;
+    return TC_OK;
 }
 
 int tc_explicit_term(tc_engine* e, int in_kind, int in_slot, int out_kind, int out_slot) {

@@ -81,6 +81,7 @@ struct Options {
     std::string checkpoint = "restart.bin";
     std::string restart;
     int timing = 0;
+    int graphs = 1;
     int print_mesh = 0;
 };
 
@@ -128,6 +129,7 @@ const OptSpec kOpts[] = {
     {"--checkpoint", "restart file to write (default restart.bin)"},
     {"--restart", "continue exactly from this restart file"},
     {"--timing", "1: report the time of each kernel class at the end"},
+    {"--graphs", "1: replay repeated steps from captured HIP graphs (default), 0: launch each kernel"},
     {"--print-mesh", "1: print the wall-normal coordinates and exit"},
 };
 
@@ -232,6 +234,7 @@ bool parse(int argc, char** argv, Options& o) {
         else if (is("--checkpoint")) o.checkpoint = val;
         else if (is("--restart")) o.restart = val;
         else if (is("--timing")) parse_num(key, val, o.timing);
+        else if (is("--graphs")) parse_num(key, val, o.graphs);
         else if (is("--print-mesh")) parse_num(key, val, o.print_mesh);
     }
     return true;
@@ -670,6 +673,7 @@ struct Run {
         if ((size_t)tc_real_size(e) != sizeof(T)) die("engine real size mismatch");
         if (!o.sgs_file.empty()) load_sgs();
         if (o.timing) check(tc_timing_enable(e, 1), "tc_timing_enable");
+        check(tc_set_graphs(e, o.graphs), "tc_set_graphs");
 
         FILE* log = nullptr;
         if (!o.restart.empty()) {

@@ -57,7 +57,9 @@ def make_engine(nx, ny, zz, model):
     kw = dict(precision=FP32_BULK, visc=visc, sour_dpdx=u_tau * u_tau, inv_length_scale=RE_TAU, hx=LX / nx,
               hy=LY / ny, stab_type=1, fd_mode=1, cfr_ubulk=1.0, visc_type=0 if model == "dns" else 1,
               turb_visc=1 if model == "smag" else 0, sgs_model=1 if model == "learned" else 0)
-    return Engine(SCHEME_AB3, nx, ny, len(zz), zz, **kw)
+    e = Engine(SCHEME_AB3, nx, ny, len(zz), zz, **kw)
+    e.graphs(True)  # repeated steps replay a captured HIP graph
+    return e
 
 
 class Run:
