@@ -1,7 +1,7 @@
 /*
  * turbchannel.h -- C ABI of the MI355X time stepper for TurbChannel's
  * staggered (Harlow-Welch 2nd order, Verstappen-Veldman 4th order) and
- * collocated BDF3/extrapolation solvers.
+ * collocated BDF3/extrapolation and IMEX Runge-Kutta (SRK) solvers.
  *
  * One opaque engine per device. It owns every device buffer, as the
  * reference's data_cuda::data does (staggered/func_global.h:944-974).
@@ -30,7 +30,14 @@ extern "C" {
 typedef struct tc_engine tc_engine;
 typedef struct tc_mlp tc_mlp;
 
-enum tc_scheme { TC_SCHEME_HW = 0, TC_SCHEME_VV = 1, TC_SCHEME_AB3 = 2 };
+enum tc_scheme { TC_SCHEME_HW = 0, TC_SCHEME_VV = 1, TC_SCHEME_AB3 = 2,
+                 TC_SCHEME_SRK = 3 /* collocated IMEX Runge-Kutta (collocated_SRK/): AB3's mesh and
+                                      FD operators, stepped by tc_srk_step */ };
+/* IMEX Runge-Kutta methods of the SRK stepper (collocated_SRK/imex.h:26-42) */
+enum tc_imex { TC_IMEX_ARS_121 = 0, TC_IMEX_ARS_232 = 1, TC_IMEX_ARS_343 = 2, TC_IMEX_MARS_343 = 3,
+               TC_IMEX_ARK3 = 4, TC_IMEX_ARK4 = 5, TC_IMEX_MARK3 = 6, TC_IMEX_BHR_553 = 7 };
+/* tIMEXtype (collocated_SRK/imex.h:9) */
+enum tc_imex_type { TC_IMEX_TYPE_A = 0, TC_IMEX_TYPE_ARS = 1, TC_IMEX_TYPE_CK = 2 };
 enum tc_precision { TC_FP64 = 0, TC_FP32_BULK = 1 };
 enum tc_kind { TC_VEL = 0, TC_FLUX = 1, TC_PRES = 2, TC_NUT = 3,
                TC_TAU = 4 /* learned-closure stress, 6 reals per node: XX YY ZZ XY XZ YZ (slot 0) */ };
@@ -70,6 +77,10 @@ typedef struct tc_params {
     int sgs_model;         /* collocated only, turb_visc = 0: 1 = learned closure (no reference
                               counterpart): each step evaluates tau from the velocity with the
                               network set by tc_set_sgs_params and subtracts its divergence */
+    int imex;              /* TC_SCHEME_SRK: tc_imex method (main_params::IMEXtype, NStages,
+                              ButcherE/I, d; collocated_SRK/main.cpp:362-372) */
+    double alpha_tau;      /* TC_SCHEME_SRK: alpha = alpha_tau / tau; <= 0 selects the reference's
+                              default 0.5 * alpha_tau_max[stab_type] (collocated_SRK/main.cpp:373) */
 } tc_params;
 
 /* Lifecycle. Replaces data::alloc_all + InitFourier + CopyDataToDevice
@@ -91,6 +102,17 @@ int tc_field_device_ptr(tc_engine* e, int kind, int slot, void** dev_ptr);
  * (staggered/func_global.h:897-942; collocated_AB3/func_global.h:387-450).
  * Reads slots cur, cur-1, cur-2 (mod 4) and writes slot (cur+1)%4. */
 int tc_step(tc_engine* e, double t, double tau, int cur, int bdf_order);
+/* One step of the collocated SRK solver: data::Step(t, tau, velocity, pressure, dpdt)
+ * (collocated_SRK/func_global.h:317-408), in place on velocity slot vel_slot and
+ * pressure slots p_slot and dpdt_slot (dpdt_slot is read and written for
+ * stab_type 1 only and must differ from p_slot). Engines of TC_SCHEME_SRK only;
+ * tc_step is refused on them. Graph capture as for tc_step, keyed by the slots and tau. */
+int tc_srk_step(tc_engine* e, double t, double tau, int vel_slot, int p_slot, int dpdt_slot);
+/* Butcher tables of one IMEX method: (s+1) x s row-major, row s = weights
+ * (tIMEXMethod::ButcherE/ButcherI, d of CalcD and alpha_tau_max; collocated_SRK/imex.cpp).
+ * Each output may be NULL; be and bi need s*(s+1) doubles, d s doubles. No device needed. */
+int tc_imex_table(int method, int* nstages, int* type, double* be, double* bi, double* d, double* alpha_tau_max);
+
 /* on != 0: tc_step captures each distinct (cur, bdf_order, tau) step into a HIP
  * graph on first use and replays it afterwards, so a step is one graph launch
  * instead of about ten kernel launches (the results are identical). Off by

@@ -13,7 +13,9 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libtc_oracle.so")
 
-SCHEME_HW, SCHEME_VV, SCHEME_AB3 = 0, 1, 2
+SCHEME_HW, SCHEME_VV, SCHEME_AB3, SCHEME_SRK = 0, 1, 2, 3
+IMEX_METHODS = ("ARS_121", "ARS_232", "ARS_343", "MARS_343", "ARK3", "ARK4", "MARK3", "BHR_553")
+IMEX_TYPE_A, IMEX_TYPE_ARS, IMEX_TYPE_CK = 0, 1, 2
 KIND_VEL, KIND_FLUX, KIND_PRES, KIND_NUT, KIND_TAU = 0, 1, 2, 3, 4
 
 
@@ -73,6 +75,8 @@ def lib():
         L.or_set_field.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.or_get_field.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.or_step.argtypes = [vp, d, d, ctypes.c_int, ctypes.c_int]
+        L.or_imex_table.argtypes = [ctypes.c_int, P(ctypes.c_int), dp, dp, dp, dp]
+        L.or_srk_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, d, d, d, dp, dp, dp]
         L.or_cfr_correct.argtypes = [vp, dp]
         L.or_cfr_correct.restype = d
         L.or_tanh_mesh.argtypes = [ctypes.c_int, d, dp]
@@ -104,6 +108,19 @@ def fft(x, sign=-1):
 
 def index_selfcheck(nx, ny, nz):
     return lib().or_index_selfcheck(nx, ny, nz)
+
+
+def imex_table(method):
+    """Butcher tables of one IMEX method (collocated_SRK/imex.cpp): dict with
+    s, type, E and I ((s+1) x s; row s = weights), d (CalcD) and alpha_tau_max."""
+    be, bi, d, am = np.zeros(42), np.zeros(42), np.zeros(6), np.zeros(2)
+    t = ctypes.c_int(0)
+    s = lib().or_imex_table(int(method), ctypes.byref(t), _p(be), _p(bi), _p(d), _p(am))
+    if s == 0:
+        raise ValueError(f"unknown IMEX method {method}")
+    n = s * (s + 1)
+    return {"s": s, "type": t.value, "E": be[:n].reshape(s + 1, s), "I": bi[:n].reshape(s + 1, s),
+            "d": d[:s].copy(), "alpha_tau_max": am}
 
 
 def set_threads(n):
@@ -219,7 +236,7 @@ class Solver:
         return lib().or_kinetic_energy(self._h, _p(np.ascontiguousarray(u)))
 
     def profiles(self, u, p):
-        w = 10 if self.scheme == SCHEME_AB3 else 12
+        w = 10 if self.scheme in (SCHEME_AB3, SCHEME_SRK) else 12
         out = np.zeros((self.nz, w))
         lib().or_profiles(self._h, _p(np.ascontiguousarray(u)), _p(np.ascontiguousarray(p)), _p(out))
         return out
@@ -248,6 +265,17 @@ class Solver:
     def step(self, t, tau, cur, order):
         if lib().or_step(self._h, t, tau, cur, order):
             raise RuntimeError(lib().or_last_error().decode())
+
+    def srk_step(self, t, tau, velocity, pressure, dpdt=None, *, method=2, stab_type=1, alpha_tau=1.0):
+        """data::Step of the collocated SRK solver (collocated_SRK/func_global.h:317-408)
+        on copies; returns (velocity, pressure, dpdt)."""
+        v = np.array(velocity, dtype=np.float64, copy=True)
+        p = np.array(pressure, dtype=np.float64, copy=True)
+        q = np.zeros_like(p) if dpdt is None else np.array(dpdt, dtype=np.float64, copy=True)
+        if lib().or_srk_step(self._h, int(method), int(stab_type), float(alpha_tau), float(t), float(tau),
+                             _p(v), _p(p), _p(q)):
+            raise RuntimeError(lib().or_last_error().decode())
+        return v, p, q
 
     def cfr_correct(self, u):
         u = np.array(u, dtype=np.float64, copy=True)

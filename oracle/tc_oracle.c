@@ -169,6 +169,8 @@ int or_index_selfcheck(int nx, int ny, int nz) {
 }
 
 #define SQR(x) ((x) * (x))
+/* collocated node layout: the AB3 and SRK solvers */
+#define IS_COLL(sc) ((sc) == OR_SCHEME_AB3 || (sc) == OR_SCHEME_SRK)
 
 /* ------------------------------------------------------------------------- */
 /* Mesh and matrix set-up                                                     */
@@ -296,8 +298,9 @@ static void init_ab3_mesh(or_solver* s) {
 }
 
 or_solver* or_create(const or_params* p, const double* zz) {
-    if (!p || !zz || p->nx < 4 || p->ny < 4 || p->nz < 6 || p->scheme < 0 || p->scheme > 2 ||
-        (p->sgs_model && p->scheme != OR_SCHEME_AB3)) {
+    if (!p || !zz || p->nx < 4 || p->ny < 4 || p->nz < 6 || p->scheme < 0 || p->scheme > 3 ||
+        (p->sgs_model && !IS_COLL(p->scheme)) ||
+        (p->scheme == OR_SCHEME_SRK && (p->visc_type || p->turb_visc || p->sgs_model || p->cfr_ubulk > 0))) {
         set_err("bad parameters");
         return NULL;
     }
@@ -308,7 +311,7 @@ or_solver* or_create(const or_params* p, const double* zz) {
     s->N3D = s->NXY * s->NZ;
     s->inv_hx = 1. / p->hx;
     s->inv_hy = 1. / p->hy;
-    int NZ = s->NZ, nzz = p->scheme == OR_SCHEME_AB3 ? NZ : NZ + 1;
+    int NZ = s->NZ, nzz = IS_COLL(p->scheme) ? NZ : NZ + 1;
     s->ZZ = (double*)calloc((size_t)nzz, sizeof(double));
     memcpy(s->ZZ, zz, sizeof(double) * (size_t)nzz);
     s->DZ = (double*)calloc((size_t)NZ, sizeof(double));
@@ -318,7 +321,7 @@ or_solver* or_create(const or_params* p, const double* zz) {
     s->GCZ = (double*)calloc((size_t)NZ, sizeof(double));
     s->MAT_A = (double*)calloc((size_t)3 * NZ, sizeof(double));
     s->KERG = (double*)calloc((size_t)NZ, sizeof(double));
-    if (p->scheme == OR_SCHEME_AB3) {
+    if (IS_COLL(p->scheme)) {
         init_ab3_mesh(s);
         s->area = p->hx * s->NX * p->hy * s->NY * (s->ZZ[NZ - 1] - s->ZZ[0]);
     } else {
@@ -358,7 +361,7 @@ void or_destroy(or_solver* s) {
 
 int or_mesh_array(const or_solver* s, const char* name, double* out, int cap) {
     const double* src = NULL;
-    int n = 0, NZ = s->NZ, ab3 = s->P.scheme == OR_SCHEME_AB3;
+    int n = 0, NZ = s->NZ, ab3 = IS_COLL(s->P.scheme);
     if (!strcmp(name, "ZZ")) { src = s->ZZ; n = ab3 ? NZ : NZ + 1; }
     else if (!strcmp(name, "DZ")) { src = s->DZ; n = ab3 ? NZ - 1 : NZ; }
     else if (!strcmp(name, "MAT_A")) { src = s->MAT_A; n = s->P.scheme == OR_SCHEME_VV ? 3 * NZ : NZ; }
@@ -1125,7 +1128,8 @@ void or_normalize_pressure(or_solver* s, double* f) {
     } else {
 #pragma omp parallel for reduction(+ : psum)
         for (int i = 0; i < N3D; i++) psum += f[i] * cell_volume(s, i);
-        double p_shift = psum * s->P.hx * s->P.hy / s->area;
+        /* SRK: psum / area (collocated_SRK/func_global.h:170-177) */
+        double p_shift = s->P.scheme == OR_SCHEME_SRK ? psum / s->area : psum * s->P.hx * s->P.hy / s->area;
 #pragma omp parallel for
         for (int i = 0; i < N3D; i++) f[i] -= p_shift;
     }
@@ -1159,7 +1163,7 @@ void or_solve_pressure(or_solver* s, double* XX, const double* f) {
 /* ------------------------------------------------------------------------- */
 void or_flux_term(or_solver* s, const double* u, int do_conv, int do_visc, double* kuhat) {
     const int N3D = s->N3D, NXY = s->NXY;
-    if (s->P.scheme == OR_SCHEME_AB3) {
+    if (IS_COLL(s->P.scheme)) {
 #pragma omp parallel for
         for (int in = 0; in < N3D; in++) flux_ab3(s, in, u, do_conv, do_visc, kuhat);
         return;
@@ -1185,7 +1189,7 @@ void or_flux_term(or_solver* s, const double* u, int do_conv, int do_visc, doubl
 
 void or_explicit_term(or_solver* s, const double* u, double* kuhat) {
     const int N3D = s->N3D, NXY = s->NXY;
-    if (s->P.scheme == OR_SCHEME_AB3) {
+    if (IS_COLL(s->P.scheme)) {
         or_flux_term(s, u, 1, 1, kuhat);
         return;
     }
@@ -1212,7 +1216,7 @@ void or_visc_flux_naive(or_solver* s, const double* u, double* f) {
 void or_apply_gradient(or_solver* s, const double* p, double* g) {
 #pragma omp parallel for
     for (int in = 0; in < s->N3D; in++) {
-        if (s->P.scheme == OR_SCHEME_AB3) grad_ab3(s, in, p, g);
+        if (IS_COLL(s->P.scheme)) grad_ab3(s, in, p, g);
         else grad_staggered(s, in, p, g);
     }
 }
@@ -1220,7 +1224,7 @@ void or_apply_gradient(or_solver* s, const double* p, double* g) {
 void or_apply_divergence(or_solver* s, const double* v, double* dv) {
 #pragma omp parallel for
     for (int in = 0; in < s->N3D; in++) {
-        if (s->P.scheme == OR_SCHEME_AB3) div_ab3(s, in, v, dv);
+        if (IS_COLL(s->P.scheme)) div_ab3(s, in, v, dv);
         else div_staggered(s, in, v, dv);
     }
 }
@@ -1235,7 +1239,7 @@ void or_recalc_turb_visc(or_solver* s, const double* u) {
     if (!s->nu_t) return;
 #pragma omp parallel for
     for (int in = 0; in < s->N3D; in++) {
-        if (s->P.scheme == OR_SCHEME_AB3) turb_visc_ab3(s, in, u);
+        if (IS_COLL(s->P.scheme)) turb_visc_ab3(s, in, u);
         else turb_visc_staggered(s, in, u);
     }
 }
@@ -1252,7 +1256,7 @@ void or_integral(or_solver* s, const double* u, double out[3]) {
     const double hx = s->P.hx, hy = s->P.hy;
 #pragma omp parallel for reduction(+ : a, b, c)
     for (int i = 0; i < s->N3D; i++) {
-        if (s->P.scheme == OR_SCHEME_AB3) {
+        if (IS_COLL(s->P.scheme)) {
             double vol = cell_volume(s, i);
             a += u[3 * i] * vol; b += u[3 * i + 1] * vol; c += u[3 * i + 2] * vol;
         } else {
@@ -1271,7 +1275,7 @@ double or_kinetic_energy(or_solver* s, const double* u) {
     const int NXY = s->NXY;
 #pragma omp parallel for reduction(+ : e)
     for (int i = 0; i < s->N3D; i++) {
-        if (s->P.scheme == OR_SCHEME_AB3) {
+        if (IS_COLL(s->P.scheme)) {
             e += (u[3 * i] * u[3 * i] + u[3 * i + 1] * u[3 * i + 1] + u[3 * i + 2] * u[3 * i + 2]) * cell_volume(s, i);
         } else {
             int iz = i / NXY;
@@ -1284,7 +1288,7 @@ double or_kinetic_energy(or_solver* s, const double* u) {
 
 /* CalcProfiles, CPU branch (staggered/func_global.h:784-807; collocated_AB3/func_global.h:272-288). */
 int or_profiles(or_solver* s, const double* vel, const double* p, double* av) {
-    const int NZ = s->NZ, NXY = s->NXY, ab3 = s->P.scheme == OR_SCHEME_AB3;
+    const int NZ = s->NZ, NXY = s->NXY, ab3 = IS_COLL(s->P.scheme);
     const int W = ab3 ? 10 : 12;
 #pragma omp parallel for
     for (int iz = 0; iz < NZ; iz++) {
@@ -1311,7 +1315,7 @@ double or_cfl(or_solver* s, const double* u, double tau, int mode) {
 #pragma omp parallel for reduction(max : cfl_max)
     for (int i = 0; i < s->N3D; i++) {
         double inv_hz, visc_coeff, fac;
-        if (s->P.scheme == OR_SCHEME_AB3) {
+        if (IS_COLL(s->P.scheme)) {
             inv_hz = 1 / cell_height(s, i);
             visc_coeff = s->P.visc + nut_at_node(s, i);
             fac = 4 * (s->P.visc_type + 1);
@@ -1363,7 +1367,7 @@ double or_cfr_correct(or_solver* s, double* u) {
     double q[3];
     or_integral(s, u, q);
     double vol_u = 0;
-    if (s->P.scheme == OR_SCHEME_AB3) {
+    if (IS_COLL(s->P.scheme)) {
         for (int iz = 1; iz < s->NZ - 1; iz++) vol_u += 0.5 * (s->DZ[iz - 1] + s->DZ[iz]);
         vol_u *= s->P.hx * s->P.hy * s->NXY;
     } else {
@@ -1372,7 +1376,7 @@ double or_cfr_correct(or_solver* s, double* u) {
     double shift = (s->P.cfr_ubulk * s->area - q[0]) / vol_u;
 #pragma omp parallel for
     for (int i = 0; i < s->N3D; i++) {
-        if (s->P.scheme == OR_SCHEME_AB3 && is_wall(s, i)) continue;
+        if (IS_COLL(s->P.scheme) && is_wall(s, i)) continue;
         u[3 * i] += shift;
     }
     return shift;
@@ -1382,6 +1386,7 @@ double or_cfr_correct(or_solver* s, double* u) {
 int or_step(or_solver* s, double t_start, double tau, int cur, int bdf_order) {
     (void)t_start;
     if (bdf_order < 1 || bdf_order > 3) { set_err("Wrong bdf_order"); return 1; }
+    if (s->P.scheme == OR_SCHEME_SRK) { set_err("the SRK solver steps with or_srk_step"); return 1; }
     const int N3D = s->N3D;
     int next = (cur + 1) % 4, prev = (cur + 3) % 4, prev2 = (cur + 2) % 4;
     const double inv_tau = 1. / tau;
@@ -1397,7 +1402,7 @@ int or_step(or_solver* s, double t_start, double tau, int cur, int bdf_order) {
         else if (bdf_order == 2) Fn[k] = 2. * Fc[k] - Fp[k] + inv_tau * (2 * Vc[k] - 0.5 * Vp[k]);
         else Fn[k] = 3 * Fc[k] - 3 * Fp[k] + Fp2[k] + inv_tau * (3 * Vc[k] - 1.5 * Vp[k] + C1_3 * Vp2[k]);
     }
-    if (s->P.scheme == OR_SCHEME_AB3 && s->P.stab_type != 0) {
+    if (IS_COLL(s->P.scheme) && s->P.stab_type != 0) {
         or_apply_gradient(s, s->pres[cur], Vn);
 #pragma omp parallel for
         for (int k = 0; k < 3 * N3D; k++) Fn[k] -= Vn[k];
@@ -1412,6 +1417,250 @@ int or_step(or_solver* s, double t_start, double tau, int cur, int bdf_order) {
 #pragma omp parallel for
     for (int k = 0; k < 3 * N3D; k++) Vn[k] = (Fn[k] - Vn[k]) * tau_over_bd0;
     if (s->P.cfr_ubulk > 0) or_cfr_correct(s, Vn);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Collocated SRK: IMEX Runge-Kutta stepper (collocated_SRK/)                 */
+/* ------------------------------------------------------------------------- */
+#define OR_MAX_STAGES 6
+
+/* Butcher tables, restated from collocated_SRK/imex.cpp:48-239 (published
+ * coefficients: Ascher-Ruuth-Spiteri 1997, Kennedy-Carpenter 2003,
+ * Boscarino-Russo 2007, Boscarino 2009). Row-major, (s+1) rows of s entries;
+ * the last row holds the weights. Returns s, 0 for an unknown method. */
+static int imex_rows(int method, int* type, double* BE, double* BI) {
+    const double gL = 0.435866521508458999416019451193556842529; /* middle root of 6x^3-18x^2+9x-1 */
+    memset(BE, 0, sizeof(double) * 42);
+    memset(BI, 0, sizeof(double) * 42);
+#define ROW(T, r, ...) do { const double v_[] = {__VA_ARGS__}; memcpy(T + (r) * S, v_, sizeof v_); } while (0)
+    int S = 0;
+    switch (method) {
+    case OR_IMEX_ARS_121: /* :48-63 */
+        S = 2; *type = OR_IMEX_TYPE_ARS;
+        ROW(BE, 1, 1., 0.); ROW(BE, 2, 0., 1.);
+        ROW(BI, 1, 0., 1.); ROW(BI, 2, 0., 1.);
+        break;
+    case OR_IMEX_ARS_232: { /* :66-84 */
+        S = 3; *type = OR_IMEX_TYPE_ARS;
+        const double g = 1. - 0.5 * sqrt(2.), dl = -2. * sqrt(2.) / 3.;
+        ROW(BE, 1, g, 0., 0.); ROW(BE, 2, dl, 1. - dl, 0.); ROW(BE, 3, 0., 1. - g, g);
+        ROW(BI, 1, 0., g, 0.); ROW(BI, 2, 0., 1. - g, g); ROW(BI, 3, 0., 1. - g, g);
+        break;
+    }
+    case OR_IMEX_ARS_343: { /* :87-112 */
+        S = 4; *type = OR_IMEX_TYPE_ARS;
+        const double g = gL;
+        const double b1 = -1.5 * g * g + 4 * g - 0.25, b2 = 1.5 * g * g - 5. * g + 1.25;
+        const double a42 = 0.5529291479, a43 = 0.5529291479;
+        const double c1 = 1. - 4.5 * g + 1.5 * g * g, c2 = 2.75 - 10.5 * g + 3.75 * g * g;
+        ROW(BI, 1, 0., g, 0., 0.); ROW(BI, 2, 0., (1. - g) / 2., g, 0.);
+        ROW(BI, 3, 0., b1, b2, g); ROW(BI, 4, 0., b1, b2, g);
+        ROW(BE, 1, g, 0., 0., 0.);
+        ROW(BE, 2, c1 * a42 + c2 * a43 - 3.5 + 13. * g - 4.5 * g * g,
+            -c1 * a42 - c2 * a43 + 4. - 12.5 * g + 4.5 * g * g, 0., 0.);
+        ROW(BE, 3, 1. - a42 - a43, a42, a43, 0.);
+        ROW(BE, 4, 0., b1, b2, g);
+        break;
+    }
+    case OR_IMEX_MARS_343: { /* :116-136 */
+        S = 4; *type = OR_IMEX_TYPE_ARS;
+        const double g = gL, b2 = 1.20849664917601;
+        ROW(BI, 1, 0., g, 0., 0.); ROW(BI, 2, 0., 0.28206673924577, g, 0.);
+        ROW(BI, 3, 0., b2, 1. - g - b2, g); ROW(BI, 4, 0., b2, 1. - g - b2, g);
+        ROW(BE, 1, g, 0., 0., 0.); ROW(BE, 2, 0.535396540307354, 0.182536720446875, 0., 0.);
+        ROW(BE, 3, 0.63041255815287, -0.83193390106308, 1.20152134291021, 0.);
+        ROW(BE, 4, 0., b2, 1. - g - b2, g);
+        break;
+    }
+    case OR_IMEX_ARK3: { /* :139-160 */
+        S = 4; *type = OR_IMEX_TYPE_CK;
+        const double g = gL;
+        const double b1 = 1471266399579. / 7840856788654., b2 = -4482444167858. / 7529755066697.;
+        const double b3 = 1. - g - b1 - b2;
+        ROW(BI, 1, g, g, 0., 0.);
+        ROW(BI, 2, 2746238789719. / 10658868560708., -640167445237. / 6845629431997., g, 0.);
+        ROW(BI, 3, b1, b2, b3, g); ROW(BI, 4, b1, b2, b3, g);
+        ROW(BE, 1, 2 * g, 0., 0., 0.);
+        ROW(BE, 2, 5535828885825. / 10492691773637., 788022342437. / 10882634858940., 0., 0.);
+        ROW(BE, 3, 6485989280629. / 16251701735622., -4246266847089. / 9704473918619.,
+            10755448449292. / 10357097424841., 0.);
+        ROW(BE, 4, b1, b2, b3, g);
+        break;
+    }
+    case OR_IMEX_ARK4: { /* :163-186 */
+        S = 6; *type = OR_IMEX_TYPE_CK;
+        const double b0 = 82889. / 524892., b2 = 15625. / 83664., b3 = 69875. / 102672., b4 = -2260. / 8211., b5 = 0.25;
+        ROW(BI, 1, 0.25, 0.25, 0., 0., 0., 0.);
+        ROW(BI, 2, 8611. / 62500., -1743. / 31250., 0.25, 0., 0., 0.);
+        ROW(BI, 3, 5012029. / 34652500., -654441. / 2922500., 174375. / 388108., 0.25, 0., 0.);
+        ROW(BI, 4, 15267082809. / 155376265600., -71443401. / 120774400., 730878875. / 902184768.,
+            2285395. / 8070912., 0.25, 0.);
+        ROW(BI, 5, b0, 0., b2, b3, b4, b5);
+        ROW(BI, 6, b0, 0., b2, b3, b4, b5);
+        ROW(BE, 1, 0.5, 0., 0., 0., 0., 0.);
+        ROW(BE, 2, 13861. / 62500., 6889. / 62500., 0., 0., 0., 0.);
+        ROW(BE, 3, -116923316275. / 2393684061468., -2731218467317. / 15368042101831.,
+            9408046702089. / 11113171139209., 0., 0., 0.);
+        ROW(BE, 4, -451086348788. / 2902428689909., -2682348792572. / 7519795681897.,
+            12662868775082. / 11960479115383., 3355817975965. / 11060851509271., 0., 0.);
+        ROW(BE, 5, 647845179188. / 3216320057751., 73281519250. / 8382639484533., 552539513391. / 3454668386233.,
+            3354512671639. / 8306763924573., 4040. / 17871., 0.);
+        ROW(BE, 6, b0, 0., b2, b3, b4, b5);
+        break;
+    }
+    case OR_IMEX_MARK3: { /* :189-209 */
+        S = 4; *type = OR_IMEX_TYPE_CK;
+        const double g = gL, b1 = 0.60424832458800;
+        ROW(BI, 1, g, g, 0., 0.); ROW(BI, 2, -4.30002662176923, 2.26541338346372, g, 0.);
+        ROW(BI, 3, b1, 0., 1. - g - b1, g); ROW(BI, 4, b1, 0., 1. - g - b1, g);
+        ROW(BE, 1, 2 * g, 0., 0., 0.); ROW(BE, 2, -3.06478674186224, 1.46604002506519, 0., 0.);
+        ROW(BE, 3, 0.21444560762133, 0.71075364965269, 0.07480074272597, 0.);
+        ROW(BE, 4, b1, 0., 1. - g - b1, g);
+        break;
+    }
+    case OR_IMEX_BHR_553: { /* :213-239 */
+        S = 5; *type = OR_IMEX_TYPE_CK;
+        const double g = 424782. / 974569.;
+        const double i41 = 487698502336740678603511. / 1181159636928185920260208.;
+        const double i43 = 302987763081184622639300143137943089. / 1535359944203293318639180129368156500.;
+        const double i44 = -105235928335100616072938218863. / 2282554452064661756575727198000.;
+        ROW(BI, 1, g, g, 0., 0., 0.);
+        ROW(BI, 2, g, -31733082319927313. / 455705377221960889379854647102., g, 0., 0.);
+        ROW(BI, 3, -3012378541084922027361996761794919360516301377809610. /
+                      45123394056585269977907753045030512597955897345819349.,
+            -62865589297807153294268. / 102559673441610672305587327019095047.,
+            418769796920855299603146267001414900945214277000. / 212454360385257708555954598099874818603217167139.,
+            g, 0.);
+        ROW(BI, 4, i41, 0., i43, i44, g);
+        ROW(BI, 5, i41, 0., i43, i44, g);
+        ROW(BE, 1, 2. * g, 0., 0., 0., 0.);
+        ROW(BE, 2, g, g, 0., 0., 0.);
+        ROW(BE, 3, -475883375220285986033264. / 594112726933437845704163., 0.,
+            1866233449822026827708736. / 594112726933437845704163., 0., 0.);
+        ROW(BE, 4, 62828845818073169585635881686091391737610308247. / 176112910684412105319781630311686343715753056000.,
+            -i43, 262315887293043739337088563996093207. / 297427554730376353252081786906492000.,
+            -987618231894176581438124717087. / 23877337660202969319526901856000., 0.);
+        ROW(BE, 5, i41, 0., i43, i44, g);
+        break;
+    }
+    default:
+        return 0;
+    }
+#undef ROW
+    return S;
+}
+
+int or_imex_table(int method, int* type, double* be, double* bi, double* d, double* alpha_tau_max) {
+    double BE[42], BI[42];
+    int ty = 0;
+    const int S = imex_rows(method, &ty, BE, BI);
+    if (!S) return 0;
+    if (type) *type = ty;
+    if (be) memcpy(be, BE, sizeof(double) * (size_t)(S * (S + 1)));
+    if (bi) memcpy(bi, BI, sizeof(double) * (size_t)(S * (S + 1)));
+    if (d) { /* CalcD (collocated_SRK/imex.cpp:35-45) */
+        const double ass = BI[S * S - 1];
+        d[0] = 1.;
+        for (int j = 1; j < S; j++) {
+            d[j] = 0.;
+            for (int k = 0; k < j; k++) d[j] -= BI[j * S + k] * d[k] / ass;
+        }
+    }
+    if (alpha_tau_max) alpha_tau_max[0] = alpha_tau_max[1] = 2.;
+    return S;
+}
+
+/* data::Step (collocated_SRK/func_global.h:317-408) with ImplicitTerm = 0 and
+ * ImplicitStage = copy (:310-316). Ku is kept, as the reference keeps it. */
+int or_srk_step(or_solver* s, int method, int stab_type, double alpha_tau, double t_start, double tau,
+                double* velocity, double* pressure, double* dpdt) {
+    (void)t_start;
+    if (s->P.scheme != OR_SCHEME_SRK) { set_err("or_srk_step needs scheme OR_SCHEME_SRK"); return 1; }
+    double BE[42], BI[42], d[OR_MAX_STAGES];
+    int type = 0;
+    const int S = or_imex_table(method, &type, BE, BI, d, NULL);
+    if (!S) { set_err("unknown IMEX method"); return 1; }
+    if (stab_type && !dpdt) { set_err("stab_type 1 needs dpdt"); return 1; }
+    const int N = s->N3D, N3 = 3 * N;
+    const int ck = type == OR_IMEX_TYPE_CK;
+    const double ass = BI[S * S - 1], inv_ass = 1. / ass;
+    const double tau_stage = tau * ass, inv_tau_stage = 1. / tau_stage;
+    const double alpha = alpha_tau / tau;
+    double *Ku[OR_MAX_STAGES], *Kuhat[OR_MAX_STAGES], *mujt[OR_MAX_STAGES], *qj[OR_MAX_STAGES];
+    for (int i = 0; i < S; i++) {
+        Ku[i] = (double*)calloc((size_t)N3, sizeof(double));
+        Kuhat[i] = (double*)calloc((size_t)N3, sizeof(double));
+        mujt[i] = (double*)calloc((size_t)N, sizeof(double));
+        qj[i] = (double*)calloc((size_t)N, sizeof(double));
+    }
+    double* nu1 = (double*)calloc((size_t)N3, sizeof(double));
+    double* bv = (double*)calloc((size_t)N3, sizeof(double));
+    double* gp = (double*)calloc((size_t)N3, sizeof(double));
+    double* u = (double*)calloc((size_t)N3, sizeof(double));
+    double* bs = (double*)calloc((size_t)N, sizeof(double));
+    double* pj = (double*)calloc((size_t)N, sizeof(double));
+    double* muj = (double*)calloc((size_t)N, sizeof(double));
+    double* pjhat = (double*)calloc((size_t)N, sizeof(double));
+
+    if (type == OR_IMEX_TYPE_CK || type == OR_IMEX_TYPE_ARS) {
+        or_explicit_term(s, velocity, Kuhat[0]);
+        or_apply_gradient(s, pressure, gp);
+        for (int k = 0; k < N3; k++) { Ku[0][k] = 0.; Kuhat[0][k] -= gp[k]; }
+    }
+    if (ck)
+        for (int k = 0; k < N3; k++) nu1[k] = Ku[0][k] + Kuhat[0][k] + alpha * velocity[k];
+    const int j0 = type == OR_IMEX_TYPE_A ? 0 : 1;
+    for (int i = j0; i < S; i++) {
+        for (int k = 0; k < N3; k++) {
+            double b = velocity[k];
+            for (int j = 0; j < i; j++) b += tau * BI[i * S + j] * Ku[j][k] + tau * BE[i * S + j] * Kuhat[j][k];
+            bv[k] = b;
+        }
+        memcpy(u, bv, sizeof(double) * (size_t)N3);                             /* ImplicitStage */
+        for (int k = 0; k < N3; k++) Ku[i][k] = (u[k] - bv[k]) * inv_tau_stage;
+        or_explicit_term(s, u, Kuhat[i]);
+        for (int n = 0; n < N; n++) {
+            if (stab_type == 0) {
+                pjhat[n] = 0.;
+                muj[n] = pressure[n];
+                if (ck) muj[n] *= 1. - alpha_tau * BI[i * S];
+                for (int j = j0; j < i; j++) muj[n] += inv_ass * BI[i * S + j] * mujt[j][n];
+                bs[n] = muj[n] - alpha * tau_stage * pressure[n];
+            } else {
+                muj[n] = tau_stage * dpdt[n];
+                if (ck) muj[n] *= 1. - alpha_tau * BI[i * S];
+                pjhat[n] = pressure[n];
+                if (ck) pjhat[n] += tau * BI[i * S] * dpdt[n];
+                for (int j = j0; j < i; j++) {
+                    muj[n] += inv_ass * BI[i * S + j] * mujt[j][n];
+                    pjhat[n] += tau * BI[i * S + j] * qj[j][n];
+                }
+                bs[n] = muj[n] + pjhat[n] - alpha * tau_stage * tau_stage * dpdt[n];
+            }
+        }
+        or_apply_gradient(s, bs, bv);
+        for (int k = 0; k < N3; k++) {
+            bv[k] = Ku[i][k] + Kuhat[i][k] - bv[k] + alpha * velocity[k];
+            if (ck) bv[k] -= d[i] * nu1[k];
+        }
+        or_unapply_gradient(s, bv, mujt[i]);
+        for (int n = 0; n < N; n++) {
+            pj[n] = mujt[i][n] + bs[n];
+            mujt[i][n] -= alpha * tau_stage * (stab_type ? tau_stage * dpdt[n] : pressure[n]);
+            if (stab_type) qj[i][n] = inv_tau_stage * (pj[n] - pjhat[n]);
+        }
+        or_apply_gradient(s, pj, bv);
+        for (int k = 0; k < N3; k++) Kuhat[i][k] -= bv[k];
+    }
+    for (int k = 0; k < N3; k++)
+        for (int j = 0; j < S; j++) velocity[k] += tau * BI[S * S + j] * Ku[j][k] + tau * BE[S * S + j] * Kuhat[j][k];
+    for (int n = 0; n < N; n++) {
+        pressure[n] = pj[n];
+        if (stab_type) dpdt[n] = qj[S - 1][n];
+    }
+    for (int i = 0; i < S; i++) { free(Ku[i]); free(Kuhat[i]); free(mujt[i]); free(qj[i]); }
+    free(nu1); free(bv); free(gp); free(u); free(bs); free(pj); free(muj); free(pjhat);
     return 0;
 }
 
@@ -1442,7 +1691,7 @@ static double gauss(uint64_t* st) {
  * discrete curl of a smoothed Gaussian vector potential, scaled to noise_rms
  * (root mean square over the three components). Not projected. */
 void or_initial_field(or_solver* s, unsigned long long seed, double ubulk, double noise_rms, double* u) {
-    const int NX = s->NX, NY = s->NY, NZ = s->NZ, NXY = s->NXY, ab3 = s->P.scheme == OR_SCHEME_AB3;
+    const int NX = s->NX, NY = s->NY, NZ = s->NZ, NXY = s->NXY, ab3 = IS_COLL(s->P.scheme);
     const int NP = ab3 ? NZ : NZ + 1; /* potential planes: faces (staggered) or nodes (AB3) */
     const double hx = s->P.hx, hy = s->P.hy;
     const double z0 = s->ZZ[0], z1 = ab3 ? s->ZZ[NZ - 1] : s->ZZ[NZ];

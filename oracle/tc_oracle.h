@@ -8,7 +8,8 @@
  * Everything is FP64 and follows the reference's CPU (OpenMP) path:
  *   staggered/func_global.h, staggered/main.cpp:12-185,
  *   collocated_AB3/func_global.h, collocated_AB3/func_fluxes.h,
- *   collocated_AB3/main.cpp:197-301.
+ *   collocated_AB3/main.cpp:197-301, collocated_SRK/func_global.h,
+ *   collocated_SRK/imex.cpp.
  * Index helpers are generalised from the reference's power-of-2 bitmask form
  * (staggered/data_global_inc.h:4-9) to any NX, NY; or_index_selfcheck()
  * compares both forms at power-of-2 sizes.
@@ -23,7 +24,12 @@
 extern "C" {
 #endif
 
-enum { OR_SCHEME_HW = 0, OR_SCHEME_VV = 1, OR_SCHEME_AB3 = 2 };
+enum { OR_SCHEME_HW = 0, OR_SCHEME_VV = 1, OR_SCHEME_AB3 = 2,
+       OR_SCHEME_SRK = 3 /* collocated_SRK: AB3's mesh and operators (FD only), zero-mean pressure */ };
+/* IMEX Runge-Kutta methods of collocated_SRK/imex.h:26-42 and their types (imex.h:9) */
+enum { OR_IMEX_ARS_121 = 0, OR_IMEX_ARS_232 = 1, OR_IMEX_ARS_343 = 2, OR_IMEX_MARS_343 = 3,
+       OR_IMEX_ARK3 = 4, OR_IMEX_ARK4 = 5, OR_IMEX_MARK3 = 6, OR_IMEX_BHR_553 = 7 };
+enum { OR_IMEX_TYPE_A = 0, OR_IMEX_TYPE_ARS = 1, OR_IMEX_TYPE_CK = 2 };
 enum { OR_KIND_VEL = 0, OR_KIND_FLUX = 1, OR_KIND_PRES = 2, OR_KIND_NUT = 3, OR_KIND_TAU = 4 };
 
 typedef struct or_params {
@@ -78,6 +84,15 @@ double or_cfl(or_solver* s, const double* u, double tau, int mode);
 int or_set_field(or_solver* s, int kind, int slot, const double* src);
 int or_get_field(or_solver* s, int kind, int slot, double* dst);
 int or_step(or_solver* s, double t, double tau, int cur, int bdf_order);
+/* SRK: Butcher tables ((s+1) x s, row s = weights), d of CalcD and alpha_tau_max
+ * (collocated_SRK/imex.cpp). Arrays may be NULL; be/bi need 42 doubles, d 6.
+ * Returns the number of stages, 0 for an unknown method. */
+int or_imex_table(int method, int* type, double* be, double* bi, double* d, double* alpha_tau_max);
+/* One step of data::Step (collocated_SRK/func_global.h:317-408) in place on
+ * caller arrays: velocity (3*N3D), pressure and dpdt (N3D; dpdt is read and
+ * written only for stab_type 1). The solver must have scheme OR_SCHEME_SRK. */
+int or_srk_step(or_solver* s, int method, int stab_type, double alpha_tau, double t_start, double tau,
+                double* velocity, double* pressure, double* dpdt);
 /* Bulk-flow correction used by or_step when cfr_ubulk > 0 (returns the shift applied). */
 double or_cfr_correct(or_solver* s, double* u);
 

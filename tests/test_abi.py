@@ -65,3 +65,35 @@ def test_mesh_array_rejects_bad_mesh():
     zz = np.array([0.0, 0.5, 0.4, 1.0, 1.5, 1.8, 2.0])  # not increasing
     p = tca.make_params(tca.SCHEME_HW, 8, 8, 6, zz, visc=0.01, hx=0.1, hy=0.1)
     assert tca.lib().tc_mesh_array(ctypes.byref(p), b"DZ", None, 0) == -1
+
+
+@pytest.mark.parametrize("m", range(len(tca.IMEX_METHODS)))
+def test_imex_tables_match_oracle(m):
+    """tc_imex_table (host only) against the oracle's restatement of
+    collocated_SRK/imex.cpp, whose order conditions tests/test_oracle_srk.py checks."""
+    import oracle
+    a, b = tca.imex_table(m), oracle.imex_table(m)
+    assert a["s"] == b["s"] and a["type"] == b["type"]
+    for k in ("E", "I", "d", "alpha_tau_max"):
+        np.testing.assert_allclose(a[k], b[k], rtol=1e-15, atol=1e-16, err_msg=k)
+    assert tca.lib().tc_imex_table(99, None, None, None, None, None, None) != 0
+
+
+def test_srk_create_rejects_unsupported_options():
+    zz = parity.problem("ab3")[4]
+    h = ctypes.c_void_p()
+    for extra, msg in ((dict(turb_visc=1), "turbulence"), (dict(visc_type=1), "FD"),
+                       (dict(cfr_ubulk=1.0), "flow-rate"), (dict(imex=8), "IMEX"), (dict(stab_type=2), "StabType")):
+        kw = dict(visc=0.01, hx=0.1, hy=0.1, stab_type=1)
+        kw.update(extra)
+        p = tca.make_params(tca.SCHEME_SRK, 16, 8, len(zz), zz, **kw)
+        assert tca.lib().tc_create(ctypes.byref(p), 0, ctypes.byref(h)) == 1 and not h.value
+        assert msg in tca.lib().tc_last_error().decode(), extra
+
+
+def test_srk_mesh_data_is_collocated():
+    zz = parity.problem("ab3")[4]
+    p = tca.make_params(tca.SCHEME_SRK, 16, 12, len(zz), zz, visc=0.01, hx=0.1, hy=0.1)
+    q = tca.make_params(tca.SCHEME_AB3, 16, 12, len(zz), zz, visc=0.01, hx=0.1, hy=0.1)
+    for n in ("ZZ", "DZ", "MAT_A"):
+        np.testing.assert_array_equal(tca.mesh_array(p, n), tca.mesh_array(q, n))

@@ -16,7 +16,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libturbchannel_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "turbchannel.h")
 
-SCHEME_HW, SCHEME_VV, SCHEME_AB3 = 0, 1, 2
+SCHEME_HW, SCHEME_VV, SCHEME_AB3, SCHEME_SRK = 0, 1, 2, 3
+# IMEX Runge-Kutta methods of the SRK solver, in tc_imex order (collocated_SRK/imex.h:26-42)
+IMEX_METHODS = ("ARS_121", "ARS_232", "ARS_343", "MARS_343", "ARK3", "ARK4", "MARK3", "BHR_553")
+IMEX_TYPE_A, IMEX_TYPE_ARS, IMEX_TYPE_CK = 0, 1, 2
 FP64, FP32_BULK = 0, 1
 VEL, FLUX, PRES, NUT, TAU = 0, 1, 2, 3, 4
 KERNELS = ["turb_visc", "flux", "div", "fft_x", "fft_y", "zsolve", "ifft_y", "ifft_x", "project", "other"]
@@ -37,6 +40,7 @@ class TcParams(ctypes.Structure):
         ("cfr_ubulk", ctypes.c_double),
         ("zz", ctypes.POINTER(ctypes.c_double)),
         ("sgs_model", ctypes.c_int),
+        ("imex", ctypes.c_int), ("alpha_tau", ctypes.c_double),
     ]
 
 
@@ -71,6 +75,8 @@ def lib():
     L.tc_field_device_ptr.argtypes = [vp, c_int, c_int, P(vp)]
     L.tc_step.argtypes = [vp, c_double, c_double, c_int, c_int]
     L.tc_set_graphs.argtypes = [vp, c_int]
+    L.tc_srk_step.argtypes = [vp, c_double, c_double, c_int, c_int, c_int]
+    L.tc_imex_table.argtypes = [c_int, P(c_int), P(c_int), dp, dp, dp, dp]
     for n in ("tc_explicit_term", "tc_apply_gradient", "tc_unapply_gradient"):
         getattr(L, n).argtypes = [vp, c_int, c_int, c_int, c_int]
     L.tc_normalize_pressure.argtypes = [vp, c_int, c_int]
@@ -117,15 +123,30 @@ def _dptr(a):
 
 
 def make_params(scheme, nx, ny, nz, zz, *, precision=FP64, visc, sour_dpdx=0.0, inv_length_scale=1.0, hx, hy,
-                turb_visc=0, visc_type=None, stab_type=1, fd_mode=1, xmin=0.0, ymin=0.0, cfr_ubulk=0.0, sgs_model=0):
-    """tc_params for a problem; keeps the mesh array alive on the returned object."""
+                turb_visc=0, visc_type=None, stab_type=1, fd_mode=1, xmin=0.0, ymin=0.0, cfr_ubulk=0.0, sgs_model=0,
+                imex=2, alpha_tau=0.0):
+    """tc_params for a problem; keeps the mesh array alive on the returned object.
+    imex and alpha_tau apply to SCHEME_SRK (method index into IMEX_METHODS; alpha_tau
+    <= 0 selects the reference's default 0.5 * alpha_tau_max[stab_type])."""
     if visc_type is None:
         visc_type = 1 if turb_visc else 0
     zz = np.ascontiguousarray(zz, dtype=np.float64)
     p = TcParams(scheme, precision, nx, ny, nz, turb_visc, visc_type, stab_type, fd_mode, visc, sour_dpdx,
-                 inv_length_scale, hx, hy, xmin, ymin, cfr_ubulk, _dptr(zz), sgs_model)
+                 inv_length_scale, hx, hy, xmin, ymin, cfr_ubulk, _dptr(zz), sgs_model, imex, alpha_tau)
     p._zz = zz
     return p
+
+
+def imex_table(method):
+    """Butcher tables of one IMEX method (tIMEXMethod, collocated_SRK/imex.cpp): dict
+    with s, type, E and I ((s+1) x s, row s = weights), d (CalcD), alpha_tau_max."""
+    n, t = ctypes.c_int(0), ctypes.c_int(0)
+    _check(lib().tc_imex_table(int(method), ctypes.byref(n), ctypes.byref(t), None, None, None, None))
+    s = n.value
+    be, bi, d, am = np.zeros(s * (s + 1)), np.zeros(s * (s + 1)), np.zeros(s), np.zeros(2)
+    _check(lib().tc_imex_table(int(method), None, None, _dptr(be), _dptr(bi), _dptr(d), _dptr(am)))
+    return {"s": s, "type": t.value, "E": be.reshape(s + 1, s), "I": bi.reshape(s + 1, s), "d": d,
+            "alpha_tau_max": am}
 
 
 def mesh_array(params, name):
@@ -217,6 +238,11 @@ class Engine:
         """data::Step (staggered/func_global.h:897-942)."""
         _check(lib().tc_step(self._h, t, tau, cur, bdf_order))
 
+    def srk_step(self, t, tau, vel_slot=0, p_slot=0, dpdt_slot=1):
+        """data::Step of the collocated SRK solver (collocated_SRK/func_global.h:317-408),
+        in place on the velocity slot and the pressure and dpdt slots (PRES kind)."""
+        _check(lib().tc_srk_step(self._h, t, tau, vel_slot, p_slot, dpdt_slot))
+
     def graphs(self, on=True):
         """Replay repeated steps from captured HIP graphs (tc_set_graphs)."""
         _check(lib().tc_set_graphs(self._h, 1 if on else 0))
@@ -252,7 +278,7 @@ class Engine:
         return out.value
 
     def profiles(self, vel, p_slot=0):
-        nv = 10 if self.scheme == SCHEME_AB3 else 12
+        nv = 10 if self.scheme in (SCHEME_AB3, SCHEME_SRK) else 12
         out = np.zeros((self.nz, nv))
         _check(lib().tc_profiles(self._h, vel[0], vel[1], p_slot, _dptr(out), out.size))
         return out
@@ -260,7 +286,7 @@ class Engine:
     def diagnostics(self, vel_slot, p_slot=None, tau=1.0, cfl_mode=3):
         """Integral, kinetic energy, CFL and profiles of velocity slot vel_slot in
         one pass (tc_diagnostics); returns a dict of the four."""
-        nv = 10 if self.scheme == SCHEME_AB3 else 12
+        nv = 10 if self.scheme in (SCHEME_AB3, SCHEME_SRK) else 12
         prof = np.zeros((self.nz, nv))
         q = np.zeros(3)
         ke, cfl = ctypes.c_double(), ctypes.c_double()

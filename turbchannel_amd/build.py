@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libturbchannel_amd.so")
 DRIVER = os.path.join(HERE, "turbch")
 DRIVER_SRC = os.path.join(HERE, "driver", "turbch.cpp")
 SOURCES = ["host_setup.cpp", "kernels_stencil.hip", "kernels_spectral.hip", "kernels_reduce.hip", "kernels_mlp.hip",
-           "engine.cpp", "mlp.cpp"]
+           "engine.cpp", "mlp.cpp", "imex.cpp"]
 
 ARCH = os.environ.get("TC_OFFLOAD_ARCH", "gfx950")
 FLAGS = [

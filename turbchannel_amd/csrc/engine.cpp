@@ -113,6 +113,18 @@ struct tc_engine {
     void *ftwx = nullptr, *ftwy = nullptr, *fspec = nullptr, *fq = nullptr;
     Timing tim;
     std::vector<void*> allocs;
+    // collocated SRK stepper (scheme TC_SCHEME_SRK; P.scheme holds TC_SCHEME_AB3
+    // for the shared kernels)
+    bool srk = false;
+    int srk_stab = 0;
+    double alpha_tau = 0;
+    ImexTable imex;
+    void* sK[kMaxImexStages] = {};     // Kuhat per stage (real3)
+    void* smujt[kMaxImexStages] = {};  // mujtilde per stage
+    void* sqj[kMaxImexStages] = {};    // qj per stage (StabType 1)
+    void* snu1 = nullptr;              // nu1tilde (methods of type CK)
+    void* swork = nullptr;             // stage velocity u, then buf_vector (real3)
+    void *sbs = nullptr, *spj = nullptr, *spjhat = nullptr;
     // captured steps, replayed when tc_step is called again with the same
     // (current slot, order, tau): one graph launch instead of ~10 kernel launches
     bool use_graphs = false;
@@ -243,6 +255,31 @@ int setup_device(tc_engine* e) {
         e->tau = p;
         if (int r = dalloc(e, &e->sgs_w, (size_t)kWBlock)) return r;
         if (int r = dalloc(e, &e->sgs_b, (size_t)kBBlock)) return r;
+    }
+    if (e->srk) {
+        const int S = e->imex.s;
+        for (int i = 0; i < S; i++) {
+            if (int r = dalloc(e, &p, 3 * n3)) return r;
+            e->sK[i] = p;
+            if (int r = dalloc(e, &p, n3)) return r;
+            e->smujt[i] = p;
+            if (e->srk_stab) {
+                if (int r = dalloc(e, &p, n3)) return r;
+                e->sqj[i] = p;
+            }
+        }
+        if (e->imex.type == TC_IMEX_TYPE_CK) {
+            if (int r = dalloc(e, &p, 3 * n3)) return r;
+            e->snu1 = p;
+        }
+        if (int r = dalloc(e, &p, 3 * n3)) return r;
+        e->swork = p;
+        if (int r = dalloc(e, &p, n3)) return r;
+        e->sbs = p;
+        if (int r = dalloc(e, &p, n3)) return r;
+        e->spj = p;
+        if (int r = dalloc(e, &p, n3)) return r;
+        e->spjhat = p;
     }
     if (int r = dalloc_bytes(e, &e->spec, (size_t)e->nm * e->nz * sizeof(Cplx<T>))) return r;
     // mesh arrays: ZZ, DZ, DZW, GC1, GC2, GCZ, cellh, each mstride long
@@ -476,9 +513,160 @@ int normalize(tc_engine* e, T* p) {
     else if (e->P.scheme == TC_SCHEME_VV) {
         coef = 1.0 / (e->hm.KERG_norm2 * nxy);
         w = (const T*)e->d_kerg;
-    } else coef = e->P.hx * e->P.hy / e->hm.area;
+    } else if (e->srk) coef = 1.0 / e->hm.area;  // zero mean (collocated_SRK/func_global.h:170-177)
+    else coef = e->P.hx * e->P.hy / e->hm.area;
     HIPCHK(launch_add_scaled_plane<T>(e->s, m, p, e->d_out, coef, w));
     return TC_OK;
+}
+
+// One step of the collocated SRK solver: data::Step (collocated_SRK/func_global.h:317-408)
+// with ImplicitTerm = 0 and ImplicitStage = copy (:310-316), so Ku is zero and the
+// stage velocity is buf_vector itself. Launches per stage: stage velocity (fused
+// with the previous stage's Kuhat -= G pj), explicit term, pressure right-hand
+// side, buf_vector, the five kernels of the pressure solve, and the pressure
+// update; the last stage's pressure and dpdt land in the caller's slots, and
+// the final velocity update applies the last Kuhat -= G pj on the fly.
+template <typename T>
+int srk_step(tc_engine* e, double tau, int vs, int ps, int qs) {
+    const ImexTable& tb = e->imex;
+    const int S = tb.s;
+    const double ass = tb.I[S * S - 1], inv_ass = 1.0 / ass;
+    const double tau_stage = tau * ass, inv_tau_stage = 1.0 / tau_stage;
+    const double alpha_tau = e->alpha_tau, alpha = alpha_tau / tau;
+    const bool ck = tb.type == TC_IMEX_TYPE_CK, stab = e->srk_stab != 0;
+    const int j0 = tb.type == TC_IMEX_TYPE_A ? 0 : 1;
+    const Mesh<T> m = e->mesh<T>();
+    T* vel = (T*)e->vel[vs];
+    T* p = (T*)e->pres[ps];
+    T* q = stab ? (T*)e->pres[qs] : nullptr;
+    T* K[kMaxImexStages];
+    for (int i = 0; i < S; i++) K[i] = (T*)e->sK[i];
+    T* work = (T*)e->swork;
+    T* nu1 = (T*)e->snu1;
+    hipEvent_t ev;
+    auto flux = [&](const T* u, T* out) -> int {
+        FluxArgs<T> a;
+        a.scheme = TC_SCHEME_AB3;
+        a.visc_type = 0;
+        a.order = 0;
+        a.vel = u;
+        a.fcur = out;
+        e->tbeg(TC_K_FLUX, ev);
+        HIPCHK(launch_flux<T>(e->s, m, a, nullptr));
+        e->tend(TC_K_FLUX, ev);
+        return TC_OK;
+    };
+    auto vec = [&](const SrkVec<T>& a) -> int {
+        e->tbeg(TC_K_PROJECT, ev);
+        HIPCHK(launch_srk_vec<T>(e->s, m, a));
+        e->tend(TC_K_PROJECT, ev);
+        return TC_OK;
+    };
+    if (j0 == 1) {  // Kuhat[0] = F(u_n) - G p_n; CK: nu1tilde = Kuhat[0] + alpha u_n
+        if (int r = flux(vel, K[0])) return r;
+        SrkVec<T> a;
+        a.KC = K[0];
+        a.S = p;
+        a.KCout = K[0];
+        if (ck) {
+            a.out = nu1;
+            a.ckc = T(1);
+            a.A = vel;
+            a.sa = (T)alpha;
+        }
+        if (int r = vec(a)) return r;
+    }
+    for (int i = j0; i < S; i++) {
+        {  // u = u_n + sum_j tau E_ij Kuhat[j]; the previous stage's Kuhat -= G pj on the way
+            SrkVec<T> a;
+            a.out = work;
+            a.X = vel;
+            const bool fuse = i > j0;
+            a.nk = fuse ? i - 1 : i;
+            for (int j = 0; j < a.nk; j++) {
+                a.K[j] = K[j];
+                a.c[j] = (T)(tau * tb.E[i * S + j]);
+            }
+            if (fuse) {
+                a.KC = K[i - 1];
+                a.S = (T*)e->spj;
+                a.KCout = K[i - 1];
+                a.ckc = (T)(tau * tb.E[i * S + i - 1]);
+            }
+            if (int r = vec(a)) return r;
+        }
+        if (int r = flux(work, K[i])) return r;
+        {
+            SrkPre<T> a;
+            a.stab = stab;
+            a.ck = ck;
+            a.p = p;
+            a.q = q;
+            a.bs = (T*)e->sbs;
+            a.pjhat = (T*)e->spjhat;
+            a.m0 = (T)(1. - alpha_tau * tb.I[i * S]);
+            a.ph0 = (T)(tau * tb.I[i * S]);
+            a.tau_stage = (T)tau_stage;
+            a.clast = stab ? (T)(alpha * tau_stage * tau_stage) : (T)(alpha * tau_stage);
+            a.nj = i - j0;
+            for (int j = j0; j < i; j++) {
+                a.mujt[j - j0] = (const T*)e->smujt[j];
+                a.cm[j - j0] = (T)(inv_ass * tb.I[i * S + j]);
+                a.qj[j - j0] = (const T*)e->sqj[j];
+                a.cq[j - j0] = (T)(tau * tb.I[i * S + j]);
+            }
+            e->tbeg(TC_K_OTHER, ev);
+            HIPCHK(launch_srk_pre<T>(e->s, m, a));
+            e->tend(TC_K_OTHER, ev);
+        }
+        {  // buf_vector = Kuhat[i] - G bs + alpha u_n (- d_i nu1tilde)
+            SrkVec<T> a;
+            a.out = work;
+            a.KC = K[i];
+            a.S = (T*)e->sbs;
+            a.ckc = T(1);
+            a.A = vel;
+            a.sa = (T)alpha;
+            if (ck) {
+                a.B = nu1;
+                a.sb = (T)(-tb.d[i]);
+            }
+            if (int r = vec(a)) return r;
+        }
+        if (int r = unapply<T>(e, work, (T*)e->smujt[i])) return r;
+        {
+            const bool last = i == S - 1;
+            SrkPost<T> a;
+            a.stab = stab;
+            a.last = last;
+            a.mujt = (T*)e->smujt[i];
+            a.bs = (T*)e->sbs;
+            a.p = p;
+            a.q = q;
+            a.pjhat = (T*)e->spjhat;
+            a.pj_out = last ? p : (T*)e->spj;
+            a.qj_out = last ? q : (T*)e->sqj[i];
+            a.c = (T)(alpha * tau_stage);
+            a.tau_stage = (T)tau_stage;
+            a.inv_tau_stage = (T)inv_tau_stage;
+            e->tbeg(TC_K_OTHER, ev);
+            HIPCHK(launch_srk_post<T>(e->s, m, a));
+            e->tend(TC_K_OTHER, ev);
+        }
+    }
+    // u_{n+1} = u_n + sum_j tau b_j Kuhat[j], the last Kuhat with its - G pj
+    SrkVec<T> a;
+    a.out = vel;
+    a.X = vel;
+    a.nk = S - 1;
+    for (int j = 0; j < S - 1; j++) {
+        a.K[j] = K[j];
+        a.c[j] = (T)(tau * tb.E[S * S + j]);
+    }
+    a.KC = K[S - 1];
+    a.S = p;
+    a.ckc = (T)(tau * tb.E[S * S + S - 1]);
+    return vec(a);
 }
 
 template <typename T>
@@ -554,8 +742,10 @@ extern "C" {
 
 const char* tc_last_error(void) { return g_err.c_str(); }
 
-int tc_mesh_array(const tc_params* p, const char* name, double* out, int cap) {
-    if (!p || !name) return -1;
+int tc_mesh_array(const tc_params* pin, const char* name, double* out, int cap) {
+    if (!pin || !name) return -1;
+    const tc_params kp = as_kernel_params(*pin);
+    const tc_params* p = &kp;
     HostMesh m;
     if (!build_mesh(*p, m).empty()) return -1;
     std::vector<double> v;
@@ -577,18 +767,28 @@ int tc_mesh_array(const tc_params* p, const char* name, double* out, int cap) {
     return (int)v.size();
 }
 
-int tc_make_initial_field(const tc_params* p, unsigned long long seed, double ubulk, double noise_rms, double* out) {
-    if (!p || !out) return fail(TC_ERR_ARG, "NULL argument");
+int tc_make_initial_field(const tc_params* pin, unsigned long long seed, double ubulk, double noise_rms, double* out) {
+    if (!pin || !out) return fail(TC_ERR_ARG, "NULL argument");
+    const tc_params p = as_kernel_params(*pin);
     HostMesh m;
-    std::string err = build_mesh(*p, m);
+    std::string err = build_mesh(p, m);
     if (!err.empty()) return fail(TC_ERR_ARG, err);
-    make_initial_field(*p, m, seed, ubulk, noise_rms, out);
+    make_initial_field(p, m, seed, ubulk, noise_rms, out);
     return TC_OK;
 }
 
-int tc_create(const tc_params* p, int device, tc_engine** out) {
-    if (!p || !out) return fail(TC_ERR_ARG, "NULL argument");
+int tc_create(const tc_params* pin, int device, tc_engine** out) {
+    if (!pin || !out) return fail(TC_ERR_ARG, "NULL argument");
     *out = nullptr;
+    ImexTable imex;
+    if (pin->scheme == TC_SCHEME_SRK) {
+        if (pin->visc_type != 0 || pin->turb_visc || pin->sgs_model || pin->cfr_ubulk > 0)
+            return fail(TC_ERR_ARG, "the SRK solver has the FD viscous term only, no turbulence model and no flow-rate control");
+        if (pin->stab_type != 0 && pin->stab_type != 1) return fail(TC_ERR_ARG, "StabType must be 0 or 1");
+        if (!imex_table(pin->imex, imex)) return fail(TC_ERR_ARG, "unknown IMEX method");
+    }
+    const tc_params kp = as_kernel_params(*pin);
+    const tc_params* p = &kp;
     if (p->precision != TC_FP64 && p->precision != TC_FP32_BULK) return fail(TC_ERR_ARG, "unknown precision");
     if (p->visc_type < 0 || p->visc_type > 1) return fail(TC_ERR_ARG, "visc_type must be 0 or 1");
     if (p->scheme != TC_SCHEME_AB3 && p->turb_visc && p->visc_type == 0)
@@ -610,6 +810,14 @@ int tc_create(const tc_params* p, int device, tc_engine** out) {
     e->P.zz = e->zz.data();
     err = build_zfactors(e->P, e->hm, e->zf);
     if (!err.empty()) return fail(TC_ERR_NUMERIC, err);
+    if (pin->scheme == TC_SCHEME_SRK) {
+        e->srk = true;
+        e->srk_stab = pin->stab_type;
+        e->imex = imex;
+        e->alpha_tau = pin->alpha_tau > 0 ? pin->alpha_tau : 0.5 * imex.alpha_tau_max[pin->stab_type];
+        // zero-mean normalisation: the AB3 shift without its hx*hy factor
+        for (double& c : e->zf.C) c /= p->hx * p->hy;
+    }
     if (!make_plan(p->nx, e->plx) || !make_plan(p->ny, e->ply)) return fail(TC_ERR_ARG, "nx and ny must have only the prime factors 2, 3 and 5");
     e->fp32 = p->precision == TC_FP32_BULK;
     e->rsz = e->fp32 ? 4 : 8;
@@ -686,18 +894,17 @@ int tc_field_device_ptr(tc_engine* e, int kind, int slot, void** dev_ptr) {
     return TC_OK;
 }
 
-int tc_step(tc_engine* e, double t, double tau, int cur, int bdf_order) {
-    (void)t;
-    if (int r = check(e)) return r;
-    if (bdf_order < 1 || bdf_order > 3) return fail(TC_ERR_ARG, "Wrong bdf_order");
-    if (cur < 0 || cur > 3) return fail(TC_ERR_ARG, "bad current_field");
-    if (!(tau > 0)) return fail(TC_ERR_ARG, "tau must be positive");
-    auto run = [&]() { return e->fp32 ? step<float>(e, tau, cur, bdf_order) : step<double>(e, tau, cur, bdf_order); };
-    if (!e->use_graphs || e->tim.on) return run();
-    if (e->P.sgs_model && !e->sgs_ready) return fail(TC_ERR_STATE, "learned closure weights are not set (tc_set_sgs_params)");
+}  // extern "C"
+
+namespace {
+
+// Runs one step through the engine's graph cache: captured on first use of
+// (a, b, tau), replayed afterwards.
+template <class F>
+int run_graph(tc_engine* e, int a, int b, double tau, F&& run) {
     unsigned long long tb;
     std::memcpy(&tb, &tau, sizeof tb);
-    const auto key = std::make_tuple(cur, bdf_order, tb);
+    const auto key = std::make_tuple(a, b, tb);
     auto it = e->graphs.find(key);
     if (it == e->graphs.end()) {
         if (e->graphs.size() >= 48) {  // step sizes changed many times: start over
@@ -721,6 +928,39 @@ int tc_step(tc_engine* e, double t, double tau, int cur, int bdf_order) {
     }
     HIPCHK(hipGraphLaunch(it->second, e->s));
     return TC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tc_step(tc_engine* e, double t, double tau, int cur, int bdf_order) {
+    (void)t;
+    if (int r = check(e)) return r;
+    if (e->srk) return fail(TC_ERR_STATE, "the SRK solver steps with tc_srk_step");
+    if (bdf_order < 1 || bdf_order > 3) return fail(TC_ERR_ARG, "Wrong bdf_order");
+    if (cur < 0 || cur > 3) return fail(TC_ERR_ARG, "bad current_field");
+    if (!(tau > 0)) return fail(TC_ERR_ARG, "tau must be positive");
+    auto run = [&]() { return e->fp32 ? step<float>(e, tau, cur, bdf_order) : step<double>(e, tau, cur, bdf_order); };
+    if (!e->use_graphs || e->tim.on) return run();
+    if (e->P.sgs_model && !e->sgs_ready) return fail(TC_ERR_STATE, "learned closure weights are not set (tc_set_sgs_params)");
+    return run_graph(e, cur, bdf_order, tau, run);
+}
+
+int tc_srk_step(tc_engine* e, double t, double tau, int vel_slot, int p_slot, int dpdt_slot) {
+    (void)t;
+    if (int r = check(e)) return r;
+    if (!e->srk) return fail(TC_ERR_STATE, "tc_srk_step needs an engine of scheme TC_SCHEME_SRK");
+    if (vel_slot < 0 || vel_slot > 3 || p_slot < 0 || p_slot > 3) return fail(TC_ERR_ARG, "bad slot");
+    if (e->srk_stab && (dpdt_slot < 0 || dpdt_slot > 3 || dpdt_slot == p_slot)) return fail(TC_ERR_ARG, "bad dpdt slot");
+    if (!(tau > 0)) return fail(TC_ERR_ARG, "tau must be positive");
+    auto run = [&]() {
+        return e->fp32 ? srk_step<float>(e, tau, vel_slot, p_slot, dpdt_slot)
+                       : srk_step<double>(e, tau, vel_slot, p_slot, dpdt_slot);
+    };
+    if (!e->use_graphs || e->tim.on) return run();
+    const int slots = vel_slot + 4 * p_slot + 16 * (e->srk_stab ? dpdt_slot : 0);
+    return run_graph(e, slots, -1, tau, run);
 }
 
 int tc_set_graphs(tc_engine* e, int on) {

@@ -54,6 +54,70 @@ template <typename T>
 hipError_t launch_add_scaled_plane(hipStream_t s, const Mesh<T>& m, T* f, const double* sum, double coef, const T* plane_w);
 
 // ---------------------------------------------------------------------------
+// Stage updates of the collocated IMEX Runge-Kutta (SRK) step, one thread per node
+// ---------------------------------------------------------------------------
+// kc = KC - G S (S == nullptr: no gradient), stored to KCout when that is set;
+// out = X + sum_{j<nk} c[j] K[j] + ckc kc + sa A + sb B, absent terms skipped.
+// out may alias X and KCout may alias KC.
+template <typename T>
+struct SrkVec {
+    const T* KC = nullptr;
+    const T* S = nullptr;
+    T* KCout = nullptr;
+    T* out = nullptr;
+    const T* X = nullptr;
+    int nk = 0;
+    const T* K[kMaxImexStages] = {};
+    T c[kMaxImexStages] = {};
+    T ckc = 0;
+    const T* A = nullptr;
+    T sa = 0;
+    const T* B = nullptr;
+    T sb = 0;
+};
+template <typename T>
+hipError_t launch_srk_vec(hipStream_t s, const Mesh<T>& m, const SrkVec<T>& a);
+
+// Right-hand side scalar of the stage pressure system (collocated_SRK/func_global.h:360-382).
+// stab 0: bs = (m0 p + sum_j cm[j] mujt[j]) - clast p          (m0 only for ck)
+// stab 1: muj = m0 (tau_stage q) + sum_j cm[j] mujt[j]; pjhat = p + ph0 q + sum_j cq[j] qj[j];
+//         bs = muj + pjhat - clast q                            (m0, ph0 only for ck)
+template <typename T>
+struct SrkPre {
+    int stab = 0, ck = 0;
+    const T* p = nullptr;
+    const T* q = nullptr;
+    T* bs = nullptr;
+    T* pjhat = nullptr;
+    T m0 = 1, ph0 = 0, tau_stage = 0, clast = 0;
+    int nj = 0;
+    const T* mujt[kMaxImexStages] = {};
+    T cm[kMaxImexStages] = {};
+    const T* qj[kMaxImexStages] = {};
+    T cq[kMaxImexStages] = {};
+};
+template <typename T>
+hipError_t launch_srk_pre(hipStream_t s, const Mesh<T>& m, const SrkPre<T>& a);
+
+// After the stage pressure solve (collocated_SRK/func_global.h:394-398):
+// pj = mujt + bs; mujt -= c (stab ? tau_stage q : p) (skipped when last);
+// stab: qj = inv_tau_stage (pj - pjhat). pj_out may alias p and qj_out q.
+template <typename T>
+struct SrkPost {
+    int stab = 0, last = 0;
+    T* mujt = nullptr;
+    const T* bs = nullptr;
+    const T* p = nullptr;
+    const T* q = nullptr;
+    const T* pjhat = nullptr;
+    T* pj_out = nullptr;
+    T* qj_out = nullptr;
+    T c = 0, tau_stage = 0, inv_tau_stage = 0;
+};
+template <typename T>
+hipError_t launch_srk_post(hipStream_t s, const Mesh<T>& m, const SrkPost<T>& a);
+
+// ---------------------------------------------------------------------------
 // Spectral pressure solve
 // ---------------------------------------------------------------------------
 // Forward x transform of the discrete divergence of v (computed on the fly; ApplyDivergence).

@@ -647,6 +647,132 @@ dim3 cell_grid(const Mesh<T>& m) {
     return dim3((m.nxy + kBlock - 1) / kBlock, m.nz);
 }
 
+// ---------------------------------------------------------------------------
+// SRK stage updates (collocated_SRK/func_global.h:317-408). Elementwise over
+// nodes except for the collocated gradient of one scalar; every read of a
+// node comes before its writes, since outputs alias inputs at the same node.
+// ---------------------------------------------------------------------------
+template <typename T, bool GRAD>
+__global__ __launch_bounds__(kBlock) void k_srk_vec(Mesh<T> m, SrkVec<T> a) {
+    const int iz = blockIdx.y;
+    int ix, iy;
+    if (!cell_of_thread(m.nx, m.nxy, ix, iy)) return;
+    Nb nb;
+    make_nb<1>(nb, ix, iy, iz, m.nx, m.ny);
+    const int in = nb.at(0, 0, 0);
+    T kc[3] = {T(0), T(0), T(0)};
+    if (a.KC) {
+        const T* k = a.KC + 3 * in;
+        kc[0] = k[0];
+        kc[1] = k[1];
+        kc[2] = k[2];
+    }
+    if (GRAD) {
+        T g[3];
+        grad_at<T, TC_SCHEME_AB3>(m, a.S, nb, iz, g);
+        kc[0] -= g[0];
+        kc[1] -= g[1];
+        kc[2] -= g[2];
+    }
+    T o[3] = {T(0), T(0), T(0)};
+    if (a.out) {
+        if (a.X) {
+            const T* x = a.X + 3 * in;
+            o[0] = x[0];
+            o[1] = x[1];
+            o[2] = x[2];
+        }
+#pragma unroll
+        for (int j = 0; j < kMaxImexStages; j++)
+            if (j < a.nk) {
+                const T* k = a.K[j] + 3 * in;
+                const T cj = a.c[j];
+                o[0] += cj * k[0];
+                o[1] += cj * k[1];
+                o[2] += cj * k[2];
+            }
+        if (a.KC) {
+            o[0] += a.ckc * kc[0];
+            o[1] += a.ckc * kc[1];
+            o[2] += a.ckc * kc[2];
+        }
+        if (a.A) {
+            const T* x = a.A + 3 * in;
+            o[0] += a.sa * x[0];
+            o[1] += a.sa * x[1];
+            o[2] += a.sa * x[2];
+        }
+        if (a.B) {
+            const T* x = a.B + 3 * in;
+            o[0] += a.sb * x[0];
+            o[1] += a.sb * x[1];
+            o[2] += a.sb * x[2];
+        }
+    }
+    if (a.KCout) {
+        T* k = a.KCout + 3 * in;
+        k[0] = kc[0];
+        k[1] = kc[1];
+        k[2] = kc[2];
+    }
+    if (a.out) {
+        T* x = a.out + 3 * in;
+        x[0] = o[0];
+        x[1] = o[1];
+        x[2] = o[2];
+    }
+}
+
+template <typename T, bool STAB, bool CK>
+__global__ __launch_bounds__(kBlock) void k_srk_pre(Mesh<T> m, SrkPre<T> a) {
+    const int iz = blockIdx.y;
+    int ix, iy;
+    if (!cell_of_thread(m.nx, m.nxy, ix, iy)) return;
+    const int in = iz * m.nxy + iy * m.nx + ix;
+    const T p = a.p[in];
+    if (!STAB) {
+        T muj = p;
+        if (CK) muj *= a.m0;
+#pragma unroll
+        for (int j = 0; j < kMaxImexStages; j++)
+            if (j < a.nj) muj += a.cm[j] * a.mujt[j][in];
+        a.bs[in] = muj - a.clast * p;
+    } else {
+        const T q = a.q[in];
+        T muj = a.tau_stage * q;
+        if (CK) muj *= a.m0;
+        T ph = p;
+        if (CK) ph += a.ph0 * q;
+#pragma unroll
+        for (int j = 0; j < kMaxImexStages; j++)
+            if (j < a.nj) {
+                muj += a.cm[j] * a.mujt[j][in];
+                ph += a.cq[j] * a.qj[j][in];
+            }
+        a.pjhat[in] = ph;
+        a.bs[in] = muj + ph - a.clast * q;
+    }
+}
+
+template <typename T, bool STAB, bool LAST>
+__global__ __launch_bounds__(kBlock) void k_srk_post(Mesh<T> m, SrkPost<T> a) {
+    const int iz = blockIdx.y;
+    int ix, iy;
+    if (!cell_of_thread(m.nx, m.nxy, ix, iy)) return;
+    const int in = iz * m.nxy + iy * m.nx + ix;
+    const T mu = a.mujt[in];
+    const T pj = mu + a.bs[in];
+    if (STAB) {
+        const T q = a.q[in];
+        const T qn = a.inv_tau_stage * (pj - a.pjhat[in]);
+        if (!LAST) a.mujt[in] = mu - a.c * (a.tau_stage * q);
+        a.qj_out[in] = qn;
+    } else {
+        if (!LAST) a.mujt[in] = mu - a.c * a.p[in];
+    }
+    a.pj_out[in] = pj;
+}
+
 }  // namespace
 
 // Upper bound of the per-block partial sums written by launch_flux (at least one z-plane per block).
@@ -727,7 +853,44 @@ hipError_t launch_add_scaled_plane(hipStream_t s, const Mesh<T>& m, T* f, const 
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_srk_vec(hipStream_t s, const Mesh<T>& m, const SrkVec<T>& a) {
+    if (a.S && !a.KC) return hipErrorInvalidValue;
+    if (a.S) hipLaunchKernelGGL((k_srk_vec<T, true>), cell_grid(m), dim3(kBlock), 0, s, m, a);
+    else hipLaunchKernelGGL((k_srk_vec<T, false>), cell_grid(m), dim3(kBlock), 0, s, m, a);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_srk_pre(hipStream_t s, const Mesh<T>& m, const SrkPre<T>& a) {
+    const dim3 g = cell_grid(m);
+    if (a.stab) {
+        if (a.ck) hipLaunchKernelGGL((k_srk_pre<T, true, true>), g, dim3(kBlock), 0, s, m, a);
+        else hipLaunchKernelGGL((k_srk_pre<T, true, false>), g, dim3(kBlock), 0, s, m, a);
+    } else {
+        if (a.ck) hipLaunchKernelGGL((k_srk_pre<T, false, true>), g, dim3(kBlock), 0, s, m, a);
+        else hipLaunchKernelGGL((k_srk_pre<T, false, false>), g, dim3(kBlock), 0, s, m, a);
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_srk_post(hipStream_t s, const Mesh<T>& m, const SrkPost<T>& a) {
+    const dim3 g = cell_grid(m);
+    if (a.stab) {
+        if (a.last) hipLaunchKernelGGL((k_srk_post<T, true, true>), g, dim3(kBlock), 0, s, m, a);
+        else hipLaunchKernelGGL((k_srk_post<T, true, false>), g, dim3(kBlock), 0, s, m, a);
+    } else {
+        if (a.last) hipLaunchKernelGGL((k_srk_post<T, false, true>), g, dim3(kBlock), 0, s, m, a);
+        else hipLaunchKernelGGL((k_srk_post<T, false, false>), g, dim3(kBlock), 0, s, m, a);
+    }
+    return hipGetLastError();
+}
+
 #define INST(T)                                                                                             \
+    template hipError_t launch_srk_vec<T>(hipStream_t, const Mesh<T>&, const SrkVec<T>&);                  \
+    template hipError_t launch_srk_pre<T>(hipStream_t, const Mesh<T>&, const SrkPre<T>&);                  \
+    template hipError_t launch_srk_post<T>(hipStream_t, const Mesh<T>&, const SrkPost<T>&);                \
     template hipError_t launch_flux<T>(hipStream_t, const Mesh<T>&, const FluxArgs<T>&, int*);              \
     template hipError_t launch_turb_visc<T>(hipStream_t, const Mesh<T>&, int, const T*, T*);                \
     template hipError_t launch_gradient<T>(hipStream_t, const Mesh<T>&, int, const T*, T*);                 \

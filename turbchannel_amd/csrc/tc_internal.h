@@ -45,6 +45,25 @@ std::string build_zfactors(const tc_params& p, const HostMesh& m, ZFactors& f);
 void make_initial_field(const tc_params& p, const HostMesh& m, unsigned long long seed, double ubulk,
                         double noise_rms, double* out);
 
+// IMEX Runge-Kutta method of the SRK stepper: E and I are (s+1) x s row-major
+// (row s = weights), d the coefficients of CalcD (collocated_SRK/imex.cpp:35-45).
+constexpr int kMaxImexStages = 6;
+struct ImexTable {
+    int type = 0, s = 0;
+    std::vector<double> E, I, d;
+    double alpha_tau_max[2] = {2., 2.};
+};
+bool imex_table(int method, ImexTable& t);
+
+// The collocated SRK solver shares the AB3 mesh, operators and pressure solver;
+// only its pressure normalisation differs (zero mean instead of the AB3
+// hx*hy-scaled shift). Mesh and factor builders see it as AB3.
+inline tc_params as_kernel_params(const tc_params& p) {
+    tc_params q = p;
+    if (q.scheme == TC_SCHEME_SRK) q.scheme = TC_SCHEME_AB3;
+    return q;
+}
+
 // ---------------------------------------------------------------------------
 // Device-side types
 // ---------------------------------------------------------------------------
