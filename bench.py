@@ -39,6 +39,12 @@ REALS_PER_POINT = {
     "project": 7,    # reads p (1), r (3); writes u_n+1 (3)
 }
 
+# The SRK stepper's explicit term is F(u) alone (reads u, writes F); its stage
+# combinations (the "project" class) move a number of reals that depends on the
+# stage and the method, so they get no figure.
+REALS_PER_POINT_SRK = dict(REALS_PER_POINT, flux=6)
+REALS_PER_POINT_SRK.pop("project")
+
 PEAK_BF16_TFLOPS = 2500.0  # dense bfloat16 MFMA
 # flops per sample of the closure network (10 -> 64 -> 64 -> 64 -> 6): forward, and
 # forward + backward (weight gradients of every layer, data gradients of layers 2-4)
@@ -55,13 +61,19 @@ WORKLOADS = {
     # reference's Smagorinsky-van Driest model on the same grid
     "les_learned_64x64x96": (tca.SCHEME_AB3, 64, 64, 96, 2.0, 21900.0, 590.0, 2e-3, dict(visc_type=1, sgs_model=1)),
     "les_smag_64x64x96": (tca.SCHEME_AB3, 64, 64, 96, 2.0, 21900.0, 590.0, 2e-3, dict(visc_type=1, turb_visc=1)),
+    # collocated IMEX Runge-Kutta (SRK) stepper, ARS(3,4,3) with StabType 1 as in
+    # collocated_SRK/main.cpp:363-373: the reference's grid size (128 x 64 x 49
+    # nodes) and a large grid
+    "srk_128x64x49": (tca.SCHEME_SRK, 128, 64, 49, 2.0, 5600.0, 180.0, 8e-4, dict(imex=2)),
+    "srk_384x384x192": (tca.SCHEME_SRK, 384, 384, 192, 2.0, 13750.0, 395.0, 5e-4, dict(imex=2)),
 }
 TRAIN_WORKLOAD = "closure_train_b65536"
 
 
 def problem(name, precision):
     scheme, nx, ny, nz, gamma, re_b, re_tau, dt, extra = WORKLOADS[name]
-    if scheme == tca.SCHEME_AB3:
+    collocated = scheme in (tca.SCHEME_AB3, tca.SCHEME_SRK)
+    if collocated:
         k = np.arange(nz)
         zz = 1.0 + np.tanh(gamma * (2.0 * k / (nz - 1) - 1.0)) / math.tanh(gamma)
     else:
@@ -72,8 +84,10 @@ def problem(name, precision):
     u_tau = re_tau * visc
     kw = dict(visc=visc, sour_dpdx=u_tau * u_tau, inv_length_scale=u_tau / visc, hx=2 * math.pi / nx,
               hy=math.pi / ny, cfr_ubulk=1.0, precision=precision)
-    if scheme == tca.SCHEME_AB3:
+    if collocated:
         kw.update(stab_type=1, fd_mode=1, visc_type=0)
+    if scheme == tca.SCHEME_SRK:  # no flow-rate control in the SRK solver
+        kw["cfr_ubulk"] = 0.0
     kw.update(extra)
     return scheme, nx, ny, nz, zz, kw, dt
 
@@ -136,8 +150,22 @@ def cpu_baseline(name, seconds_target=15.0):
     kw.pop("precision")
     threads = min(os.cpu_count() or 1, 16)
     oracle.set_threads(threads)
+    srk = scheme == tca.SCHEME_SRK
+    method = kw.pop("imex", 2)
     sol = oracle.Solver(scheme, nx, ny, nz, zz, **kw)
     u0 = sol.project(sol.initial_field(seed=42))
+    if srk:  # SRK: the oracle's or_srk_step on the same state, timed the same way
+        st = [u0, np.zeros(sol.n3d), np.zeros(sol.n3d)]
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            st = list(sol.srk_step(n * dt, dt, *st, method=method, stab_type=kw["stab_type"], alpha_tau=1.0))
+            n += 1
+            el = time.perf_counter() - t0
+            if el > seconds_target:
+                break
+        return {"value": nx * ny * nz * n / el, "unit": "grid-pt steps/s", "cores": threads, "kind": "port",
+                "sample": f"{n} SRK steps of {name} (FP64 oracle, OpenMP {threads} threads, {el:.1f} s)"}
     sol.set_field(oracle.KIND_VEL, 0, u0)
     closure = None
     if kw.get("sgs_model"):  # learned closure: the network forward (numpy) feeds each step's tau
@@ -232,11 +260,23 @@ def main():
         eng.set_sgs_params(closure_weights())
     eng.initial_field(0, seed=42, ubulk=1.0, noise_rms=0.1)
     eng.graphs(not args.no_graphs)
+    srk = scheme == tca.SCHEME_SRK
+    reals = REALS_PER_POINT_SRK if srk else REALS_PER_POINT
     cur = 0
+
+    def advance(it, order=3):
+        """One step; returns the slot of the new velocity."""
+        nonlocal cur
+        if srk:  # in place: velocity slot 0, pressure slot 0, dp/dt in pressure slot 1
+            eng.srk_step(it * dt, dt, 0, 0, 1)
+            return 0
+        eng.step(it * dt, dt, cur, order)
+        cur = (cur + 1) % 4
+        return cur
+
     # warm-up, long enough to capture the four order-3 steps (one per ring slot)
     for it in range(max(args.warmup, 7)):
-        eng.step(it * dt, dt, cur, min(it + 1, 3))
-        cur = (cur + 1) % 4
+        advance(it, min(it + 1, 3))
     eng.sync()
 
     try:
@@ -251,8 +291,7 @@ def main():
     sync_torch()
     t0 = time.perf_counter()
     for it in range(args.steps):
-        eng.step(it * dt, dt, cur, 3)
-        cur = (cur + 1) % 4
+        advance(it)
     eng.sync()
     sync_torch()
     el = time.perf_counter() - t0
@@ -262,8 +301,7 @@ def main():
     eng.timing_reset()
     eng.timing(True)
     for it in range(min(args.steps, 50)):
-        eng.step(it * dt, dt, cur, 3)
-        cur = (cur + 1) % 4
+        advance(it)
     eng.sync()
     eng.timing(False)
     kt = eng.timing_read()
@@ -274,9 +312,8 @@ def main():
     eng.sync()
     t1 = time.perf_counter()
     for it in range(nd):
-        eng.step(it * dt, dt, cur, 3)
-        cur = (cur + 1) % 4
-        eng.diagnostics(cur, cur, dt, 3)
+        v = advance(it)
+        eng.diagnostics(v, v, dt, 3)
     eng.sync()
     el_diag = time.perf_counter() - t1
 
@@ -287,14 +324,14 @@ def main():
             continue
         avg = ms / n
         d = {"avg_us": round(1e3 * avg, 2), "launches": n}
-        if k in REALS_PER_POINT:
-            gbs = REALS_PER_POINT[k] * rsz * n3d / (avg * 1e-3) / 1e9
+        if k in reals:
+            gbs = reals[k] * rsz * n3d / (avg * 1e-3) / 1e9
             d["gbs"] = round(gbs, 1)
             d["frac_hbm"] = round(gbs / PEAK_HBM_GBS, 4)
         per_kernel[k] = d
-    dom = max((k for k in per_kernel if k in REALS_PER_POINT), key=lambda k: kt[k][0])
+    dom = max((k for k in per_kernel if k in reals), key=lambda k: kt[k][0])
     dom_avg_s = kt[dom][0] / kt[dom][1] * 1e-3
-    achieved = REALS_PER_POINT[dom] * rsz * n3d / dom_avg_s / 1e9
+    achieved = reals[dom] * rsz * n3d / dom_avg_s / 1e9
 
     value = n3d * args.steps / el
     traffic = measured_traffic(args.workload, args.precision, dom)
@@ -321,11 +358,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if precision == tca.FP64 else "f32 (f64 low-mode solves, normalisation, flow rate, statistics)",
         "data": "synthetic (seeded divergence-free initial field)",
-        "config": {"workload": args.workload, "scheme": ["HW", "VV", "AB3"][scheme], "grid": [nx, ny, nz],
+        "config": {"workload": args.workload,
+                   "scheme": "SRK " + tca.IMEX_METHODS[kw["imex"]] if srk else ["HW", "VV", "AB3"][scheme],
+                   "grid": [nx, ny, nz],
                    "dt": dt, "parallelism": "single GPU", "graphs": not args.no_graphs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "algorithmic_bytes": REALS_PER_POINT[dom] * rsz * n3d,
+                     "algorithmic_bytes": reals[dom] * rsz * n3d,
                      "traffic": traffic["bytes"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None},
         "cpu_baseline": cpu,

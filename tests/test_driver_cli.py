@@ -104,3 +104,23 @@ def test_driver_without_device_reports_and_exits(tmp_path, monkeypatch):
 def test_driver_binary_exists():
     import os
     assert os.access(DRIVER, os.X_OK)
+
+
+def test_driver_srk_defaults_are_the_reference_mesh(tmp_path):
+    """--scheme srk without mesh options: collocated_SRK/main.cpp:320-340 (Re_tau 180,
+    first step 1, ratio 1.3, steps capped at 10 wall units), 49 nodes."""
+    n, z = mesh(tmp_path, "--scheme", "srk")
+    zp = [0.0, 1.0]
+    while (zp[-1] - zp[-2]) * 1.3 < 10.0:
+        zp.append(zp[-1] + (zp[-1] - zp[-2]) * 1.3)
+    m = int((180.0 - zp[-1]) / 10.0) + 1
+    hb = (180.0 - zp[-1]) / m
+    zp += [zp[-1] + hb * (i + 1) for i in range(m - 1)] + [180.0]
+    zp += [360.0 - v for v in zp[-2::-1]]
+    assert n == len(zp) == 49
+    np.testing.assert_allclose(z * 180.0, zp, rtol=1e-14, atol=1e-12)
+
+
+def test_driver_srk_rejects_unknown_method(tmp_path):
+    r = run_driver(["--scheme", "srk", "--imex", "rk4", "--print-mesh", 1], tmp_path)
+    assert r.returncode == 0 and "unknown IMEX method rk4" in r.stdout

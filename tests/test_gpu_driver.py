@@ -194,3 +194,56 @@ def test_driver_learned_closure_from_file(tmp_path):
     np.zeros(npar - 1, dtype="<f4").tofile(tmp_path / "short.bin")
     r = run_driver([*base, "--sgs", "short.bin"], tmp_path)
     assert f"expected {npar} float32 network parameters" in r.stdout
+
+
+def test_driver_srk_matches_oracle_replay(tmp_path):
+    """--scheme srk: the fixed-step loop of collocated_SRK/main.cpp:466-549 against
+    or_srk_step replayed on the oracle (log lines, 5-value VTK with dp/dt,
+    profiles, time-averaged profiles from t = 0 and the friction velocity)."""
+    import oracle
+    nx, ny, nz = 16, 12, 18
+    zz = cases.ab3_mesh_tanh(nz, 1.8)
+    kw = cases.bulk_kwargs(nx, ny, 5600, 180)
+    sol = oracle.Solver(oracle.SCHEME_SRK, nx, ny, nz, zz, stab_type=1, fd_mode=1, visc_type=0, **kw)
+    u0 = sol.project(sol.initial_field(seed=9, ubulk=1.0, noise_rms=0.1))
+    u0, p0 = write_vtk(tmp_path / "input.vtk", nx, ny, nz, u0, np.zeros(nx * ny * nz))
+    tau, steps, alpha_tau = 2e-3, 9, 0.8
+    r = run_driver(["--scheme", "srk", "--Re_b", 5600, "--Re_tau", 180, "--Lx", repr(2 * math.pi), "--Ly",
+                    repr(math.pi), "--mesh", "tanh", "--gamma", 1.8, "--nx", nx, "--ny", ny, "--nz", nz, "--imex",
+                    "ark3", "--stab", 1, "--alpha_tau", alpha_tau, "--tau", repr(tau), "--TimeStepsMax", steps,
+                    "-s", 2, "-f", 4], tmp_path)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0 and "Initial data read" in r.stdout, r.stdout + r.stderr
+    area = kw["hx"] * nx * kw["hy"] * ny * (zz[-1] - zz[0])
+    dz = np.diff(zz)
+    u, p, q = u0, p0, np.zeros_like(p0)
+    rows, av_sum = [], 0.0
+    for it in range(steps):
+        u, p, q = sol.srk_step(it * tau, tau, u, p, q, method=4, stab_type=1, alpha_tau=alpha_tau)
+        av = sol.profiles(u, p)
+        av_sum = av_sum + tau * av
+        t = (it + 1) * tau
+        dudy, dudy_int = av[1, 0] / dz[0], av_sum[1, 0] / (dz[0] * t)
+        if it % 2 == 0 or it == steps - 1:
+            rows.append((t, sol.integral(u)[0] / area, math.sqrt(max(dudy, 0) * kw["visc"]),
+                         math.sqrt(max(dudy_int, 0) * kw["visc"])))
+        if it == 4:
+            u4, p4, q4 = u, p, q
+    got = read_table(tmp_path / "log.dat")
+    assert got.shape == (len(rows), 4)
+    np.testing.assert_allclose(got, np.array(rows), rtol=0, atol=1.5e-6)
+    for name, (uu, pp, qq) in (("q00004.vtk", (u4, p4, q4)), ("output.vtk", (u, p, q))):
+        dims, _, vals = read_vtk(tmp_path / name)
+        assert dims == (nx, ny, nz) and vals.shape[1] == 5
+        np.testing.assert_allclose(vals[:, :3], uu, rtol=1e-6, atol=1e-6 * np.abs(uu).max())
+        np.testing.assert_allclose(vals[:, 3], pp, rtol=1e-6, atol=1e-6 * np.abs(pp).max())
+        np.testing.assert_allclose(vals[:, 4], qq, rtol=1e-6, atol=1e-6 * np.abs(qq).max())
+    assert (tmp_path / "q00008.vtk").exists() and (tmp_path / "r00008.dat").exists()  # the final step
+    u_tau = 180 * kw["visc"]
+    _check_profiles(tmp_path / "r00008.dat", profile_columns(av, zz, kw["visc"], True, u_tau))
+    _check_profiles(tmp_path / "res.dat", profile_columns(av_sum / (steps * tau), zz, kw["visc"], True, u_tau))
+    m = re.search(r"u_fric: obtained=([0-9.]+), expected=([0-9.]+)", r.stdout)
+    a = av_sum / (steps * tau)
+    dudy = 1.5 * a[1, 0] / dz[0] - 0.5 * (a[2, 0] - a[1, 0]) / dz[1]
+    assert float(m.group(1)) == pytest.approx(math.sqrt(kw["visc"] * max(dudy, 0)), abs=1.5e-6)
+    assert float(m.group(2)) == pytest.approx(u_tau, abs=1e-6)

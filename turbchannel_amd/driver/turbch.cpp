@@ -3,7 +3,8 @@
 // The host program of the reference, restated over the C ABI of
 // include/turbchannel.h: the same options (staggered/io.cpp:107-126,
 // collocated_AB3/io.cpp:104-123), the same time loop with the BDF order ramp and
-// CFL-driven step control (staggered/main.cpp:288-371, collocated_AB3/main.cpp:392-476),
+// CFL-driven step control (staggered/main.cpp:288-371, collocated_AB3/main.cpp:392-476)
+// or, for the IMEX Runge-Kutta solver, with a fixed step (collocated_SRK/main.cpp:466-549),
 // the same screen and log.dat lines, the same files (qNNNNN.vtk, rNNNNN.dat,
 // output.vtk, output.dat, res.dat; staggered/io.cpp:6-96, collocated_AB3/io.cpp:9-101)
 // and the same start from input.vtk when it exists (staggered/io.cpp:31-69).
@@ -17,6 +18,7 @@
 // file wall-normal meshes, a seeded divergence-free start, the learned subgrid
 // closure, and an exact binary restart of the whole solver state.
 #include <algorithm>
+#include <cctype>
 #include <cerrno>
 #include <chrono>
 #include <cmath>
@@ -52,26 +54,28 @@ struct Options {
     // the reference's t_cmdline_params (staggered/config.h:75-87; collocated_AB3/io.h:10-21)
     int device = 0;
     int nz = -1;  // staggered: cells (default 32); collocated: nodes (default from the mesh)
-    double time_max = 100.0;
+    double time_max = -1;  // 100 (0.1 for srk)
     double time_start_averaging = 0.0;
     long long time_steps_max = 0x7FFFFFFF;
-    double tau = 1e-3;
+    double tau = -1;       // 1e-3 (0.8e-3 for srk)
     double cfl_min = 0.05, cfl_max = 0.5;
     int s_progress = -1, f_progress = -1, p_progress = -1;  // defaults depend on the solver
     // what the reference sets at compile time
-    std::string scheme = "vv";  // hw | vv | ab3
+    std::string scheme = "vv";  // hw | vv | ab3 | srk
     std::string precision = "fp64";
     int nx = -1, ny = -1;
     int les = -1;     // ENABLE_TURB_VISC: 1 (both reference configs) unless the learned closure is on
     int visc = -1;    // staggered 0 CROSS / 1 NAIVE, collocated 0 FD / 1 Galerkin; default follows les
     int stab = 1, fd_mode = 1;
-    double re_tau = -1;  // 395 unless given
+    double re_tau = -1;  // 395 (180 for srk) unless given
+    std::string imex = "ars343";  // srk: IMEX Runge-Kutta method (collocated_SRK/main.cpp:363)
+    double alpha_tau = 0;         // srk: <= 0 selects 0.5 * alpha_tau_max[stab]
     // options with no reference counterpart
     double re_b = 0;     // > 0: bulk units, nu = 2 / Re_b
     double lx = -1, ly = -1;
     std::string mesh;    // sinh | tanh | uniform | geometric | file
     double stretch = 6.5, gamma = 1.8;
-    double geo_y1 = 50.0, geo_ratio = 1.05, geo_ybulk = 2.2;  // wall units
+    double geo_y1 = -1, geo_ratio = -1, geo_ybulk = -1;  // wall units; 50, 1.05, 2.2 (srk: 1, 1.3, 10)
     std::string zfile;
     double cfr = 0;      // > 0: hold the bulk velocity at this value
     long long seed = -1; // >= 0: seeded divergence-free start instead of the analytic one
@@ -102,23 +106,25 @@ const OptSpec kOpts[] = {
     {"-s,--SProgress", "number of timesteps between writing to screen and log"},
     {"-f,--FProgress", "number of timesteps between writing a field"},
     {"-p,--PProgress", "number of timesteps between writing a profile"},
-    {"--scheme", "hw | vv (staggered) | ab3 (collocated); default vv"},
+    {"--scheme", "hw | vv (staggered) | ab3 | srk (collocated BDF3 / IMEX Runge-Kutta); default vv"},
+    {"--imex", "srk: ars121 | ars232 | ars343 | mars343 | ark3 | ark4 | mark3 | bhr553 (default ars343)"},
+    {"--alpha_tau", "srk: pressure stabilisation alpha * tau (default 0.5 * the method's limit)"},
     {"--precision", "fp64 | fp32 (FP32 bulk, FP64 islands); default fp64"},
-    {"--nx", "cells (nodes) in x; default 64"},
+    {"--nx", "cells (nodes) in x; default 64 (srk 128)"},
     {"--ny", "cells (nodes) in y; default 32 staggered, 64 collocated"},
     {"--les", "1: Smagorinsky-van Driest eddy viscosity (default without --sgs), 0: none"},
     {"--visc", "viscous term: staggered 0 CROSS / 1 NAIVE, collocated 0 FD / 1 Galerkin"},
     {"--stab", "collocated StabType 0 | 1 (default 1)"},
-    {"--Re_tau", "friction Reynolds number (default 395)"},
+    {"--Re_tau", "friction Reynolds number (default 395, srk 180)"},
     {"--Re_b", "bulk units: H = U_b = 1, nu = 2/Re_b"},
     {"--Lx", "box length in x over H (default 4 pi)"},
     {"--Ly", "box length in y over H (default 2 pi staggered, 4 pi/3 collocated)"},
     {"--mesh", "sinh | tanh | uniform | geometric | file"},
     {"--stretch", "sinh mesh stretching (default 6.5)"},
     {"--gamma", "tanh mesh parameter (default 1.8)"},
-    {"--geo_y1", "geometric mesh: first step in wall units (default 50)"},
-    {"--geo_ratio", "geometric mesh: growth ratio (default 1.05)"},
-    {"--geo_ybulk", "geometric mesh: largest step in wall units (default 2.2)"},
+    {"--geo_y1", "geometric mesh: first step in wall units (default 50, srk 1)"},
+    {"--geo_ratio", "geometric mesh: growth ratio (default 1.05, srk 1.3)"},
+    {"--geo_ybulk", "geometric mesh: largest step in wall units (default 2.2, srk 10)"},
     {"--zfile", "wall-normal coordinates, one per line (faces staggered, nodes collocated)"},
     {"--cfr", "hold the bulk velocity at this value (constant flow rate)"},
     {"--seed", "seeded divergence-free start (laminar profile + random vortices)"},
@@ -208,6 +214,8 @@ bool parse(int argc, char** argv, Options& o) {
         else if (is("--FProgress")) parse_num(key, val, o.f_progress);
         else if (is("--PProgress")) parse_num(key, val, o.p_progress);
         else if (is("--scheme")) o.scheme = val;
+        else if (is("--imex")) o.imex = val;
+        else if (is("--alpha_tau")) parse_num(key, val, o.alpha_tau);
         else if (is("--precision")) o.precision = val;
         else if (is("--nx")) parse_num(key, val, o.nx);
         else if (is("--ny")) parse_num(key, val, o.ny);
@@ -318,14 +326,26 @@ Problem setup(const Options& o) {
     if (o.scheme == "hw") P.scheme = TC_SCHEME_HW;
     else if (o.scheme == "vv") P.scheme = TC_SCHEME_VV;
     else if (o.scheme == "ab3") P.scheme = TC_SCHEME_AB3;
+    else if (o.scheme == "srk") P.scheme = TC_SCHEME_SRK;
     else die("unknown scheme " + o.scheme);
+    const bool srk = P.scheme == TC_SCHEME_SRK;
+    int imex = -1;
+    {
+        static const char* names[] = {"ars121", "ars232", "ars343", "mars343", "ark3", "ark4", "mark3", "bhr553"};
+        std::string k;
+        for (char c : o.imex)
+            if (c != '_') k += (char)std::tolower((unsigned char)c);
+        for (int i = 0; i < 8; i++)
+            if (k == names[i]) imex = i;
+        if (imex < 0) die("unknown IMEX method " + o.imex);
+    }
     if (o.precision == "fp64") P.precision = TC_FP64;
     else if (o.precision == "fp32") P.precision = TC_FP32_BULK;
     else die("unknown precision " + o.precision);
-    P.collocated = P.scheme == TC_SCHEME_AB3;
-    P.nx = o.nx > 0 ? o.nx : 64;
+    P.collocated = P.scheme == TC_SCHEME_AB3 || srk;
+    P.nx = o.nx > 0 ? o.nx : (srk ? 128 : 64);
     P.ny = o.ny > 0 ? o.ny : (P.collocated ? 64 : 32);
-    P.re_tau = o.re_tau > 0 ? o.re_tau : 395.0;
+    P.re_tau = o.re_tau > 0 ? o.re_tau : (srk ? 180.0 : 395.0);
     // reference units (staggered/main.cpp:200-205): u_tau = 1, nu = 1/Re_tau;
     // bulk units: U_b = 1, nu = 2/Re_b, u_tau = Re_tau nu
     P.visc = o.re_b > 0 ? 2.0 / o.re_b : 1.0 / P.re_tau;
@@ -337,7 +357,11 @@ Problem setup(const Options& o) {
     P.hy = P.Ly / P.ny;
     P.xmin = 0.0;
     P.ymin = -0.5 * P.Ly;
-    P.zz = make_mesh(o, P.collocated, P.nz, P.u_tau / P.visc);
+    Options om = o;  // geometric mesh defaults: collocated_AB3/main.cpp:209-211, collocated_SRK/main.cpp:324-326
+    if (om.geo_y1 <= 0) om.geo_y1 = srk ? 1.0 : 50.0;
+    if (om.geo_ratio <= 0) om.geo_ratio = srk ? 1.3 : 1.05;
+    if (om.geo_ybulk <= 0) om.geo_ybulk = srk ? 10.0 : 2.2;
+    P.zz = make_mesh(om, P.collocated, P.nz, P.u_tau / P.visc);
     if (P.collocated) {
         for (int i = 0; i + 1 < P.nz; i++) P.dz.push_back(P.zz[i + 1] - P.zz[i]);
         P.area = P.Lx * P.Ly * (P.zz[P.nz - 1] - P.zz[0]);
@@ -351,7 +375,7 @@ Problem setup(const Options& o) {
     p.nx = P.nx;
     p.ny = P.ny;
     p.nz = P.nz;
-    const int les = o.les >= 0 ? o.les : (o.sgs_file.empty() ? 1 : 0);
+    const int les = o.les >= 0 ? o.les : (o.sgs_file.empty() && !srk ? 1 : 0);
     p.turb_visc = les;
     p.visc_type = o.visc >= 0 ? o.visc : (les ? 1 : 0);  // VISC_APPR_TYPE, VISC_TERM_GALERKIN
     p.stab_type = o.stab;
@@ -366,6 +390,8 @@ Problem setup(const Options& o) {
     p.cfr_ubulk = o.cfr;
     p.zz = P.zz.data();
     p.sgs_model = o.sgs_file.empty() ? 0 : 1;
+    p.imex = imex;
+    p.alpha_tau = o.alpha_tau;
     return P;
 }
 
@@ -379,7 +405,7 @@ struct Run {
     tc_engine* e = nullptr;
     size_t n3d;
     int nprof;                        // values per plane from tc_profiles
-    std::vector<T> vel, pres;         // host copies of one slot (real3 AoS, real)
+    std::vector<T> vel, pres, qres;   // host copies of one slot (real3 AoS, real); qres: srk dp/dt
     std::vector<T> av;                // last profiles, in real (the reference's av_u)
     std::vector<double> av_sum_d;     // time-integrated profiles, staggered (double)
     std::vector<T> av_sum_r;          // time-integrated profiles, collocated (real)
@@ -394,6 +420,7 @@ struct Run {
         nprof = P.collocated ? 10 : 12;
         vel.assign(3 * n3d, T(0));
         pres.assign(n3d, T(0));
+        qres.assign(n3d, T(0));
         av.assign((size_t)nprof * P.nz, T(0));
         av_sum_d.assign(av.size(), 0.0);
         av_sum_r.assign(av.size(), T(0));
@@ -449,8 +476,30 @@ struct Run {
         }
     }
 
+    // The SRK start (collocated_SRK/main.cpp:437-447): laminar parabola, a
+    // spanwise velocity 0.1 sin(x/2) z+ exp(-0.01 z+^2), and 0.1 rand()/RAND_MAX
+    // added to every component away from the walls (the C library's rand(),
+    // unseeded, as the reference).
+    void srk_start() {
+        const double zmin = 0.0, Hz = 2.0;
+        const double re = P.u_tau / P.visc;
+        for (size_t in = 0; in < n3d; in++) {
+            const int ix = in % P.nx, iz = in / ((size_t)P.nx * P.ny);
+            const double x = ix * P.hx, z = P.zz[iz];
+            const double zp = (P.zz[iz] - P.zz[0]) * re;
+            const double zplus = std::min(zp, re * 2 - zp);
+            T u[3] = {T(0.5 * (z - zmin) * (zmin + Hz - z) * P.sour_dpdx / P.visc),
+                      T(1e-1 * std::sin(0.5 * x) * zplus * std::exp(-0.01 * zplus * zplus)), T(0)};
+            for (int d = 0; d < 3; d++)
+                if (iz != 0 && iz != P.nz - 1) u[d] += T(1e-1 * double(rand()) / RAND_MAX);
+            vel[3 * in] = u[0], vel[3 * in + 1] = u[1], vel[3 * in + 2] = u[2];
+            pres[in] = T(0);
+        }
+    }
+
     // DumpData (staggered/io.cpp:6-29, collocated_AB3/io.cpp:9-32): ASCII VTK
-    // rectilinear grid; z coordinates are the faces (staggered) or nodes.
+    // rectilinear grid; z coordinates are the faces (staggered) or nodes. The SRK
+    // solver's file has dp/dt as a fifth value (collocated_SRK/main.cpp:247-268).
     void dump_vtk(const char* fname) const {
         FILE* f = fopen(fname, "wt");
         if (!f) die(std::string("cannot write ") + fname);
@@ -466,12 +515,16 @@ struct Run {
                 fprintf(f, "%e ", d == 0 ? P.xmin + P.hx * i : d == 1 ? P.ymin + P.hy * i : P.zz[i]);
             fprintf(f, "\n");
         }
+        const bool q = P.scheme == TC_SCHEME_SRK;
         fprintf(f, "POINT_DATA %i\n", (int)n3d);
-        fprintf(f, "SCALARS volume_scalars float %i\n", 4);
+        fprintf(f, "SCALARS volume_scalars float %i\n", q ? 5 : 4);
         fprintf(f, "LOOKUP_TABLE default\n");
-        for (size_t in = 0; in < n3d; in++)
-            fprintf(f, "%e %e %e %e\n", (double)vel[3 * in], (double)vel[3 * in + 1], (double)vel[3 * in + 2],
+        for (size_t in = 0; in < n3d; in++) {
+            fprintf(f, "%e %e %e %e", (double)vel[3 * in], (double)vel[3 * in + 1], (double)vel[3 * in + 2],
                     (double)pres[in]);
+            if (q) fprintf(f, " %e", (double)qres[in]);
+            fprintf(f, "\n");
+        }
         fclose(f);
     }
 
@@ -582,6 +635,8 @@ struct Run {
     void download(int slot) {
         check(tc_get_field(e, TC_VEL, slot, vel.data(), vel.size() * sizeof(T)), "tc_get_field");
         check(tc_get_field(e, TC_PRES, slot, pres.data(), pres.size() * sizeof(T)), "tc_get_field");
+        if (P.scheme == TC_SCHEME_SRK)  // dp/dt lives in pressure slot 1
+            check(tc_get_field(e, TC_PRES, 1, qres.data(), qres.size() * sizeof(T)), "tc_get_field");
     }
 
     // ---- exact restart: every ring slot plus the loop state --------------
@@ -663,6 +718,93 @@ struct Run {
         check(tc_set_sgs_params(e, w.data(), w.size()), "tc_set_sgs_params");
     }
 
+    void report_timing() {
+        if (!o.timing) return;
+        static const char* names[TC_K_COUNT] = {"turb_visc", "flux", "div", "fft_x", "fft_y",
+                                                "zsolve", "ifft_y", "ifft_x", "project", "other"};
+        printf("Kernels (ms):");
+        for (int k = 0; k < TC_K_COUNT; k++) {
+            double ms = 0;
+            long long n = 0;
+            check(tc_timing_read(e, k, &ms, &n), "tc_timing_read");
+            if (n) printf(" %s %.3f", names[k], ms);
+        }
+        printf("\n");
+    }
+
+    // The SRK time loop (collocated_SRK/main.cpp:466-549): a fixed step, the bulk
+    // velocity and the friction velocity from the first node plane, averaging
+    // from t = 0 with t = (step + 1) tau, fields and profiles every FProgress
+    // steps and at the final step. Velocity in slot 0, pressure in pressure slot
+    // 0, dp/dt (StabType 1) in pressure slot 1.
+    int srk_loop(FILE* log) {
+        const int sprog = o.s_progress > 0 ? o.s_progress : 10;
+        const int fprog = o.f_progress > 0 ? o.f_progress : 10000;
+        const int pprog = o.p_progress > 0 ? o.p_progress : fprog;
+        std::vector<double> prof((size_t)nprof * P.nz);
+        printf("Start of time integration (N=%i,%i,%i)\n", P.nx, P.ny, P.nz);
+        const double t_total0 = now();
+        for (bool final_step = false; !final_step; it++) {
+            const double t0 = now();
+            check(tc_srk_step(e, it * tau, tau, 0, 0, 1), "tc_srk_step");
+            check(tc_sync(e), "tc_sync");
+            const double t1 = now();
+            sec_step += t1 - t0;
+            double q[3];
+            check(tc_diagnostics(e, 0, 0, tau, 3, q, nullptr, nullptr, prof.data(), prof.size()), "tc_diagnostics");
+            const double flow_rate = q[0] / P.area;
+            for (size_t i = 0; i < prof.size(); i++) av[i] = T(prof[i]);
+            for (size_t i = 0; i < av.size(); i++) av_sum_r[i] += T(tau) * av[i];
+            const double t3 = now();
+            sec_red += t3 - t1;
+
+            t = (it + 1) * tau;
+            intergal_time = t;
+            // the reference's loop ends at TimeMax only (its step limit is fixed at 10^6);
+            // here the step limit also makes the final step, as in the other loops
+            if (t >= o.time_max || it + 1 == o.time_steps_max) final_step = true;
+            const double dudy_cur = (av[nprof] - 0.) / P.dz[0];
+            const double ufr_cur = dudy_cur < 0. ? 0. : std::sqrt(dudy_cur * P.visc);
+            const double dudy_int = av_sum_r[nprof] / (P.dz[0] * t);
+            const double ufr_int = dudy_int < 0. ? 0. : std::sqrt(dudy_int * P.visc);
+            if (it % sprog == 0 || final_step) {
+                printf("T=%f Ubulk=%.04f Ufric=%.04f UfricAv=%.04f\n", t, flow_rate, ufr_cur, ufr_int);
+                fprintf(log, "%f %f %f %f\n", t, flow_rate, ufr_cur, ufr_int);
+                fflush(stdout);
+            }
+            if ((it != 0 && it % fprog == 0) || final_step) {
+                char fname[64];
+                snprintf(fname, sizeof fname, "q%05lld.vtk", it);
+                download(0);
+                dump_vtk(fname);
+            }
+            if ((it != 0 && it % pprog == 0) || final_step) {
+                char fname[64];
+                snprintf(fname, sizeof fname, "r%05lld.dat", it);
+                dump_profiles(av.data(), fname);
+            }
+            sec_out += now() - t3;
+            if (o.checkpoint_every > 0 && ((it + 1) % o.checkpoint_every == 0 || final_step)) write_restart();
+        }
+        fclose(log);
+        check(tc_sync(e), "tc_sync");
+        printf("Time: total %.04f, step %.04f, diagnostics %.04f, output %.04f\n", now() - t_total0, sec_step, sec_red,
+               sec_out);
+        report_timing();
+        download(0);
+        dump_vtk("output.vtk");
+        if (t > 0.) {
+            for (T& v : av_sum_r) v *= T(1. / t);
+            dump_profiles(av_sum_r.data(), "res.dat");
+            const double a1 = av_sum_r[nprof], a2 = av_sum_r[2 * nprof];
+            const double dudy = 1.5 * (a1 - 0.) / P.dz[0] - 0.5 * (a2 - a1) / P.dz[1];
+            const double u_fric = dudy < 0. ? 0. : std::sqrt(P.visc * dudy);
+            printf("u_fric: obtained=%.6f, expected=%.6f\n", u_fric, P.u_tau);
+        }
+        check(tc_destroy(e), "tc_destroy");
+        return 0;
+    }
+
     static double now() {
         using namespace std::chrono;
         return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -700,7 +842,8 @@ struct Run {
                     }
                     check(tc_initial_field(e, 0, (unsigned long long)o.seed, o.ubulk, o.noise), "tc_initial_field");
                 } else {
-                    analytic_start();
+                    if (P.scheme == TC_SCHEME_SRK) srk_start();
+                    else analytic_start();
                     std::vector<T> zv(3 * n3d, T(0)), zp(n3d, T(0));
                     for (int s = 0; s < 4; s++) {
                         check(tc_set_field(e, TC_VEL, s, s == 0 ? vel.data() : zv.data(), vel.size() * sizeof(T)),
@@ -714,6 +857,7 @@ struct Run {
         }
         if (!log) die("cannot open log.dat");
         check(tc_sync(e), "tc_sync");
+        if (P.scheme == TC_SCHEME_SRK) return srk_loop(log);
 
         const int sprog = o.s_progress > 0 ? o.s_progress : (P.collocated ? 10 : 200);
         const int fprog = o.f_progress > 0 ? o.f_progress : (P.collocated ? 10000 : 1000000);
@@ -801,18 +945,7 @@ struct Run {
             printf("Time: total %.04f, step %.04f, diagnostics %.04f, output %.04f\n", total, sec_step, sec_red,
                    sec_out);
         }
-        if (o.timing) {
-            static const char* names[TC_K_COUNT] = {"turb_visc", "flux", "div", "fft_x", "fft_y",
-                                                    "zsolve", "ifft_y", "ifft_x", "project", "other"};
-            printf("Kernels (ms):");
-            for (int k = 0; k < TC_K_COUNT; k++) {
-                double ms = 0;
-                long long n = 0;
-                check(tc_timing_read(e, k, &ms, &n), "tc_timing_read");
-                if (n) printf(" %s %.3f", names[k], ms);
-            }
-            printf("\n");
-        }
+        report_timing();
 
         download(cur);
         dump_vtk("output.vtk");
@@ -843,6 +976,9 @@ struct Run {
 int main(int argc, char** argv) {
     Options o;
     if (!parse(argc, argv, o)) return 0;
+    // collocated_SRK/main.cpp:452-454: TimeMax 0.1, tau 0.8e-3
+    if (o.time_max < 0) o.time_max = o.scheme == "srk" ? 0.10000000001 : 100.0;
+    if (o.tau < 0) o.tau = o.scheme == "srk" ? 0.8e-3 : 1e-3;
     if (o.tau <= 0 || o.cfl_min < 0 || o.cfl_max <= 0) die("tau and CFLmax must be positive");
     if (o.s_progress == 0 || o.f_progress == 0 || o.p_progress == 0) die("progress intervals must be positive");
     Problem P = setup(o);
