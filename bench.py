@@ -62,7 +62,7 @@ WORKLOADS = {
     "les_learned_64x64x96": (tca.SCHEME_AB3, 64, 64, 96, 2.0, 21900.0, 590.0, 2e-3, dict(visc_type=1, sgs_model=1)),
     "les_smag_64x64x96": (tca.SCHEME_AB3, 64, 64, 96, 2.0, 21900.0, 590.0, 2e-3, dict(visc_type=1, turb_visc=1)),
     # collocated IMEX Runge-Kutta (SRK) stepper, ARS(3,4,3) with StabType 1 as in
-    # collocated_SRK/main.cpp:363-373: the reference's grid size (128 x 64 x 49
+    # collocated_SRK/main.cpp:342-352: the reference's grid size (128 x 64 x 49
     # nodes) and a large grid
     "srk_128x64x49": (tca.SCHEME_SRK, 128, 64, 49, 2.0, 5600.0, 180.0, 8e-4, dict(imex=2)),
     "srk_384x384x192": (tca.SCHEME_SRK, 384, 384, 192, 2.0, 13750.0, 395.0, 5e-4, dict(imex=2)),

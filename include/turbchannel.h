@@ -78,9 +78,9 @@ typedef struct tc_params {
                               counterpart): each step evaluates tau from the velocity with the
                               network set by tc_set_sgs_params and subtracts its divergence */
     int imex;              /* TC_SCHEME_SRK: tc_imex method (main_params::IMEXtype, NStages,
-                              ButcherE/I, d; collocated_SRK/main.cpp:362-372) */
+                              ButcherE/I, d; collocated_SRK/main.cpp:342-350) */
     double alpha_tau;      /* TC_SCHEME_SRK: alpha = alpha_tau / tau; <= 0 selects the reference's
-                              default 0.5 * alpha_tau_max[stab_type] (collocated_SRK/main.cpp:373) */
+                              default 0.5 * alpha_tau_max[stab_type] (collocated_SRK/main.cpp:352) */
 } tc_params;
 
 /* Lifecycle. Replaces data::alloc_all + InitFourier + CopyDataToDevice

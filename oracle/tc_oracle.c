@@ -1128,7 +1128,7 @@ void or_normalize_pressure(or_solver* s, double* f) {
     } else {
 #pragma omp parallel for reduction(+ : psum)
         for (int i = 0; i < N3D; i++) psum += f[i] * cell_volume(s, i);
-        /* SRK: psum / area (collocated_SRK/func_global.h:170-177) */
+        /* SRK: psum / area (collocated_SRK/func_global.h:130-137) */
         double p_shift = s->P.scheme == OR_SCHEME_SRK ? psum / s->area : psum * s->P.hx * s->P.hy / s->area;
 #pragma omp parallel for
         for (int i = 0; i < N3D; i++) f[i] -= p_shift;
@@ -1436,19 +1436,19 @@ static int imex_rows(int method, int* type, double* BE, double* BI) {
 #define ROW(T, r, ...) do { const double v_[] = {__VA_ARGS__}; memcpy(T + (r) * S, v_, sizeof v_); } while (0)
     int S = 0;
     switch (method) {
-    case OR_IMEX_ARS_121: /* :48-63 */
+    case OR_IMEX_ARS_121: /* :48-62 */
         S = 2; *type = OR_IMEX_TYPE_ARS;
         ROW(BE, 1, 1., 0.); ROW(BE, 2, 0., 1.);
         ROW(BI, 1, 0., 1.); ROW(BI, 2, 0., 1.);
         break;
-    case OR_IMEX_ARS_232: { /* :66-84 */
+    case OR_IMEX_ARS_232: { /* :65-83 */
         S = 3; *type = OR_IMEX_TYPE_ARS;
         const double g = 1. - 0.5 * sqrt(2.), dl = -2. * sqrt(2.) / 3.;
         ROW(BE, 1, g, 0., 0.); ROW(BE, 2, dl, 1. - dl, 0.); ROW(BE, 3, 0., 1. - g, g);
         ROW(BI, 1, 0., g, 0.); ROW(BI, 2, 0., 1. - g, g); ROW(BI, 3, 0., 1. - g, g);
         break;
     }
-    case OR_IMEX_ARS_343: { /* :87-112 */
+    case OR_IMEX_ARS_343: { /* :86-111 */
         S = 4; *type = OR_IMEX_TYPE_ARS;
         const double g = gL;
         const double b1 = -1.5 * g * g + 4 * g - 0.25, b2 = 1.5 * g * g - 5. * g + 1.25;
@@ -1463,7 +1463,7 @@ static int imex_rows(int method, int* type, double* BE, double* BI) {
         ROW(BE, 4, 0., b1, b2, g);
         break;
     }
-    case OR_IMEX_MARS_343: { /* :116-136 */
+    case OR_IMEX_MARS_343: { /* :115-135 */
         S = 4; *type = OR_IMEX_TYPE_ARS;
         const double g = gL, b2 = 1.20849664917601;
         ROW(BI, 1, 0., g, 0., 0.); ROW(BI, 2, 0., 0.28206673924577, g, 0.);
@@ -1473,7 +1473,7 @@ static int imex_rows(int method, int* type, double* BE, double* BI) {
         ROW(BE, 4, 0., b2, 1. - g - b2, g);
         break;
     }
-    case OR_IMEX_ARK3: { /* :139-160 */
+    case OR_IMEX_ARK3: { /* :138-160 */
         S = 4; *type = OR_IMEX_TYPE_CK;
         const double g = gL;
         const double b1 = 1471266399579. / 7840856788654., b2 = -4482444167858. / 7529755066697.;
@@ -1559,7 +1559,7 @@ int or_imex_table(int method, int* type, double* be, double* bi, double* d, doub
     if (type) *type = ty;
     if (be) memcpy(be, BE, sizeof(double) * (size_t)(S * (S + 1)));
     if (bi) memcpy(bi, BI, sizeof(double) * (size_t)(S * (S + 1)));
-    if (d) { /* CalcD (collocated_SRK/imex.cpp:35-45) */
+    if (d) { /* CalcD (collocated_SRK/imex.cpp:34-44) */
         const double ass = BI[S * S - 1];
         d[0] = 1.;
         for (int j = 1; j < S; j++) {

@@ -107,7 +107,7 @@ def test_driver_binary_exists():
 
 
 def test_driver_srk_defaults_are_the_reference_mesh(tmp_path):
-    """--scheme srk without mesh options: collocated_SRK/main.cpp:320-340 (Re_tau 180,
+    """--scheme srk without mesh options: collocated_SRK/main.cpp:312-333 (Re_tau 180,
     first step 1, ratio 1.3, steps capped at 10 wall units), 49 nodes."""
     n, z = mesh(tmp_path, "--scheme", "srk")
     zp = [0.0, 1.0]

@@ -197,7 +197,7 @@ def test_driver_learned_closure_from_file(tmp_path):
 
 
 def test_driver_srk_matches_oracle_replay(tmp_path):
-    """--scheme srk: the fixed-step loop of collocated_SRK/main.cpp:466-549 against
+    """--scheme srk: the fixed-step loop of collocated_SRK/main.cpp:472-550 against
     or_srk_step replayed on the oracle (log lines, 5-value VTK with dp/dt,
     profiles, time-averaged profiles from t = 0 and the friction velocity)."""
     import oracle

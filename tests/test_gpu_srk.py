@@ -19,7 +19,7 @@ TOL32 = 1e-5
 CASES = [
     (0, 0, 1.0, 16, 12, 18),
     (1, 1, 0.0, 16, 12, 18),   # alpha_tau <= 0: the reference's default 0.5 * alpha_tau_max
-    (2, 1, 1.0, 16, 12, 18),   # ARS(3,4,3), StabType 1: the reference's own choice (main.cpp:363-373)
+    (2, 1, 1.0, 16, 12, 18),   # ARS(3,4,3), StabType 1: the reference's own choice (main.cpp:342-352)
     (2, 0, 0.6, 30, 10, 14),
     (3, 0, 1.0, 16, 12, 18),
     (4, 1, 1.0, 16, 12, 18),
@@ -139,7 +139,7 @@ def test_srk_slots_and_refusals():
 
 def test_srk_diagnostics_are_collocated():
     """Reductions on an SRK engine are the collocated ones (CalcIntegral,
-    CalcVelocityProfile of collocated_SRK/func_global.h:180-276)."""
+    CalcVelocityProfile of collocated_SRK/func_global.h:145-231)."""
     sol, eng, a, u, p, q = setup(*CASES[2])
     put(eng, u, p, q)
     ref = sol.integral(u)

@@ -513,7 +513,7 @@ int normalize(tc_engine* e, T* p) {
     else if (e->P.scheme == TC_SCHEME_VV) {
         coef = 1.0 / (e->hm.KERG_norm2 * nxy);
         w = (const T*)e->d_kerg;
-    } else if (e->srk) coef = 1.0 / e->hm.area;  // zero mean (collocated_SRK/func_global.h:170-177)
+    } else if (e->srk) coef = 1.0 / e->hm.area;  // zero mean (collocated_SRK/func_global.h:130-137)
     else coef = e->P.hx * e->P.hy / e->hm.area;
     HIPCHK(launch_add_scaled_plane<T>(e->s, m, p, e->d_out, coef, w));
     return TC_OK;

@@ -143,7 +143,7 @@ bool imex_table(int method, ImexTable& t) {
         default:
             return false;
     }
-    // d_j of the methods of type CK (collocated_SRK/imex.cpp:39-49): d_0 = 1,
+    // d_j of the methods of type CK (collocated_SRK/imex.cpp:34-44): d_0 = 1,
     // d_j = -sum_{k<j} a_jk d_k / a_ss with a_ss the last diagonal entry of I
     const int s = t.s;
     const double ass = t.I[s * s - 1];

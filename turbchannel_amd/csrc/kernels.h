@@ -78,7 +78,7 @@ struct SrkVec {
 template <typename T>
 hipError_t launch_srk_vec(hipStream_t s, const Mesh<T>& m, const SrkVec<T>& a);
 
-// Right-hand side scalar of the stage pressure system (collocated_SRK/func_global.h:360-382).
+// Right-hand side scalar of the stage pressure system (collocated_SRK/func_global.h:357-384).
 // stab 0: bs = (m0 p + sum_j cm[j] mujt[j]) - clast p          (m0 only for ck)
 // stab 1: muj = m0 (tau_stage q) + sum_j cm[j] mujt[j]; pjhat = p + ph0 q + sum_j cq[j] qj[j];
 //         bs = muj + pjhat - clast q                            (m0, ph0 only for ck)
@@ -99,7 +99,7 @@ struct SrkPre {
 template <typename T>
 hipError_t launch_srk_pre(hipStream_t s, const Mesh<T>& m, const SrkPre<T>& a);
 
-// After the stage pressure solve (collocated_SRK/func_global.h:394-398):
+// After the stage pressure solve (collocated_SRK/func_global.h:389-393):
 // pj = mujt + bs; mujt -= c (stab ? tau_stage q : p) (skipped when last);
 // stab: qj = inv_tau_stage (pj - pjhat). pj_out may alias p and qj_out q.
 template <typename T>

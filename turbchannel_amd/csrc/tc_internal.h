@@ -46,7 +46,7 @@ void make_initial_field(const tc_params& p, const HostMesh& m, unsigned long lon
                         double noise_rms, double* out);
 
 // IMEX Runge-Kutta method of the SRK stepper: E and I are (s+1) x s row-major
-// (row s = weights), d the coefficients of CalcD (collocated_SRK/imex.cpp:35-45).
+// (row s = weights), d the coefficients of CalcD (collocated_SRK/imex.cpp:34-44).
 constexpr int kMaxImexStages = 6;
 struct ImexTable {
     int type = 0, s = 0;

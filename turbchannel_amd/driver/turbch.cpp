@@ -4,7 +4,7 @@
 // include/turbchannel.h: the same options (staggered/io.cpp:107-126,
 // collocated_AB3/io.cpp:104-123), the same time loop with the BDF order ramp and
 // CFL-driven step control (staggered/main.cpp:288-371, collocated_AB3/main.cpp:392-476)
-// or, for the IMEX Runge-Kutta solver, with a fixed step (collocated_SRK/main.cpp:466-549),
+// or, for the IMEX Runge-Kutta solver, with a fixed step (collocated_SRK/main.cpp:472-550),
 // the same screen and log.dat lines, the same files (qNNNNN.vtk, rNNNNN.dat,
 // output.vtk, output.dat, res.dat; staggered/io.cpp:6-96, collocated_AB3/io.cpp:9-101)
 // and the same start from input.vtk when it exists (staggered/io.cpp:31-69).
@@ -68,7 +68,7 @@ struct Options {
     int visc = -1;    // staggered 0 CROSS / 1 NAIVE, collocated 0 FD / 1 Galerkin; default follows les
     int stab = 1, fd_mode = 1;
     double re_tau = -1;  // 395 (180 for srk) unless given
-    std::string imex = "ars343";  // srk: IMEX Runge-Kutta method (collocated_SRK/main.cpp:363)
+    std::string imex = "ars343";  // srk: IMEX Runge-Kutta method (collocated_SRK/main.cpp:342)
     double alpha_tau = 0;         // srk: <= 0 selects 0.5 * alpha_tau_max[stab]
     // options with no reference counterpart
     double re_b = 0;     // > 0: bulk units, nu = 2 / Re_b
@@ -357,7 +357,7 @@ Problem setup(const Options& o) {
     P.hy = P.Ly / P.ny;
     P.xmin = 0.0;
     P.ymin = -0.5 * P.Ly;
-    Options om = o;  // geometric mesh defaults: collocated_AB3/main.cpp:209-211, collocated_SRK/main.cpp:324-326
+    Options om = o;  // geometric mesh defaults: collocated_AB3/main.cpp:209-211, collocated_SRK/main.cpp:315-317
     if (om.geo_y1 <= 0) om.geo_y1 = srk ? 1.0 : 50.0;
     if (om.geo_ratio <= 0) om.geo_ratio = srk ? 1.3 : 1.05;
     if (om.geo_ybulk <= 0) om.geo_ybulk = srk ? 10.0 : 2.2;
@@ -476,7 +476,7 @@ struct Run {
         }
     }
 
-    // The SRK start (collocated_SRK/main.cpp:437-447): laminar parabola, a
+    // The SRK start (collocated_SRK/main.cpp:441-451): laminar parabola, a
     // spanwise velocity 0.1 sin(x/2) z+ exp(-0.01 z+^2), and 0.1 rand()/RAND_MAX
     // added to every component away from the walls (the C library's rand(),
     // unseeded, as the reference).
@@ -499,7 +499,7 @@ struct Run {
 
     // DumpData (staggered/io.cpp:6-29, collocated_AB3/io.cpp:9-32): ASCII VTK
     // rectilinear grid; z coordinates are the faces (staggered) or nodes. The SRK
-    // solver's file has dp/dt as a fifth value (collocated_SRK/main.cpp:247-268).
+    // solver's file has dp/dt as a fifth value (collocated_SRK/main.cpp:238-259).
     void dump_vtk(const char* fname) const {
         FILE* f = fopen(fname, "wt");
         if (!f) die(std::string("cannot write ") + fname);
@@ -732,7 +732,7 @@ struct Run {
         printf("\n");
     }
 
-    // The SRK time loop (collocated_SRK/main.cpp:466-549): a fixed step, the bulk
+    // The SRK time loop (collocated_SRK/main.cpp:472-550): a fixed step, the bulk
     // velocity and the friction velocity from the first node plane, averaging
     // from t = 0 with t = (step + 1) tau, fields and profiles every FProgress
     // steps and at the final step. Velocity in slot 0, pressure in pressure slot
@@ -976,7 +976,7 @@ struct Run {
 int main(int argc, char** argv) {
     Options o;
     if (!parse(argc, argv, o)) return 0;
-    // collocated_SRK/main.cpp:452-454: TimeMax 0.1, tau 0.8e-3
+    // collocated_SRK/main.cpp:437-438: TimeMax 0.1, tau 0.8e-3
     if (o.time_max < 0) o.time_max = o.scheme == "srk" ? 0.10000000001 : 100.0;
     if (o.tau < 0) o.tau = o.scheme == "srk" ? 0.8e-3 : 1e-3;
     if (o.tau <= 0 || o.cfl_min < 0 || o.cfl_max <= 0) die("tau and CFLmax must be positive");
