@@ -156,6 +156,7 @@ struct tc_engine {
         m.GC2 = b + 4 * mstride;
         m.GCZ = b + 5 * mstride;
         m.cellh = b + 6 * mstride;
+        m.xcd_swz = (size_t)nxy * 3 * sizeof(T) * 4 <= ((size_t)4 << 20) ? 1 : 0;
         return m;
     }
 

@@ -5,6 +5,14 @@
 
 namespace tc {
 
+// Block b of a launch runs on XCD b % 8. Renumbering b -> xcd_swizzle(b, nb)
+// gives each XCD one contiguous range of logical blocks, so blocks that read
+// each other's neighbourhoods (adjacent rows and planes) share an L2.
+__device__ __forceinline__ int xcd_swizzle(int b, int nb) {
+    const int q = nb / 8, r = nb % 8, x = b % 8, i = b / 8;
+    return x * q + min(x, r) + i;
+}
+
 // Explicit term (convection + viscosity + source) fused with the BDF3/EX3
 // right-hand side. order == 0: write only kuhat = F(vel) to fcur.
 template <typename T>

@@ -233,7 +233,9 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
     const int N = pl.n, nrows = m.ny * m.nz;
     Cplx<T>* b1 = b0 + lds_elems(lpb * N);
     Cplx<T>* twl = b1 + lds_elems(lpb * N + 2 * lpb);
-    const int q0 = blockIdx.x * lpb;
+    // XCD-swizzled: the y and z taps of the divergence read rows of neighbouring
+    // blocks, which then run on the same XCD
+    const int q0 = (m.xcd_swz ? xcd_swizzle(blockIdx.x, gridDim.x) : (int)blockIdx.x) * lpb;
     TwStage<T> tws;
     tws.load(tw, N);
     const int tot = lpb * N;

@@ -23,8 +23,19 @@ struct Nb {
     __device__ __forceinline__ int at(int dx, int dy, int dz) const { return base + dz * nxy + y[dy + 3] + x[dx + 3]; }
 };
 
-__device__ __forceinline__ bool cell_of_thread(int nx, int nxy, int& ix, int& iy) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
+// Kernels over nodes launch a (plane chunk, plane) grid; with swz the logical
+// block is XCD-swizzled so that each XCD walks a contiguous range of planes.
+struct PlaneBlock {
+    int bx, iz;
+};
+__device__ __forceinline__ PlaneBlock plane_block(int swz) {
+    if (!swz) return {(int)blockIdx.x, (int)blockIdx.y};
+    const int lb = xcd_swizzle(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    return {lb % (int)gridDim.x, lb / (int)gridDim.x};
+}
+
+__device__ __forceinline__ bool cell_of_thread(int nx, int nxy, int& ix, int& iy, int bx) {
+    const int t = bx * kBlock + threadIdx.x;
     if (t >= nxy) return false;
     iy = t / nx;
     ix = t - iy * nx;
@@ -129,10 +140,6 @@ constexpr int flux_min_waves() {
     return 1;
 }
 
-__device__ __forceinline__ int xcd_swizzle(int b, int nb) {
-    const int q = nb / 8, r = nb % 8, x = b % 8, i = b / 8;
-    return x * q + min(x, r) + i;
-}
 
 // nu_t at an offset; an offset past a wall reads the wall plane (the stencils
 // read such values unconditionally and discard them).
@@ -495,9 +502,10 @@ inline FluxGrid flux_grid(int nx, int ny, int nz, int ty, int slots, int halo) {
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_turb_staggered(Mesh<T> m, const T* __restrict__ vel, T* __restrict__ nut) {
     // CalcAbsS + RecalcTurbVisc (staggered/func_global.h:823-890)
-    const int iz = blockIdx.y, NZ = m.nz;
+    const PlaneBlock pb = plane_block(m.xcd_swz);
+    const int iz = pb.iz, NZ = m.nz;
     int ix, iy;
-    if (!cell_of_thread(m.nx, m.nxy, ix, iy)) return;
+    if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx)) return;
     Nb nb;
     make_nb<1>(nb, ix, iy, iz, m.nx, m.ny);
     const T dz = m.DZ[iz];
@@ -548,9 +556,10 @@ __global__ __launch_bounds__(kBlock) void k_turb_staggered(Mesh<T> m, const T* _
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_turb_ab3(Mesh<T> m, const T* __restrict__ vel, T* __restrict__ nut) {
     // element strain rate (collocated_AB3/func_global.h:342-384); plane nz-1 has no element
-    const int iz = blockIdx.y, NZ = m.nz;
+    const PlaneBlock pb = plane_block(m.xcd_swz);
+    const int iz = pb.iz, NZ = m.nz;
     int ix, iy;
-    if (iz >= NZ - 1 || !cell_of_thread(m.nx, m.nxy, ix, iy)) return;
+    if (iz >= NZ - 1 || !cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx)) return;
     Nb nb;
     make_nb<1>(nb, ix, iy, iz, m.nx, m.ny);
     const T hz = m.DZ[iz];
@@ -581,9 +590,10 @@ __global__ __launch_bounds__(kBlock) void k_turb_ab3(Mesh<T> m, const T* __restr
 
 template <typename T, int SCHEME>
 __global__ __launch_bounds__(kBlock) void k_grad(Mesh<T> m, const T* __restrict__ p, T* __restrict__ g) {
-    const int iz = blockIdx.y;
+    const PlaneBlock pb = plane_block(m.xcd_swz);
+    const int iz = pb.iz;
     int ix, iy;
-    if (!cell_of_thread(m.nx, m.nxy, ix, iy)) return;
+    if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx)) return;
     Nb nb;
     make_nb<2>(nb, ix, iy, iz, m.nx, m.ny);
     T r[3];
@@ -597,9 +607,10 @@ __global__ __launch_bounds__(kBlock) void k_grad(Mesh<T> m, const T* __restrict_
 // K7: velocity correction (staggered/func_global.h:937-941; collocated_AB3/func_global.h:428-449)
 template <typename T, int SCHEME, bool STAB>
 __global__ __launch_bounds__(kBlock) void k_project(Mesh<T> m, ProjectArgs<T> a) {
-    const int iz = blockIdx.y;
+    const PlaneBlock pb = plane_block(m.xcd_swz);
+    const int iz = pb.iz;
     int ix, iy;
-    if (!cell_of_thread(m.nx, m.nxy, ix, iy)) return;
+    if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx)) return;
     Nb nb;
     make_nb<2>(nb, ix, iy, iz, m.nx, m.ny);
     const int in = nb.at(0, 0, 0);
@@ -623,9 +634,10 @@ __global__ __launch_bounds__(kBlock) void k_project(Mesh<T> m, ProjectArgs<T> a)
 
 template <typename T, int SCHEME>
 __global__ __launch_bounds__(kBlock) void k_add_u(Mesh<T> m, T* __restrict__ v, const double* __restrict__ shift) {
-    const int iz = blockIdx.y;
+    const PlaneBlock pb = plane_block(m.xcd_swz);
+    const int iz = pb.iz;
     int ix, iy;
-    if (!cell_of_thread(m.nx, m.nxy, ix, iy)) return;
+    if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx)) return;
     if (SCHEME == TC_SCHEME_AB3 && (iz == 0 || iz == m.nz - 1)) return;
     v[3 * (iz * m.nxy + iy * m.nx + ix)] += T(*shift);
 }
@@ -634,9 +646,10 @@ __global__ __launch_bounds__(kBlock) void k_add_u(Mesh<T> m, T* __restrict__ v, 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_shift_plane(Mesh<T> m, T* __restrict__ f, const double* __restrict__ sum,
                                                          double coef, const T* __restrict__ w) {
-    const int iz = blockIdx.y;
+    const PlaneBlock pb = plane_block(m.xcd_swz);
+    const int iz = pb.iz;
     int ix, iy;
-    if (!cell_of_thread(m.nx, m.nxy, ix, iy)) return;
+    if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx)) return;
     const int in = iz * m.nxy + iy * m.nx + ix;
     const T s = T(sum[0] * coef);
     f[in] -= w ? s * w[iz] : s;
@@ -654,9 +667,10 @@ dim3 cell_grid(const Mesh<T>& m) {
 // ---------------------------------------------------------------------------
 template <typename T, bool GRAD>
 __global__ __launch_bounds__(kBlock) void k_srk_vec(Mesh<T> m, SrkVec<T> a) {
-    const int iz = blockIdx.y;
+    const PlaneBlock pb = plane_block(m.xcd_swz);
+    const int iz = pb.iz;
     int ix, iy;
-    if (!cell_of_thread(m.nx, m.nxy, ix, iy)) return;
+    if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx)) return;
     Nb nb;
     make_nb<1>(nb, ix, iy, iz, m.nx, m.ny);
     const int in = nb.at(0, 0, 0);
@@ -725,9 +739,10 @@ __global__ __launch_bounds__(kBlock) void k_srk_vec(Mesh<T> m, SrkVec<T> a) {
 
 template <typename T, bool STAB, bool CK>
 __global__ __launch_bounds__(kBlock) void k_srk_pre(Mesh<T> m, SrkPre<T> a) {
-    const int iz = blockIdx.y;
+    const PlaneBlock pb = plane_block(m.xcd_swz);
+    const int iz = pb.iz;
     int ix, iy;
-    if (!cell_of_thread(m.nx, m.nxy, ix, iy)) return;
+    if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx)) return;
     const int in = iz * m.nxy + iy * m.nx + ix;
     const T p = a.p[in];
     if (!STAB) {
@@ -756,9 +771,10 @@ __global__ __launch_bounds__(kBlock) void k_srk_pre(Mesh<T> m, SrkPre<T> a) {
 
 template <typename T, bool STAB, bool LAST>
 __global__ __launch_bounds__(kBlock) void k_srk_post(Mesh<T> m, SrkPost<T> a) {
-    const int iz = blockIdx.y;
+    const PlaneBlock pb = plane_block(m.xcd_swz);
+    const int iz = pb.iz;
     int ix, iy;
-    if (!cell_of_thread(m.nx, m.nxy, ix, iy)) return;
+    if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx)) return;
     const int in = iz * m.nxy + iy * m.nx + ix;
     const T mu = a.mujt[in];
     const T pj = mu + a.bs[in];

@@ -110,6 +110,11 @@ struct Mesh {
     T hx, hy, inv_hx, inv_hy, visc, sour, inv_len;
     T zlo, zhi;  // wall coordinates
     const T *ZZ, *DZ, *DZW, *GC1, *GC2, *GCZ, *cellh;
+    // 1: kernels that read neighbouring rows and planes renumber their blocks
+    // so that each XCD works on a contiguous range of planes (set when four
+    // planes of a real3 field fit one XCD's 4 MiB L2, so the neighbours hit
+    // there); 0: plain order, every XCD streaming the same region
+    int xcd_swz = 0;
 };
 
 }  // namespace tc
