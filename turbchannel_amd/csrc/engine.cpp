@@ -156,6 +156,12 @@ struct tc_engine {
         m.GC2 = b + 4 * mstride;
         m.GCZ = b + 5 * mstride;
         m.cellh = b + 6 * mstride;
+        m.IDZ = b + 7 * mstride;
+        m.IHDN = b + 8 * mstride;
+        m.IHUP = b + 9 * mstride;
+        m.IDZM = b + 10 * mstride;
+        m.IVOL = b + 11 * mstride;
+        m.IHC = b + 12 * mstride;
         m.xcd_swz = (size_t)nxy * 3 * sizeof(T) * 4 <= ((size_t)4 << 20) ? 1 : 0;
         return m;
     }
@@ -283,10 +289,11 @@ int setup_device(tc_engine* e) {
         e->spjhat = p;
     }
     if (int r = dalloc_bytes(e, &e->spec, (size_t)e->nm * e->nz * sizeof(Cplx<T>))) return r;
-    // mesh arrays: ZZ, DZ, DZW, GC1, GC2, GCZ, cellh, each mstride long
+    // mesh arrays: ZZ, DZ, DZW, GC1, GC2, GCZ, cellh, then the per-plane
+    // quotients IDZ, IHDN, IHUP, IDZM, IVOL, IHC, each mstride long
     const HostMesh& hm = e->hm;
     e->mstride = (size_t)e->nz + 8;
-    std::vector<double> mh(7 * e->mstride, 0.0);
+    std::vector<double> mh(kMeshArrays * e->mstride, 0.0);
     auto put = [&](int k, const std::vector<double>& v) {
         for (size_t i = 0; i < v.size() && i < e->mstride; i++) mh[k * e->mstride + i] = v[i];
     };
@@ -297,6 +304,24 @@ int setup_device(tc_engine* e) {
     put(4, hm.GC2);
     put(5, hm.GCZ);
     put(6, hm.cellh);
+    {
+        // quotients in T, as the device would form them (stored exactly in double)
+        const int nz = e->nz;
+        const bool coll = e->P.scheme == TC_SCHEME_AB3;
+        auto dzt = [&](int i) -> T { return (i < 0 || i >= (int)hm.DZ.size()) ? T(0) : (T)hm.DZ[i]; };
+        auto q = [](T num, T den) -> T { return den == T(0) ? T(0) : num / den; };
+        const T hx = (T)e->P.hx, hy = (T)e->P.hy;
+        for (int iz = 0; iz < nz; iz++) {
+            const T dz = dzt(iz), dzm = dzt(iz - 1), dzp = iz + 1 < nz ? dzt(iz + 1) : T(0);
+            const T dzw = iz < (int)hm.DZW.size() ? (T)hm.DZW[iz] : T(0);
+            mh[7 * e->mstride + iz] = (double)q(T(1), dz);
+            mh[8 * e->mstride + iz] = (double)q(T(2), dz + dzm);
+            mh[9 * e->mstride + iz] = (double)q(T(2), dz + dzp);
+            mh[10 * e->mstride + iz] = (double)q(T(1), dzt(std::max(iz - 1, 0)));
+            mh[11 * e->mstride + iz] = (double)q(T(-1), T(32) * (dz - T(1.0 / 9.0) * dzw) * hx * hy);
+            mh[12 * e->mstride + iz] = (coll && iz > 0 && iz < nz - 1) ? (double)q(T(2), dzm + dz) : 0.0;
+        }
+    }
     if (int r = dalloc_bytes(e, &e->dmesh, mh.size() * sizeof(T))) return r;
     if (int r = upload_as<T>(e, e->dmesh, mh.data(), mh.size())) return r;
     // normalisation weights (NormalizePressure)

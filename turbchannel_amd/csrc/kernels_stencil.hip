@@ -61,7 +61,7 @@ __device__ __forceinline__ void grad_at(const Mesh<T>& m, const T* __restrict__ 
         const T a0 = a[nb.at(0, 0, 0)];
         g[0] = m.inv_hx * (a0 - a[nb.at(-1, 0, 0)]);
         g[1] = m.inv_hy * (a0 - a[nb.at(0, -1, 0)]);
-        g[2] = iz == 0 ? T(0) : (a0 - a[nb.at(0, 0, -1)]) * (T(2) / (m.DZ[iz] + m.DZ[iz - 1]));
+        g[2] = iz == 0 ? T(0) : (a0 - a[nb.at(0, 0, -1)]) * m.IHDN[iz];  // 2 / (DZ[iz] + DZ[iz-1])
     } else if (SCHEME == TC_SCHEME_VV) {
         const T a0 = a[nb.at(0, 0, 0)];
         const T g1 = m.GC1[iz], g2 = m.GC2[iz];
@@ -115,6 +115,7 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // XCD works on neighbouring tiles and shares their halos in its L2.
 // ---------------------------------------------------------------------------
 constexpr int kTX = 32;
+constexpr int kFluxMeshArrays = 10;  // DZ, DZW, GCZ, cellh and the six per-plane quotients
 // Tile height (one thread per column of the tile).
 template <typename T, int SCHEME, int VISC>
 constexpr int flux_ty() {
@@ -227,21 +228,26 @@ __global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>()), (flux_min_waves<
     constexpr int SLOT = ND * NT * 4 / (int)sizeof(T);  // reals per ring slot
     __shared__ __align__(16) T ring[NS * SLOT];
     __shared__ double sh[NT / 64];
-    // The mesh arrays the stencils read, mirrored in LDS (dynamic, 4 nz reals):
-    // LDS reads wait on their own counter, never on the global loads in flight.
+    // The mesh arrays the stencils read, mirrored in LDS (dynamic, kFluxMeshArrays
+    // nz reals): LDS reads wait on their own counter, never on the global loads
+    // in flight.
     extern __shared__ __align__(16) unsigned char mesh_raw[];
     {
         T* ml = reinterpret_cast<T*>(mesh_raw);
-        for (int i = threadIdx.x; i < m.nz; i += NT) {
-            ml[i] = m.DZ[i];
-            ml[m.nz + i] = m.DZW[i];
-            ml[2 * m.nz + i] = m.GCZ[i];
-            ml[3 * m.nz + i] = m.cellh[i];
-        }
+        const T* src[kFluxMeshArrays] = {m.DZ, m.DZW, m.GCZ, m.cellh, m.IDZ, m.IHDN, m.IHUP, m.IDZM, m.IVOL, m.IHC};
+        for (int i = threadIdx.x; i < m.nz; i += NT)
+#pragma unroll
+            for (int k = 0; k < kFluxMeshArrays; k++) ml[k * m.nz + i] = src[k][i];
         m.DZ = ml;
         m.DZW = ml + m.nz;
         m.GCZ = ml + 2 * m.nz;
         m.cellh = ml + 3 * m.nz;
+        m.IDZ = ml + 4 * m.nz;
+        m.IHDN = ml + 5 * m.nz;
+        m.IHUP = ml + 6 * m.nz;
+        m.IDZM = ml + 7 * m.nz;
+        m.IVOL = ml + 8 * m.nz;
+        m.IHC = ml + 9 * m.nz;
     }
     const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tile = lb % ntiles, chunk = lb / ntiles;
@@ -377,7 +383,7 @@ __global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>()), (flux_min_waves<
         T k[3] = {0, 0, 0};
         if (SCHEME == TC_SCHEME_AB3) {
             if (iz != 0 && iz != m.nz - 1) {
-                const T inv_hz = T(2) / (m.DZ[iz - 1] + m.DZ[iz]);
+                const T inv_hz = m.IHC[iz];  // 2 / (DZ[iz-1] + DZ[iz])
                 conv_ab3(m, acc, iz, inv_hz, k);
                 if (VISC == 1) {
                     T g[3];
@@ -392,7 +398,7 @@ __global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>()), (flux_min_waves<
                         const T a0 = acc(c, 0, 0, 0);
                         k[c] += ihx2 * (acc(c, 1, 0, 0) + acc(c, -1, 0, 0) - T(2) * a0);
                         k[c] += ihy2 * (acc(c, 0, 1, 0) + acc(c, 0, -1, 0) - T(2) * a0);
-                        k[c] += m.visc * ((acc(c, 0, 0, 1) - a0) / m.DZ[iz] - (a0 - acc(c, 0, 0, -1)) / m.DZ[iz - 1]) * inv_hz;
+                        k[c] += m.visc * ((acc(c, 0, 0, 1) - a0) * m.IDZ[iz] - (a0 - acc(c, 0, 0, -1)) * m.IDZM[iz]) * inv_hz;
                     }
                 }
                 if (sgs) sgs_divergence(m, tq, iz, k);
@@ -799,7 +805,7 @@ hipError_t launch_flux(hipStream_t s, const Mesh<T>& m, const FluxArgs<T>& a, in
 #define L(SCH, VIS, TRB)                                                                                       \
     {                                                                                                          \
         constexpr int ty = flux_ty<T, SCH, VIS>();                                                             \
-        const size_t lds = 4 * (size_t)m.nz * sizeof(T);                                                     \
+        const size_t lds = kFluxMeshArrays * (size_t)m.nz * sizeof(T);                                       \
         const FluxGrid fg = flux_grid(m.nx, m.ny, m.nz, ty, flux_slots<T, SCH, VIS, TRB>(lds), SCH == TC_SCHEME_VV ? 5 : 3);               \
         if (nblocks) *nblocks = fg.nblocks;                                                                    \
         hipLaunchKernelGGL((k_flux<T, SCH, VIS, TRB>), dim3(fg.nblocks), dim3(flux_threads<T, SCH, VIS>()), lds, s, m, a, fg.ntx, \

@@ -29,8 +29,8 @@ __device__ __forceinline__ void conv_hw(const Mesh<T>& m, const A& acc, int iz, 
     const int NZ = m.nz;
     const bool bot = iz == 0, top = iz == NZ - 1;
     const T dz = m.DZ[iz], dzm = bot ? T(0) : m.DZ[zcl(iz - 1, NZ)];
-    const T inv_hz = T(1) / dz;
-    const T inv_hz_alt = T(2) / (dz + dzm);
+    const T inv_hz = m.IDZ[iz];        // 1 / DZ[iz]
+    const T inv_hz_alt = m.IHDN[iz];   // 2 / (DZ[iz] + DZ[iz-1]), DZ[-1] = 0
     const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
     const T up = UU(0, 0, 1), um = UU(0, 0, -1), vp = VV(0, 0, 1), vm = VV(0, 0, -1);
     const T wp = WW(0, 0, 1), wm = WW(0, 0, -1), wxp = WW(-1, 0, 1), wyp = WW(0, -1, 1);
@@ -200,7 +200,7 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
             f2 += C27 * bbl * ql;
         }
     }
-    const T minus_inv_vol = T(-1) / (T(32) * (dz - T(1.0 / 9.0) * dzw) * hx * hy);
+    const T minus_inv_vol = m.IVOL[iz];  // -1 / (32 (dz - dzw/9) hx hy)
     f[0] = f0 * minus_inv_vol;
     f[1] = f1 * minus_inv_vol;
     f[2] = f2 * (-T(1. / 32.) * m.inv_hx * m.inv_hy * m.GCZ[iz]);
@@ -213,9 +213,8 @@ template <typename T, class A>
 __device__ __forceinline__ void visc_cross(const Mesh<T>& m, const A& acc, int iz, T f[3], int only = -1) {
     const int NZ = m.nz;
     const bool bot = iz == 0, top = iz == NZ - 1;
-    const T dz = m.DZ[iz], dzm = m.DZ[zcl(iz - 1, NZ)], dzp = m.DZ[zcl(iz + 1, NZ)];
-    const T inv_hz = T(1) / dz;
-    const T inv_hz_alt = T(2) / (dz + (bot ? T(0) : dzm));
+    const T inv_hz = m.IDZ[iz];        // 1 / DZ[iz]
+    const T inv_hz_alt = m.IHDN[iz];   // 2 / (DZ[iz] + DZ[iz-1]), DZ[-1] = 0
     const T ihx2 = m.inv_hx * m.inv_hx, ihy2 = m.inv_hy * m.inv_hy;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -231,7 +230,7 @@ __device__ __forceinline__ void visc_cross(const Mesh<T>& m, const A& acc, int i
     const T up = UU(0, 0, 1), vp = VV(0, 0, 1), wp = WW(0, 0, 1);
     const T um = UU(0, 0, -1), vm = VV(0, 0, -1), wm = WW(0, 0, -1);
     {  // upper neighbour; past the top wall the no-slip ghost (-u) enters as 0 - u
-        const T inv_hz_up = T(2) / (dz + (top ? T(0) : dzp));
+        const T inv_hz_up = m.IHUP[iz];  // 2 / (DZ[iz] + DZ[iz+1]), DZ[nz] = 0
         f[0] += ((top ? T(0) : up) - u0) * inv_hz_up * inv_hz;
         f[1] += ((top ? T(0) : vp) - v0) * inv_hz_up * inv_hz;
         f[2] += ((top ? T(0) : wp) - w0) * inv_hz * inv_hz_alt;
@@ -239,7 +238,7 @@ __device__ __forceinline__ void visc_cross(const Mesh<T>& m, const A& acc, int i
     {  // lower neighbour; at the bottom wall there is no w-equation
         f[0] += ((bot ? T(0) : um) - u0) * inv_hz_alt * inv_hz;
         f[1] += ((bot ? T(0) : vm) - v0) * inv_hz_alt * inv_hz;
-        const T f2b = f[2] + (wm - w0) * (T(1) / dzm) * inv_hz_alt;
+        const T f2b = f[2] + (wm - w0) * m.IDZM[iz] * inv_hz_alt;  // 1 / DZ[iz-1]
         f[2] = bot ? f[2] : f2b;
     }
 #pragma unroll
@@ -256,10 +255,10 @@ __device__ __forceinline__ void visc_naive(const Mesh<T>& m, const A& acc, const
     const T visc = m.visc;
     auto mu = [&](int dx, int dy, int dz) -> T { return TURB ? visc + nu(dx, dy, dz) : visc; };
     const bool bot = iz == 0, top = iz == NZ - 1;
-    const T dz = m.DZ[iz], dzm = m.DZ[zcl(iz - 1, NZ)], dzp = m.DZ[zcl(iz + 1, NZ)];
-    const T inv_hz = T(1) / dz;
-    const T inv_hz_up = T(2) / (dz + (top ? T(0) : dzp));
-    const T inv_hz_down = T(2) / (dz + (bot ? T(0) : dzm));
+    const T dz = m.DZ[iz];
+    const T inv_hz = m.IDZ[iz];         // 1 / DZ[iz]
+    const T inv_hz_up = m.IHUP[iz];     // 2 / (DZ[iz] + DZ[iz+1]), DZ[nz] = 0
+    const T inv_hz_down = m.IHDN[iz];   // 2 / (DZ[iz] + DZ[iz-1]), DZ[-1] = 0
     const T ihx = m.inv_hx, ihy = m.inv_hy;
     const T q = T(0.25);
     const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);

@@ -90,6 +90,10 @@ struct FastDiv {
     }
 };
 
+// Device mesh arrays per engine: ZZ, DZ, DZW, GC1, GC2, GCZ, cellh and the six
+// per-plane quotients of Mesh.
+constexpr int kMeshArrays = 13;
+
 constexpr int kMaxStages = 12;
 // Mixed-radix in-place decimation-in-frequency plan. Stage st has radix
 // radix[st] and works on sub-transforms of length L = n / (product of the
@@ -110,6 +114,14 @@ struct Mesh {
     T hx, hy, inv_hx, inv_hy, visc, sour, inv_len;
     T zlo, zhi;  // wall coordinates
     const T *ZZ, *DZ, *DZW, *GC1, *GC2, *GCZ, *cellh;
+    // Per-plane quotients the stencils use, formed on the host in T (the same
+    // correctly rounded values the device division gives):
+    // IDZ = 1/DZ[iz]; IHDN = 2/(DZ[iz] + DZ[iz-1]) (DZ[-1] = 0);
+    // IHUP = 2/(DZ[iz] + DZ[iz+1]) (DZ[nz] = 0); IDZM = 1/DZ[max(iz-1, 0)];
+    // IVOL = -1/(32 (DZ[iz] - DZW[iz]/9) hx hy) (fourth-order convection);
+    // IHC = 2/(DZ[iz-1] + DZ[iz]) at the interior collocated nodes.
+    // A zero denominator gives 0 (those entries are never used).
+    const T *IDZ, *IHDN, *IHUP, *IDZM, *IVOL, *IHC;
     // 1: kernels that read neighbouring rows and planes renumber their blocks
     // so that each XCD works on a contiguous range of planes (set when four
     // planes of a real3 field fit one XCD's 4 MiB L2, so the neighbours hit
