@@ -124,7 +124,7 @@ struct tc_engine {
     void* sqj[kMaxImexStages] = {};    // qj per stage (StabType 1)
     void* snu1 = nullptr;              // nu1tilde (methods of type CK)
     void* swork = nullptr;             // stage velocity u, then buf_vector (real3)
-    void *sbs = nullptr, *spj = nullptr, *spjhat = nullptr;
+    void *sbs = nullptr, *spjhat = nullptr;
     // captured steps, replayed when tc_step is called again with the same
     // (current slot, order, tau): one graph launch instead of ~10 kernel launches
     bool use_graphs = false;
@@ -283,8 +283,6 @@ int setup_device(tc_engine* e) {
         e->swork = p;
         if (int r = dalloc(e, &p, n3)) return r;
         e->sbs = p;
-        if (int r = dalloc(e, &p, n3)) return r;
-        e->spj = p;
         if (int r = dalloc(e, &p, n3)) return r;
         e->spjhat = p;
     }
@@ -547,11 +545,13 @@ int normalize(tc_engine* e, T* p) {
 
 // One step of the collocated SRK solver: data::Step (collocated_SRK/func_global.h:317-408)
 // with ImplicitTerm = 0 and ImplicitStage = copy (:310-316), so Ku is zero and the
-// stage velocity is buf_vector itself. Launches per stage: stage velocity (fused
-// with the previous stage's Kuhat -= G pj), explicit term, pressure right-hand
-// side, buf_vector, the five kernels of the pressure solve, and the pressure
-// update; the last stage's pressure and dpdt land in the caller's slots, and
-// the final velocity update applies the last Kuhat -= G pj on the fly.
+// stage velocity is buf_vector itself. Launches per stage: the stage velocity
+// (which also finishes the previous stage: Kuhat -= G p_j, q_j, from p_j =
+// mujtilde + buf_scalar formed at the nodes it reads), the explicit term, the
+// pressure right-hand side (which applies the previous stages' mujtilde update
+// of :391 as it reads them), buf_vector, and the five kernels of the pressure
+// solve. The final update finishes the last stage the same way and writes p and
+// dp/dt into the caller's slots. Terms with a zero Butcher coefficient are not read.
 template <typename T>
 int srk_step(tc_engine* e, double tau, int vs, int ps, int qs) {
     const ImexTable& tb = e->imex;
@@ -569,6 +569,8 @@ int srk_step(tc_engine* e, double tau, int vs, int ps, int qs) {
     for (int i = 0; i < S; i++) K[i] = (T*)e->sK[i];
     T* work = (T*)e->swork;
     T* nu1 = (T*)e->snu1;
+    T* bs = (T*)e->sbs;
+    T* pjhat = (T*)e->spjhat;
     hipEvent_t ev;
     auto flux = [&](const T* u, T* out) -> int {
         FluxArgs<T> a;
@@ -588,6 +590,27 @@ int srk_step(tc_engine* e, double tau, int vs, int ps, int qs) {
         e->tend(TC_K_PROJECT, ev);
         return TC_OK;
     };
+    // sum_{j<n} c_row[j] Kuhat[j] with the zero coefficients left out
+    auto combo = [&](SrkVec<T>& a, const double* row, int n) {
+        a.nk = 0;
+        for (int j = 0; j < n; j++)
+            if (row[j] != 0.0) {
+                a.K[a.nk] = K[j];
+                a.c[a.nk++] = (T)(tau * row[j]);
+            }
+    };
+    // finish stage i: Kuhat[i] -= G p_i with p_i = mujtilde[i] + buf_scalar; q_i or dp/dt
+    auto finish = [&](SrkVec<T>& a, int i, T* pj_out, T* qj_out) {
+        a.KC = K[i];
+        a.S = (const T*)e->smujt[i];
+        a.S2 = bs;
+        a.pj_out = pj_out;
+        if (stab) {
+            a.qj_out = qj_out;
+            a.pjhat = pjhat;
+            a.inv_tau_stage = (T)inv_tau_stage;
+        }
+    };
     if (j0 == 1) {  // Kuhat[0] = F(u_n) - G p_n; CK: nu1tilde = Kuhat[0] + alpha u_n
         if (int r = flux(vel, K[0])) return r;
         SrkVec<T> a;
@@ -603,19 +626,14 @@ int srk_step(tc_engine* e, double tau, int vs, int ps, int qs) {
         if (int r = vec(a)) return r;
     }
     for (int i = j0; i < S; i++) {
-        {  // u = u_n + sum_j tau E_ij Kuhat[j]; the previous stage's Kuhat -= G pj on the way
+        {  // u = u_n + sum_j tau E_ij Kuhat[j], finishing stage i - 1 on the way
             SrkVec<T> a;
             a.out = work;
             a.X = vel;
             const bool fuse = i > j0;
-            a.nk = fuse ? i - 1 : i;
-            for (int j = 0; j < a.nk; j++) {
-                a.K[j] = K[j];
-                a.c[j] = (T)(tau * tb.E[i * S + j]);
-            }
+            combo(a, &tb.E[i * S], fuse ? i - 1 : i);
             if (fuse) {
-                a.KC = K[i - 1];
-                a.S = (T*)e->spj;
+                finish(a, i - 1, nullptr, (T*)e->sqj[i - 1]);
                 a.KCout = K[i - 1];
                 a.ckc = (T)(tau * tb.E[i * S + i - 1]);
             }
@@ -628,12 +646,13 @@ int srk_step(tc_engine* e, double tau, int vs, int ps, int qs) {
             a.ck = ck;
             a.p = p;
             a.q = q;
-            a.bs = (T*)e->sbs;
-            a.pjhat = (T*)e->spjhat;
+            a.bs = bs;
+            a.pjhat = pjhat;
             a.m0 = (T)(1. - alpha_tau * tb.I[i * S]);
             a.ph0 = (T)(tau * tb.I[i * S]);
             a.tau_stage = (T)tau_stage;
             a.clast = stab ? (T)(alpha * tau_stage * tau_stage) : (T)(alpha * tau_stage);
+            a.cfix = (T)(alpha * tau_stage);
             a.nj = i - j0;
             for (int j = j0; j < i; j++) {
                 a.mujt[j - j0] = (const T*)e->smujt[j];
@@ -649,7 +668,7 @@ int srk_step(tc_engine* e, double tau, int vs, int ps, int qs) {
             SrkVec<T> a;
             a.out = work;
             a.KC = K[i];
-            a.S = (T*)e->sbs;
+            a.S = bs;
             a.ckc = T(1);
             a.A = vel;
             a.sa = (T)alpha;
@@ -660,37 +679,13 @@ int srk_step(tc_engine* e, double tau, int vs, int ps, int qs) {
             if (int r = vec(a)) return r;
         }
         if (int r = unapply<T>(e, work, (T*)e->smujt[i])) return r;
-        {
-            const bool last = i == S - 1;
-            SrkPost<T> a;
-            a.stab = stab;
-            a.last = last;
-            a.mujt = (T*)e->smujt[i];
-            a.bs = (T*)e->sbs;
-            a.p = p;
-            a.q = q;
-            a.pjhat = (T*)e->spjhat;
-            a.pj_out = last ? p : (T*)e->spj;
-            a.qj_out = last ? q : (T*)e->sqj[i];
-            a.c = (T)(alpha * tau_stage);
-            a.tau_stage = (T)tau_stage;
-            a.inv_tau_stage = (T)inv_tau_stage;
-            e->tbeg(TC_K_OTHER, ev);
-            HIPCHK(launch_srk_post<T>(e->s, m, a));
-            e->tend(TC_K_OTHER, ev);
-        }
     }
-    // u_{n+1} = u_n + sum_j tau b_j Kuhat[j], the last Kuhat with its - G pj
+    // u_{n+1} = u_n + sum_j tau b_j Kuhat[j], finishing the last stage: p = p_s, dp/dt = q_s
     SrkVec<T> a;
     a.out = vel;
     a.X = vel;
-    a.nk = S - 1;
-    for (int j = 0; j < S - 1; j++) {
-        a.K[j] = K[j];
-        a.c[j] = (T)(tau * tb.E[S * S + j]);
-    }
-    a.KC = K[S - 1];
-    a.S = p;
+    combo(a, &tb.E[S * S], S - 1);
+    finish(a, S - 1, p, q);
     a.ckc = (T)(tau * tb.E[S * S + S - 1]);
     return vec(a);
 }

@@ -64,13 +64,16 @@ hipError_t launch_add_scaled_plane(hipStream_t s, const Mesh<T>& m, T* f, const 
 // ---------------------------------------------------------------------------
 // Stage updates of the collocated IMEX Runge-Kutta (SRK) step, one thread per node
 // ---------------------------------------------------------------------------
-// kc = KC - G S (S == nullptr: no gradient), stored to KCout when that is set;
-// out = X + sum_{j<nk} c[j] K[j] + ckc kc + sa A + sb B, absent terms skipped.
-// out may alias X and KCout may alias KC.
+// kc = KC - G s with s = S (+ S2 when set; S == nullptr: no gradient), stored to
+// KCout when that is set; out = X + sum_{j<nk} c[j] K[j] + ckc kc + sa A + sb B,
+// absent terms skipped. With pj_out: pj_out = S + S2 at the node, and with
+// qj_out: qj_out = inv_tau_stage (S + S2 - pjhat) (the stage pressure and its
+// rate, collocated_SRK/func_global.h:389-393). out may alias X, KCout KC.
 template <typename T>
 struct SrkVec {
     const T* KC = nullptr;
     const T* S = nullptr;
+    const T* S2 = nullptr;
     T* KCout = nullptr;
     T* out = nullptr;
     const T* X = nullptr;
@@ -82,14 +85,20 @@ struct SrkVec {
     T sa = 0;
     const T* B = nullptr;
     T sb = 0;
+    T* pj_out = nullptr;
+    T* qj_out = nullptr;
+    const T* pjhat = nullptr;
+    T inv_tau_stage = 0;
 };
 template <typename T>
 hipError_t launch_srk_vec(hipStream_t s, const Mesh<T>& m, const SrkVec<T>& a);
 
-// Right-hand side scalar of the stage pressure system (collocated_SRK/func_global.h:357-384).
-// stab 0: bs = (m0 p + sum_j cm[j] mujt[j]) - clast p          (m0 only for ck)
-// stab 1: muj = m0 (tau_stage q) + sum_j cm[j] mujt[j]; pjhat = p + ph0 q + sum_j cq[j] qj[j];
-//         bs = muj + pjhat - clast q                            (m0, ph0 only for ck)
+// Right-hand side scalar of the stage pressure system (collocated_SRK/func_global.h:357-384),
+// with the stored mujtilde[j] taken as mujt[j] - cfix X, X = (stab ? tau_stage q : p)
+// (the update of :391, applied here instead of in place):
+// stab 0: bs = (m0 p + sum_j cm[j] mujt~[j]) - clast p          (m0 only for ck)
+// stab 1: muj = m0 (tau_stage q) + sum_j cm[j] mujt~[j]; pjhat = p + ph0 q + sum_j cq[j] qj[j];
+//         bs = muj + pjhat - clast q                             (m0, ph0 only for ck)
 template <typename T>
 struct SrkPre {
     int stab = 0, ck = 0;
@@ -97,7 +106,7 @@ struct SrkPre {
     const T* q = nullptr;
     T* bs = nullptr;
     T* pjhat = nullptr;
-    T m0 = 1, ph0 = 0, tau_stage = 0, clast = 0;
+    T m0 = 1, ph0 = 0, tau_stage = 0, clast = 0, cfix = 0;
     int nj = 0;
     const T* mujt[kMaxImexStages] = {};
     T cm[kMaxImexStages] = {};
@@ -106,24 +115,6 @@ struct SrkPre {
 };
 template <typename T>
 hipError_t launch_srk_pre(hipStream_t s, const Mesh<T>& m, const SrkPre<T>& a);
-
-// After the stage pressure solve (collocated_SRK/func_global.h:389-393):
-// pj = mujt + bs; mujt -= c (stab ? tau_stage q : p) (skipped when last);
-// stab: qj = inv_tau_stage (pj - pjhat). pj_out may alias p and qj_out q.
-template <typename T>
-struct SrkPost {
-    int stab = 0, last = 0;
-    T* mujt = nullptr;
-    const T* bs = nullptr;
-    const T* p = nullptr;
-    const T* q = nullptr;
-    const T* pjhat = nullptr;
-    T* pj_out = nullptr;
-    T* qj_out = nullptr;
-    T c = 0, tau_stage = 0, inv_tau_stage = 0;
-};
-template <typename T>
-hipError_t launch_srk_post(hipStream_t s, const Mesh<T>& m, const SrkPost<T>& a);
 
 // ---------------------------------------------------------------------------
 // Spectral pressure solve

@@ -671,6 +671,23 @@ dim3 cell_grid(const Mesh<T>& m) {
 // nodes except for the collocated gradient of one scalar; every read of a
 // node comes before its writes, since outputs alias inputs at the same node.
 // ---------------------------------------------------------------------------
+// Collocated gradient (the AB3 branch of grad_at) of s = S + S2 (S2 may be nullptr).
+template <typename T>
+__device__ __forceinline__ void grad_sum_ab3(const Mesh<T>& m, const T* __restrict__ S, const T* __restrict__ S2,
+                                             const Nb& nb, int iz, T g[3]) {
+    if (iz == 0 || iz == m.nz - 1) {
+        g[0] = g[1] = g[2] = T(0);
+        return;
+    }
+    auto at = [&](int dx, int dy, int dz) {
+        const int i = nb.at(dx, dy, dz);
+        return S2 ? S[i] + S2[i] : S[i];
+    };
+    g[0] = T(0.5) * m.inv_hx * (at(1, 0, 0) - at(-1, 0, 0));
+    g[1] = T(0.5) * m.inv_hy * (at(0, 1, 0) - at(0, -1, 0));
+    g[2] = T(0.5) * (at(0, 0, 1) - at(0, 0, -1)) / m.cellh[iz];
+}
+
 template <typename T, bool GRAD>
 __global__ __launch_bounds__(kBlock) void k_srk_vec(Mesh<T> m, SrkVec<T> a) {
     const PlaneBlock pb = plane_block(m.xcd_swz);
@@ -689,10 +706,15 @@ __global__ __launch_bounds__(kBlock) void k_srk_vec(Mesh<T> m, SrkVec<T> a) {
     }
     if (GRAD) {
         T g[3];
-        grad_at<T, TC_SCHEME_AB3>(m, a.S, nb, iz, g);
+        grad_sum_ab3<T>(m, a.S, a.S2, nb, iz, g);
         kc[0] -= g[0];
         kc[1] -= g[1];
         kc[2] -= g[2];
+    }
+    T pj = T(0), qn = T(0);
+    if (a.pj_out || a.qj_out) {
+        pj = a.S2 ? a.S[in] + a.S2[in] : a.S[in];
+        if (a.qj_out) qn = a.inv_tau_stage * (pj - a.pjhat[in]);
     }
     T o[3] = {T(0), T(0), T(0)};
     if (a.out) {
@@ -741,6 +763,8 @@ __global__ __launch_bounds__(kBlock) void k_srk_vec(Mesh<T> m, SrkVec<T> a) {
         x[1] = o[1];
         x[2] = o[2];
     }
+    if (a.pj_out) a.pj_out[in] = pj;
+    if (a.qj_out) a.qj_out[in] = qn;
 }
 
 template <typename T, bool STAB, bool CK>
@@ -752,14 +776,16 @@ __global__ __launch_bounds__(kBlock) void k_srk_pre(Mesh<T> m, SrkPre<T> a) {
     const int in = iz * m.nxy + iy * m.nx + ix;
     const T p = a.p[in];
     if (!STAB) {
+        const T fix = a.cfix * p;
         T muj = p;
         if (CK) muj *= a.m0;
 #pragma unroll
         for (int j = 0; j < kMaxImexStages; j++)
-            if (j < a.nj) muj += a.cm[j] * a.mujt[j][in];
+            if (j < a.nj) muj += a.cm[j] * (a.mujt[j][in] - fix);
         a.bs[in] = muj - a.clast * p;
     } else {
         const T q = a.q[in];
+        const T fix = a.cfix * (a.tau_stage * q);
         T muj = a.tau_stage * q;
         if (CK) muj *= a.m0;
         T ph = p;
@@ -767,32 +793,12 @@ __global__ __launch_bounds__(kBlock) void k_srk_pre(Mesh<T> m, SrkPre<T> a) {
 #pragma unroll
         for (int j = 0; j < kMaxImexStages; j++)
             if (j < a.nj) {
-                muj += a.cm[j] * a.mujt[j][in];
+                muj += a.cm[j] * (a.mujt[j][in] - fix);
                 ph += a.cq[j] * a.qj[j][in];
             }
         a.pjhat[in] = ph;
         a.bs[in] = muj + ph - a.clast * q;
     }
-}
-
-template <typename T, bool STAB, bool LAST>
-__global__ __launch_bounds__(kBlock) void k_srk_post(Mesh<T> m, SrkPost<T> a) {
-    const PlaneBlock pb = plane_block(m.xcd_swz);
-    const int iz = pb.iz;
-    int ix, iy;
-    if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx)) return;
-    const int in = iz * m.nxy + iy * m.nx + ix;
-    const T mu = a.mujt[in];
-    const T pj = mu + a.bs[in];
-    if (STAB) {
-        const T q = a.q[in];
-        const T qn = a.inv_tau_stage * (pj - a.pjhat[in]);
-        if (!LAST) a.mujt[in] = mu - a.c * (a.tau_stage * q);
-        a.qj_out[in] = qn;
-    } else {
-        if (!LAST) a.mujt[in] = mu - a.c * a.p[in];
-    }
-    a.pj_out[in] = pj;
 }
 
 }  // namespace
@@ -896,23 +902,9 @@ hipError_t launch_srk_pre(hipStream_t s, const Mesh<T>& m, const SrkPre<T>& a) {
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_srk_post(hipStream_t s, const Mesh<T>& m, const SrkPost<T>& a) {
-    const dim3 g = cell_grid(m);
-    if (a.stab) {
-        if (a.last) hipLaunchKernelGGL((k_srk_post<T, true, true>), g, dim3(kBlock), 0, s, m, a);
-        else hipLaunchKernelGGL((k_srk_post<T, true, false>), g, dim3(kBlock), 0, s, m, a);
-    } else {
-        if (a.last) hipLaunchKernelGGL((k_srk_post<T, false, true>), g, dim3(kBlock), 0, s, m, a);
-        else hipLaunchKernelGGL((k_srk_post<T, false, false>), g, dim3(kBlock), 0, s, m, a);
-    }
-    return hipGetLastError();
-}
-
 #define INST(T)                                                                                             \
     template hipError_t launch_srk_vec<T>(hipStream_t, const Mesh<T>&, const SrkVec<T>&);                  \
     template hipError_t launch_srk_pre<T>(hipStream_t, const Mesh<T>&, const SrkPre<T>&);                  \
-    template hipError_t launch_srk_post<T>(hipStream_t, const Mesh<T>&, const SrkPost<T>&);                \
     template hipError_t launch_flux<T>(hipStream_t, const Mesh<T>&, const FluxArgs<T>&, int*);              \
     template hipError_t launch_turb_visc<T>(hipStream_t, const Mesh<T>&, int, const T*, T*);                \
     template hipError_t launch_gradient<T>(hipStream_t, const Mesh<T>&, int, const T*, T*);                 \
