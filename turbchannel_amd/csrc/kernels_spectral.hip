@@ -630,7 +630,8 @@ __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFac
     if (mode >= nm) return;
     if (lowidx && lowidx[mode] >= 0) return;  // solved in FP64 by the low-mode blocks
     T* x = reinterpret_cast<T*>(spec) + 2 * mode + part;
-    constexpr int NB = sizeof(T) == 8 ? 2 : 3;
+    // batches of 8 rows in flight: NB - 1 (FP64 lanes hold 5 reals per row of W = 3)
+    constexpr int NB = 3;
     lane_solve<T, T, W, 8, NB>(x, xs, f.F + (size_t)mode * FW, (size_t)nm * FW, f.U + (size_t)mode * UW,
                                (size_t)nm * UW, x, xs, nz, mode == 0 && part == 0 && norm.s00 != nullptr, norm);
 }
