@@ -101,7 +101,7 @@ struct tc_engine {
     double* d_shift = nullptr;  // flow-rate shift
     double* d_prof = nullptr;   // profiles result (nz * 12)
     double* d_diag = nullptr;   // fused diagnostics, nz records
-    std::vector<double> h_diag;
+    double* h_diag = nullptr;   // pinned host copy of d_diag (hipHostMalloc: the per-step copy skips staging)
     // learned closure (collocated, sgs_model == 1)
     void* tau = nullptr;                // [node][6] closure stress
     unsigned short* sgs_w = nullptr;    // bfloat16 weight block
@@ -387,6 +387,8 @@ int setup_device(tc_engine* e) {
     if (int r = dalloc(e, &e->d_shift, 4)) return r;
     if (int r = dalloc(e, &e->d_prof, (size_t)e->nz * 12)) return r;
     if (int r = dalloc(e, &e->d_diag, (size_t)e->nz * diag_record(e->P.scheme))) return r;
+    HIPCHK(hipHostMalloc((void**)&e->h_diag, (size_t)e->nz * diag_record(e->P.scheme) * sizeof(double),
+                         hipHostMallocDefault));
     return TC_OK;
 }
 
@@ -872,6 +874,7 @@ int tc_destroy(tc_engine* e) {
     for (auto ev : e->tim.pool) hipEventDestroy(ev);
     for (auto& g : e->graphs) hipGraphExecDestroy(g.second);
     for (void* p : e->allocs) hipFree(p);
+    if (e->h_diag) hipHostFree(e->h_diag);
     if (e->s) hipStreamDestroy(e->s);
     delete e;
     return TC_OK;
@@ -1145,15 +1148,14 @@ int tc_diagnostics(tc_engine* e, int vel_slot, int p_slot, double tau, int cfl_m
     else
         HIPCHK(launch_diag<double>(e->s, e->mesh<double>(), e->P.scheme, (const double*)v, (const double*)p,
                                    (const double*)e->nut, visc_fac, e->d_partial, e->d_diag));
-    e->h_diag.resize((size_t)e->nz * nrec);
-    HIPCHK(hipMemcpyAsync(e->h_diag.data(), e->d_diag, e->h_diag.size() * sizeof(double), hipMemcpyDeviceToHost, e->s));
+    HIPCHK(hipMemcpyAsync(e->h_diag, e->d_diag, (size_t)e->nz * nrec * sizeof(double), hipMemcpyDeviceToHost, e->s));
     HIPCHK(hipStreamSynchronize(e->s));
     // volumes as the device kernels see them: the mesh in the engine's precision
     auto r = [&](double x) { return e->fp32 ? (double)(float)x : x; };
     const double hxy = r(e->P.hx) * r(e->P.hy), inv_nxy = 1.0 / (double)e->nxy;
     double q[3] = {0, 0, 0}, ke = 0, cmax = 0;
     for (int iz = 0; iz < e->nz; iz++) {
-        const double* d = e->h_diag.data() + (size_t)iz * nrec;
+        const double* d = e->h_diag + (size_t)iz * nrec;
         const double sw = d[nv], sww = d[nv + 1];
         if (ab3) {
             const double vol = r(e->hm.cellh[iz]) * hxy;

@@ -188,47 +188,71 @@ __global__ __launch_bounds__(kThreads) void k_diag(Mesh<T> m, const T* __restric
     const double kv = visc_fac * (ihx * ihx + ihy * ihy + inv_hz * inv_hz);
     const bool top = iz == m.nz - 1;
     const bool nut_node = SCHEME == TC_SCHEME_AB3 && nut && iz != 0 && !top;
-    for (int j = blockIdx.x * kThreads + threadIdx.x; j < m.nxy; j += nch * kThreads) {
-        const int in = base + j;
-        const double u = vel[3 * in], v = vel[3 * in + 1], wr = vel[3 * in + 2];
-        double w = wr;
-        if (SCHEME != TC_SCHEME_AB3) w = 0.5 * (wr + (top ? 0.0 : (double)vel[3 * (in + m.nxy) + 2]));
-        s[0] += u;
-        s[1] += v;
-        s[2] += w;
-        s[3] += u * u;
-        s[4] += v * v;
-        s[5] += w * w;
-        s[6] += u * v;
-        s[7] += u * w;
-        s[8] += v * w;
-        if (nut) s[9] += nut[in];
-        if (SCHEME != TC_SCHEME_AB3) {
-            const double pp = p[in];
-            s[10] += pp;
-            s[11] += pp * pp;
-        }
-        s[NV] += wr;
-        s[NV + 1] += wr * wr;
-        double vc = m.visc;
-        if (SCHEME == TC_SCHEME_AB3) {
-            if (nut_node) {  // GetNuTAtNode: mean of the 8 elements around the node
-                const int iy = j / m.nx, ix = j - iy * m.nx;
-                const int xl = wrapi<T>(ix - 1, m.nx), yl = wrapi<T>(iy - 1, m.ny);
-                const int p0 = iz * m.nxy, p1 = (iz - 1) * m.nxy;
-                const T* n = nut;
-                vc += 0.125 * ((double)n[p0 + iy * m.nx + ix] + n[p0 + iy * m.nx + xl] + n[p0 + yl * m.nx + ix] +
-                               n[p0 + yl * m.nx + xl] + n[p1 + iy * m.nx + ix] + n[p1 + iy * m.nx + xl] +
-                               n[p1 + yl * m.nx + ix] + n[p1 + yl * m.nx + xl]);
+    // U cells per thread per pass: every load of a pass is issued before the
+    // first sum (the cells of a thread are visited in the same order as a plain
+    // strided loop, so the sums are bitwise those of one; a cell past the plane
+    // reads the pass's first cell and adds nothing)
+    constexpr int U = 4;
+    const int stride = nch * kThreads;
+    for (int j0 = blockIdx.x * kThreads + threadIdx.x; j0 < m.nxy; j0 += U * stride) {
+        double u_[U], v_[U], wr_[U], wn_[U], p_[U], nt_[U], vc_[U];
+        bool ok[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const int j = j0 + k * stride;
+            ok[k] = j < m.nxy;
+            const int jj = ok[k] ? j : j0;
+            const int in = base + jj;
+            u_[k] = vel[3 * in];
+            v_[k] = vel[3 * in + 1];
+            wr_[k] = vel[3 * in + 2];
+            wn_[k] = (SCHEME != TC_SCHEME_AB3 && !top) ? (double)vel[3 * (in + m.nxy) + 2] : 0.0;
+            p_[k] = SCHEME != TC_SCHEME_AB3 ? (double)p[in] : 0.0;
+            nt_[k] = nut ? (double)nut[in] : 0.0;
+            vc_[k] = m.visc;
+            if (SCHEME == TC_SCHEME_AB3) {
+                if (nut_node) {  // GetNuTAtNode: mean of the 8 elements around the node
+                    const int iy = jj / m.nx, ix = jj - iy * m.nx;
+                    const int xl = wrapi<T>(ix - 1, m.nx), yl = wrapi<T>(iy - 1, m.ny);
+                    const int p0 = iz * m.nxy, p1 = (iz - 1) * m.nxy;
+                    const T* n = nut;
+                    vc_[k] += 0.125 * ((double)n[p0 + iy * m.nx + ix] + n[p0 + iy * m.nx + xl] + n[p0 + yl * m.nx + ix] +
+                                       n[p0 + yl * m.nx + xl] + n[p1 + iy * m.nx + ix] + n[p1 + iy * m.nx + xl] +
+                                       n[p1 + yl * m.nx + ix] + n[p1 + yl * m.nx + xl]);
+                }
+            } else if (nut) {
+                vc_[k] = (double)m.visc + nt_[k];
             }
-        } else if (nut) {
-            vc = (double)m.visc + (double)nut[in];
         }
-        const double cc = fabs(u) * ihx + fabs(v) * ihy + fabs(wr) * inv_hz;
-        const double cv = kv * vc;
-        mx[0] = fmax(mx[0], cc);
-        mx[1] = fmax(mx[1], cv);
-        mx[2] = fmax(mx[2], cc + cv);
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            if (!ok[k]) continue;
+            const double u = u_[k], v = v_[k], wr = wr_[k];
+            double w = wr;
+            if (SCHEME != TC_SCHEME_AB3) w = 0.5 * (wr + wn_[k]);
+            s[0] += u;
+            s[1] += v;
+            s[2] += w;
+            s[3] += u * u;
+            s[4] += v * v;
+            s[5] += w * w;
+            s[6] += u * v;
+            s[7] += u * w;
+            s[8] += v * w;
+            if (nut) s[9] += nt_[k];
+            if (SCHEME != TC_SCHEME_AB3) {
+                const double pp = p_[k];
+                s[10] += pp;
+                s[11] += pp * pp;
+            }
+            s[NV] += wr;
+            s[NV + 1] += wr * wr;
+            const double cc = fabs(u) * ihx + fabs(v) * ihy + fabs(wr) * inv_hz;
+            const double cv = kv * vc_[k];
+            mx[0] = fmax(mx[0], cc);
+            mx[1] = fmax(mx[1], cv);
+            mx[2] = fmax(mx[2], cc + cv);
+        }
     }
     block_reduce<NS>(s, sh, false);
     block_reduce<3>(mx, shm, true);
