@@ -264,13 +264,21 @@ __global__ __launch_bounds__(kThreads) void k_diag(Mesh<T> m, const T* __restric
     }
 }
 
+// Chunks combined in order, 8 loads issued before they are added (a chunk
+// past nch reads chunk 0 and is skipped).
 __global__ void k_diag_final(const double* partial, int nch, int nx_rec, int nsum, double* out) {
     const int iz = blockIdx.x, k = threadIdx.x;
     if (k >= nx_rec) return;
+    constexpr int U = 8;
+    const double* src = partial + (size_t)iz * nch * nx_rec + k;
     double r = 0.0;
-    for (int c = 0; c < nch; c++) {
-        const double x = partial[((size_t)iz * nch + c) * nx_rec + k];
-        r = k < nsum ? r + x : fmax(r, x);
+    for (int c0 = 0; c0 < nch; c0 += U) {
+        double x[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) x[u] = src[(size_t)(c0 + u < nch ? c0 + u : 0) * nx_rec];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (c0 + u < nch) r = k < nsum ? r + x[u] : fmax(r, x[u]);
     }
     out[iz * nx_rec + k] = r;
 }
