@@ -5,8 +5,9 @@ FP32-bulk bar: <= 1e-5 relative per step against the FP64 engine."""
 import numpy as np
 import pytest
 
+import oracle
 import turbchannel_amd as tca
-from tests import parity
+from tests import cases, parity
 
 pytestmark = pytest.mark.gpu
 
@@ -271,3 +272,60 @@ def test_bad_slot_and_order_raise():
     short = np.zeros(5)
     rc = tca.lib().tc_set_field(eng._h, tca.PRES, 0, short.ctypes.data, short.nbytes)
     assert rc == 1 and "size mismatch" in tca.lib().tc_last_error().decode()
+
+
+# Planes of 256 x 192 (FP64) and 384 x 256 (FP32) points: too large for an XCD's
+# L2 to hold four of them, so the kernels run in their plain block order, and
+# an odd number of line pairs per transform block (3 at N = 256 in FP64). A few
+# planes keep the oracle fast.
+LARGE = [(tca.SCHEME_HW, 256, 192, 8), (tca.SCHEME_VV, 256, 192, 8), (tca.SCHEME_AB3, 256, 192, 9)]
+
+
+def divergent_field(u, scheme, nxy):
+    """u plus a pattern with a divergence (the seeded field alone is a discrete
+    curl, divergence-free in the HW discretisation); no flux through the bottom
+    wall for the staggered schemes."""
+    n = u.shape[0]
+    v = u + 0.3 * np.stack([np.cos(np.arange(n) * 0.37 + c) for c in range(3)], axis=1)
+    if scheme != tca.SCHEME_AB3:
+        v[:nxy, 2] = 0.0
+    return v
+
+
+def large_problem(scheme, nx, ny, nz):
+    if scheme == tca.SCHEME_AB3:
+        zz = cases.ab3_mesh_tanh(nz, 1.8)
+        kw = cases.bulk_kwargs(nx, ny, 5600, 180)
+        kw.update(stab_type=1, fd_mode=1, visc_type=0)
+    else:
+        zz = cases.tanh_faces(nz, 1.8)
+        kw = cases.bulk_kwargs(nx, ny, 5600, 180)
+    return zz, kw
+
+
+@pytest.mark.parametrize("case", LARGE, ids=lambda c: f"{['hw', 'vv', 'ab3'][c[0]]}-{c[1]}x{c[2]}x{c[3]}")
+def test_pressure_solve_large_planes(case):
+    scheme, nx, ny, nz = case
+    zz, kw = large_problem(scheme, nx, ny, nz)
+    sol = oracle.Solver(scheme, nx, ny, nz, zz, **kw)
+    eng = tca.Engine(scheme, nx, ny, nz, zz, **kw)
+    u = divergent_field(sol.initial_field(seed=5, ubulk=1.0, noise_rms=0.3), scheme, nx * ny)
+    eng.set_field(tca.VEL, 0, u)
+    eng.unapply_gradient((tca.VEL, 0), (tca.PRES, 0))
+    assert parity.rel(eng.get_field(tca.PRES, 0), sol.unapply_gradient(u)) < TOL64
+
+
+@pytest.mark.parametrize("case", LARGE, ids=lambda c: f"{['hw', 'vv', 'ab3'][c[0]]}-{c[1]}x{c[2]}x{c[3]}")
+def test_pressure_solve_large_planes_fp32(case):
+    scheme, _, _, nz = case
+    nx, ny = 384, 256
+    zz, kw = large_problem(scheme, nx, ny, nz)
+    e64 = tca.Engine(scheme, nx, ny, nz, zz, **kw)
+    e32 = tca.Engine(scheme, nx, ny, nz, zz, precision=tca.FP32_BULK, **kw)
+    u = divergent_field(tca.make_initial_field(e64.params, seed=6, ubulk=1.0, noise_rms=0.3), scheme,
+                        nx * ny).astype(np.float32)
+    e64.set_field(tca.VEL, 0, u.astype(np.float64))
+    e32.set_field(tca.VEL, 0, u)
+    for e in (e64, e32):
+        e.unapply_gradient((tca.VEL, 0), (tca.PRES, 0))
+    assert parity.rel(e32.get_field(tca.PRES, 0), e64.get_field(tca.PRES, 0)) < TOL32
