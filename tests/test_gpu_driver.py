@@ -146,8 +146,13 @@ def test_driver_analytic_start(tmp_path, name):
     _check_field(tmp_path / "output.vtk", u, p)
 
 
-@pytest.mark.parametrize("args", [STAG_ARGS + ["--precision", "fp64"], AB3_ARGS + ["--precision", "fp32"]],
-                         ids=["vv_fp64", "ab3_fp32"])
+SRK_ARGS = ["--scheme", "srk", "--Re_b", 5600, "--Re_tau", 180, "--Lx", repr(2 * math.pi), "--Ly", repr(math.pi),
+            "--mesh", "tanh", "--gamma", 1.8, "--nx", 16, "--ny", 12, "--nz", 18, "--imex", "ark3", "--stab", 1]
+
+
+@pytest.mark.parametrize("args", [STAG_ARGS + ["--precision", "fp64"], AB3_ARGS + ["--precision", "fp32"],
+                                  SRK_ARGS + ["--precision", "fp64"]],
+                         ids=["vv_fp64", "ab3_fp32", "srk_fp64"])
 def test_driver_restart_is_exact(tmp_path, args):
     common = [*args, "--seed", 3, "--tau", "2e-3", "-s", 1, "-f", 1000, "-p", 1000]
     full, first = tmp_path / "full", tmp_path / "split"
@@ -158,7 +163,8 @@ def test_driver_restart_is_exact(tmp_path, args):
     assert r.returncode == 0 and (first / "restart.bin").exists(), r.stdout + r.stderr
     r = run_driver([*common, "--TimeStepsMax", 10, "--restart", "restart.bin"], first)
     assert r.returncode == 0 and "Restarted from restart.bin at step 5" in r.stdout, r.stdout + r.stderr
-    for name in ("output.vtk", "output.dat", "res.dat", "log.dat"):
+    names = ("output.vtk", "res.dat", "log.dat") if "srk" in args else ("output.vtk", "output.dat", "res.dat", "log.dat")
+    for name in names:  # the SRK loop writes no output.dat (collocated_SRK/main.cpp:543-551)
         assert (full / name).read_bytes() == (first / name).read_bytes(), name
 
 
