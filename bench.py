@@ -307,8 +307,14 @@ def main():
     kt = eng.timing_read()
 
     # steps plus the reference loop's per-step diagnostics (flow rate, plane
-    # statistics, CFL: staggered/main.cpp:312-338) in one fused pass each
+    # statistics, CFL: staggered/main.cpp:312-338), formed by the step's
+    # projection (the SRK stepper: one pass of their own)
     nd = max(1, args.steps // 4)
+    if not srk:
+        eng.step_diagnostics(True)
+    for it in range(4):  # captures of the steps that form them
+        v = advance(it)
+        eng.diagnostics(v, v, dt, 3)
     eng.sync()
     t1 = time.perf_counter()
     for it in range(nd):

@@ -143,6 +143,16 @@ int tc_profiles(tc_engine* e, int vel_kind, int vel_slot, int p_slot, double* ho
  * as a single call. */
 int tc_diagnostics(tc_engine* e, int vel_slot, int p_slot, double tau, int cfl_mode, double integral[3],
                    double* kinetic_energy, double* cfl, double* host_av, size_t count);
+/* on != 0: tc_step also forms these diagnostics of the fields it writes, in its
+ * projection kernel (staggered/main.cpp:312-338 reads the new velocity three
+ * more times; here the values are summed while they are stored), and a
+ * following tc_diagnostics of the step's new velocity slot (and the same
+ * pressure slot, staggered) copies them instead of reading the fields again.
+ * Any call that may change a field or nu_t (tc_set_field, tc_field_device_ptr,
+ * tc_srk_step, ...) drops the step's record; writes through a device pointer
+ * obtained earlier are not seen. Off by default; refused on SRK engines. No
+ * reference counterpart. */
+int tc_step_diagnostics(tc_engine* e, int on);
 
 /* Constant-flow-rate correction of one velocity field (no reference counterpart;
  * tc_step applies it itself when cfr_ubulk > 0). shift may be NULL. */

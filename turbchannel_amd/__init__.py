@@ -75,6 +75,7 @@ def lib():
     L.tc_field_device_ptr.argtypes = [vp, c_int, c_int, P(vp)]
     L.tc_step.argtypes = [vp, c_double, c_double, c_int, c_int]
     L.tc_set_graphs.argtypes = [vp, c_int]
+    L.tc_step_diagnostics.argtypes = [vp, c_int]
     L.tc_srk_step.argtypes = [vp, c_double, c_double, c_int, c_int, c_int]
     L.tc_imex_table.argtypes = [c_int, P(c_int), P(c_int), dp, dp, dp, dp]
     for n in ("tc_explicit_term", "tc_apply_gradient", "tc_unapply_gradient"):
@@ -246,6 +247,11 @@ class Engine:
     def graphs(self, on=True):
         """Replay repeated steps from captured HIP graphs (tc_set_graphs)."""
         _check(lib().tc_set_graphs(self._h, 1 if on else 0))
+
+    def step_diagnostics(self, on=True):
+        """Form the diagnostics of each step's new fields in its projection
+        (tc_step_diagnostics); diagnostics() of that slot then reads them."""
+        _check(lib().tc_step_diagnostics(self._h, 1 if on else 0))
 
     def explicit_term(self, src, dst):
         _check(lib().tc_explicit_term(self._h, src[0], src[1], dst[0], dst[1]))

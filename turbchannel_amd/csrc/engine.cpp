@@ -14,6 +14,7 @@
 #include <tuple>
 #include <vector>
 
+#include "diag.h"
 #include "kernels.h"
 #include "mlp.h"
 
@@ -102,6 +103,11 @@ struct tc_engine {
     double* d_prof = nullptr;   // profiles result (nz * 12)
     double* d_diag = nullptr;   // fused diagnostics, nz records
     double* h_diag = nullptr;   // pinned host copy of d_diag (hipHostMalloc: the per-step copy skips staging)
+    // tc_step also forms the diagnostics of the new fields in its projection
+    // (tc_step_diagnostics); d_diag then holds those of velocity (and pressure)
+    // slot diag_slot until a call changes a field, or -1
+    bool step_diag = false;
+    int diag_slot = -1;
     // learned closure (collocated, sgs_model == 1)
     void* tau = nullptr;                // [node][6] closure stress
     unsigned short* sgs_w = nullptr;    // bfloat16 weight block
@@ -382,6 +388,7 @@ int setup_device(tc_engine* e) {
     if (int r = upload_twiddles<T>(e, e->ny, &e->twy)) return r;
     e->npartial = std::max<size_t>((size_t)kRedBlocks * 3, (size_t)flux_grid_blocks(e->nx, e->ny, e->nz));
     e->npartial = std::max<size_t>(e->npartial, (size_t)profiles_chunks(e->nxy) * e->nz * diag_record(e->P.scheme));
+    e->npartial = std::max<size_t>(e->npartial, (size_t)project_diag_chunks(e->nxy) * e->nz * diag_record(e->P.scheme));
     if (int r = dalloc(e, &e->d_partial, e->npartial)) return r;
     if (int r = dalloc(e, &e->d_out, 16)) return r;
     if (int r = dalloc(e, &e->d_shift, 4)) return r;
@@ -516,6 +523,12 @@ int step(tc_engine* e, double tau, int cur, int order) {
     if (stab) {
         pa.pcur = (const T*)e->pres[cur];
         pa.pnext = (T*)e->pres[next];
+    }
+    if (e->step_diag) {
+        pa.diag = e->d_partial;
+        pa.diag_out = e->d_diag;
+        pa.visc_fac = diag_visc_fac(e->P.scheme, e->P.visc_type);
+        pa.nut = (const T*)e->nut;
     }
     e->tbeg(TC_K_PROJECT, ev);
     HIPCHK(launch_project<T>(e->s, m, pa));
@@ -890,6 +903,7 @@ int tc_sync(tc_engine* e) {
 
 int tc_set_field(tc_engine* e, int kind, int slot, const void* host, size_t bytes) {
     if (int r = check(e)) return r;
+    e->diag_slot = -1;
     void* d = field_ptr(e, kind, slot);
     if (!d || !host) return fail(TC_ERR_ARG, "bad field");
     const size_t need = ncomp(kind) * e->n3d * e->rsz;
@@ -914,6 +928,7 @@ int tc_field_device_ptr(tc_engine* e, int kind, int slot, void** dev_ptr) {
     if (!e || !dev_ptr) return fail(TC_ERR_ARG, "NULL argument");
     void* d = field_ptr(e, kind, slot);
     if (!d) return fail(TC_ERR_ARG, "bad field");
+    e->diag_slot = -1;  // the caller may write through it
     *dev_ptr = d;
     return TC_OK;
 }
@@ -965,10 +980,26 @@ int tc_step(tc_engine* e, double t, double tau, int cur, int bdf_order) {
     if (bdf_order < 1 || bdf_order > 3) return fail(TC_ERR_ARG, "Wrong bdf_order");
     if (cur < 0 || cur > 3) return fail(TC_ERR_ARG, "bad current_field");
     if (!(tau > 0)) return fail(TC_ERR_ARG, "tau must be positive");
+    e->diag_slot = -1;
     auto run = [&]() { return e->fp32 ? step<float>(e, tau, cur, bdf_order) : step<double>(e, tau, cur, bdf_order); };
-    if (!e->use_graphs || e->tim.on) return run();
-    if (e->P.sgs_model && !e->sgs_ready) return fail(TC_ERR_STATE, "learned closure weights are not set (tc_set_sgs_params)");
-    return run_graph(e, cur, bdf_order, tau, run);
+    int r;
+    if (!e->use_graphs || e->tim.on) {
+        r = run();
+    } else {
+        if (e->P.sgs_model && !e->sgs_ready)
+            return fail(TC_ERR_STATE, "learned closure weights are not set (tc_set_sgs_params)");
+        r = run_graph(e, cur, bdf_order + 8 * e->step_diag, tau, run);
+    }
+    if (r == TC_OK && e->step_diag) e->diag_slot = (cur + 1) % 4;
+    return r;
+}
+
+int tc_step_diagnostics(tc_engine* e, int on) {
+    if (int r = check(e)) return r;
+    if (e->srk) return fail(TC_ERR_STATE, "the SRK solver has no fused step diagnostics");
+    e->step_diag = on != 0;
+    e->diag_slot = -1;
+    return TC_OK;
 }
 
 int tc_srk_step(tc_engine* e, double t, double tau, int vel_slot, int p_slot, int dpdt_slot) {
@@ -978,6 +1009,7 @@ int tc_srk_step(tc_engine* e, double t, double tau, int vel_slot, int p_slot, in
     if (vel_slot < 0 || vel_slot > 3 || p_slot < 0 || p_slot > 3) return fail(TC_ERR_ARG, "bad slot");
     if (e->srk_stab && (dpdt_slot < 0 || dpdt_slot > 3 || dpdt_slot == p_slot)) return fail(TC_ERR_ARG, "bad dpdt slot");
     if (!(tau > 0)) return fail(TC_ERR_ARG, "tau must be positive");
+    e->diag_slot = -1;
     auto run = [&]() {
         return e->fp32 ? srk_step<float>(e, tau, vel_slot, p_slot, dpdt_slot)
                        : srk_step<double>(e, tau, vel_slot, p_slot, dpdt_slot);
@@ -995,6 +1027,7 @@ int tc_set_graphs(tc_engine* e, int on) {
 
 int tc_explicit_term(tc_engine* e, int in_kind, int in_slot, int out_kind, int out_slot) {
     if (int r = check(e)) return r;
+    e->diag_slot = -1;
     void* in = field_ptr(e, in_kind, in_slot);
     void* out = field_ptr(e, out_kind, out_slot);
     if (!in || !out || !is_vec(in_kind) || !is_vec(out_kind) || in == out) return fail(TC_ERR_ARG, "bad field");
@@ -1020,6 +1053,7 @@ int tc_explicit_term(tc_engine* e, int in_kind, int in_slot, int out_kind, int o
 
 int tc_apply_gradient(tc_engine* e, int p_kind, int p_slot, int out_kind, int out_slot) {
     if (int r = check(e)) return r;
+    e->diag_slot = -1;
     void* p = field_ptr(e, p_kind, p_slot);
     void* out = field_ptr(e, out_kind, out_slot);
     if (!p || !out || !is_scal(p_kind) || !is_vec(out_kind)) return fail(TC_ERR_ARG, "bad field");
@@ -1030,6 +1064,7 @@ int tc_apply_gradient(tc_engine* e, int p_kind, int p_slot, int out_kind, int ou
 
 int tc_unapply_gradient(tc_engine* e, int v_kind, int v_slot, int p_kind, int p_slot) {
     if (int r = check(e)) return r;
+    e->diag_slot = -1;
     void* v = field_ptr(e, v_kind, v_slot);
     void* p = field_ptr(e, p_kind, p_slot);
     if (!v || !p || !is_vec(v_kind) || !is_scal(p_kind)) return fail(TC_ERR_ARG, "bad field");
@@ -1038,6 +1073,7 @@ int tc_unapply_gradient(tc_engine* e, int v_kind, int v_slot, int p_kind, int p_
 
 int tc_normalize_pressure(tc_engine* e, int kind, int slot) {
     if (int r = check(e)) return r;
+    e->diag_slot = -1;
     void* p = field_ptr(e, kind, slot);
     if (!p || !is_scal(kind)) return fail(TC_ERR_ARG, "bad field");
     return e->fp32 ? normalize<float>(e, (float*)p) : normalize<double>(e, (double*)p);
@@ -1045,6 +1081,7 @@ int tc_normalize_pressure(tc_engine* e, int kind, int slot) {
 
 int tc_recalc_turb_visc(tc_engine* e, int kind, int slot) {
     if (int r = check(e)) return r;
+    e->diag_slot = -1;
     void* v = field_ptr(e, kind, slot);
     if (!v || !is_vec(kind)) return fail(TC_ERR_ARG, "bad field");
     if (e->P.sgs_model) {  // the learned closure's counterpart: tau from this field
@@ -1105,7 +1142,7 @@ int tc_cfl(tc_engine* e, int kind, int slot, double tau, int mode, double* out) 
         a.nut = (const T*)e->nut;
         a.tau = tau;
         a.mode = mode;
-        a.visc_fac = e->P.scheme == TC_SCHEME_AB3 ? 4.0 * (e->P.visc_type + 1) : 4.0 * ((e->P.visc_type != 0) + 1);
+        a.visc_fac = diag_visc_fac(e->P.scheme, e->P.visc_type);
         return reduce_to_host<T>(e, a, out, 1);
     };
     return e->fp32 ? run(0.0f) : run(0.0);
@@ -1141,13 +1178,17 @@ int tc_diagnostics(tc_engine* e, int vel_slot, int p_slot, double tau, int cfl_m
     if (!ab3 && !p) return fail(TC_ERR_ARG, "bad pressure slot");
     if (host_av && count != (size_t)nv * e->nz) return fail(TC_ERR_ARG, "count must be nz*" + std::to_string(nv));
     if (cfl && (cfl_mode < 1 || cfl_mode > 3)) return fail(TC_ERR_ARG, "cfl_mode must be 1, 2 or 3");
-    const double visc_fac = ab3 ? 4.0 * (e->P.visc_type + 1) : 4.0 * ((e->P.visc_type != 0) + 1);
-    if (e->fp32)
-        HIPCHK(launch_diag<float>(e->s, e->mesh<float>(), e->P.scheme, (const float*)v, (const float*)p,
-                                  (const float*)e->nut, visc_fac, e->d_partial, e->d_diag));
-    else
-        HIPCHK(launch_diag<double>(e->s, e->mesh<double>(), e->P.scheme, (const double*)v, (const double*)p,
-                                   (const double*)e->nut, visc_fac, e->d_partial, e->d_diag));
+    // the step's own record when it holds these fields, else one pass now
+    if (!(e->diag_slot == vel_slot && (ab3 || p_slot == vel_slot))) {
+        e->diag_slot = -1;
+        const double visc_fac = diag_visc_fac(e->P.scheme, e->P.visc_type);
+        if (e->fp32)
+            HIPCHK(launch_diag<float>(e->s, e->mesh<float>(), e->P.scheme, (const float*)v, (const float*)p,
+                                      (const float*)e->nut, visc_fac, e->d_partial, e->d_diag));
+        else
+            HIPCHK(launch_diag<double>(e->s, e->mesh<double>(), e->P.scheme, (const double*)v, (const double*)p,
+                                       (const double*)e->nut, visc_fac, e->d_partial, e->d_diag));
+    }
     HIPCHK(hipMemcpyAsync(e->h_diag, e->d_diag, (size_t)e->nz * nrec * sizeof(double), hipMemcpyDeviceToHost, e->s));
     HIPCHK(hipStreamSynchronize(e->s));
     // volumes as the device kernels see them: the mesh in the engine's precision
@@ -1184,6 +1225,7 @@ int tc_diagnostics(tc_engine* e, int vel_slot, int p_slot, double tau, int cfl_m
 
 int tc_cfr_correct(tc_engine* e, int kind, int slot, double* shift) {
     if (int r = check(e)) return r;
+    e->diag_slot = -1;
     void* v = field_ptr(e, kind, slot);
     if (!v || !is_vec(kind)) return fail(TC_ERR_ARG, "bad field");
     if (!(e->P.cfr_ubulk > 0)) return fail(TC_ERR_STATE, "cfr_ubulk is not set");
@@ -1200,6 +1242,7 @@ int tc_cfr_correct(tc_engine* e, int kind, int slot, double* shift) {
 
 int tc_project(tc_engine* e, int kind, int slot) {
     if (int r = check(e)) return r;
+    e->diag_slot = -1;
     void* v = field_ptr(e, kind, slot);
     if (!v || !is_vec(kind)) return fail(TC_ERR_ARG, "bad field");
     return e->fp32 ? project<float>(e, (float*)v) : project<double>(e, (double*)v);
@@ -1207,6 +1250,7 @@ int tc_project(tc_engine* e, int kind, int slot) {
 
 int tc_initial_field(tc_engine* e, int slot, unsigned long long seed, double ubulk, double noise_rms) {
     if (int r = check(e)) return r;
+    e->diag_slot = -1;
     void* v = field_ptr(e, TC_VEL, slot);
     if (!v) return fail(TC_ERR_ARG, "bad slot");
     std::vector<double> u(3 * e->n3d);

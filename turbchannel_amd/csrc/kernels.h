@@ -50,6 +50,13 @@ struct ProjectArgs {
     const double* shift = nullptr;  // flow-rate shift on u (nullptr: none)
     const T* pcur = nullptr;        // StabType 1
     T* pnext = nullptr;             // StabType 1
+    // The step's diagnostics (nullptr: off): the projection also writes one
+    // partial diagnostics record per block for the new velocity (and pressure,
+    // nu_t), which diag_out then receives per plane, as launch_diag would.
+    double* diag = nullptr;
+    double* diag_out = nullptr;
+    double visc_fac = 0;
+    const T* nut = nullptr;
 };
 template <typename T>
 hipError_t launch_project(hipStream_t s, const Mesh<T>& m, const ProjectArgs<T>& a);
@@ -201,5 +208,9 @@ int diag_record(int scheme);
 template <typename T>
 hipError_t launch_diag(hipStream_t s, const Mesh<T>& m, int scheme, const T* v, const T* p, const T* nut,
                        double visc_fac, double* partial, double* out);
+// Per-plane records from nch partial records per plane (any nch).
+hipError_t launch_diag_final_wide(hipStream_t s, int scheme, const double* partial, int nch, int nz, double* out);
+// Partial records the projection writes when it forms the diagnostics (see ProjectArgs::diag).
+int project_diag_chunks(int nxy);
 
 }  // namespace tc

@@ -2,6 +2,7 @@
 // pressure-normalisation sum, CFL and per-plane statistics
 // (staggered/func_global.h:657-808; collocated_AB3/func_global.h:162-289).
 // Two stages with a fixed grid, so results do not depend on scheduling.
+#include "diag.h"
 #include "kernels.h"
 
 namespace tc {
@@ -174,14 +175,12 @@ __global__ void k_profiles_final(const double* partial, int nch, int nv, double 
 template <typename T, int SCHEME>
 __global__ __launch_bounds__(kThreads) void k_diag(Mesh<T> m, const T* __restrict__ vel, const T* __restrict__ p,
                                                    const T* __restrict__ nut, double visc_fac, double* partial) {
-    constexpr int NV = SCHEME == TC_SCHEME_AB3 ? 10 : 12;
-    constexpr int NS = NV + 2, NX = NS + 3;
-    __shared__ double sh[NS][kThreads / 64];
+    using D = DiagSums<SCHEME>;
+    __shared__ double sh[D::NS][kThreads / 64];
     __shared__ double shm[3][kThreads / 64];
     const int iz = blockIdx.y, nch = gridDim.x;
-    double s[NS], mx[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int k = 0; k < NS; k++) s[k] = 0.0;
+    D acc;
+    acc.init();
     const int base = iz * m.nxy;
     const double ihx = m.inv_hx, ihy = m.inv_hy;
     const double inv_hz = SCHEME == TC_SCHEME_AB3 ? 1.0 / (double)m.cellh[iz] : 1.0 / (double)m.DZ[iz];
@@ -227,40 +226,17 @@ __global__ __launch_bounds__(kThreads) void k_diag(Mesh<T> m, const T* __restric
 #pragma unroll
         for (int k = 0; k < U; k++) {
             if (!ok[k]) continue;
-            const double u = u_[k], v = v_[k], wr = wr_[k];
-            double w = wr;
-            if (SCHEME != TC_SCHEME_AB3) w = 0.5 * (wr + wn_[k]);
-            s[0] += u;
-            s[1] += v;
-            s[2] += w;
-            s[3] += u * u;
-            s[4] += v * v;
-            s[5] += w * w;
-            s[6] += u * v;
-            s[7] += u * w;
-            s[8] += v * w;
-            if (nut) s[9] += nt_[k];
-            if (SCHEME != TC_SCHEME_AB3) {
-                const double pp = p_[k];
-                s[10] += pp;
-                s[11] += pp * pp;
-            }
-            s[NV] += wr;
-            s[NV + 1] += wr * wr;
-            const double cc = fabs(u) * ihx + fabs(v) * ihy + fabs(wr) * inv_hz;
-            const double cv = kv * vc_[k];
-            mx[0] = fmax(mx[0], cc);
-            mx[1] = fmax(mx[1], cv);
-            mx[2] = fmax(mx[2], cc + cv);
+            const double w = SCHEME != TC_SCHEME_AB3 ? 0.5 * (wr_[k] + wn_[k]) : wr_[k];
+            acc.add(u_[k], v_[k], w, wr_[k], nut != nullptr, nt_[k], p_[k], vc_[k], ihx, ihy, inv_hz, kv);
         }
     }
-    block_reduce<NS>(s, sh, false);
-    block_reduce<3>(mx, shm, true);
+    block_reduce_diag<D::NS>(acc.s, sh, false);
+    block_reduce_diag<3>(acc.mx, shm, true);
     if (threadIdx.x == 0) {
-        double* o = partial + ((size_t)iz * nch + blockIdx.x) * NX;
+        double* o = partial + ((size_t)iz * nch + blockIdx.x) * D::NX;
 #pragma unroll
-        for (int k = 0; k < NS; k++) o[k] = s[k];
-        for (int k = 0; k < 3; k++) o[NS + k] = mx[k];
+        for (int k = 0; k < D::NS; k++) o[k] = acc.s[k];
+        for (int k = 0; k < 3; k++) o[D::NS + k] = acc.mx[k];
     }
 }
 
@@ -283,9 +259,44 @@ __global__ void k_diag_final(const double* partial, int nch, int nx_rec, int nsu
     out[iz * nx_rec + k] = r;
 }
 
+// The same reduction for many chunks per plane (the projection's records: one
+// per block of a plane): a block per plane, chunks strided over its threads.
+template <int SCHEME>
+__global__ __launch_bounds__(kThreads) void k_diag_final_wide(const double* partial, int nch, double* out) {
+    using D = DiagSums<SCHEME>;
+    __shared__ double sh[D::NS][kThreads / 64];
+    __shared__ double shm[3][kThreads / 64];
+    const int iz = blockIdx.x;
+    D acc;
+    acc.init();
+    for (int c = threadIdx.x; c < nch; c += kThreads) {
+        const double* r = partial + ((size_t)iz * nch + c) * D::NX;
+#pragma unroll
+        for (int k = 0; k < D::NS; k++) acc.s[k] += r[k];
+#pragma unroll
+        for (int k = 0; k < 3; k++) acc.mx[k] = fmax(acc.mx[k], r[D::NS + k]);
+    }
+    block_reduce_diag<D::NS>(acc.s, sh, false);
+    block_reduce_diag<3>(acc.mx, shm, true);
+    if (threadIdx.x == 0) {
+        double* o = out + (size_t)iz * D::NX;
+#pragma unroll
+        for (int k = 0; k < D::NS; k++) o[k] = acc.s[k];
+        for (int k = 0; k < 3; k++) o[D::NS + k] = acc.mx[k];
+    }
+}
+
 }  // namespace
 
 int diag_record(int scheme) { return (scheme == TC_SCHEME_AB3 ? 10 : 12) + 5; }
+
+hipError_t launch_diag_final_wide(hipStream_t s, int scheme, const double* partial, int nch, int nz, double* out) {
+    if (scheme == TC_SCHEME_AB3)
+        hipLaunchKernelGGL((k_diag_final_wide<TC_SCHEME_AB3>), dim3(nz), dim3(kThreads), 0, s, partial, nch, out);
+    else
+        hipLaunchKernelGGL((k_diag_final_wide<TC_SCHEME_HW>), dim3(nz), dim3(kThreads), 0, s, partial, nch, out);
+    return hipGetLastError();
+}
 
 template <typename T>
 hipError_t launch_diag(hipStream_t s, const Mesh<T>& m, int scheme, const T* v, const T* p, const T* nut,

@@ -6,6 +6,7 @@
 // wavefront and every mesh coefficient is a scalar load.
 #include <algorithm>
 
+#include "diag.h"
 #include "kernels.h"
 #include "stencil.h"
 
@@ -53,6 +54,20 @@ __device__ __forceinline__ void make_nb(Nb& nb, int ix, int iy, int iz, int nx, 
     nb.base = iz * nb.nxy;
 }
 
+// z component of the staggered gradient at the cell dzo planes above nb's
+// cell; iz is that cell's plane (staggered/func_global.h:302, :319-321).
+template <typename T, int SCHEME>
+__device__ __forceinline__ T grad_z_stag(const Mesh<T>& m, const T* __restrict__ a, const Nb& nb, int iz, int dzo) {
+    const int NZ = m.nz;
+    if (iz == 0) return T(0);
+    const T a0 = a[nb.at(0, 0, dzo)];
+    const T am = a[nb.at(0, 0, dzo - 1)];
+    if (SCHEME == TC_SCHEME_HW) return (a0 - am) * m.IHDN[iz];  // 2 / (DZ[iz] + DZ[iz-1])
+    const T ar = iz == NZ - 1 ? -a0 : a[nb.at(0, 0, dzo + 1)];
+    const T all = iz == 1 ? -am : a[nb.at(0, 0, dzo - 2)];
+    return ((a0 - am) - T(1.0 / 27.0) * (ar - all)) * m.GCZ[iz];
+}
+
 // Gradient at one cell (staggered/func_global.h:297-303, :316-322; collocated_AB3/func_global.h:298-303).
 template <typename T, int SCHEME>
 __device__ __forceinline__ void grad_at(const Mesh<T>& m, const T* __restrict__ a, const Nb& nb, int iz, T g[3]) {
@@ -61,20 +76,13 @@ __device__ __forceinline__ void grad_at(const Mesh<T>& m, const T* __restrict__ 
         const T a0 = a[nb.at(0, 0, 0)];
         g[0] = m.inv_hx * (a0 - a[nb.at(-1, 0, 0)]);
         g[1] = m.inv_hy * (a0 - a[nb.at(0, -1, 0)]);
-        g[2] = iz == 0 ? T(0) : (a0 - a[nb.at(0, 0, -1)]) * m.IHDN[iz];  // 2 / (DZ[iz] + DZ[iz-1])
+        g[2] = grad_z_stag<T, SCHEME>(m, a, nb, iz, 0);
     } else if (SCHEME == TC_SCHEME_VV) {
         const T a0 = a[nb.at(0, 0, 0)];
         const T g1 = m.GC1[iz], g2 = m.GC2[iz];
         g[0] = m.inv_hx * (g1 * (a0 - a[nb.at(-1, 0, 0)]) + g2 * (a[nb.at(1, 0, 0)] - a[nb.at(-2, 0, 0)]));
         g[1] = m.inv_hy * (g1 * (a0 - a[nb.at(0, -1, 0)]) + g2 * (a[nb.at(0, 1, 0)] - a[nb.at(0, -2, 0)]));
-        if (iz == 0) {
-            g[2] = T(0);
-        } else {
-            const T am = a[nb.at(0, 0, -1)];
-            const T ar = iz == NZ - 1 ? -a0 : a[nb.at(0, 0, 1)];
-            const T all = iz == 1 ? -am : a[nb.at(0, 0, -2)];
-            g[2] = ((a0 - am) - T(1.0 / 27.0) * (ar - all)) * m.GCZ[iz];
-        }
+        g[2] = grad_z_stag<T, SCHEME>(m, a, nb, iz, 0);
     } else {
         if (iz == 0 || iz == NZ - 1) {
             g[0] = g[1] = g[2] = T(0);
@@ -610,13 +618,23 @@ __global__ __launch_bounds__(kBlock) void k_grad(Mesh<T> m, const T* __restrict_
     o[2] = r[2];
 }
 
-// K7: velocity correction (staggered/func_global.h:937-941; collocated_AB3/func_global.h:428-449)
-template <typename T, int SCHEME, bool STAB>
-__global__ __launch_bounds__(kBlock) void k_project(Mesh<T> m, ProjectArgs<T> a) {
-    const PlaneBlock pb = plane_block(m.xcd_swz);
-    const int iz = pb.iz;
-    int ix, iy;
-    if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx)) return;
+// Cells per thread of the projection that also forms the diagnostics: one
+// block reduction of the record per kProjDiagCells * kBlock cells of a plane.
+constexpr int kProjDiagCells = 4;
+
+// Per-plane factors of the diagnostics' CFL terms.
+struct DiagPlane {
+    double ihx, ihy, inv_hz, kv;
+};
+
+// K7 at one cell: velocity correction (staggered/func_global.h:937-941;
+// collocated_AB3/func_global.h:428-449). DIAG: also adds the cell's terms of
+// the diagnostics of the new velocity (the terms k_diag adds per cell, diag.h);
+// the staggered cell-centre w needs the new w one plane up, formed here as the
+// thread that stores it forms it (r must not alias the output then, as in a step).
+template <typename T, int SCHEME, bool STAB, bool DIAG>
+__device__ __forceinline__ void project_cell(const Mesh<T>& m, const ProjectArgs<T>& a, int ix, int iy, int iz,
+                                             DiagSums<SCHEME>* acc, const DiagPlane& dp) {
     Nb nb;
     make_nb<2>(nb, ix, iy, iz, m.nx, m.ny);
     const int in = nb.at(0, 0, 0);
@@ -629,13 +647,75 @@ __global__ __launch_bounds__(kBlock) void k_project(Mesh<T> m, ProjectArgs<T> a)
     if (STAB) pn = a.p[in] + a.pcur[in];
     T g[3];
     grad_at<T, SCHEME>(m, a.p, nb, iz, g);
-    T* o = a.vnext + 3 * in;
+    T wup = T(0);  // staggered: the new w one plane up (0 past the top wall)
+    if (DIAG && SCHEME != TC_SCHEME_AB3 && iz < m.nz - 1)
+        wup = (a.r[3 * (in + m.nxy) + 2] - grad_z_stag<T, SCHEME>(m, a.p, nb, iz + 1, 1)) * a.tau_over_bd0;
+    double nt = 0.0, vc = m.visc, ps = 0.0;
+    if (DIAG) {
+        if (a.nut) nt = a.nut[in];
+        if (SCHEME == TC_SCHEME_AB3) {
+            if (a.nut && iz != 0 && iz != m.nz - 1) {  // GetNuTAtNode: mean of the 8 elements around the node
+                const T* n = a.nut;
+                const int c0 = in, cx = nb.at(-1, 0, 0), cy = nb.at(0, -1, 0), cxy = nb.at(-1, -1, 0);
+                const int dz = m.nxy;
+                vc += 0.125 * ((double)n[c0] + n[cx] + n[cy] + n[cxy] + n[c0 - dz] + n[cx - dz] + n[cy - dz] +
+                               n[cxy - dz]);
+            }
+        } else {
+            if (a.nut) vc = (double)m.visc + nt;
+            ps = a.p[in];
+        }
+    }
     T u = (r0 - g[0]) * a.tau_over_bd0;
     if (a.shift && !(SCHEME == TC_SCHEME_AB3 && (iz == 0 || iz == m.nz - 1))) u += sh;
+    const T v = (r1 - g[1]) * a.tau_over_bd0;
+    const T w = (r2 - g[2]) * a.tau_over_bd0;
+    T* o = a.vnext + 3 * in;
     o[0] = u;
-    o[1] = (r1 - g[1]) * a.tau_over_bd0;
-    o[2] = (r2 - g[2]) * a.tau_over_bd0;
+    o[1] = v;
+    o[2] = w;
     if (STAB) a.pnext[in] = pn;
+    if (DIAG) {
+        const double wc = SCHEME != TC_SCHEME_AB3 ? 0.5 * ((double)w + (double)wup) : (double)w;
+        acc->add(u, v, wc, w, a.nut != nullptr, nt, ps, vc, dp.ihx, dp.ihy, dp.inv_hz, dp.kv);
+    }
+}
+
+// K7 over a (plane chunk, plane) grid; DIAG: kProjDiagCells chunks of kBlock
+// cells per block, and one partial diagnostics record per block.
+template <typename T, int SCHEME, bool STAB, bool DIAG>
+__global__ __launch_bounds__(kBlock) void k_project(Mesh<T> m, ProjectArgs<T> a) {
+    const PlaneBlock pb = plane_block(m.xcd_swz);
+    const int iz = pb.iz;
+    DiagPlane dp;
+    if (!DIAG) {
+        int ix, iy;
+        if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx)) return;
+        project_cell<T, SCHEME, STAB, false>(m, a, ix, iy, iz, nullptr, dp);
+        return;
+    }
+    using D = DiagSums<SCHEME>;
+    __shared__ double shs[D::NS][kBlock / 64];
+    __shared__ double shm[3][kBlock / 64];
+    dp.ihx = m.inv_hx;
+    dp.ihy = m.inv_hy;
+    dp.inv_hz = SCHEME == TC_SCHEME_AB3 ? 1.0 / (double)m.cellh[iz] : 1.0 / (double)m.DZ[iz];
+    dp.kv = a.visc_fac * (dp.ihx * dp.ihx + dp.ihy * dp.ihy + dp.inv_hz * dp.inv_hz);
+    D acc;
+    acc.init();
+    for (int k = 0; k < kProjDiagCells; k++) {
+        int ix, iy;
+        if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx * kProjDiagCells + k)) break;
+        project_cell<T, SCHEME, STAB, true>(m, a, ix, iy, iz, &acc, dp);
+    }
+    block_reduce_diag<D::NS>(acc.s, shs, false);
+    block_reduce_diag<3>(acc.mx, shm, true);
+    if (threadIdx.x == 0) {
+        double* o = a.diag + ((size_t)iz * gridDim.x + pb.bx) * D::NX;
+#pragma unroll
+        for (int k = 0; k < D::NS; k++) o[k] = acc.s[k];
+        for (int k = 0; k < 3; k++) o[D::NS + k] = acc.mx[k];
+    }
 }
 
 template <typename T, int SCHEME>
@@ -857,13 +937,27 @@ hipError_t launch_gradient(hipStream_t s, const Mesh<T>& m, int scheme, const T*
     return hipGetLastError();
 }
 
+int project_diag_chunks(int nxy) { return (nxy + kBlock * kProjDiagCells - 1) / (kBlock * kProjDiagCells); }
+
 template <typename T>
 hipError_t launch_project(hipStream_t s, const Mesh<T>& m, const ProjectArgs<T>& a) {
-    const dim3 g = cell_grid(m);
-    if (a.scheme == TC_SCHEME_HW) hipLaunchKernelGGL((k_project<T, TC_SCHEME_HW, false>), g, dim3(kBlock), 0, s, m, a);
-    else if (a.scheme == TC_SCHEME_VV) hipLaunchKernelGGL((k_project<T, TC_SCHEME_VV, false>), g, dim3(kBlock), 0, s, m, a);
-    else if (a.pnext) hipLaunchKernelGGL((k_project<T, TC_SCHEME_AB3, true>), g, dim3(kBlock), 0, s, m, a);
-    else hipLaunchKernelGGL((k_project<T, TC_SCHEME_AB3, false>), g, dim3(kBlock), 0, s, m, a);
+    dim3 g = cell_grid(m);
+#define LP(SCH, STAB, DIAG) hipLaunchKernelGGL((k_project<T, SCH, STAB, DIAG>), g, dim3(kBlock), 0, s, m, a)
+    if (a.diag) {
+        g.x = project_diag_chunks(m.nxy);
+        if (a.scheme == TC_SCHEME_HW) LP(TC_SCHEME_HW, false, true);
+        else if (a.scheme == TC_SCHEME_VV) LP(TC_SCHEME_VV, false, true);
+        else if (a.pnext) LP(TC_SCHEME_AB3, true, true);
+        else LP(TC_SCHEME_AB3, false, true);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        return launch_diag_final_wide(s, a.scheme, a.diag, (int)g.x, m.nz, a.diag_out);
+    }
+    if (a.scheme == TC_SCHEME_HW) LP(TC_SCHEME_HW, false, false);
+    else if (a.scheme == TC_SCHEME_VV) LP(TC_SCHEME_VV, false, false);
+    else if (a.pnext) LP(TC_SCHEME_AB3, true, false);
+    else LP(TC_SCHEME_AB3, false, false);
+#undef LP
     return hipGetLastError();
 }
 

@@ -86,6 +86,7 @@ struct Options {
     std::string restart;
     int timing = 0;
     int graphs = 1;
+    int fused_diag = 1;
     int print_mesh = 0;
 };
 
@@ -136,6 +137,7 @@ const OptSpec kOpts[] = {
     {"--restart", "continue exactly from this restart file"},
     {"--timing", "1: report the time of each kernel class at the end"},
     {"--graphs", "1: replay repeated steps from captured HIP graphs (default), 0: launch each kernel"},
+    {"--fused_diag", "1: the step's projection forms the per-step diagnostics (default), 0: a pass of its own"},
     {"--print-mesh", "1: print the wall-normal coordinates and exit"},
 };
 
@@ -243,6 +245,7 @@ bool parse(int argc, char** argv, Options& o) {
         else if (is("--restart")) o.restart = val;
         else if (is("--timing")) parse_num(key, val, o.timing);
         else if (is("--graphs")) parse_num(key, val, o.graphs);
+        else if (is("--fused_diag")) parse_num(key, val, o.fused_diag);
         else if (is("--print-mesh")) parse_num(key, val, o.print_mesh);
     }
     return true;
@@ -858,6 +861,7 @@ struct Run {
         if (!log) die("cannot open log.dat");
         check(tc_sync(e), "tc_sync");
         if (P.scheme == TC_SCHEME_SRK) return srk_loop(log);
+        check(tc_step_diagnostics(e, o.fused_diag), "tc_step_diagnostics");
 
         const int sprog = o.s_progress > 0 ? o.s_progress : (P.collocated ? 10 : 200);
         const int fprog = o.f_progress > 0 ? o.f_progress : (P.collocated ? 10000 : 1000000);
