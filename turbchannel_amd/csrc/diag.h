@@ -82,6 +82,52 @@ __device__ __forceinline__ void block_reduce_diag(double (&v)[NV], double (*sh)[
         }
 }
 
+// Reduces P values (P a power of two, 2 <= P <= 64) over a wavefront: value k
+// is summed when bit k of summask is set, else maximised. Each of the first
+// log2(P) steps gives away half of a lane's remaining values to the lane 32,
+// 16, ... apart and combines the other half with what that lane gave, so that
+// after P - 1 exchanges lane L holds one value, index L >> (6 - log2 P), over
+// the lanes that share L's high bits; log2(64 / P) more exchanges finish it.
+// The result: lane L with its low 6 - log2(P) bits clear holds value
+// L >> (6 - log2 P) over the wave (P - 1 + log2(64 / P) exchanges instead of
+// 6 per value). Fixed order, so results do not depend on scheduling.
+// One halving step of wave_reduce_transpose: 2H values per lane -> H, with the
+// lane OFF = 64 H / P apart; every index a compile-time constant (a dynamically
+// indexed array would live in scratch memory).
+template <int P, int H>
+__device__ __forceinline__ void transpose_halve(double (&v)[P], int& base, unsigned long long summask, int lane) {
+    if constexpr (H >= 1) {
+        constexpr int OFF = 64 * H / P;
+        const bool hi = (lane & OFF) != 0;
+#pragma unroll
+        for (int j = 0; j < H; j++) {
+            const double keep = hi ? v[H + j] : v[j];
+            const double give = hi ? v[j] : v[H + j];
+            const double got = __shfl_xor(give, OFF, 64);
+            const int idx = base + (hi ? H : 0) + j;
+            v[j] = ((summask >> idx) & 1) ? keep + got : fmax(keep, got);
+        }
+        if (hi) base += H;
+        transpose_halve<P, H / 2>(v, base, summask, lane);
+    }
+}
+
+template <int P>
+__device__ __forceinline__ double wave_reduce_transpose(double (&v)[P], unsigned long long summask) {
+    static_assert(P >= 2 && P <= 64 && (P & (P - 1)) == 0, "P must be a power of two in [2, 64]");
+    const int lane = threadIdx.x & 63;
+    int base = 0;  // index of v[0] among the values
+    transpose_halve<P, P / 2>(v, base, summask, lane);
+    double x = v[0];
+    const bool sum = (summask >> base) & 1;
+#pragma unroll
+    for (int off = 32 / P; off >= 1; off >>= 1) {
+        const double y = __shfl_xor(x, off, 64);
+        x = sum ? x + y : fmax(x, y);
+    }
+    return x;
+}
+
 // visc_fac of the viscous CFL number (CalcCFL_loc: 4 (nu + nu_t) per unit
 // squared mesh step, doubled for the Galerkin / NAIVE viscous terms).
 inline double diag_visc_fac(int scheme, int visc_type) {

@@ -54,18 +54,29 @@ __device__ __forceinline__ void make_nb(Nb& nb, int ix, int iy, int iz, int nx, 
     nb.base = iz * nb.nxy;
 }
 
+// z offset dz from plane ib, clamped into the grid: every load is valid, and the
+// wall rules select away what a clamped load reads (no branch around a load).
+__device__ __forceinline__ int zclamp(int ib, int dz, int nz) { return min(max(dz, -ib), nz - 1 - ib); }
+
 // z component of the staggered gradient at the cell dzo planes above nb's
-// cell; iz is that cell's plane (staggered/func_global.h:302, :319-321).
+// cell; iz is that cell's plane (staggered/func_global.h:302, :319-321). Past
+// the top plane (iz == nz) the value is meaningless; the caller discards it.
 template <typename T, int SCHEME>
 __device__ __forceinline__ T grad_z_stag(const Mesh<T>& m, const T* __restrict__ a, const Nb& nb, int iz, int dzo) {
-    const int NZ = m.nz;
-    if (iz == 0) return T(0);
-    const T a0 = a[nb.at(0, 0, dzo)];
-    const T am = a[nb.at(0, 0, dzo - 1)];
-    if (SCHEME == TC_SCHEME_HW) return (a0 - am) * m.IHDN[iz];  // 2 / (DZ[iz] + DZ[iz-1])
-    const T ar = iz == NZ - 1 ? -a0 : a[nb.at(0, 0, dzo + 1)];
-    const T all = iz == 1 ? -am : a[nb.at(0, 0, dzo - 2)];
-    return ((a0 - am) - T(1.0 / 27.0) * (ar - all)) * m.GCZ[iz];
+    const int NZ = m.nz, ib = iz - dzo, it = min(iz, NZ - 1);
+    const T a0 = a[nb.at(0, 0, zclamp(ib, dzo, NZ))];
+    const T am = a[nb.at(0, 0, zclamp(ib, dzo - 1, NZ))];
+    T g;
+    if (SCHEME == TC_SCHEME_HW) {
+        g = (a0 - am) * m.IHDN[it];  // 2 / (DZ[iz] + DZ[iz-1])
+    } else {
+        const T arl = a[nb.at(0, 0, zclamp(ib, dzo + 1, NZ))];
+        const T alll = a[nb.at(0, 0, zclamp(ib, dzo - 2, NZ))];
+        const T ar = iz == NZ - 1 ? -a0 : arl;
+        const T all = iz == 1 ? -am : alll;
+        g = ((a0 - am) - T(1.0 / 27.0) * (ar - all)) * m.GCZ[it];
+    }
+    return iz == 0 ? T(0) : g;
 }
 
 // Gradient at one cell (staggered/func_global.h:297-303, :316-322; collocated_AB3/func_global.h:298-303).
@@ -84,13 +95,13 @@ __device__ __forceinline__ void grad_at(const Mesh<T>& m, const T* __restrict__ 
         g[1] = m.inv_hy * (g1 * (a0 - a[nb.at(0, -1, 0)]) + g2 * (a[nb.at(0, 1, 0)] - a[nb.at(0, -2, 0)]));
         g[2] = grad_z_stag<T, SCHEME>(m, a, nb, iz, 0);
     } else {
-        if (iz == 0 || iz == NZ - 1) {
-            g[0] = g[1] = g[2] = T(0);
-            return;
-        }
-        g[0] = T(0.5) * m.inv_hx * (a[nb.at(1, 0, 0)] - a[nb.at(-1, 0, 0)]);
-        g[1] = T(0.5) * m.inv_hy * (a[nb.at(0, 1, 0)] - a[nb.at(0, -1, 0)]);
-        g[2] = T(0.5) * (a[nb.at(0, 0, 1)] - a[nb.at(0, 0, -1)]) / m.cellh[iz];
+        const bool wall = iz == 0 || iz == NZ - 1;
+        const T gx = T(0.5) * m.inv_hx * (a[nb.at(1, 0, 0)] - a[nb.at(-1, 0, 0)]);
+        const T gy = T(0.5) * m.inv_hy * (a[nb.at(0, 1, 0)] - a[nb.at(0, -1, 0)]);
+        const T gz = T(0.5) * (a[nb.at(0, 0, zclamp(iz, 1, NZ))] - a[nb.at(0, 0, zclamp(iz, -1, NZ))]) / m.cellh[iz];
+        g[0] = wall ? T(0) : gx;
+        g[1] = wall ? T(0) : gy;
+        g[2] = wall ? T(0) : gz;
     }
 }
 
@@ -142,6 +153,13 @@ constexpr int flux_nc() {
 template <typename T, int SCHEME, int VISC>
 constexpr int flux_threads() {
     return kTX * flux_ty<T, SCHEME, VISC>() * flux_nc<T, SCHEME>();
+}
+// Planes of the LDS ring either side of the computed one. The fourth-order
+// scheme reaches z+-3 on the cell's own column only; a ring of +-3 planes holds
+// those too, so that no global load sits in the stencil.
+template <typename T, int SCHEME>
+constexpr int flux_rz() {
+    return SCHEME == TC_SCHEME_VV ? 3 : 1;
 }
 // Waves per SIMD the register allocation must allow.
 template <typename T, int SCHEME, int VISC>
@@ -217,12 +235,16 @@ __device__ __forceinline__ void sgs_divergence(const Mesh<T>& m, const T q[3][7]
     }
 }
 
-template <typename T, int SCHEME, int VISC, bool TURB>
-__global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>()), (flux_min_waves<T, SCHEME, VISC>())) void k_flux(
-    Mesh<T> m, FluxArgs<T> a, int ntx, int ntiles, int zchunk) {
+// The plane loop of K1 for one block; COMP is the momentum component of the
+// calling wave when a cell's three equations go to three waves (else -1), a
+// compile-time value so that each wave's registers hold only its own equation.
+template <typename T, int SCHEME, int VISC, bool TURB, int COMP>
+__device__ __forceinline__ void flux_planes(const Mesh<T>& m, const FluxArgs<T>& a, int ntx, int ntiles, int zchunk,
+                                            T* ring, double* sh) {
     constexpr int TY = flux_ty<T, SCHEME, VISC>();
     constexpr int RXY = SCHEME == TC_SCHEME_VV ? 3 : 1;
-    constexpr int RZ = SCHEME == TC_SCHEME_VV ? 2 : 1;
+    constexpr int RZ = flux_rz<T, SCHEME>();
+    constexpr bool COLUMN = SCHEME == TC_SCHEME_VV && RZ < 3;  // z+-3 of the own column from global memory
     constexpr int NS = 2 * RZ + 2;  // planes iz-RZ..iz+RZ plus the one in flight
     using Acc = RingAcc<T, RXY, RZ, TY>;
     constexpr int W = Acc::W, PS = Acc::PS;
@@ -234,35 +256,12 @@ __global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>()), (flux_min_waves<
     // slot stride: every lane writes ND dwords per plane fill (the surplus into
     // padding), so the fill is the same instruction stream for every plane
     constexpr int SLOT = ND * NT * 4 / (int)sizeof(T);  // reals per ring slot
-    __shared__ __align__(16) T ring[NS * SLOT];
-    __shared__ double sh[NT / 64];
-    // The mesh arrays the stencils read, mirrored in LDS (dynamic, kFluxMeshArrays
-    // nz reals): LDS reads wait on their own counter, never on the global loads
-    // in flight.
-    extern __shared__ __align__(16) unsigned char mesh_raw[];
-    {
-        T* ml = reinterpret_cast<T*>(mesh_raw);
-        const T* src[kFluxMeshArrays] = {m.DZ, m.DZW, m.GCZ, m.cellh, m.IDZ, m.IHDN, m.IHUP, m.IDZM, m.IVOL, m.IHC};
-        for (int i = threadIdx.x; i < m.nz; i += NT)
-#pragma unroll
-            for (int k = 0; k < kFluxMeshArrays; k++) ml[k * m.nz + i] = src[k][i];
-        m.DZ = ml;
-        m.DZW = ml + m.nz;
-        m.GCZ = ml + 2 * m.nz;
-        m.cellh = ml + 3 * m.nz;
-        m.IDZ = ml + 4 * m.nz;
-        m.IHDN = ml + 5 * m.nz;
-        m.IHUP = ml + 6 * m.nz;
-        m.IDZM = ml + 7 * m.nz;
-        m.IVOL = ml + 8 * m.nz;
-        m.IHC = ml + 9 * m.nz;
-    }
     const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tile = lb % ntiles, chunk = lb / ntiles;
     const int x0 = (tile % ntx) * kTX, y0 = (tile / ntx) * TY;
     const int z0 = chunk * zchunk, z1 = min(m.nz, z0 + zchunk);
     const int tcell = threadIdx.x % (kTX * TY);
-    const int comp = NC == 3 ? (int)(threadIdx.x / (kTX * TY)) : -1;  // wave-uniform
+    const int comp = COMP;
     const int tx = tcell % kTX, ty = tcell / kTX;
     const int ix = x0 + tx, iy = y0 + ty;
     const bool active = ix < m.nx && iy < m.ny;
@@ -356,7 +355,7 @@ __global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>()), (flux_min_waves<
             }
         }
         Acc acc;
-        {
+        if (COLUMN) {
             const int zp = min(iz + RZ + 1, m.nz - 1), zm = max(iz - RZ - 1, 0);
             const T* cpp = a.vel + 3 * ((size_t)zp * m.nxy + cy * m.nx + cx);
             const T* cmp = a.vel + 3 * ((size_t)zm * m.nxy + cy * m.nx + cx);
@@ -454,6 +453,57 @@ __global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>()), (flux_min_waves<
     if (a.cfr_partial) {
         const double s = block_sum(cfr, sh);
         if (threadIdx.x == 0) a.cfr_partial[blockIdx.x] = s;
+    }
+}
+
+template <typename T, int SCHEME, int VISC, bool TURB>
+__global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>()), (flux_min_waves<T, SCHEME, VISC>())) void k_flux(
+    Mesh<T> m, FluxArgs<T> a, int ntx, int ntiles, int zchunk) {
+    constexpr int TY = flux_ty<T, SCHEME, VISC>();
+    constexpr int RXY = SCHEME == TC_SCHEME_VV ? 3 : 1;
+    constexpr int RZ = flux_rz<T, SCHEME>();
+    constexpr bool COLUMN = SCHEME == TC_SCHEME_VV && RZ < 3;  // z+-3 of the own column from global memory
+    constexpr int NS = 2 * RZ + 2;  // planes iz-RZ..iz+RZ plus the one in flight
+    using Acc = RingAcc<T, RXY, RZ, TY>;
+    constexpr int W = Acc::W, PS = Acc::PS;
+    constexpr int NC = flux_nc<T, SCHEME>();
+    constexpr int NT = kTX * TY * NC;
+    constexpr int DPC = 3 * (int)sizeof(T) / 4;  // dwords per real3 cell
+    constexpr int DPS = PS * DPC;                // dwords per plane of the padded tile
+    constexpr int ND = (DPS + NT - 1) / NT;
+    // slot stride: every lane writes ND dwords per plane fill (the surplus into
+    // padding), so the fill is the same instruction stream for every plane
+    constexpr int SLOT = ND * NT * 4 / (int)sizeof(T);  // reals per ring slot
+    __shared__ __align__(16) T ring[NS * SLOT];
+    __shared__ double sh[NT / 64];
+    // The mesh arrays the stencils read, mirrored in LDS (dynamic, kFluxMeshArrays
+    // nz reals): LDS reads wait on their own counter, never on the global loads
+    // in flight.
+    extern __shared__ __align__(16) unsigned char mesh_raw[];
+    {
+        T* ml = reinterpret_cast<T*>(mesh_raw);
+        const T* src[kFluxMeshArrays] = {m.DZ, m.DZW, m.GCZ, m.cellh, m.IDZ, m.IHDN, m.IHUP, m.IDZM, m.IVOL, m.IHC};
+        for (int i = threadIdx.x; i < m.nz; i += NT)
+#pragma unroll
+            for (int k = 0; k < kFluxMeshArrays; k++) ml[k * m.nz + i] = src[k][i];
+        m.DZ = ml;
+        m.DZW = ml + m.nz;
+        m.GCZ = ml + 2 * m.nz;
+        m.cellh = ml + 3 * m.nz;
+        m.IDZ = ml + 4 * m.nz;
+        m.IHDN = ml + 5 * m.nz;
+        m.IHUP = ml + 6 * m.nz;
+        m.IDZM = ml + 7 * m.nz;
+        m.IVOL = ml + 8 * m.nz;
+        m.IHC = ml + 9 * m.nz;
+    }
+    if (NC == 3) {
+        const int comp = threadIdx.x / (kTX * TY);  // wave-uniform
+        if (comp == 0) flux_planes<T, SCHEME, VISC, TURB, 0>(m, a, ntx, ntiles, zchunk, ring, sh);
+        else if (comp == 1) flux_planes<T, SCHEME, VISC, TURB, 1>(m, a, ntx, ntiles, zchunk, ring, sh);
+        else flux_planes<T, SCHEME, VISC, TURB, 2>(m, a, ntx, ntiles, zchunk, ring, sh);
+    } else {
+        flux_planes<T, SCHEME, VISC, TURB, -1>(m, a, ntx, ntiles, zchunk, ring, sh);
     }
 }
 
@@ -618,71 +668,95 @@ __global__ __launch_bounds__(kBlock) void k_grad(Mesh<T> m, const T* __restrict_
     o[2] = r[2];
 }
 
-// Cells per thread of the projection that also forms the diagnostics: one
-// block reduction of the record per kProjDiagCells * kBlock cells of a plane.
-constexpr int kProjDiagCells = 4;
-
 // Per-plane factors of the diagnostics' CFL terms.
 struct DiagPlane {
     double ihx, ihy, inv_hz, kv;
 };
 
 // K7 at one cell: velocity correction (staggered/func_global.h:937-941;
-// collocated_AB3/func_global.h:428-449). DIAG: also adds the cell's terms of
-// the diagnostics of the new velocity (the terms k_diag adds per cell, diag.h);
-// the staggered cell-centre w needs the new w one plane up, formed here as the
-// thread that stores it forms it (r must not alias the output then, as in a step).
+// collocated_AB3/func_global.h:428-449), in two parts: project_eval does every
+// load and the arithmetic, project_store the stores (and the diagnostics
+// terms), so that a thread can issue the loads of several cells before its
+// first store. DIAG: also the values of the cell's diagnostics terms (the
+// terms k_diag adds per cell, diag.h); the staggered cell-centre w needs the
+// new w one plane up, formed here as the thread that stores it forms it (r
+// must not alias the output then, as in a step).
+template <typename T>
+struct CellProj {
+    int in;
+    T u, v, w, pn, wup;
+    double nt, vc, ps;
+};
+
 template <typename T, int SCHEME, bool STAB, bool DIAG>
-__device__ __forceinline__ void project_cell(const Mesh<T>& m, const ProjectArgs<T>& a, int ix, int iy, int iz,
-                                             DiagSums<SCHEME>* acc, const DiagPlane& dp) {
+__device__ __forceinline__ void project_eval(const Mesh<T>& m, const ProjectArgs<T>& a, int ix, int iy, int iz,
+                                             CellProj<T>& c) {
     Nb nb;
     make_nb<2>(nb, ix, iy, iz, m.nx, m.ny);
     const int in = nb.at(0, 0, 0);
-    // every read before the first write (the output may alias an input, so a
-    // read after a write would wait for that write)
+    c.in = in;
     const T* r = a.r + 3 * in;
     const T r0 = r[0], r1 = r[1], r2 = r[2];
     const T sh = a.shift ? T(*a.shift) : T(0);
-    T pn = T(0);
-    if (STAB) pn = a.p[in] + a.pcur[in];
+    c.pn = T(0);
+    if (STAB) c.pn = a.p[in] + a.pcur[in];
     T g[3];
     grad_at<T, SCHEME>(m, a.p, nb, iz, g);
-    T wup = T(0);  // staggered: the new w one plane up (0 past the top wall)
-    if (DIAG && SCHEME != TC_SCHEME_AB3 && iz < m.nz - 1)
-        wup = (a.r[3 * (in + m.nxy) + 2] - grad_z_stag<T, SCHEME>(m, a.p, nb, iz + 1, 1)) * a.tau_over_bd0;
-    double nt = 0.0, vc = m.visc, ps = 0.0;
+    c.wup = T(0);  // staggered: the new w one plane up (0 past the top wall)
+    if (DIAG && SCHEME != TC_SCHEME_AB3) {
+        const T rup = a.r[3 * nb.at(0, 0, zclamp(iz, 1, m.nz)) + 2];
+        const T wu = (rup - grad_z_stag<T, SCHEME>(m, a.p, nb, iz + 1, 1)) * a.tau_over_bd0;
+        c.wup = iz < m.nz - 1 ? wu : T(0);
+    }
+    c.nt = 0.0;
+    c.vc = m.visc;
+    c.ps = 0.0;
     if (DIAG) {
-        if (a.nut) nt = a.nut[in];
+        if (a.nut) c.nt = a.nut[in];
         if (SCHEME == TC_SCHEME_AB3) {
-            if (a.nut && iz != 0 && iz != m.nz - 1) {  // GetNuTAtNode: mean of the 8 elements around the node
+            if (a.nut) {  // GetNuTAtNode: mean of the 8 elements around the node (interior nodes)
                 const T* n = a.nut;
                 const int c0 = in, cx = nb.at(-1, 0, 0), cy = nb.at(0, -1, 0), cxy = nb.at(-1, -1, 0);
-                const int dz = m.nxy;
-                vc += 0.125 * ((double)n[c0] + n[cx] + n[cy] + n[cxy] + n[c0 - dz] + n[cx - dz] + n[cy - dz] +
-                               n[cxy - dz]);
+                const int dz = zclamp(iz, -1, m.nz) * m.nxy;
+                const double s8 = (double)n[c0] + n[cx] + n[cy] + n[cxy] + n[c0 + dz] + n[cx + dz] + n[cy + dz] +
+                                  n[cxy + dz];
+                if (iz != 0 && iz != m.nz - 1) c.vc += 0.125 * s8;
             }
         } else {
-            if (a.nut) vc = (double)m.visc + nt;
-            ps = a.p[in];
+            if (a.nut) c.vc = (double)m.visc + c.nt;
+            c.ps = a.p[in];
         }
     }
     T u = (r0 - g[0]) * a.tau_over_bd0;
     if (a.shift && !(SCHEME == TC_SCHEME_AB3 && (iz == 0 || iz == m.nz - 1))) u += sh;
-    const T v = (r1 - g[1]) * a.tau_over_bd0;
-    const T w = (r2 - g[2]) * a.tau_over_bd0;
-    T* o = a.vnext + 3 * in;
-    o[0] = u;
-    o[1] = v;
-    o[2] = w;
-    if (STAB) a.pnext[in] = pn;
+    c.u = u;
+    c.v = (r1 - g[1]) * a.tau_over_bd0;
+    c.w = (r2 - g[2]) * a.tau_over_bd0;
+}
+
+template <typename T, int SCHEME, bool STAB, bool DIAG>
+__device__ __forceinline__ void project_store(const ProjectArgs<T>& a, const CellProj<T>& c, DiagSums<SCHEME>* acc,
+                                              const DiagPlane& dp) {
+    T* o = a.vnext + 3 * c.in;
+    o[0] = c.u;
+    o[1] = c.v;
+    o[2] = c.w;
+    if (STAB) a.pnext[c.in] = c.pn;
     if (DIAG) {
-        const double wc = SCHEME != TC_SCHEME_AB3 ? 0.5 * ((double)w + (double)wup) : (double)w;
-        acc->add(u, v, wc, w, a.nut != nullptr, nt, ps, vc, dp.ihx, dp.ihy, dp.inv_hz, dp.kv);
+        const double wc = SCHEME != TC_SCHEME_AB3 ? 0.5 * ((double)c.w + (double)c.wup) : (double)c.w;
+        acc->add(c.u, c.v, wc, c.w, a.nut != nullptr, c.nt, c.ps, c.vc, dp.ihx, dp.ihy, dp.inv_hz, dp.kv);
     }
 }
 
-// K7 over a (plane chunk, plane) grid; DIAG: kProjDiagCells chunks of kBlock
-// cells per block, and one partial diagnostics record per block.
+// Cells per thread of the projection that also forms the diagnostics: the
+// loads of all of them go out before the first store, and one reduction of the
+// record serves kProjDiagCells * kBlock cells of a plane.
+constexpr int kProjDiagCells = 4;
+
+// K7 over a (plane chunk, plane) grid, one cell per thread; DIAG: kProjDiagCells
+// chunks of kBlock cells of a plane per block and one partial diagnostics record
+// per block (each wave's sums by transposing exchanges, then the block's four
+// waves through LDS).
 template <typename T, int SCHEME, bool STAB, bool DIAG>
 __global__ __launch_bounds__(kBlock) void k_project(Mesh<T> m, ProjectArgs<T> a) {
     const PlaneBlock pb = plane_block(m.xcd_swz);
@@ -691,30 +765,48 @@ __global__ __launch_bounds__(kBlock) void k_project(Mesh<T> m, ProjectArgs<T> a)
     if (!DIAG) {
         int ix, iy;
         if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx)) return;
-        project_cell<T, SCHEME, STAB, false>(m, a, ix, iy, iz, nullptr, dp);
+        CellProj<T> c;
+        project_eval<T, SCHEME, STAB, false>(m, a, ix, iy, iz, c);
+        project_store<T, SCHEME, STAB, false>(a, c, nullptr, dp);
         return;
     }
     using D = DiagSums<SCHEME>;
-    __shared__ double shs[D::NS][kBlock / 64];
-    __shared__ double shm[3][kBlock / 64];
+    constexpr int P = D::NX <= 16 ? 16 : 32;
+    __shared__ double shr[kBlock / 64][P];
     dp.ihx = m.inv_hx;
     dp.ihy = m.inv_hy;
     dp.inv_hz = SCHEME == TC_SCHEME_AB3 ? 1.0 / (double)m.cellh[iz] : 1.0 / (double)m.DZ[iz];
     dp.kv = a.visc_fac * (dp.ihx * dp.ihx + dp.ihy * dp.ihy + dp.inv_hz * dp.inv_hz);
+    CellProj<T> c[kProjDiagCells];
+    bool act[kProjDiagCells];
+#pragma unroll
+    for (int k = 0; k < kProjDiagCells; k++) {  // a cell past the plane computes on the plane's last cell
+        const int t = (pb.bx * kProjDiagCells + k) * kBlock + threadIdx.x;
+        act[k] = t < m.nxy;
+        const int tt = act[k] ? t : m.nxy - 1;
+        const int iy = tt / m.nx, ix = tt - iy * m.nx;
+        project_eval<T, SCHEME, STAB, true>(m, a, ix, iy, iz, c[k]);
+    }
     D acc;
     acc.init();
-    for (int k = 0; k < kProjDiagCells; k++) {
-        int ix, iy;
-        if (!cell_of_thread(m.nx, m.nxy, ix, iy, pb.bx * kProjDiagCells + k)) break;
-        project_cell<T, SCHEME, STAB, true>(m, a, ix, iy, iz, &acc, dp);
-    }
-    block_reduce_diag<D::NS>(acc.s, shs, false);
-    block_reduce_diag<3>(acc.mx, shm, true);
-    if (threadIdx.x == 0) {
-        double* o = a.diag + ((size_t)iz * gridDim.x + pb.bx) * D::NX;
 #pragma unroll
-        for (int k = 0; k < D::NS; k++) o[k] = acc.s[k];
-        for (int k = 0; k < 3; k++) o[D::NS + k] = acc.mx[k];
+    for (int k = 0; k < kProjDiagCells; k++)
+        if (act[k]) project_store<T, SCHEME, STAB, true>(a, c[k], &acc, dp);
+    double v[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) v[k] = k < D::NS ? acc.s[k] : k < D::NX ? acc.mx[k - D::NS] : 0.0;
+    const unsigned long long summask = (1ull << D::NS) - 1;  // then 3 maxima (and padding)
+    const double r = wave_reduce_transpose<P>(v, summask);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    constexpr int LS = 64 / P;  // lanes per value
+    if (lane % LS == 0) shr[wid][lane / LS] = r;
+    __syncthreads();
+    if (threadIdx.x < D::NX) {
+        const int k = threadIdx.x;
+        double x = shr[0][k];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; w++) x = k < D::NS ? x + shr[w][k] : fmax(x, shr[w][k]);
+        a.diag[((size_t)iz * gridDim.x + pb.bx) * D::NX + k] = x;
     }
 }
 
@@ -892,7 +984,7 @@ hipError_t launch_flux(hipStream_t s, const Mesh<T>& m, const FluxArgs<T>& a, in
     {                                                                                                          \
         constexpr int ty = flux_ty<T, SCH, VIS>();                                                             \
         const size_t lds = kFluxMeshArrays * (size_t)m.nz * sizeof(T);                                       \
-        const FluxGrid fg = flux_grid(m.nx, m.ny, m.nz, ty, flux_slots<T, SCH, VIS, TRB>(lds), SCH == TC_SCHEME_VV ? 5 : 3);               \
+        const FluxGrid fg = flux_grid(m.nx, m.ny, m.nz, ty, flux_slots<T, SCH, VIS, TRB>(lds), 2 * flux_rz<T, SCH>() + 1);               \
         if (nblocks) *nblocks = fg.nblocks;                                                                    \
         hipLaunchKernelGGL((k_flux<T, SCH, VIS, TRB>), dim3(fg.nblocks), dim3(flux_threads<T, SCH, VIS>()), lds, s, m, a, fg.ntx, \
                            fg.ntiles, fg.zchunk);                                                              \
