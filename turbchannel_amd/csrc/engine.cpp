@@ -341,7 +341,7 @@ int setup_device(tc_engine* e) {
     if (int r = upload_as<T>(e, e->d_kerg, hm.KERG.data(), e->nz)) return r;
     // wall-normal factors
     const ZFactors& zf = e->zf;
-    const size_t fz = (size_t)e->nz * e->nm;
+    const size_t fz = (size_t)e->nz * zf.ncols;
     const int FW = zfac_fw(zf.W), UW = zfac_uw(zf.W);
     // packed per row: F[i][FW] = (M1..M_W, D), U[i][UW] = (U1..U_W), i = iz * ncols + col
     auto pack = [&](auto* F, auto* U, size_t n, auto idx) {
@@ -377,9 +377,9 @@ int setup_device(tc_engine* e) {
         HIPCHK(hipMemcpy(e->d_lowmode, low.data(), low.size() * sizeof(int), hipMemcpyHostToDevice));
         const size_t lz = (size_t)e->nz * e->nlow;
         std::vector<double> lf((FW + UW) * lz);
-        const int nlow = e->nlow, nm = e->nm;
+        const int nlow = e->nlow, ncols = zf.ncols, ny = e->ny;
         pack(lf.data(), lf.data() + FW * lz, lz,
-             [&](size_t i) { return (i / nlow) * nm + low[i % nlow]; });
+             [&](size_t i) { return (i / nlow) * ncols + zfac_col(low[i % nlow], ny); });
         if (int r = dalloc(e, &e->zfac_low, lf.size())) return r;
         HIPCHK(hipMemcpy(e->zfac_low, lf.data(), lf.size() * sizeof(double), hipMemcpyHostToDevice));
         if (int r = dalloc(e, &e->ylow, 2 * lz)) return r;
@@ -404,8 +404,10 @@ ZFacDev<T> zfac_dev(tc_engine* e) {
     ZFacDev<T> f;
     f.W = e->zf.W;
     f.nz = e->nz;
-    f.ncols = e->nm;
-    const size_t fz = (size_t)e->nz * e->nm;
+    f.ncols = e->zf.ncols;
+    f.nmodes = e->nm;
+    f.ny = e->ny;
+    const size_t fz = (size_t)e->nz * f.ncols;
     f.F = (const T*)e->zfac;
     f.U = f.F + zfac_fw(f.W) * fz;
     return f;

@@ -161,12 +161,12 @@ std::string build_mesh(const tc_params& p, HostMesh& m) {
 // factor is the reference's, so substitution with them reproduces its solves.
 // ---------------------------------------------------------------------------
 std::string build_zfactors(const tc_params& p, const HostMesh& m, ZFactors& f) {
-    const int nx = p.nx, ny = p.ny, nz = p.nz, nxh = nx / 2 + 1;
-    const int nm = nxh * ny;
+    const int nx = p.nx, ny = p.ny, nz = p.nz, nxh = nx / 2 + 1, nyh = ny / 2 + 1;
+    const int nm = nxh * nyh;  // columns (kx, |ky|)
     const int W = p.scheme == TC_SCHEME_VV ? 3 : 1;
     f.W = W;
     f.nz = nz;
-    f.nmodes = nm;
+    f.ncols = nm;
     for (int k = 0; k < 3; k++) {
         f.M[k].assign(k < W ? (size_t)nz * nm : 0, 0.0);
         f.U[k].assign(k < W ? (size_t)nz * nm : 0, 0.0);
@@ -177,7 +177,7 @@ std::string build_zfactors(const tc_params& p, const HostMesh& m, ZFactors& f) {
     const double* A = m.MAT_A.data();
     std::string err;
     for (int mode = 0; mode < nm; mode++) {
-        const int ix = mode / ny, iy = mode % ny;  // spectrum order: ky fastest
+        const int ix = mode / nyh, iy = mode % nyh;  // column (kx, |ky|), ky <= ny / 2
         auto at = [&](int iz) { return (size_t)iz * nm + mode; };
         if (p.scheme == TC_SCHEME_HW || p.scheme == TC_SCHEME_AB3) {
             const double cphi = std::cos(2. * kPi * ix / nx), cpsi = std::cos(2. * kPi * iy / ny);

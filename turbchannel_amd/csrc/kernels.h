@@ -152,9 +152,15 @@ hipError_t launch_spec_crop(hipStream_t s, const Cplx<T>* fine, int nxh, int ny,
 // load instructions: F[iz][col][FW] = (M1, M2, M3, D) for W = 3, (M1, D) for
 // W = 1; U[iz][col][UW] = (U1, U2, U3) or (U1). col = mode (or index in the
 // low-mode list).
+// Column of the z-solve factors of spectrum mode kx * ny + ky (ny even).
+__host__ __device__ inline int zfac_col(int mode, int ny) {
+    const int kx = mode / ny, ky = mode - kx * ny;
+    return kx * (ny / 2 + 1) + (ky <= ny - ky ? ky : ny - ky);
+}
 template <typename CT>
 struct ZFacDev {
-    int W = 1, nz = 0, ncols = 0;
+    int W = 1, nz = 0, ncols = 0;  // factor columns
+    int nmodes = 0, ny = 0;        // spectrum modes, and modes per kx (columns by zfac_col)
     const CT* F = nullptr;
     const CT* U = nullptr;
 };

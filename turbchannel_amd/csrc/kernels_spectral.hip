@@ -613,7 +613,7 @@ template <typename T, int W>
 __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFacDev<double> flow, const int* lowidx,
                                                 const int* lowmode, int nlow, double* ylow, ZNorm norm, int nbulk) {
     constexpr int FW = zfac_fw(W), UW = zfac_uw(W);
-    const int nm = f.ncols, nz = f.nz;
+    const int nm = f.nmodes, nz = f.nz;
     const size_t xs = 2 * (size_t)nm;  // stride between planes, in reals
     if ((int)blockIdx.x >= nbulk) {    // FP64 low-mode blocks
         const int t = (blockIdx.x - nbulk) * blockDim.x + threadIdx.x;
@@ -630,10 +630,12 @@ __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFac
     if (mode >= nm) return;
     if (lowidx && lowidx[mode] >= 0) return;  // solved in FP64 by the low-mode blocks
     T* x = reinterpret_cast<T*>(spec) + 2 * mode + part;
+    const int col = zfac_col(mode, f.ny);  // the factors of ky and ny - ky are one column
+    const size_t nc = f.ncols;
     // batches of 8 rows in flight: NB - 1 (FP64 lanes hold 5 reals per row of W = 3)
     constexpr int NB = 3;
-    lane_solve<T, T, W, 8, NB>(x, xs, f.F + (size_t)mode * FW, (size_t)nm * FW, f.U + (size_t)mode * UW,
-                               (size_t)nm * UW, x, xs, nz, mode == 0 && part == 0 && norm.s00 != nullptr, norm);
+    lane_solve<T, T, W, 8, NB>(x, xs, f.F + (size_t)col * FW, nc * FW, f.U + (size_t)col * UW, nc * UW, x, xs, nz,
+                               mode == 0 && part == 0 && norm.s00 != nullptr, norm);
 }
 
 // Forward x transform of one component (cb < 0) or of the product of two
@@ -758,7 +760,7 @@ hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cpl
 template <typename T>
 hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, const ZFacDev<double>& flow,
                          const int* lowidx, const int* lowmode, int nlow, double* ylow, ZNorm norm) {
-    const int nbulk = (2 * f.ncols + 63) / 64;
+    const int nbulk = (2 * f.nmodes + 63) / 64;
     const int nlowb = lowidx ? (2 * nlow + 63) / 64 : 0;
     const dim3 grid(nbulk + nlowb);
     if (f.W == 3)

@@ -462,7 +462,6 @@ __global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>()), (flux_min_waves<
     constexpr int TY = flux_ty<T, SCHEME, VISC>();
     constexpr int RXY = SCHEME == TC_SCHEME_VV ? 3 : 1;
     constexpr int RZ = flux_rz<T, SCHEME>();
-    constexpr bool COLUMN = SCHEME == TC_SCHEME_VV && RZ < 3;  // z+-3 of the own column from global memory
     constexpr int NS = 2 * RZ + 2;  // planes iz-RZ..iz+RZ plus the one in flight
     using Acc = RingAcc<T, RXY, RZ, TY>;
     constexpr int W = Acc::W, PS = Acc::PS;

@@ -28,13 +28,16 @@ std::string build_mesh(const tc_params& p, HostMesh& m);
 int solve_banded(int N, int NB, double* M, double* x, double tiny);
 
 // Precomputed elimination of the wall-normal pressure systems, one column per
-// (kx, ky) mode with kx in [0, nx/2], ky in [0, ny). W = bandwidth (1 or 3).
+// (kx, |ky|): kx in [0, nx/2], ky in [0, ny/2] (the system of ky and ny - ky is
+// the same: it depends on ky only through cos(2 pi ky / ny), or sin(pi ky / ny)
+// for the fourth-order scheme); zfac_col maps a mode to its column.
+// W = bandwidth (1 or 3).
 // Forward:  x[i] = (((x[i] + M3 x[i-3]) + M2 x[i-2]) + M1 x[i-1]) * D[i]
 // Backward: x[i] -= U1 x[i+1];  x[i] -= U2 x[i+2];  x[i] -= U3 x[i+3]
 // Arrays are [coef][iz][mode]. Mode-independent M1 of the tridiagonal solvers is
 // stored per mode too, to keep one kernel.
 struct ZFactors {
-    int W = 1, nz = 0, nmodes = 0;
+    int W = 1, nz = 0, ncols = 0;
     std::vector<double> M[3], D, U[3];
     // normalisation of the (0,0) mode in spectral space: X00[iz] -= psum * C[iz],
     // psum = sum_iz Wt[iz] * X00[iz]
