@@ -625,7 +625,9 @@ __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFac
                                         ylow + 2 * li + part, 2 * (size_t)nlow, nz, mode == 0 && part == 0, norm);
         return;
     }
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-contiguous block order: the blocks of one kx line run on one XCD, so
+    // modes ky and ny - ky (one factor column) read the factors through one L2
+    const int t = xcd_swizzle(blockIdx.x, nbulk) * blockDim.x + threadIdx.x;
     const int mode = t >> 1, part = t & 1;
     if (mode >= nm) return;
     if (lowidx && lowidx[mode] >= 0) return;  // solved in FP64 by the low-mode blocks
