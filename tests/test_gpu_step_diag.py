@@ -17,11 +17,15 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-11
 TOL_ORDER = 1e-13
 
-# the parity cases, plus planes of several 256-node blocks with a ragged last one
-NAMES = ["hw", "hw_les", "vv", "vv_les", "vv_odd", "ab3", "ab3_les", "ab3_stab0", "hw_wide", "ab3_wide"]
+# the parity cases, plus planes of several 256-cell chunks with a ragged last one,
+# and of more than one 1024-cell block (4 cells per thread) with a ragged last block
+NAMES = ["hw", "hw_les", "vv", "vv_les", "vv_odd", "ab3", "ab3_les", "ab3_stab0", "hw_wide", "ab3_wide", "vv_wide2",
+         "ab3_wide2"]
 WIDE = {
     "hw_wide": (tca.SCHEME_HW, 48, 20, 12, dict(turb_visc=1, visc_type=1)),
     "ab3_wide": (tca.SCHEME_AB3, 40, 24, 14, dict(stab_type=1, fd_mode=1, visc_type=0)),
+    "vv_wide2": (tca.SCHEME_VV, 50, 24, 12, dict()),
+    "ab3_wide2": (tca.SCHEME_AB3, 50, 24, 12, dict(stab_type=0, fd_mode=1, visc_type=1, turb_visc=1)),
 }
 
 
