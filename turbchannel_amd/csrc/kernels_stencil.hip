@@ -128,10 +128,10 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // K1: explicit term (+ BDF3/EX3 right-hand side), 2.5D-blocked.
 // A block owns a kTX x TY tile of (x, y) columns and marches along z over a
 // chunk of planes. The velocity planes iz-RZ..iz+RZ of the tile plus a halo of
-// RXY cells sit in an LDS ring (one component per sub-array, so a wavefront
-// reads consecutive words); the fourth-order scheme's z+-3 values of the own
-// column come straight from global memory. Blocks are ordered so that each
-// XCD works on neighbouring tiles and shares their halos in its L2.
+// RXY cells sit in an LDS ring (real3 AoS); RZ covers the stencils' reach in z
+// (3 for the fourth-order scheme, whose z+-3 terms read the cell's own column
+// only). Blocks are ordered so that each XCD works on neighbouring tiles and
+// shares their halos in its L2.
 // ---------------------------------------------------------------------------
 constexpr int kTX = 32;
 constexpr int kFluxMeshArrays = 10;  // DZ, DZW, GCZ, cellh and the six per-plane quotients
@@ -180,8 +180,8 @@ struct GlobalNu {
     }
 };
 
-// The stencils reach beyond the ring in z only on the cell's own column
-// (the fourth-order scheme's z+-3 values), held in registers.
+// Ring accessor. With a ring narrower than the fourth-order reach (RZ < 3)
+// the own column's values at dz = +-(RZ+1) come from registers (cp, cm).
 template <typename T, int RXY, int RZ, int TY>
 struct RingAcc {
     static constexpr int W = kTX + 2 * RXY, H = TY + 2 * RXY, PS = W * H;
