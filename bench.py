@@ -142,13 +142,37 @@ def bench_training(args):
     print(json.dumps(rec))
 
 
-def cpu_baseline(name, seconds_target=15.0):
+def device_copy_rate(nbytes=256 << 20, reps=20):
+    """Read + write rate of a plain device-to-device copy of nbytes (torch), the
+    practical ceiling of a streaming kernel on this GPU; None without torch."""
+    try:
+        import torch
+
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        del a, b
+        gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+        return {"bytes": nbytes, "gbs": round(gbs, 1), "frac_of_peak": round(gbs / PEAK_HBM_GBS, 4)}
+    except Exception as exc:  # no torch device: reported, not fatal
+        return {"error": repr(exc)}
+
+
+def cpu_baseline(name, seconds_target=15.0, threads=None):
     """FP64 OpenMP oracle on the host cores, on a bounded number of steps."""
     import oracle
 
     scheme, nx, ny, nz, zz, kw, dt = problem(name, tca.FP64)
     kw.pop("precision")
-    threads = min(os.cpu_count() or 1, 16)
+    threads = threads or min(os.cpu_count() or 1, 16)
     oracle.set_threads(threads)
     srk = scheme == tca.SCHEME_SRK
     method = kw.pop("imex", 2)
@@ -252,6 +276,7 @@ def main():
     if args.workload == TRAIN_WORKLOAD:
         bench_training(args)
         return
+    copy = device_copy_rate()  # before the engine: torch sees the GPU only if it starts HIP first
     precision = tca.FP64 if args.precision == "fp64" else tca.FP32_BULK
     scheme, nx, ny, nz, zz, kw, dt = problem(args.workload, precision)
     n3d = nx * ny * nz
@@ -348,9 +373,11 @@ def main():
         closure = {"kernel": "k_sgs_tau", "avg_us": round(avg_s * 1e6, 2), "bound": "mfma",
                    "achieved": round(tf, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                    "frac": round(tf / PEAK_BF16_TFLOPS, 5)}
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.workload, args.cpu_seconds)
+        # the reference's own setting: one OpenMP thread (staggered/main.cpp:279)
+        cpu1 = cpu_baseline(args.workload, min(args.cpu_seconds, 5.0), threads=1)
     rec = {
         "metric": "grid-pt steps/s",
         "value": value,
@@ -374,6 +401,8 @@ def main():
                      "traffic": traffic["bytes"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None},
         "cpu_baseline": cpu,
+        "cpu_baseline_one_thread": cpu1,
+        "device_copy": copy,
         "closure": closure,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "with_diagnostics": {"value": n3d * nd / el_diag, "ms_per_step": 1e3 * el_diag / nd, "steps": nd},
