@@ -217,7 +217,11 @@ __device__ __forceinline__ void store_half_spectra(const Cplx<T>* res, const FFT
         const Cplx<T> out = (rr & 1) ? Cplx<T>{T(0.5) * (z.y + zc.y), T(-0.5) * (z.x - zc.x)}
                                      : Cplx<T>{T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
         const int iz = dny.div(row), iy = row - iz * ny;
-        spec[((size_t)iz * nxh + k) * ny + iy] = out;
+        // streaming store: the spectrum is not read again by this kernel, so it
+        // should not displace the rows whose y and z taps other blocks still read
+        T* sp = reinterpret_cast<T*>(spec + ((size_t)iz * nxh + k) * ny + iy);
+        __builtin_nontemporal_store(out.x, sp);
+        __builtin_nontemporal_store(out.y, sp + 1);
     }
 }
 
