@@ -736,11 +736,13 @@ __device__ __forceinline__ void project_eval(const Mesh<T>& m, const ProjectArgs
 template <typename T, int SCHEME, bool STAB, bool DIAG>
 __device__ __forceinline__ void project_store(const ProjectArgs<T>& a, const CellProj<T>& c, DiagSums<SCHEME>* acc,
                                               const DiagPlane& dp) {
+    // streaming stores: the new fields are not read again here, so they should not
+    // displace the p and r other blocks still read as stencil taps
     T* o = a.vnext + 3 * c.in;
-    o[0] = c.u;
-    o[1] = c.v;
-    o[2] = c.w;
-    if (STAB) a.pnext[c.in] = c.pn;
+    __builtin_nontemporal_store(c.u, o);
+    __builtin_nontemporal_store(c.v, o + 1);
+    __builtin_nontemporal_store(c.w, o + 2);
+    if (STAB) __builtin_nontemporal_store(c.pn, a.pnext + c.in);
     if (DIAG) {
         const double wc = SCHEME != TC_SCHEME_AB3 ? 0.5 * ((double)c.w + (double)c.wup) : (double)c.w;
         acc->add(c.u, c.v, wc, c.w, a.nut != nullptr, c.nt, c.ps, c.vc, dp.ihx, dp.ihy, dp.inv_hz, dp.kv);
