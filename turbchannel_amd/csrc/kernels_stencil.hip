@@ -883,9 +883,9 @@ __global__ __launch_bounds__(kBlock) void k_srk_vec(Mesh<T> m, SrkVec<T> a) {
     T kc[3] = {T(0), T(0), T(0)};
     if (a.KC) {
         const T* k = a.KC + 3 * in;
-        kc[0] = k[0];
-        kc[1] = k[1];
-        kc[2] = k[2];
+        kc[0] = load_once(k);
+        kc[1] = load_once(k + 1);
+        kc[2] = load_once(k + 2);
     }
     if (GRAD) {
         T g[3];
@@ -897,24 +897,24 @@ __global__ __launch_bounds__(kBlock) void k_srk_vec(Mesh<T> m, SrkVec<T> a) {
     T pj = T(0), qn = T(0);
     if (a.pj_out || a.qj_out) {
         pj = a.S2 ? a.S[in] + a.S2[in] : a.S[in];
-        if (a.qj_out) qn = a.inv_tau_stage * (pj - a.pjhat[in]);
+        if (a.qj_out) qn = a.inv_tau_stage * (pj - load_once(a.pjhat + in));
     }
     T o[3] = {T(0), T(0), T(0)};
     if (a.out) {
         if (a.X) {
             const T* x = a.X + 3 * in;
-            o[0] = x[0];
-            o[1] = x[1];
-            o[2] = x[2];
+            o[0] = load_once(x);
+            o[1] = load_once(x + 1);
+            o[2] = load_once(x + 2);
         }
 #pragma unroll
         for (int j = 0; j < kMaxImexStages; j++)
             if (j < a.nk) {
                 const T* k = a.K[j] + 3 * in;
                 const T cj = a.c[j];
-                o[0] += cj * k[0];
-                o[1] += cj * k[1];
-                o[2] += cj * k[2];
+                o[0] += cj * load_once(k);
+                o[1] += cj * load_once(k + 1);
+                o[2] += cj * load_once(k + 2);
             }
         if (a.KC) {
             o[0] += a.ckc * kc[0];
@@ -923,31 +923,30 @@ __global__ __launch_bounds__(kBlock) void k_srk_vec(Mesh<T> m, SrkVec<T> a) {
         }
         if (a.A) {
             const T* x = a.A + 3 * in;
-            o[0] += a.sa * x[0];
-            o[1] += a.sa * x[1];
-            o[2] += a.sa * x[2];
+            o[0] += a.sa * load_once(x);
+            o[1] += a.sa * load_once(x + 1);
+            o[2] += a.sa * load_once(x + 2);
         }
         if (a.B) {
             const T* x = a.B + 3 * in;
-            o[0] += a.sb * x[0];
-            o[1] += a.sb * x[1];
-            o[2] += a.sb * x[2];
+            o[0] += a.sb * load_once(x);
+            o[1] += a.sb * load_once(x + 1);
+            o[2] += a.sb * load_once(x + 2);
         }
     }
+    // streaming stores: nothing here reads them again (the gradient taps of S stay in L2)
     if (a.KCout) {
         T* k = a.KCout + 3 * in;
-        k[0] = kc[0];
-        k[1] = kc[1];
-        k[2] = kc[2];
+#pragma unroll
+        for (int c = 0; c < 3; c++) __builtin_nontemporal_store(kc[c], k + c);
     }
     if (a.out) {
         T* x = a.out + 3 * in;
-        x[0] = o[0];
-        x[1] = o[1];
-        x[2] = o[2];
+#pragma unroll
+        for (int c = 0; c < 3; c++) __builtin_nontemporal_store(o[c], x + c);
     }
-    if (a.pj_out) a.pj_out[in] = pj;
-    if (a.qj_out) a.qj_out[in] = qn;
+    if (a.pj_out) __builtin_nontemporal_store(pj, a.pj_out + in);
+    if (a.qj_out) __builtin_nontemporal_store(qn, a.qj_out + in);
 }
 
 template <typename T, bool STAB, bool CK>
