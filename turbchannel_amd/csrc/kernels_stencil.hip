@@ -5,6 +5,7 @@
 // (the near-wall closures of the fourth-order scheme) is uniform across a
 // wavefront and every mesh coefficient is a scalar load.
 #include <algorithm>
+#include <type_traits>
 
 #include "diag.h"
 #include "kernels.h"
@@ -190,6 +191,10 @@ struct GlobalNu {
     }
 };
 
+// The unit of the LDS ring fill: one real (4- or 8-byte loads and stores).
+template <typename T>
+using RingWord = typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type;
+
 // Ring accessor. With a ring narrower than the fourth-order reach (RZ < 3)
 // the own column's values at dz = +-(RZ+1) come from registers (cp, cm).
 template <typename T, int RXY, int RZ, int TY>
@@ -260,12 +265,14 @@ __device__ __forceinline__ void flux_planes(const Mesh<T>& m, const FluxArgs<T>&
     constexpr int W = Acc::W, PS = Acc::PS;
     constexpr int NC = flux_nc<T, SCHEME>();
     constexpr int NT = kTX * TY * NC;
-    constexpr int DPC = 3 * (int)sizeof(T) / 4;  // dwords per real3 cell
-    constexpr int DPS = PS * DPC;                // dwords per plane of the padded tile
+    using Word = RingWord<T>;                   // one real: a fill moves whole reals
+    constexpr int DPC = 3;                      // words per real3 cell
+    constexpr int DPS = PS * DPC;               // words per plane of the padded tile
     constexpr int ND = (DPS + NT - 1) / NT;
-    // slot stride: every lane writes ND dwords per plane fill (the surplus into
-    // padding), so the fill is the same instruction stream for every plane
-    constexpr int SLOT = ND * NT * 4 / (int)sizeof(T);  // reals per ring slot
+    // slot: the plane's words, then 64 spare words that the lanes past the plane
+    // write (one per lane position, so no two lanes of a wave collide); every
+    // lane writes ND words per fill, the same instruction stream for every plane
+    constexpr int SLOT = (DPS + 64 + 1) / 2 * 2;  // reals per ring slot (16-byte multiple)
     const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tile = lb % ntiles, chunk = lb / ntiles;
     const int x0 = (tile % ntx) * kTX, y0 = (tile / ntx) * TY;
@@ -278,9 +285,9 @@ __device__ __forceinline__ void flux_planes(const Mesh<T>& m, const FluxArgs<T>&
     Nb nb;  // global neighbours for nu_t and the collocated pressure gradient
     make_nb<1>(nb, active ? ix : 0, active ? iy : 0, 0, m.nx, m.ny);
 
-    // The padded tile of a plane is copied into LDS by direct global->LDS dword
-    // loads: LDS dword d of the slot (cell d / DPC, word d % DPC of the real3 AoS
-    // cell) comes from the cell's periodic image; each lane holds the global dword
+    // The padded tile of a plane is copied into LDS word by word (a word is one
+    // real): word d of the slot (cell d / DPC, component d % DPC of the real3 AoS
+    // cell) comes from the cell's periodic image; each lane holds the global word
     // offsets of its ND words within a plane, the same for every plane.
     int gdw[ND];
 #pragma unroll
@@ -294,21 +301,24 @@ __device__ __forceinline__ void flux_planes(const Mesh<T>& m, const FluxArgs<T>&
         if (gy < 0) gy += m.ny;
         gdw[k] = (gy * m.nx + gx) * DPC + wd;
     }
-    // A plane fill is ND dword loads per lane into registers, written to the
+    // A plane fill is ND word loads per lane into registers, written to the
     // ring slot after the plane in between has been computed: the loads in
     // flight target registers, so no LDS read of the computation waits for them.
-    // Lanes past the plane re-read a valid word into the slot's padding.
-    unsigned pf[ND];
+    // Lanes past the plane re-read a valid word into the slot's spare words.
+    Word pf[ND];
     auto fetch = [&](int p) {  // plane min(p, nz-1) -> registers
-        const unsigned* __restrict__ src =
-            reinterpret_cast<const unsigned*>(a.vel) + (size_t)min(p, m.nz - 1) * m.nxy * DPC;
+        const Word* __restrict__ src =
+            reinterpret_cast<const Word*>(a.vel) + (size_t)min(p, m.nz - 1) * m.nxy * DPC;
 #pragma unroll
         for (int k = 0; k < ND; k++) pf[k] = src[gdw[k]];
     };
     auto commit = [&](int p) {  // registers -> the ring slot of plane p
-        unsigned* dst = reinterpret_cast<unsigned*>(ring + (p % NS) * SLOT);
+        Word* dst = reinterpret_cast<Word*>(ring + (p % NS) * SLOT);
 #pragma unroll
-        for (int k = 0; k < ND; k++) dst[threadIdx.x + k * NT] = pf[k];
+        for (int k = 0; k < ND; k++) {
+            const int d = threadIdx.x + k * NT;
+            dst[(k + 1) * NT <= DPS || d < DPS ? d : DPS + (threadIdx.x & 63)] = pf[k];
+        }
     };
     for (int p = max(0, z0 - RZ); p <= min(m.nz - 1, z0 + RZ); p++) {
         fetch(p);
@@ -477,12 +487,14 @@ __global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>()), (flux_min_waves<
     constexpr int W = Acc::W, PS = Acc::PS;
     constexpr int NC = flux_nc<T, SCHEME>();
     constexpr int NT = kTX * TY * NC;
-    constexpr int DPC = 3 * (int)sizeof(T) / 4;  // dwords per real3 cell
-    constexpr int DPS = PS * DPC;                // dwords per plane of the padded tile
+    using Word = RingWord<T>;                   // one real: a fill moves whole reals
+    constexpr int DPC = 3;                      // words per real3 cell
+    constexpr int DPS = PS * DPC;               // words per plane of the padded tile
     constexpr int ND = (DPS + NT - 1) / NT;
-    // slot stride: every lane writes ND dwords per plane fill (the surplus into
-    // padding), so the fill is the same instruction stream for every plane
-    constexpr int SLOT = ND * NT * 4 / (int)sizeof(T);  // reals per ring slot
+    // slot: the plane's words, then 64 spare words that the lanes past the plane
+    // write (one per lane position, so no two lanes of a wave collide); every
+    // lane writes ND words per fill, the same instruction stream for every plane
+    constexpr int SLOT = (DPS + 64 + 1) / 2 * 2;  // reals per ring slot (16-byte multiple)
     __shared__ __align__(16) T ring[NS * SLOT];
     __shared__ double sh[NT / 64];
     // The mesh arrays the stencils read, mirrored in LDS (dynamic, kFluxMeshArrays
