@@ -103,6 +103,7 @@ struct tc_engine {
     double* d_prof = nullptr;   // profiles result (nz * 12)
     double* d_diag = nullptr;   // fused diagnostics, nz records
     double* h_diag = nullptr;   // pinned host copy of d_diag (hipHostMalloc: the per-step copy skips staging)
+    double* hd_diag = nullptr;  // h_diag's device address: the step's diagnostics land there directly
     // tc_step also forms the diagnostics of the new fields in its projection
     // (tc_step_diagnostics); d_diag then holds those of velocity (and pressure)
     // slot diag_slot until a call changes a field, or -1
@@ -395,7 +396,8 @@ int setup_device(tc_engine* e) {
     if (int r = dalloc(e, &e->d_prof, (size_t)e->nz * 12)) return r;
     if (int r = dalloc(e, &e->d_diag, (size_t)e->nz * diag_record(e->P.scheme))) return r;
     HIPCHK(hipHostMalloc((void**)&e->h_diag, (size_t)e->nz * diag_record(e->P.scheme) * sizeof(double),
-                         hipHostMallocDefault));
+                         hipHostMallocMapped));
+    HIPCHK(hipHostGetDevicePointer((void**)&e->hd_diag, e->h_diag, 0));
     return TC_OK;
 }
 
@@ -528,7 +530,7 @@ int step(tc_engine* e, double tau, int cur, int order) {
     }
     if (e->step_diag) {
         pa.diag = e->d_partial;
-        pa.diag_out = e->d_diag;
+        pa.diag_out = e->hd_diag;  // the final reduction writes the host buffer: no copy per step
         pa.visc_fac = diag_visc_fac(e->P.scheme, e->P.visc_type);
         pa.nut = (const T*)e->nut;
     }
@@ -1180,7 +1182,8 @@ int tc_diagnostics(tc_engine* e, int vel_slot, int p_slot, double tau, int cfl_m
     if (!ab3 && !p) return fail(TC_ERR_ARG, "bad pressure slot");
     if (host_av && count != (size_t)nv * e->nz) return fail(TC_ERR_ARG, "count must be nz*" + std::to_string(nv));
     if (cfl && (cfl_mode < 1 || cfl_mode > 3)) return fail(TC_ERR_ARG, "cfl_mode must be 1, 2 or 3");
-    // the step's own record when it holds these fields, else one pass now
+    // the step's own record (already in h_diag once the stream is done) when it
+    // holds these fields, else one pass now and a copy
     if (!(e->diag_slot == vel_slot && (ab3 || p_slot == vel_slot))) {
         e->diag_slot = -1;
         const double visc_fac = diag_visc_fac(e->P.scheme, e->P.visc_type);
@@ -1190,8 +1193,8 @@ int tc_diagnostics(tc_engine* e, int vel_slot, int p_slot, double tau, int cfl_m
         else
             HIPCHK(launch_diag<double>(e->s, e->mesh<double>(), e->P.scheme, (const double*)v, (const double*)p,
                                        (const double*)e->nut, visc_fac, e->d_partial, e->d_diag));
+        HIPCHK(hipMemcpyAsync(e->h_diag, e->d_diag, (size_t)e->nz * nrec * sizeof(double), hipMemcpyDeviceToHost, e->s));
     }
-    HIPCHK(hipMemcpyAsync(e->h_diag, e->d_diag, (size_t)e->nz * nrec * sizeof(double), hipMemcpyDeviceToHost, e->s));
     HIPCHK(hipStreamSynchronize(e->s));
     // volumes as the device kernels see them: the mesh in the engine's precision
     auto r = [&](double x) { return e->fp32 ? (double)(float)x : x; };
