@@ -101,11 +101,12 @@ struct tc_engine {
     double* d_out = nullptr;    // small results (16 doubles)
     double* d_shift = nullptr;  // flow-rate shift
     double* d_prof = nullptr;   // profiles result (nz * 12)
-    double* d_diag = nullptr;   // fused diagnostics, nz records
-    double* h_diag = nullptr;   // pinned host copy of d_diag (hipHostMalloc: the per-step copy skips staging)
-    double* hd_diag = nullptr;  // h_diag's device address: the step's diagnostics land there directly
+    // fused diagnostics, nz records, in pinned host memory that the final
+    // reductions write through its device address hd_diag (no copy to the host)
+    double* h_diag = nullptr;
+    double* hd_diag = nullptr;
     // tc_step also forms the diagnostics of the new fields in its projection
-    // (tc_step_diagnostics); d_diag then holds those of velocity (and pressure)
+    // (tc_step_diagnostics); h_diag then holds those of velocity (and pressure)
     // slot diag_slot until a call changes a field, or -1
     bool step_diag = false;
     int diag_slot = -1;
@@ -394,7 +395,6 @@ int setup_device(tc_engine* e) {
     if (int r = dalloc(e, &e->d_out, 16)) return r;
     if (int r = dalloc(e, &e->d_shift, 4)) return r;
     if (int r = dalloc(e, &e->d_prof, (size_t)e->nz * 12)) return r;
-    if (int r = dalloc(e, &e->d_diag, (size_t)e->nz * diag_record(e->P.scheme))) return r;
     HIPCHK(hipHostMalloc((void**)&e->h_diag, (size_t)e->nz * diag_record(e->P.scheme) * sizeof(double),
                          hipHostMallocMapped));
     HIPCHK(hipHostGetDevicePointer((void**)&e->hd_diag, e->h_diag, 0));
@@ -530,7 +530,7 @@ int step(tc_engine* e, double tau, int cur, int order) {
     }
     if (e->step_diag) {
         pa.diag = e->d_partial;
-        pa.diag_out = e->hd_diag;  // the final reduction writes the host buffer: no copy per step
+        pa.diag_out = e->hd_diag;
         pa.visc_fac = diag_visc_fac(e->P.scheme, e->P.visc_type);
         pa.nut = (const T*)e->nut;
     }
@@ -1182,18 +1182,17 @@ int tc_diagnostics(tc_engine* e, int vel_slot, int p_slot, double tau, int cfl_m
     if (!ab3 && !p) return fail(TC_ERR_ARG, "bad pressure slot");
     if (host_av && count != (size_t)nv * e->nz) return fail(TC_ERR_ARG, "count must be nz*" + std::to_string(nv));
     if (cfl && (cfl_mode < 1 || cfl_mode > 3)) return fail(TC_ERR_ARG, "cfl_mode must be 1, 2 or 3");
-    // the step's own record (already in h_diag once the stream is done) when it
-    // holds these fields, else one pass now and a copy
+    // the step's own record when it holds these fields, else one pass now; either
+    // final reduction writes h_diag through its device mapping (no copy)
     if (!(e->diag_slot == vel_slot && (ab3 || p_slot == vel_slot))) {
         e->diag_slot = -1;
         const double visc_fac = diag_visc_fac(e->P.scheme, e->P.visc_type);
         if (e->fp32)
             HIPCHK(launch_diag<float>(e->s, e->mesh<float>(), e->P.scheme, (const float*)v, (const float*)p,
-                                      (const float*)e->nut, visc_fac, e->d_partial, e->d_diag));
+                                      (const float*)e->nut, visc_fac, e->d_partial, e->hd_diag));
         else
             HIPCHK(launch_diag<double>(e->s, e->mesh<double>(), e->P.scheme, (const double*)v, (const double*)p,
-                                       (const double*)e->nut, visc_fac, e->d_partial, e->d_diag));
-        HIPCHK(hipMemcpyAsync(e->h_diag, e->d_diag, (size_t)e->nz * nrec * sizeof(double), hipMemcpyDeviceToHost, e->s));
+                                       (const double*)e->nut, visc_fac, e->d_partial, e->hd_diag));
     }
     HIPCHK(hipStreamSynchronize(e->s));
     // volumes as the device kernels see them: the mesh in the engine's precision
