@@ -148,10 +148,13 @@ int tc_diagnostics(tc_engine* e, int vel_slot, int p_slot, double tau, int cfl_m
  * more times; here the values are summed while they are stored), and a
  * following tc_diagnostics of the step's new velocity slot (and the same
  * pressure slot, staggered) copies them instead of reading the fields again.
- * Any call that may change a field or nu_t (tc_set_field, tc_field_device_ptr,
- * tc_srk_step, ...) drops the step's record; writes through a device pointer
- * obtained earlier are not seen. Off by default; refused on SRK engines. No
- * reference counterpart. */
+ * The records of the last two steps are kept (each with the nu_t of its own
+ * step), and tc_diagnostics waits only for the step whose record it reads, so a
+ * caller may launch step n+1 before reading step n's diagnostics; a third step
+ * replaces the older record. Any call that may change a field or nu_t
+ * (tc_set_field, tc_field_device_ptr, tc_srk_step, ...) drops both records;
+ * writes through a device pointer obtained earlier are not seen. Off by
+ * default; refused on SRK engines. No reference counterpart. */
 int tc_step_diagnostics(tc_engine* e, int on);
 
 /* Constant-flow-rate correction of one velocity field (no reference counterpart;

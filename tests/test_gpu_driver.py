@@ -253,3 +253,26 @@ def test_driver_srk_matches_oracle_replay(tmp_path):
     dudy = 1.5 * a[1, 0] / dz[0] - 0.5 * (a[2, 0] - a[1, 0]) / dz[1]
     assert float(m.group(1)) == pytest.approx(math.sqrt(kw["visc"] * max(dudy, 0)), abs=1.5e-6)
     assert float(m.group(2)) == pytest.approx(u_tau, abs=1e-6)
+
+
+# a narrow CFL band: the control changes tau on most steps, in both directions
+@pytest.mark.parametrize("args", [STAG_ARGS + ["--les", 1, "--CFLmax", 0.06, "--CFLmin", 0.05],
+                                  AB3_ARGS + ["--CFLmax", 0.3, "--CFLmin", 0.25]], ids=["vv", "ab3"])
+def test_driver_pipelined_loop_writes_the_same_files(tmp_path, args):
+    """Launching step n+1 before reading step n's diagnostics (and launching it
+    again when the step-size control changes tau) changes no output byte."""
+    common = [*args, "--seed", 3, "--tau", "2e-3", "-s", 1, "-f", 4, "-p", 3, "--TimeStepsMax", 11,
+              "--checkpoint-every", 5]
+    dirs = {}
+    for pipe in (0, 1):
+        d = tmp_path / f"pipe{pipe}"
+        d.mkdir()
+        r = run_driver([*common, "--pipeline", pipe], d)
+        assert r.returncode == 0, r.stdout + r.stderr
+        dirs[pipe] = d
+    taus = _log_rows(dirs[1] / "log.dat")[:, 2]
+    assert len(set(taus.tolist())) > 1  # the control changed the step
+    names = sorted(p.name for p in dirs[0].iterdir())
+    assert names == sorted(p.name for p in dirs[1].iterdir()) and "q00004.vtk" in names
+    for name in names:
+        assert (dirs[0] / name).read_bytes() == (dirs[1] / name).read_bytes(), name

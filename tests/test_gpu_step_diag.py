@@ -173,3 +173,27 @@ def test_step_diagnostics_refused_on_srk():
     eng = tca.Engine(tca.SCHEME_SRK, 16, 12, 14, zz, imex=2, alpha_tau=1.0, **kw)
     with pytest.raises(tca.TCError):
         eng.step_diagnostics(True)
+
+
+@pytest.mark.parametrize("name", ["vv", "hw_les", "ab3_les"])
+@pytest.mark.parametrize("latest_first", [False, True])
+def test_records_of_the_last_two_steps(name, latest_first):
+    """Each of the last two steps keeps the record it formed (with the nu_t of its
+    own step), so a caller may read a step's diagnostics after launching the next."""
+    sol, eng = make(name)
+    eng.step_diagnostics(True)
+    cur, tau = 0, 1e-3
+    load_history(sol, [eng], cur)
+    sol.step(0.0, tau, 0, 1)
+    exp1 = expected(sol, 1, tau, 3)
+    sol.step(tau, tau, 1, 2)
+    exp2 = expected(sol, 2, tau, 3)
+    eng.step(0.0, tau, 0, 1)
+    eng.step(tau, tau, 1, 2)
+    order = [(2, exp2), (1, exp1)] if latest_first else [(1, exp1), (2, exp2)]
+    for slot, exp in order * 2:  # each read leaves the records in place
+        assert_close(eng.diagnostics(slot, slot, tau, 3), exp, TOL)
+    eng.step(2 * tau, tau, 2, 3)  # the third step's record replaces the first's
+    sol.step(2 * tau, tau, 2, 3)
+    assert_close(eng.diagnostics(3, 3, tau, 3), expected(sol, 3, tau, 3), TOL)
+    assert_close(eng.diagnostics(2, 2, tau, 3), exp2, TOL)

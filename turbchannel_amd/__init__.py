@@ -250,7 +250,9 @@ class Engine:
 
     def step_diagnostics(self, on=True):
         """Form the diagnostics of each step's new fields in its projection
-        (tc_step_diagnostics); diagnostics() of that slot then reads them."""
+        (tc_step_diagnostics); diagnostics() of that slot then reads them. The
+        records of the last two steps are kept, and reading one waits only for
+        its own step."""
         _check(lib().tc_step_diagnostics(self._h, 1 if on else 0))
 
     def explicit_term(self, src, dst):

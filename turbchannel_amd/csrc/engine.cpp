@@ -101,15 +101,20 @@ struct tc_engine {
     double* d_out = nullptr;    // small results (16 doubles)
     double* d_shift = nullptr;  // flow-rate shift
     double* d_prof = nullptr;   // profiles result (nz * 12)
-    // fused diagnostics, nz records, in pinned host memory that the final
-    // reductions write through its device address hd_diag (no copy to the host)
-    double* h_diag = nullptr;
-    double* hd_diag = nullptr;
+    // diagnostics, nz records each, in pinned host memory that the final
+    // reductions write through its device address (no copy to the host):
+    // buffers 0 and 1 hold the records the last two steps formed (tc_step
+    // alternates), buffer 2 the record of a separate pass
+    double* h_diag[3] = {};
+    double* hd_diag[3] = {};
+    hipEvent_t diag_ev[2] = {};  // recorded after the step that wrote buffer 0 / 1
     // tc_step also forms the diagnostics of the new fields in its projection
-    // (tc_step_diagnostics); h_diag then holds those of velocity (and pressure)
-    // slot diag_slot until a call changes a field, or -1
+    // (tc_step_diagnostics); buffer b then holds those of velocity (and
+    // pressure) slot diag_rec[b] until a call changes a field, or -1
     bool step_diag = false;
-    int diag_slot = -1;
+    int diag_rec[2] = {-1, -1};
+    int diag_cur = 0;  // the buffer the next step writes
+    void drop_records() { diag_rec[0] = diag_rec[1] = -1; }
     // learned closure (collocated, sgs_model == 1)
     void* tau = nullptr;                // [node][6] closure stress
     unsigned short* sgs_w = nullptr;    // bfloat16 weight block
@@ -395,9 +400,12 @@ int setup_device(tc_engine* e) {
     if (int r = dalloc(e, &e->d_out, 16)) return r;
     if (int r = dalloc(e, &e->d_shift, 4)) return r;
     if (int r = dalloc(e, &e->d_prof, (size_t)e->nz * 12)) return r;
-    HIPCHK(hipHostMalloc((void**)&e->h_diag, (size_t)e->nz * diag_record(e->P.scheme) * sizeof(double),
-                         hipHostMallocMapped));
-    HIPCHK(hipHostGetDevicePointer((void**)&e->hd_diag, e->h_diag, 0));
+    for (int k = 0; k < 3; k++) {
+        HIPCHK(hipHostMalloc((void**)&e->h_diag[k], (size_t)e->nz * diag_record(e->P.scheme) * sizeof(double),
+                             hipHostMallocMapped));
+        HIPCHK(hipHostGetDevicePointer((void**)&e->hd_diag[k], e->h_diag[k], 0));
+    }
+    for (int k = 0; k < 2; k++) HIPCHK(hipEventCreateWithFlags(&e->diag_ev[k], hipEventDisableTiming));
     return TC_OK;
 }
 
@@ -530,7 +538,7 @@ int step(tc_engine* e, double tau, int cur, int order) {
     }
     if (e->step_diag) {
         pa.diag = e->d_partial;
-        pa.diag_out = e->hd_diag;
+        pa.diag_out = e->hd_diag[e->diag_cur];
         pa.visc_fac = diag_visc_fac(e->P.scheme, e->P.visc_type);
         pa.nut = (const T*)e->nut;
     }
@@ -891,7 +899,10 @@ int tc_destroy(tc_engine* e) {
     for (auto ev : e->tim.pool) hipEventDestroy(ev);
     for (auto& g : e->graphs) hipGraphExecDestroy(g.second);
     for (void* p : e->allocs) hipFree(p);
-    if (e->h_diag) hipHostFree(e->h_diag);
+    for (int k = 0; k < 3; k++)
+        if (e->h_diag[k]) hipHostFree(e->h_diag[k]);
+    for (int k = 0; k < 2; k++)
+        if (e->diag_ev[k]) hipEventDestroy(e->diag_ev[k]);
     if (e->s) hipStreamDestroy(e->s);
     delete e;
     return TC_OK;
@@ -907,7 +918,7 @@ int tc_sync(tc_engine* e) {
 
 int tc_set_field(tc_engine* e, int kind, int slot, const void* host, size_t bytes) {
     if (int r = check(e)) return r;
-    e->diag_slot = -1;
+    e->drop_records();
     void* d = field_ptr(e, kind, slot);
     if (!d || !host) return fail(TC_ERR_ARG, "bad field");
     const size_t need = ncomp(kind) * e->n3d * e->rsz;
@@ -932,7 +943,7 @@ int tc_field_device_ptr(tc_engine* e, int kind, int slot, void** dev_ptr) {
     if (!e || !dev_ptr) return fail(TC_ERR_ARG, "NULL argument");
     void* d = field_ptr(e, kind, slot);
     if (!d) return fail(TC_ERR_ARG, "bad field");
-    e->diag_slot = -1;  // the caller may write through it
+    e->drop_records();  // the caller may write through it
     *dev_ptr = d;
     return TC_OK;
 }
@@ -984,7 +995,8 @@ int tc_step(tc_engine* e, double t, double tau, int cur, int bdf_order) {
     if (bdf_order < 1 || bdf_order > 3) return fail(TC_ERR_ARG, "Wrong bdf_order");
     if (cur < 0 || cur > 3) return fail(TC_ERR_ARG, "bad current_field");
     if (!(tau > 0)) return fail(TC_ERR_ARG, "tau must be positive");
-    e->diag_slot = -1;
+    const int b = e->diag_cur;  // the record buffer this step writes; the other keeps the last step's
+    e->diag_rec[b] = -1;
     auto run = [&]() { return e->fp32 ? step<float>(e, tau, cur, bdf_order) : step<double>(e, tau, cur, bdf_order); };
     int r;
     if (!e->use_graphs || e->tim.on) {
@@ -992,9 +1004,19 @@ int tc_step(tc_engine* e, double t, double tau, int cur, int bdf_order) {
     } else {
         if (e->P.sgs_model && !e->sgs_ready)
             return fail(TC_ERR_STATE, "learned closure weights are not set (tc_set_sgs_params)");
-        r = run_graph(e, cur, bdf_order + 8 * e->step_diag, tau, run);
+        r = run_graph(e, cur, bdf_order + 8 * e->step_diag + 16 * b, tau, run);
     }
-    if (r == TC_OK && e->step_diag) e->diag_slot = (cur + 1) % 4;
+    if (r != TC_OK) {
+        e->drop_records();
+        return r;
+    }
+    if (e->step_diag) {
+        HIPCHK(hipEventRecord(e->diag_ev[b], e->s));
+        const int next = (cur + 1) % 4;
+        if (e->diag_rec[1 - b] == next) e->diag_rec[1 - b] = -1;  // that slot is rewritten
+        e->diag_rec[b] = next;
+        e->diag_cur = 1 - b;
+    }
     return r;
 }
 
@@ -1002,7 +1024,7 @@ int tc_step_diagnostics(tc_engine* e, int on) {
     if (int r = check(e)) return r;
     if (e->srk) return fail(TC_ERR_STATE, "the SRK solver has no fused step diagnostics");
     e->step_diag = on != 0;
-    e->diag_slot = -1;
+    e->drop_records();
     return TC_OK;
 }
 
@@ -1013,7 +1035,7 @@ int tc_srk_step(tc_engine* e, double t, double tau, int vel_slot, int p_slot, in
     if (vel_slot < 0 || vel_slot > 3 || p_slot < 0 || p_slot > 3) return fail(TC_ERR_ARG, "bad slot");
     if (e->srk_stab && (dpdt_slot < 0 || dpdt_slot > 3 || dpdt_slot == p_slot)) return fail(TC_ERR_ARG, "bad dpdt slot");
     if (!(tau > 0)) return fail(TC_ERR_ARG, "tau must be positive");
-    e->diag_slot = -1;
+    e->drop_records();
     auto run = [&]() {
         return e->fp32 ? srk_step<float>(e, tau, vel_slot, p_slot, dpdt_slot)
                        : srk_step<double>(e, tau, vel_slot, p_slot, dpdt_slot);
@@ -1031,7 +1053,7 @@ int tc_set_graphs(tc_engine* e, int on) {
 
 int tc_explicit_term(tc_engine* e, int in_kind, int in_slot, int out_kind, int out_slot) {
     if (int r = check(e)) return r;
-    e->diag_slot = -1;
+    e->drop_records();
     void* in = field_ptr(e, in_kind, in_slot);
     void* out = field_ptr(e, out_kind, out_slot);
     if (!in || !out || !is_vec(in_kind) || !is_vec(out_kind) || in == out) return fail(TC_ERR_ARG, "bad field");
@@ -1057,7 +1079,7 @@ int tc_explicit_term(tc_engine* e, int in_kind, int in_slot, int out_kind, int o
 
 int tc_apply_gradient(tc_engine* e, int p_kind, int p_slot, int out_kind, int out_slot) {
     if (int r = check(e)) return r;
-    e->diag_slot = -1;
+    e->drop_records();
     void* p = field_ptr(e, p_kind, p_slot);
     void* out = field_ptr(e, out_kind, out_slot);
     if (!p || !out || !is_scal(p_kind) || !is_vec(out_kind)) return fail(TC_ERR_ARG, "bad field");
@@ -1068,7 +1090,7 @@ int tc_apply_gradient(tc_engine* e, int p_kind, int p_slot, int out_kind, int ou
 
 int tc_unapply_gradient(tc_engine* e, int v_kind, int v_slot, int p_kind, int p_slot) {
     if (int r = check(e)) return r;
-    e->diag_slot = -1;
+    e->drop_records();
     void* v = field_ptr(e, v_kind, v_slot);
     void* p = field_ptr(e, p_kind, p_slot);
     if (!v || !p || !is_vec(v_kind) || !is_scal(p_kind)) return fail(TC_ERR_ARG, "bad field");
@@ -1077,7 +1099,7 @@ int tc_unapply_gradient(tc_engine* e, int v_kind, int v_slot, int p_kind, int p_
 
 int tc_normalize_pressure(tc_engine* e, int kind, int slot) {
     if (int r = check(e)) return r;
-    e->diag_slot = -1;
+    e->drop_records();
     void* p = field_ptr(e, kind, slot);
     if (!p || !is_scal(kind)) return fail(TC_ERR_ARG, "bad field");
     return e->fp32 ? normalize<float>(e, (float*)p) : normalize<double>(e, (double*)p);
@@ -1085,7 +1107,7 @@ int tc_normalize_pressure(tc_engine* e, int kind, int slot) {
 
 int tc_recalc_turb_visc(tc_engine* e, int kind, int slot) {
     if (int r = check(e)) return r;
-    e->diag_slot = -1;
+    e->drop_records();
     void* v = field_ptr(e, kind, slot);
     if (!v || !is_vec(kind)) return fail(TC_ERR_ARG, "bad field");
     if (e->P.sgs_model) {  // the learned closure's counterpart: tau from this field
@@ -1182,25 +1204,31 @@ int tc_diagnostics(tc_engine* e, int vel_slot, int p_slot, double tau, int cfl_m
     if (!ab3 && !p) return fail(TC_ERR_ARG, "bad pressure slot");
     if (host_av && count != (size_t)nv * e->nz) return fail(TC_ERR_ARG, "count must be nz*" + std::to_string(nv));
     if (cfl && (cfl_mode < 1 || cfl_mode > 3)) return fail(TC_ERR_ARG, "cfl_mode must be 1, 2 or 3");
-    // the step's own record when it holds these fields, else one pass now; either
-    // final reduction writes h_diag through its device mapping (no copy)
-    if (!(e->diag_slot == vel_slot && (ab3 || p_slot == vel_slot))) {
-        e->diag_slot = -1;
+    // the record a step formed of these fields (waiting for that step only, so a
+    // later step may already run), else one pass now into buffer 2; either final
+    // reduction writes the host buffer through its device mapping (no copy)
+    int k = -1;
+    for (int j = 0; j < 2; j++)
+        if (e->diag_rec[j] == vel_slot && (ab3 || p_slot == vel_slot)) k = j;
+    if (k >= 0) {
+        HIPCHK(hipEventSynchronize(e->diag_ev[k]));
+    } else {
+        k = 2;
         const double visc_fac = diag_visc_fac(e->P.scheme, e->P.visc_type);
         if (e->fp32)
             HIPCHK(launch_diag<float>(e->s, e->mesh<float>(), e->P.scheme, (const float*)v, (const float*)p,
-                                      (const float*)e->nut, visc_fac, e->d_partial, e->hd_diag));
+                                      (const float*)e->nut, visc_fac, e->d_partial, e->hd_diag[2]));
         else
             HIPCHK(launch_diag<double>(e->s, e->mesh<double>(), e->P.scheme, (const double*)v, (const double*)p,
-                                       (const double*)e->nut, visc_fac, e->d_partial, e->hd_diag));
+                                       (const double*)e->nut, visc_fac, e->d_partial, e->hd_diag[2]));
+        HIPCHK(hipStreamSynchronize(e->s));
     }
-    HIPCHK(hipStreamSynchronize(e->s));
     // volumes as the device kernels see them: the mesh in the engine's precision
     auto r = [&](double x) { return e->fp32 ? (double)(float)x : x; };
     const double hxy = r(e->P.hx) * r(e->P.hy), inv_nxy = 1.0 / (double)e->nxy;
     double q[3] = {0, 0, 0}, ke = 0, cmax = 0;
     for (int iz = 0; iz < e->nz; iz++) {
-        const double* d = e->h_diag + (size_t)iz * nrec;
+        const double* d = e->h_diag[k] + (size_t)iz * nrec;
         const double sw = d[nv], sww = d[nv + 1];
         if (ab3) {
             const double vol = r(e->hm.cellh[iz]) * hxy;
@@ -1229,7 +1257,7 @@ int tc_diagnostics(tc_engine* e, int vel_slot, int p_slot, double tau, int cfl_m
 
 int tc_cfr_correct(tc_engine* e, int kind, int slot, double* shift) {
     if (int r = check(e)) return r;
-    e->diag_slot = -1;
+    e->drop_records();
     void* v = field_ptr(e, kind, slot);
     if (!v || !is_vec(kind)) return fail(TC_ERR_ARG, "bad field");
     if (!(e->P.cfr_ubulk > 0)) return fail(TC_ERR_STATE, "cfr_ubulk is not set");
@@ -1246,7 +1274,7 @@ int tc_cfr_correct(tc_engine* e, int kind, int slot, double* shift) {
 
 int tc_project(tc_engine* e, int kind, int slot) {
     if (int r = check(e)) return r;
-    e->diag_slot = -1;
+    e->drop_records();
     void* v = field_ptr(e, kind, slot);
     if (!v || !is_vec(kind)) return fail(TC_ERR_ARG, "bad field");
     return e->fp32 ? project<float>(e, (float*)v) : project<double>(e, (double*)v);
@@ -1254,7 +1282,7 @@ int tc_project(tc_engine* e, int kind, int slot) {
 
 int tc_initial_field(tc_engine* e, int slot, unsigned long long seed, double ubulk, double noise_rms) {
     if (int r = check(e)) return r;
-    e->diag_slot = -1;
+    e->drop_records();
     void* v = field_ptr(e, TC_VEL, slot);
     if (!v) return fail(TC_ERR_ARG, "bad slot");
     std::vector<double> u(3 * e->n3d);

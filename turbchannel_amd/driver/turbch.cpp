@@ -87,6 +87,7 @@ struct Options {
     int timing = 0;
     int graphs = 1;
     int fused_diag = 1;
+    int pipeline = 1;
     int print_mesh = 0;
 };
 
@@ -138,6 +139,7 @@ const OptSpec kOpts[] = {
     {"--timing", "1: report the time of each kernel class at the end"},
     {"--graphs", "1: replay repeated steps from captured HIP graphs (default), 0: launch each kernel"},
     {"--fused_diag", "1: the step's projection forms the per-step diagnostics (default), 0: a pass of its own"},
+    {"--pipeline", "1: launch the next step before reading this step's diagnostics (default; needs --fused_diag 1)"},
     {"--print-mesh", "1: print the wall-normal coordinates and exit"},
 };
 
@@ -246,6 +248,7 @@ bool parse(int argc, char** argv, Options& o) {
         else if (is("--timing")) parse_num(key, val, o.timing);
         else if (is("--graphs")) parse_num(key, val, o.graphs);
         else if (is("--fused_diag")) parse_num(key, val, o.fused_diag);
+        else if (is("--pipeline")) parse_num(key, val, o.pipeline);
         else if (is("--print-mesh")) parse_num(key, val, o.print_mesh);
     }
     return true;
@@ -871,12 +874,29 @@ struct Run {
 
         printf("Start of time integration (N=%i,%i,%i)\n", P.nx, P.ny, P.nz);
         const double t_total0 = now();
+        // Pipelining: step it+1 is launched before step it's diagnostics are read
+        // (they wait for step it only), unless step it ends the run or writes a
+        // field or a checkpoint. It writes only the ring slot after it and what it
+        // recomputes from its input, so if the CFL control then changes tau it is
+        // launched again from the same slot with the new tau at order 1.
+        const bool pipe = o.pipeline && o.fused_diag;
+        bool launched = false;  // step it already launched
         for (bool final_step = false; !final_step; it++) {
             double t0 = now();
             const int order = (int)std::min<long long>(it - it_start + 1, max_bdf_order);
-            check(tc_step(e, it * tau, tau, cur, order), "tc_step");
+            if (!launched) check(tc_step(e, it * tau, tau, cur, order), "tc_step");
+            launched = false;
             cur = (cur + 1) % 4;
-            check(tc_sync(e), "tc_sync");
+            const bool ends = t + tau >= o.time_max || it + 1 == o.time_steps_max;
+            const bool writes = (it != 0 && it % fprog == 0) ||
+                                (o.checkpoint_every > 0 && (it + 1) % o.checkpoint_every == 0);
+            if (pipe && !ends && !writes) {
+                const int order1 = (int)std::min<long long>(it + 1 - it_start + 1, max_bdf_order);
+                check(tc_step(e, (it + 1) * tau, tau, cur, order1), "tc_step");
+                launched = true;
+            } else {
+                check(tc_sync(e), "tc_sync");
+            }
             double t1 = now();
             sec_step += t1 - t0;
 
@@ -926,6 +946,7 @@ struct Run {
                 snprintf(fname, sizeof fname, "r%05lld.dat", it);
                 dump_profiles(av.data(), fname);
             }
+            const double tau_used = tau;
             while (cfl > o.cfl_max) {
                 cfl *= 0.5;
                 tau *= 0.5;
@@ -936,6 +957,8 @@ struct Run {
                 tau *= 1.5;
                 it_start = it + 1;
             }
+            if (launched && tau != tau_used)  // step it+1 ran with the old tau: again, order 1
+                check(tc_step(e, (it + 1) * tau, tau, cur, 1), "tc_step");
             sec_out += now() - t3;
             if (o.checkpoint_every > 0 && ((it + 1) % o.checkpoint_every == 0 || final_step)) write_restart();
         }
