@@ -16,10 +16,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "turbchannel_amd", "turbch")
 
 
-def run_driver(args, cwd, timeout=120):
+def run_driver(args, cwd, timeout=120, env=None):
+    """Runs the command-line solver; env adds variables to the inherited environment."""
     if not os.path.exists(DRIVER):
         raise RuntimeError(f"{DRIVER} is not built (python -c 'import __graft_entry__ as g; g.build()')")
-    r = subprocess.run([DRIVER, *[str(a) for a in args]], cwd=cwd, capture_output=True, text=True, timeout=timeout)
+    full_env = dict(os.environ, **env) if env else None
+    r = subprocess.run([DRIVER, *[str(a) for a in args]], cwd=cwd, capture_output=True, text=True, timeout=timeout,
+                       env=full_env)
     return r
 
 

@@ -40,17 +40,33 @@ def test_workload_definitions(name):
         assert kw["cfr_ubulk"] == 1.0
 
 
-def test_headline_workload_is_the_fourth_order_dns():
-    scheme, nx, ny, nz, zz, kw, dt = bench.problem("vv_192x160x128", tca.FP64)
-    assert (scheme, nx, ny, nz, dt) == (tca.SCHEME_VV, 192, 160, 128, 1e-3)
-    assert math.isclose(2.0 / kw["visc"], 5600.0)
+def test_headline_workload_is_the_largest_config():
+    """The default line is the collocated BDF3/EX3 DNS at 384 x 384 x 192 nodes in
+    FP32-bulk, the largest single-GPU configuration; the staggered figures ride
+    along as extra workloads."""
+    assert bench.DEFAULT_WORKLOAD == "ab3_384x384x192"
+    assert bench.DEFAULT_PRECISION[bench.DEFAULT_WORKLOAD] == "fp32"
+    scheme, nx, ny, nz, zz, kw, dt = bench.problem(bench.DEFAULT_WORKLOAD, tca.FP32_BULK)
+    assert (scheme, nx, ny, nz, dt) == (tca.SCHEME_AB3, 384, 384, 192, 5e-4)
+    assert math.isclose(2.0 / kw["visc"], 13750.0) and kw["stab_type"] == 1
+    assert bench.EXTRA_WORKLOADS == [("vv_384x384x192", "fp32"), ("vv_192x160x128", "fp64")]
+    assert bench.SHIPPED_THREADS[tca.SCHEME_AB3] == 2 and bench.SHIPPED_THREADS[tca.SCHEME_VV] == 1
 
 
 def test_algorithmic_reals_per_step():
-    """Reals per grid point and step of the staggered DNS: 42 with the z-solve's
-    two sweeps over the spectrum counted (4), 40 with the spectrum counted once."""
+    """Reals per grid point and step, each array element counted once: 40 for the
+    staggered DNS, 43 for the collocated solver with StabType 1 (its explicit term
+    and projection also read p_n, the projection writes p_n+1)."""
     r = bench.REALS_PER_POINT
-    assert r["flux"] == 21 and r["project"] == 7 and r["zsolve"] == 4
+    assert r["flux"] == 21 and r["project"] == 7 and r["zsolve"] == 2
     assert r["fft_y"] == r["ifft_y"] == 2
-    total = r["flux"] + r["fft_x"] + r["fft_y"] + r["zsolve"] + r["ifft_y"] + r["ifft_x"] + r["project"]
-    assert total == 42 and total - 2 == 40
+    assert sum(r.values()) == 40
+    a = bench.REALS_PER_POINT_AB3
+    assert a["flux"] == 22 and a["project"] == 9 and sum(a.values()) == 43
+
+
+def test_baseline_threads_use_the_job_share():
+    cores = bench.host_cores()
+    assert cores["allowed"] >= 1
+    t = bench.baseline_threads()
+    assert 1 <= t <= min(cores["allowed"], 16) or t == int(os.environ.get("OMP_NUM_THREADS", "0"))

@@ -276,3 +276,23 @@ def test_driver_pipelined_loop_writes_the_same_files(tmp_path, args):
     assert names == sorted(p.name for p in dirs[1].iterdir()) and "q00004.vtk" in names
     for name in names:
         assert (dirs[0] / name).read_bytes() == (dirs[1] / name).read_bytes(), name
+
+
+@pytest.mark.parametrize("args", [STAG_ARGS + ["--les", 1, "--CFLmax", 0.06, "--CFLmin", 0.05]], ids=["vv"])
+def test_driver_graph_cache_clear_with_a_step_in_flight(tmp_path, args):
+    """With the graph cache capped at 2 entries, the step-size control makes the
+    engine drop its captured graphs on most steps while the pipelined loop has the
+    next step in flight; the files are those of the unpipelined loop."""
+    common = [*args, "--seed", 3, "--tau", "2e-3", "-s", 1, "-f", 4, "-p", 3, "--TimeStepsMax", 11]
+    dirs = {}
+    for pipe in (0, 1):
+        d = tmp_path / f"pipe{pipe}"
+        d.mkdir()
+        r = run_driver([*common, "--pipeline", pipe], d, env={"TC_GRAPH_CACHE_MAX": "2"})
+        assert r.returncode == 0, r.stdout + r.stderr
+        dirs[pipe] = d
+    assert len(set(_log_rows(dirs[1] / "log.dat")[:, 2].tolist())) > 2
+    names = sorted(p.name for p in dirs[0].iterdir())
+    assert names == sorted(p.name for p in dirs[1].iterdir())
+    for name in names:
+        assert (dirs[0] / name).read_bytes() == (dirs[1] / name).read_bytes(), name

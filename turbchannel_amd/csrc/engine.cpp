@@ -7,6 +7,7 @@
 // and all reductions.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <map>
@@ -142,6 +143,7 @@ struct tc_engine {
     // (current slot, order, tau): one graph launch instead of ~10 kernel launches
     bool use_graphs = false;
     std::map<std::tuple<int, int, unsigned long long>, hipGraphExec_t> graphs;
+    size_t graph_cap = 48;  // captured steps kept; TC_GRAPH_CACHE_MAX (>= 1) overrides
 
     template <typename T>
     Mesh<T> mesh() const {
@@ -876,6 +878,10 @@ int tc_create(const tc_params* pin, int device, tc_engine** out) {
     e->nxh = p->nx / 2 + 1;
     e->nm = e->nxh * p->ny;
     e->n3d = (size_t)e->nxy * p->nz;
+    if (const char* cap = std::getenv("TC_GRAPH_CACHE_MAX")) {
+        const long c = std::strtol(cap, nullptr, 10);
+        if (c >= 1) e->graph_cap = (size_t)c;
+    }
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking));
     int r = e->fp32 ? setup_device<float>(e.get()) : setup_device<double>(e.get());
@@ -961,7 +967,10 @@ int run_graph(tc_engine* e, int a, int b, double tau, F&& run) {
     const auto key = std::make_tuple(a, b, tb);
     auto it = e->graphs.find(key);
     if (it == e->graphs.end()) {
-        if (e->graphs.size() >= 48) {  // step sizes changed many times: start over
+        if (e->graphs.size() >= e->graph_cap) {  // step sizes changed many times: start over
+            // a step launched from one of these graphs may still run (the driver
+            // launches the next step before it reads the last one's record)
+            HIPCHK(hipStreamSynchronize(e->s));
             for (auto& g : e->graphs) hipGraphExecDestroy(g.second);
             e->graphs.clear();
         }
@@ -1244,7 +1253,7 @@ int tc_diagnostics(tc_engine* e, int vel_slot, int p_slot, double tau, int cfl_m
             q[2] += sw * dzw * hxy;
             ke += ((d[3] + d[4]) * dz + sww * dzw) * hxy;
         }
-        cmax = std::max(cmax, d[nv + 1 + cfl_mode]);
+        if (cfl) cmax = std::max(cmax, d[nv + 1 + cfl_mode]);
         if (host_av)
             for (int k = 0; k < nv; k++) host_av[(size_t)iz * nv + k] = d[k] * inv_nxy;
     }

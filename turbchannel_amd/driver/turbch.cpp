@@ -972,6 +972,9 @@ struct Run {
             printf("Time: total %.04f, step %.04f, diagnostics %.04f, output %.04f\n", total, sec_step, sec_red,
                    sec_out);
         }
+        if (pipe)  // the host clock sees launches only; each wait lands in tc_diagnostics
+            printf("(pipelined loop: the step channel times the launches, the diagnostics channel includes the "
+                   "wait for each step's kernels; --pipeline 0 separates them)\n");
         report_timing();
 
         download(cur);
