@@ -206,6 +206,17 @@ int tc_mlp_gradient(tc_mlp* m, const float* dev_x, const float* dev_t, long long
                     double* loss);
 /* One Adam step on the batch; loss (may be NULL) is the batch loss before the step. */
 int tc_mlp_train_step(tc_mlp* m, const float* dev_x, const float* dev_t, long long n, float lr, double* loss);
+/* tc_mlp_train_step that allocates nothing: no counterpart in the reference. For
+ * callers that cannot allocate device memory (training loops under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). The gradient kernel's
+ * per-block partial records go to a static device array of this library (one per
+ * GPU, 512 records = 19.3 MB, held until the process ends) instead of the
+ * trainer's own buffer, which tc_mlp_train_step grows with hipMalloc when a batch
+ * needs more records. Every batch size fits (the kernel writes at most 512
+ * records; n above 2^31 - 1 is rejected). Results equal tc_mlp_train_step's bit
+ * for bit. Calls that overlap on one GPU (any trainers) share the array and race. */
+int tc_mlp_train_step_noalloc(tc_mlp* m, const float* dev_x, const float* dev_t, long long n, float lr,
+                              double* loss);
 int tc_mlp_steps(const tc_mlp* m, long long* steps);
 
 /* Per-kernel timing with HIP events on the engine's stream. */

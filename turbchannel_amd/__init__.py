@@ -109,6 +109,7 @@ def lib():
     L.tc_mlp_forward.argtypes = [vp, vp, ll, vp]
     L.tc_mlp_gradient.argtypes = [vp, vp, vp, ll, fp, ctypes.c_size_t, dp]
     L.tc_mlp_train_step.argtypes = [vp, vp, vp, ll, ctypes.c_float, dp]
+    L.tc_mlp_train_step_noalloc.argtypes = [vp, vp, vp, ll, ctypes.c_float, dp]
     L.tc_mlp_steps.argtypes = [vp, P(ll)]
     _lib = L
     return L
@@ -389,9 +390,11 @@ class Mlp:
                                      g.size, ctypes.byref(loss)))
         return loss.value, g
 
-    def train_step(self, dev_x, dev_t, n, lr=1e-3):
+    def train_step(self, dev_x, dev_t, n, lr=1e-3, no_alloc=False):
+        """One Adam step; no_alloc uses tc_mlp_train_step_noalloc (no device allocation)."""
         loss = ctypes.c_double()
-        _check(lib().tc_mlp_train_step(self._h, dev_x, dev_t, n, lr, ctypes.byref(loss)))
+        fn = lib().tc_mlp_train_step_noalloc if no_alloc else lib().tc_mlp_train_step
+        _check(fn(self._h, dev_x, dev_t, n, lr, ctypes.byref(loss)))
         return loss.value
 
     def steps(self):

@@ -621,7 +621,7 @@ hipError_t launch_mlp_forward(hipStream_t s, MlpDev w, const float* x, long long
 
 int mlp_train_blocks(long long n) {
     const long long nch = (n + 63) / 64;
-    return (int)(nch < 512 ? (nch < 1 ? 1 : nch) : 512);
+    return (int)(nch < kMlpTrainMaxBlocks ? (nch < 1 ? 1 : nch) : kMlpTrainMaxBlocks);
 }
 
 hipError_t launch_mlp_partials(hipStream_t s, MlpDev w, const float* x, const float* t, long long n, float* partial) {

@@ -1,8 +1,10 @@
 // C ABI of the learned-closure trainer (tc_mlp_*): FP32 master weights, Adam
 // state and bfloat16 device copies on one GPU. No reference counterpart.
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "kernels.h"
@@ -22,6 +24,7 @@ struct tc_mlp {
     size_t partial_recs = 0;
     double* tmp = nullptr;
     double* loss = nullptr;
+    float* scratch = nullptr;  // this device's copy of the no-allocation scratch (set at create)
     long long steps = 0;
     MlpDev dev_view() const {
         MlpDev d;
@@ -31,7 +34,34 @@ struct tc_mlp {
     }
 };
 
+// Scratch of tc_mlp_train_step_noalloc: the gradient kernel's per-block partial
+// records, as many as that kernel ever launches blocks for (kMlpTrainMaxBlocks x
+// kMlpRec floats = 19.3 MB), so every batch size fits. Each GPU that loads this
+// library holds one copy until the process ends; host code reaches it only through
+// the address hipGetSymbolAddress returns on that device.
+constexpr size_t kNoAllocRecs = kMlpTrainMaxBlocks;
+static __device__ __align__(256) float scratch[kNoAllocRecs * kMlpRec];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+float* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// The scratch's address on `device`, looked up once per device; the device must be current.
+hipError_t scratch_address(int device, float** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = (float*)p;
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
 
 #define MCHK(x)                                                                      \
     do {                                                                             \
@@ -101,13 +131,26 @@ int ensure_partial(tc_mlp* m, long long n) {
     return TC_OK;
 }
 
-int grad_and_maybe_adam(tc_mlp* m, const float* x, const float* t, long long n, const AdamArgs& a, double* loss) {
+// Gradient (and Adam step) on n samples. The partial records go to the trainer's
+// own buffer (grown on demand), or with no_alloc to this device's static scratch.
+int grad_and_maybe_adam(tc_mlp* m, const float* x, const float* t, long long n, const AdamArgs& a, double* loss,
+                        bool no_alloc = false) {
     if (!m) return arg_error("null trainer");
     if (n <= 0 || !x || !t) return arg_error("need n > 0 samples and device inputs/targets");
     MCHK(hipSetDevice(m->dev));
-    if (int r = ensure_partial(m, n)) return r;
-    MCHK(launch_mlp_partials(m->s, m->dev_view(), x, t, n, m->partial));
-    MCHK(launch_mlp_reduce(m->s, m->partial, mlp_train_blocks(n), m->tmp, m->grad, m->loss, a, m->params, m->m, m->v,
+    float* partial = nullptr;
+    if (no_alloc) {
+        if (n > (long long)INT_MAX) return arg_error("no-allocation training step: n must be at most 2^31 - 1 samples");
+        if ((size_t)mlp_train_blocks(n) > kNoAllocRecs)
+            return arg_error("no-allocation training step: batch needs more partial records than the scratch holds");
+        if (!m->scratch) return arg_error("no-allocation training step: no scratch on this device");
+        partial = m->scratch;
+    } else {
+        if (int r = ensure_partial(m, n)) return r;
+        partial = m->partial;
+    }
+    MCHK(launch_mlp_partials(m->s, m->dev_view(), x, t, n, partial));
+    MCHK(launch_mlp_reduce(m->s, partial, mlp_train_blocks(n), m->tmp, m->grad, m->loss, a, m->params, m->m, m->v,
                            m->w, m->b));
     double l = 0;
     MCHK(hipMemcpyAsync(&l, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->s));
@@ -140,6 +183,13 @@ int tc_mlp_create(int device, unsigned long long seed, tc_mlp** out) {
         return r;
     };
     if (hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking) != hipSuccess) return fail(TC_ERR_HIP);
+    if (device < kMaxDevices) {  // the no-allocation step's scratch on this device
+        const hipError_t e = scratch_address(device, &m->scratch);
+        if (e != hipSuccess) {
+            tc_set_error("hipGetSymbolAddress of the training scratch failed");
+            return fail(TC_ERR_HIP);
+        }
+    }
     if (hipMalloc(&m->params, kMlpParams * sizeof(float)) != hipSuccess ||
         hipMalloc(&m->m, kMlpParams * sizeof(float)) != hipSuccess ||
         hipMalloc(&m->v, kMlpParams * sizeof(float)) != hipSuccess ||
@@ -208,7 +258,8 @@ int tc_mlp_gradient(tc_mlp* m, const float* dev_x, const float* dev_t, long long
     return TC_OK;
 }
 
-int tc_mlp_train_step(tc_mlp* m, const float* dev_x, const float* dev_t, long long n, float lr, double* loss) {
+static int train_step(tc_mlp* m, const float* dev_x, const float* dev_t, long long n, float lr, double* loss,
+                      bool no_alloc) {
     if (!m) return arg_error("null trainer");
     if (!(lr > 0.f)) return arg_error("learning rate must be positive");
     AdamArgs a;
@@ -217,9 +268,18 @@ int tc_mlp_train_step(tc_mlp* m, const float* dev_x, const float* dev_t, long lo
     const long long step = m->steps + 1;
     a.bc1 = (float)(1.0 - std::pow((double)a.b1, (double)step));
     a.bc2 = (float)(1.0 - std::pow((double)a.b2, (double)step));
-    if (int r = grad_and_maybe_adam(m, dev_x, dev_t, n, a, loss)) return r;
+    if (int r = grad_and_maybe_adam(m, dev_x, dev_t, n, a, loss, no_alloc)) return r;
     m->steps = step;
     return TC_OK;
+}
+
+int tc_mlp_train_step(tc_mlp* m, const float* dev_x, const float* dev_t, long long n, float lr, double* loss) {
+    return train_step(m, dev_x, dev_t, n, lr, loss, false);
+}
+
+int tc_mlp_train_step_noalloc(tc_mlp* m, const float* dev_x, const float* dev_t, long long n, float lr,
+                              double* loss) {
+    return train_step(m, dev_x, dev_t, n, lr, loss, true);
 }
 
 int tc_mlp_steps(const tc_mlp* m, long long* steps) {

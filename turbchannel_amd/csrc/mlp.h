@@ -48,6 +48,9 @@ void mlp_pack_device(const float* params, std::vector<unsigned short>& w, std::v
 // y[n][6] = net(x[n][10]) (FP32 in and out).
 hipError_t launch_mlp_forward(hipStream_t s, MlpDev w, const float* x, long long n, float* y);
 // Loss (mean square over n x 6) gradient: per-block partial records (nblocks x kMlpRec floats).
+// Blocks of the gradient kernel for n samples: one per 64-sample chunk, at most
+// kMlpTrainMaxBlocks (each writes one partial record).
+constexpr int kMlpTrainMaxBlocks = 512;
 int mlp_train_blocks(long long n);
 hipError_t launch_mlp_partials(hipStream_t s, MlpDev w, const float* x, const float* t, long long n, float* partial);
 struct AdamArgs {
