@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libtc_oracle.so")
+# TC_ORACLE_LIB names another build of the same source (the mutation tests load
+# copies of the oracle with one planted mistake each this way).
+_LIB_PATH = os.environ.get("TC_ORACLE_LIB") or os.path.join(_HERE, "libtc_oracle.so")
 
 SCHEME_HW, SCHEME_VV, SCHEME_AB3, SCHEME_SRK = 0, 1, 2, 3
 IMEX_METHODS = ("ARS_121", "ARS_232", "ARS_343", "MARS_343", "ARK3", "ARK4", "MARK3", "BHR_553")
