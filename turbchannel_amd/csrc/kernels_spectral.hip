@@ -206,22 +206,29 @@ struct TwStage {
 template <typename T>
 __device__ __forceinline__ void store_half_spectra(const Cplx<T>* res, const FFTPlan& pl, int ny, FastDiv dny,
                                                    FastDiv d2l, int q0, int nrows, Cplx<T>* __restrict__ spec) {
-    const int N = pl.n, nr = (int)d2l.d, nxh = N / 2 + 1;
-    for (int t = threadIdx.x; t < nxh * nr; t += blockDim.x) {
-        const int k = d2l.div(t), rr = t - k * nr;
-        const int l = rr >> 1, row = 2 * q0 + rr;
+    const int N = pl.n, nl = (int)d2l.d / 2, nxh = N / 2 + 1;
+    // one thread per (kx, line): the line's two rows are neighbours in y (NY is
+    // even), so their two spectral values are one 2-complex store
+    for (int t = threadIdx.x; t < nxh * nl; t += blockDim.x) {
+        const int k = d2l.div(2 * t), l = t - k * nl;
+        const int row = 2 * (q0 + l);
         if (row >= nrows) continue;
         const Cplx<T> z = res[ldsi(l * N + dif_pos(pl, k))];
         const Cplx<T> zc = res[ldsi(l * N + dif_pos(pl, k == 0 ? 0 : N - k))];
         // row 2l: A = (Z + conj Zc)/2; row 2l+1: B = -i (Z - conj Zc)/2
-        const Cplx<T> out = (rr & 1) ? Cplx<T>{T(0.5) * (z.y + zc.y), T(-0.5) * (z.x - zc.x)}
-                                     : Cplx<T>{T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
+        const Cplx<T> a = {T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
+        const Cplx<T> b = {T(0.5) * (z.y + zc.y), T(-0.5) * (z.x - zc.x)};
         const int iz = dny.div(row), iy = row - iz * ny;
-        // streaming store: the spectrum is not read again by this kernel, so it
+        // streaming stores: the spectrum is not read again by this kernel, so it
         // should not displace the rows whose y and z taps other blocks still read
         T* sp = reinterpret_cast<T*>(spec + ((size_t)iz * nxh + k) * ny + iy);
-        __builtin_nontemporal_store(out.x, sp);
-        __builtin_nontemporal_store(out.y, sp + 1);
+        if (row + 1 < nrows) {
+            using V4 = __attribute__((ext_vector_type(4))) T;
+            __builtin_nontemporal_store(V4{a.x, a.y, b.x, b.y}, reinterpret_cast<V4*>(sp));
+        } else {
+            __builtin_nontemporal_store(a.x, sp);
+            __builtin_nontemporal_store(a.y, sp + 1);
+        }
     }
 }
 
@@ -382,7 +389,9 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = pl.n, nxh = N / 2 + 1, half = N / 2;
     Cplx<T>* twl = b0 + lds_elems(lpb * N);
-    const int q0 = blockIdx.x * lpb;
+    // XCD-contiguous block order: a block reads 2*lpb values of each kx, part of a
+    // cache line whose rest the next block reads, which then runs on the same XCD
+    const int q0 = xcd_swizzle(blockIdx.x, gridDim.x) * lpb;
     struct alignas(2 * sizeof(Cplx<T>)) Pair { Cplx<T> a, b; };
     constexpr int U = 4;
     TwStage<T> tws;
@@ -784,8 +793,8 @@ hipError_t launch_fft_x_fwd_q(hipStream_t s, const FFTPlan& pl, const Cplx<T>* t
     const int nlines = (nrows + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
     const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
-    hipLaunchKernelGGL((k_fft_x_fwd_q<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, v, ca, cb, nrows, spec, lpb, ny,
-                       FastDiv((unsigned)ny), FastDiv((unsigned)(2 * lpb)));
+    hipLaunchKernelGGL((k_fft_x_fwd_q<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, v, ca, cb, nrows, spec, lpb,
+                       ny, FastDiv((unsigned)ny), FastDiv((unsigned)(2 * lpb)));
     return hipGetLastError();
 }
 

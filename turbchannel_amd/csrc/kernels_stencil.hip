@@ -1054,8 +1054,12 @@ hipError_t launch_gradient(hipStream_t s, const Mesh<T>& m, int scheme, const T*
 int project_diag_chunks(int nxy) { return (nxy + kBlock * kProjDiagCells - 1) / (kBlock * kProjDiagCells); }
 
 template <typename T>
-hipError_t launch_project(hipStream_t s, const Mesh<T>& m, const ProjectArgs<T>& a) {
-    dim3 g = cell_grid(m);
+hipError_t launch_project(hipStream_t s, const Mesh<T>& mesh, const ProjectArgs<T>& a) {
+    dim3 g = cell_grid(mesh);
+    // XCD-contiguous block order at every plane size: the y taps of p sit in the
+    // rows of the neighbouring chunks of the plane, which then share an L2
+    Mesh<T> m = mesh;
+    m.xcd_swz = 1;
 #define LP(SCH, STAB, DIAG) hipLaunchKernelGGL((k_project<T, SCH, STAB, DIAG>), g, dim3(kBlock), 0, s, m, a)
     if (a.diag) {
         g.x = project_diag_chunks(m.nxy);
