@@ -38,7 +38,6 @@ hipError_t launch_turb_visc(hipStream_t s, const Mesh<T>& m, int scheme, const T
 template <typename T>
 hipError_t launch_gradient(hipStream_t s, const Mesh<T>& m, int scheme, const T* p, T* g);
 
-
 // u_next = (r - G p) * tau_over_bd0 (+ flow-rate shift on u); collocated StabType 1
 // also forms p_next = dp + p_cur.
 template <typename T>

@@ -281,7 +281,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
     }
     tws.store(twl, tw, N);
     __syncthreads();
-    constexpr int U = SCHEME == TC_SCHEME_VV ? 2 : 4;
+    constexpr int U = 2;  // 2 elements in flight per thread: 1, 4 and 6 measured slower
     const int ny = m.ny, nz = m.nz;
     for (int t0 = threadIdx.x; t0 < tot; t0 += U * kFftThreads) {
         T va[U], vb[U], wa[U], wb[U], vm1[U] = {}, vp2[U], vp3[U] = {};
