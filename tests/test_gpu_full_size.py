@@ -7,9 +7,10 @@ Workloads (bench.py's table; reference axes x, span, wall-normal):
 * vv_192x160x128   fourth-order staggered DNS;
 * ab3_384x384x192  collocated BDF3/EX3 DNS at bulk Re 13750;
 * vv_384x384x192   fourth-order staggered DNS at the same size.
-At 384 x 384 points per plane the node-grid kernels run in plain block order
-(four planes of a vector field exceed one XCD's L2), so these cases cover that
-launch order; the smaller ones cover the XCD-contiguous order.
+At 384 x 384 points per plane the node-grid kernels other than the projection
+run in plain block order (four planes of a vector field exceed one XCD's L2), so
+these cases cover that launch order; the smaller ones cover the XCD-contiguous
+order.
 
 Bars, each checked below:
 * FP64: max-norm relative error of velocity and pressure <= 1e-12 after the

@@ -449,7 +449,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
     const int L0 = blockIdx.x * lpb;
     const int nl = min(lpb, nlines - L0);
     Cplx<T>* base = spec + (size_t)L0 * N;
-    constexpr int U = 4;
+    constexpr int U = 6;  // all of a 4 x 384 block's loads in one batch (2 and 4 measured slower)
     TwStage<T> tws;
     tws.load(tw, N);
     const int tot = lpb * N, tin = nl * N;
