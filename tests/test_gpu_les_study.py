@@ -18,3 +18,24 @@ def test_les_study_pipeline_runs():
         for k in ("mean_u", "rms_u", "uw", "ms_per_step", "seconds_per_flow_through"):
             assert math.isfinite(r[k]) and r[k] >= 0, (name, k)
         assert r["mean_u"] < 0.2, name
+
+
+def test_closure_protocol_stages_run(tmp_path):
+    """The three stages of the closure configurations (DNS + training set +
+    training, judge DNS, LES comparison) at toy sizes, chained through their
+    JSON outputs as on the full-size runs."""
+    from turbchannel_amd import closure_protocol as cp
+    log = lambda s: print(s, flush=True)  # noqa: E731
+    net = cp.stage_dns395(cp.QUICK["dns395"], log)
+    assert len(net["params"]) == 9414
+    assert net["train_loss"][1] < net["train_loss"][0]
+    assert net["holdout_mse_learned"] < net["holdout_mse_zero"]
+    assert max(net["stats_fp32_vs_fp64"].values()) < 1e-3  # same start, a few steps apart
+    judge = cp.stage_judge590(cp.QUICK["judge590"], log)
+    assert judge["records"] > 0 and judge["filtered_fields"] > 0 and judge["re_tau"] > 0
+    res = cp.stage_les590(cp.QUICK["les590"], net, judge, log)
+    assert len(res["les"]) == 4
+    for name, r in res["les"].items():
+        assert "diverged" not in r, name
+        for k in ("mean_u_err", "mean_u_plus_err", "u_tau_err", "rms_u_err", "uw_err", "seconds_per_flow_through"):
+            assert math.isfinite(r[k]) and r[k] >= 0, (name, k)
