@@ -869,6 +869,7 @@ int tc_create(const tc_params* pin, int device, tc_engine** out) {
         for (double& c : e->zf.C) c /= p->hx * p->hy;
     }
     if (!make_plan(p->nx, e->plx) || !make_plan(p->ny, e->ply)) return fail(TC_ERR_ARG, "nx and ny must have only the prime factors 2, 3 and 5");
+    if (p->nx > 2048) return fail(TC_ERR_ARG, "nx must be at most 2048 (lines of the x transforms)");
     e->fp32 = p->precision == TC_FP32_BULK;
     e->rsz = e->fp32 ? 4 : 8;
     e->nx = p->nx;

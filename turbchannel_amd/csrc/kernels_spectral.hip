@@ -235,7 +235,13 @@ __device__ __forceinline__ void store_half_spectra(const Cplx<T>* res, const FFT
 // Forward x transform of the divergence of v: rows 2q, 2q+1 (row = iy + NY*iz)
 // form the real and imaginary parts of line q; the divergence is evaluated
 // while the lines are staged into LDS.
-template <typename T, int SCHEME>
+// One real3 cell (12 or 24 bytes, aligned to one real): a single wider load.
+template <typename T>
+struct Cell3 {
+    T x, y, z;
+};
+
+template <typename T, int SCHEME, int IT>
 __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const Cplx<T>* __restrict__ tw, Mesh<T> m,
                                                                 const T* __restrict__ v, Cplx<T>* __restrict__ spec,
                                                                 int lpb, FastDiv dny, FastDiv d2l) {
@@ -249,42 +255,52 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
     const int q0 = (m.xcd_swz ? xcd_swizzle(blockIdx.x, gridDim.x) : (int)blockIdx.x) * lpb;
     TwStage<T> tws;
     tws.load(tw, N);
-    const int tot = lpb * N;
+    const int tot = lpb * N;  // elements (line, x); at most IT * U * kFftThreads
     // Rows 2l, 2l+1 of line l are neighbours in y within one plane (NY is even).
-    // (1) u of the block's rows -> LDS (ub[row * N + x]), so the x taps of the
-    // divergence come from LDS; (2) per element the own v and w, the v of the
-    // rows y-1, y+2, y+3 around the pair (the pair shares its own v as each
-    // other's y taps) and the w of the z taps, all loaded before any is used.
+    // (1) each element's two own cells (u, v, w of rows 2l and 2l+1 at x) in one
+    // load per cell: u goes to LDS (ub[row * N + x]) for the x taps of the
+    // divergence, v and w stay in registers; (2) the v of the rows y-1, y+2,
+    // y+3 around the pair (the pair shares its own v as each other's y taps) and
+    // the w of the z taps, all loaded before any is used.
     T* ub = reinterpret_cast<T*>(b1);
+    constexpr int U = 2;  // elements per batch of the divergence
+    const int ny = m.ny, nz = m.nz;
+    T va[IT][U], wa[IT][U], vb[IT][U], wb[IT][U];
     {
-        constexpr int U = 4;
-        for (int t0 = threadIdx.x; t0 < tot; t0 += U * kFftThreads) {
-            T a[U], b[U];
+        Cell3<T> ca[IT][U], cb[IT][U];
+#pragma unroll
+        for (int it = 0; it < IT; it++)
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int t = min(t0 + u * kFftThreads, tot - 1);
+                const int t = min((int)threadIdx.x + (it * U + u) * kFftThreads, tot - 1);
                 const int l = pl.dn.div(t), n = t - l * N;
-                const int ra = min(2 * (q0 + l), nrows - 1), rb = min(2 * (q0 + l) + 1, nrows - 1);
-                a[u] = v[3 * ((size_t)ra * N + n)];
-                b[u] = v[3 * ((size_t)rb * N + n)];
+                const int ra = min(2 * (q0 + l), nrows - 2);  // rows ra, ra + 1 of one plane
+                ca[it][u] = *reinterpret_cast<const Cell3<T>*>(v + 3 * ((size_t)ra * N + n));
+                cb[it][u] = *reinterpret_cast<const Cell3<T>*>(v + 3 * ((size_t)(ra + 1) * N + n));
             }
 #pragma unroll
+        for (int it = 0; it < IT; it++)
+#pragma unroll
             for (int u = 0; u < U; u++) {
-                const int t = t0 + u * kFftThreads;
+                const int t = (int)threadIdx.x + (it * U + u) * kFftThreads;
                 if (t < tot) {
                     const int l = pl.dn.div(t), n = t - l * N;
-                    ub[2 * l * N + n] = a[u];
-                    ub[(2 * l + 1) * N + n] = b[u];
+                    ub[2 * l * N + n] = ca[it][u].x;
+                    ub[(2 * l + 1) * N + n] = cb[it][u].x;
                 }
+                va[it][u] = ca[it][u].y;
+                wa[it][u] = ca[it][u].z;
+                vb[it][u] = cb[it][u].y;
+                wb[it][u] = cb[it][u].z;
             }
-        }
     }
     tws.store(twl, tw, N);
     __syncthreads();
-    constexpr int U = 2;  // 2 elements in flight per thread: 1, 4 and 6 measured slower
-    const int ny = m.ny, nz = m.nz;
-    for (int t0 = threadIdx.x; t0 < tot; t0 += U * kFftThreads) {
-        T va[U], vb[U], wa[U], wb[U], vm1[U] = {}, vp2[U], vp3[U] = {};
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int t0 = (int)threadIdx.x + it * U * kFftThreads;
+        if (t0 >= tot) break;
+        T vm1[U] = {}, vp2[U], vp3[U] = {};
         T wa1[U], wb1[U], wa2[U] = {}, wb2[U] = {}, wam[U] = {}, wbm[U] = {};
         T dzv[U] = {}, dzwv[U] = {}, chv[U] = {};
         int iz[U], l_[U], n_[U];
@@ -299,10 +315,6 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
             n_[u] = n;
             auto at = [&](int zz, int yy) { return 3 * (((size_t)zz * ny + yy) * N + n); };
             auto wrapy = [&](int yy) { return yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy); };
-            va[u] = v[at(z, iy) + 1];
-            wa[u] = v[at(z, iy) + 2];
-            vb[u] = v[at(z, iy + 1) + 1];
-            wb[u] = v[at(z, iy + 1) + 2];
             vp2[u] = v[at(z, wrapy(iy + 2)) + 1];
             if (SCHEME != TC_SCHEME_HW) vm1[u] = v[at(z, wrapy(iy - 1)) + 1];
             if (SCHEME == TC_SCHEME_VV) vp3[u] = v[at(z, wrapy(iy + 3)) + 1];
@@ -345,20 +357,20 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
                 qb.v[2] = ubb[xpp];
             }
             // v taps y+1, y, y+2, y-1
-            qa.v[4] = vb[u];
-            qa.v[5] = va[u];
+            qa.v[4] = vb[it][u];
+            qa.v[5] = va[it][u];
             qa.v[6] = vp2[u];
             qa.v[7] = vm1[u];
             qb.v[4] = vp2[u];
-            qb.v[5] = vb[u];
+            qb.v[5] = vb[it][u];
             qb.v[6] = vp3[u];
-            qb.v[7] = va[u];
+            qb.v[7] = va[it][u];
             // w taps z, z+1, z+2, z-1
-            qa.v[8] = wa[u];
+            qa.v[8] = wa[it][u];
             qa.v[9] = wa1[u];
             qa.v[10] = wa2[u];
             qa.v[11] = wam[u];
-            qb.v[8] = wb[u];
+            qb.v[8] = wb[it][u];
             qb.v[9] = wb1[u];
             qb.v[10] = wb2[u];
             qb.v[11] = wbm[u];
@@ -727,6 +739,22 @@ int lines_per_block(int n) {
 
 }  // namespace
 
+template <typename T, int SCHEME>
+hipError_t launch_fft_x_fwd_div_s(hipStream_t s, int it, dim3 grid, size_t sm, const FFTPlan& pl, const Cplx<T>* tw,
+                                  const Mesh<T>& m, const T* v, Cplx<T>* spec, int lpb, FastDiv dny, FastDiv d2l) {
+#define LFD(IT) hipLaunchKernelGGL((k_fft_x_fwd_div<T, SCHEME, IT>), grid, dim3(kFftThreads), sm, s, pl, tw, m, v, spec, \
+                                   lpb, dny, d2l)
+    switch (it) {
+        case 1: LFD(1); break;
+        case 2: LFD(2); break;
+        case 3: LFD(3); break;
+        case 4: LFD(4); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef LFD
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_fft_x_fwd_div(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const Mesh<T>& m, int scheme,
                                 const T* v, Cplx<T>* spec) {
@@ -736,16 +764,13 @@ hipError_t launch_fft_x_fwd_div(hipStream_t s, const FFTPlan& pl, const Cplx<T>*
     const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems((size_t)lpb * pl.n + 2 * lpb) + lds_elems(pl.n)) *
                       sizeof(Cplx<T>);
     const FastDiv dny((unsigned)m.ny), d2l((unsigned)(2 * lpb));
+    // batches of 2 elements per thread: up to 4 (lines of up to 2048 points)
+    const int it = (lpb * pl.n + 2 * kFftThreads - 1) / (2 * kFftThreads);
     if (scheme == TC_SCHEME_HW)
-        hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_HW>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb,
-                           dny, d2l);
-    else if (scheme == TC_SCHEME_VV)
-        hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_VV>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb,
-                           dny, d2l);
-    else
-        hipLaunchKernelGGL((k_fft_x_fwd_div<T, TC_SCHEME_AB3>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, m, v, spec, lpb,
-                           dny, d2l);
-    return hipGetLastError();
+        return launch_fft_x_fwd_div_s<T, TC_SCHEME_HW>(s, it, dim3(grid), sm, pl, tw, m, v, spec, lpb, dny, d2l);
+    if (scheme == TC_SCHEME_VV)
+        return launch_fft_x_fwd_div_s<T, TC_SCHEME_VV>(s, it, dim3(grid), sm, pl, tw, m, v, spec, lpb, dny, d2l);
+    return launch_fft_x_fwd_div_s<T, TC_SCHEME_AB3>(s, it, dim3(grid), sm, pl, tw, m, v, spec, lpb, dny, d2l);
 }
 
 template <typename T>
