@@ -633,8 +633,10 @@ __device__ __forceinline__ void lane_solve(T* __restrict__ x, size_t xs, const C
 // per block so the waves spread over all CUs, and pipelined row batches. The
 // FP32-bulk engine's FP64 low modes run in blocks of their own, after the bulk
 // blocks, so that no FP32 wave carries a divergent FP64 lane; their forward
-// intermediate stays in double (ylow: nz * nlow * 2).
-template <typename T, int W>
+// intermediate stays in double (ylow: nz * nlow * 2). NBL = 0: the bulk
+// forward intermediate is written over the spectrum; NBL > 0: it stays in LDS
+// (lane-fastest, nz x 64 reals), with NBL batches of rows in the pipeline.
+template <typename T, int W, int NBL>
 __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFacDev<double> flow, const int* lowidx,
                                                 const int* lowmode, int nlow, double* ylow, ZNorm norm, int nbulk) {
     constexpr int FW = zfac_fw(W), UW = zfac_uw(W);
@@ -659,10 +661,17 @@ __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFac
     T* x = reinterpret_cast<T*>(spec) + 2 * mode + part;
     const int col = zfac_col(mode, f.ny);  // the factors of ky and ny - ky are one column
     const size_t nc = f.ncols;
-    // batches of 8 rows in flight: NB - 1 (FP64 lanes hold 5 reals per row of W = 3)
-    constexpr int NB = 3;
-    lane_solve<T, T, W, 8, NB>(x, xs, f.F + (size_t)col * FW, nc * FW, f.U + (size_t)col * UW, nc * UW, x, xs, nz,
-                               mode == 0 && part == 0 && norm.s00 != nullptr, norm);
+    if constexpr (NBL == 0) {
+        // batches of 8 rows in flight: NB - 1 (FP64 lanes hold 5 reals per row of W = 3)
+        constexpr int NB = 3;
+        lane_solve<T, T, W, 8, NB>(x, xs, f.F + (size_t)col * FW, nc * FW, f.U + (size_t)col * UW, nc * UW, x, xs, nz,
+                                   mode == 0 && part == 0 && norm.s00 != nullptr, norm);
+    } else {
+        extern __shared__ __align__(16) unsigned char smem_raw[];
+        T* yl = reinterpret_cast<T*>(smem_raw) + threadIdx.x;
+        lane_solve<T, T, W, 8, NBL>(x, xs, f.F + (size_t)col * FW, nc * FW, f.U + (size_t)col * UW, nc * UW, yl, 64,
+                                    nz, mode == 0 && part == 0 && norm.s00 != nullptr, norm);
+    }
 }
 
 // Forward x transform of one component (cb < 0) or of the product of two
@@ -797,16 +806,47 @@ hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cpl
     return hipGetLastError();
 }
 
+// Blocks of the LDS-intermediate z-solve resident at once on the device.
+template <typename T, int W>
+int zsolve_slots(size_t dyn_lds) {
+    static int slots = 0;
+    static size_t slots_lds = 0;
+    if (slots == 0 || slots_lds != dyn_lds) {
+        int per_cu = 0, dev = 0, ncu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_zsolve<T, W, 3>, 64, dyn_lds) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            ncu < 1)
+            ncu = 0;
+        slots = per_cu * ncu;
+        slots_lds = dyn_lds;
+    }
+    return slots;
+}
+
 template <typename T>
 hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, const ZFacDev<double>& flow,
                          const int* lowidx, const int* lowmode, int nlow, double* ylow, ZNorm norm) {
     const int nbulk = (2 * f.nmodes + 63) / 64;
     const int nlowb = lowidx ? (2 * nlow + 63) / 64 : 0;
     const dim3 grid(nbulk + nlowb);
-    if (f.W == 3)
-        hipLaunchKernelGGL((k_zsolve<T, 3>), grid, dim3(64), 0, s, spec, f, flow, lowidx, lowmode, nlow, ylow, norm, nbulk);
-    else
-        hipLaunchKernelGGL((k_zsolve<T, 1>), grid, dim3(64), 0, s, spec, f, flow, lowidx, lowmode, nlow, ylow, norm, nbulk);
+    // The forward intermediate goes to LDS (64 lanes x nz reals per block)
+    // when every block of the launch is resident at once; otherwise the LDS
+    // would cut the resident waves below what the row latency needs, and the
+    // intermediate goes over the spectrum instead.
+    const size_t sm = 64 * (size_t)f.nz * sizeof(T);
+    const bool lds = sm <= 64 * 1024 && (int)grid.x <= (f.W == 3 ? zsolve_slots<T, 3>(sm) : zsolve_slots<T, 1>(sm));
+#define LZS(W_, N_) hipLaunchKernelGGL((k_zsolve<T, W_, N_>), grid, dim3(64), N_ ? sm : 0, s, spec, f, flow, lowidx, \
+                                       lowmode, nlow, ylow, norm, nbulk)
+    if (f.W == 3) {
+        if (lds) LZS(3, 3);
+        else LZS(3, 0);
+    } else {
+        if (lds) LZS(1, 3);
+        else LZS(1, 0);
+    }
+#undef LZS
     return hipGetLastError();
 }
 
