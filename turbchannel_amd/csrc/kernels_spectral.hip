@@ -175,7 +175,143 @@ __device__ __forceinline__ int dif_pos(const FFTPlan& pl, int k) {
     return p;
 }
 
-constexpr int kFftThreads = 256;
+constexpr int kFftThreadsC = 256;  // threads of the transform blocks
+// ---------------------------------------------------------------------------
+// The same transform with the length known at compile time (the lengths of the
+// benchmark grids): the radix sequence make_plan chooses, as constants, so
+// that the butterfly numbering, the LDS addresses, the twiddle indices and the
+// digit reversal of the output are constant divisions and offsets instead of
+// run-time FastDiv chains. The arithmetic of every butterfly is that of
+// dif_pass.
+// ---------------------------------------------------------------------------
+struct CPlan {
+    int nst = 0;
+    int radix[kMaxStages] = {}, span[kMaxStages] = {}, tws[kMaxStages] = {};
+};
+constexpr CPlan cplan(int n) {
+    CPlan p;
+    int r = n;
+    while (r % 8 == 0) {
+        p.radix[p.nst++] = 8;
+        r /= 8;
+    }
+    if (r % 4 == 0) {
+        p.radix[p.nst++] = 4;
+        r /= 4;
+    }
+    if (r % 6 == 0) {
+        p.radix[p.nst++] = 6;
+        r /= 6;
+    }
+    for (int f = 2; f <= 5; f++) {
+        if (f == 4) continue;
+        while (r % f == 0) {
+            p.radix[p.nst++] = f;
+            r /= f;
+        }
+    }
+    int L = n;
+    for (int st = 0; st < p.nst; st++) {
+        p.tws[st] = n / L;
+        p.span[st] = L / p.radix[st];
+        L /= p.radix[st];
+    }
+    return p;
+}
+
+template <typename T, int N, int ST>
+__device__ __forceinline__ void dif_pass_c(Cplx<T>* buf, int nlines, const Cplx<T>* __restrict__ tw, T s) {
+    constexpr CPlan P = cplan(N);
+    constexpr int R = P.radix[ST], S = P.span[ST], L = S * R, TWS = P.tws[ST], NR = N / R;
+    const int total = NR * nlines;
+    for (int t = threadIdx.x; t < total; t += kFftThreadsC) {
+        const int l = t / NR, b = t - l * NR;
+        const int blk = b / S, j = b - blk * S;
+        const int e0 = l * N + blk * L + j;
+        Cplx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = buf[ldsi(e0 + r * S)];
+        bfly<T, R>(v, s);
+        if (S > 1 && j != 0) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                Cplx<T> w = tw[ldsi(r * j * TWS)];
+                w.y *= -s;  // forward table; conjugate for the inverse
+                v[r] = cmul(v[r], w);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) buf[ldsi(e0 + r * S)] = v[r];
+    }
+}
+
+template <typename T, int N, int ST = 0>
+__device__ __forceinline__ void lds_fft_c(Cplx<T>* buf, int nlines, const Cplx<T>* __restrict__ tw, T s) {
+    constexpr CPlan P = cplan(N);
+    if constexpr (ST < P.nst) {
+        dif_pass_c<T, N, ST>(buf, nlines, tw, s);
+        __syncthreads();
+        lds_fft_c<T, N, ST + 1>(buf, nlines, tw, s);
+    }
+}
+
+template <int N>
+__device__ __forceinline__ int dif_pos_c(int k) {
+    constexpr CPlan P = cplan(N);
+    int p = 0;
+#pragma unroll
+    for (int st = 0; st < P.nst; st++) {
+        const int q = k / P.radix[st];
+        p += (k - q * P.radix[st]) * P.span[st];
+        k = q;
+    }
+    return p;
+}
+
+// NC > 0: the compile-time length NC; NC = 0: the run-time plan.
+template <typename T, int NC>
+__device__ __forceinline__ void fft_lines(Cplx<T>* buf, const FFTPlan& pl, int nlines, const Cplx<T>* __restrict__ tw,
+                                          T s) {
+    if constexpr (NC > 0)
+        lds_fft_c<T, NC>(buf, nlines, tw, s);
+    else
+        lds_fft(buf, pl, nlines, FastDiv(), pl.n, 1, tw, s);
+}
+template <int NC>
+__device__ __forceinline__ int fft_pos(const FFTPlan& pl, int k) {
+    if constexpr (NC > 0)
+        return dif_pos_c<NC>(k);
+    else
+        return dif_pos(pl, k);
+}
+template <int NC>
+__device__ __forceinline__ int div_n(const FFTPlan& pl, int t) {
+    if constexpr (NC > 0)
+        return t / NC;
+    else
+        return pl.dn.div(t);
+}
+
+// Line lengths with a compile-time transform (the benchmark grids' NX and NY).
+#define TC_FFT_LENGTHS(X)          \
+    case 64: X(64); break;         \
+    case 96: X(96); break;         \
+    case 128: X(128); break;       \
+    case 160: X(160); break;       \
+    case 192: X(192); break;       \
+    case 384: X(384); break;       \
+    case 512: X(512); break;
+
+// Lines per transform block: 12 KiB of LDS for the lines (fwd_div adds as much
+// for its u rows); 12 and 48 KiB measured slower at 192x160 and 384x384.
+template <typename T>
+constexpr int lines_per_block(int n) {
+    const int per_line = 2 * n * (int)sizeof(Cplx<T>);
+    const int l = (24 * 1024) / per_line;
+    return l < 1 ? 1 : (l > 16 ? 16 : l);
+}
+
+constexpr int kFftThreads = kFftThreadsC;
 constexpr int kTwRegs = 4;  // twiddle loads per thread issued with the data loads (N <= 1024 in one round)
 
 // Staging into LDS: every global load of a thread's share is issued before any
@@ -450,13 +586,14 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
 // y transform of lpb contiguous lines (line L = iz*NXH + kx, NY values), in place.
 // With psum != nullptr (inverse transform of the pressure solve) the (0,0) mode
 // of each plane first gets the normalisation X00[iz] -= psum * C[iz] (FP64).
-template <typename T>
+template <typename T, int NC>
 __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>* __restrict__ tw, Cplx<T>* spec,
-                                                        int nlines, FastDiv dnxh, int lpb, T s,
+                                                        int nlines, FastDiv dnxh, int lpb_, T s,
                                                         const double* __restrict__ psum, const double* __restrict__ C) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
-    const int N = pl.n;
+    const int N = NC > 0 ? NC : pl.n;
+    const int lpb = NC > 0 ? lines_per_block<T>(NC) : lpb_;
     Cplx<T>* twl = b0 + lds_elems(lpb * N);
     const int L0 = blockIdx.x * lpb;
     const int nl = min(lpb, nlines - L0);
@@ -475,7 +612,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
             if (t >= tot) continue;
             Cplx<T> w = t < tin ? v[u] : Cplx<T>{T(0), T(0)};
             if (psum) {
-                const int l = pl.dn.div(t), n = t - l * N, L = L0 + l;
+                const int l = div_n<NC>(pl, t), n = t - l * N, L = L0 + l;
                 const int iz = dnxh.div(L);
                 if (n == 0 && L == iz * (int)dnxh.d && l < nl) w.x = T((double)w.x - psum[0] * C[iz]);
             }
@@ -484,10 +621,10 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_y(FFTPlan pl, const Cplx<T>
     }
     tws.store(twl, tw, N);
     __syncthreads();
-    lds_fft(b0, pl, lpb, FastDiv(), N, 1, twl, s);
-    for (int t = threadIdx.x; t < tin; t += blockDim.x) {
-        const int l = pl.dn.div(t), n = t - l * N;
-        base[t] = b0[ldsi(l * N + dif_pos(pl, n))];
+    fft_lines<T, NC>(b0, pl, lpb, twl, s);
+    for (int t = threadIdx.x; t < tin; t += kFftThreads) {
+        const int l = div_n<NC>(pl, t), n = t - l * N;
+        base[t] = b0[ldsi(l * N + fft_pos<NC>(pl, n))];
     }
 }
 
@@ -738,13 +875,6 @@ __global__ void k_spec_crop(const Cplx<T>* __restrict__ fine, int nxh, int ny, C
     coarse[t] = v;
 }
 
-template <typename T>
-int lines_per_block(int n) {
-    // 12 KiB of LDS per block for the lines (fwd_div adds as much for its u rows)
-    const int per_line = 2 * n * (int)sizeof(Cplx<T>);
-    int l = (24 * 1024) / per_line;  // 12 and 48 KiB measured slower at 192x160 and 384x384
-    return l < 1 ? 1 : (l > 16 ? 16 : l);
-}
 
 }  // namespace
 
@@ -801,8 +931,13 @@ hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cpl
     const int nlines = nxh * nz;
     const int grid = (nlines + lpb - 1) / lpb;
     const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
-    hipLaunchKernelGGL((k_fft_y<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, spec, nlines, FastDiv((unsigned)nxh),
-                       lpb, T(inverse ? 1 : -1), psum, C);
+#define LFY(NC_) hipLaunchKernelGGL((k_fft_y<T, NC_>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, spec, nlines, \
+                                   FastDiv((unsigned)nxh), lpb, T(inverse ? 1 : -1), psum, C)
+    switch (pl.n) {
+        TC_FFT_LENGTHS(LFY)
+        default: LFY(0);
+    }
+#undef LFY
     return hipGetLastError();
 }
 
