@@ -339,18 +339,18 @@ struct TwStage {
 // rows 2l and 2l+1 as real and imaginary parts) to spec[(iz*NXH + kx)*NY + iy]:
 // consecutive threads take consecutive rows of one kx, so each store
 // instruction writes runs of 2*lpb consecutive values.
-template <typename T>
+template <typename T, int NC>
 __device__ __forceinline__ void store_half_spectra(const Cplx<T>* res, const FFTPlan& pl, int ny, FastDiv dny,
                                                    FastDiv d2l, int q0, int nrows, Cplx<T>* __restrict__ spec) {
-    const int N = pl.n, nl = (int)d2l.d / 2, nxh = N / 2 + 1;
+    const int N = NC > 0 ? NC : pl.n, nl = NC > 0 ? lines_per_block<T>(NC) : (int)d2l.d / 2, nxh = N / 2 + 1;
     // one thread per (kx, line): the line's two rows are neighbours in y (NY is
     // even), so their two spectral values are one 2-complex store
-    for (int t = threadIdx.x; t < nxh * nl; t += blockDim.x) {
-        const int k = d2l.div(2 * t), l = t - k * nl;
+    for (int t = threadIdx.x; t < nxh * nl; t += kFftThreads) {
+        const int k = NC > 0 ? t / nl : d2l.div(2 * t), l = t - k * nl;
         const int row = 2 * (q0 + l);
         if (row >= nrows) continue;
-        const Cplx<T> z = res[ldsi(l * N + dif_pos(pl, k))];
-        const Cplx<T> zc = res[ldsi(l * N + dif_pos(pl, k == 0 ? 0 : N - k))];
+        const Cplx<T> z = res[ldsi(l * N + fft_pos<NC>(pl, k))];
+        const Cplx<T> zc = res[ldsi(l * N + fft_pos<NC>(pl, k == 0 ? 0 : N - k))];
         // row 2l: A = (Z + conj Zc)/2; row 2l+1: B = -i (Z - conj Zc)/2
         const Cplx<T> a = {T(0.5) * (z.x + zc.x), T(0.5) * (z.y - zc.y)};
         const Cplx<T> b = {T(0.5) * (z.y + zc.y), T(-0.5) * (z.x - zc.x)};
@@ -377,13 +377,14 @@ struct Cell3 {
     T x, y, z;
 };
 
-template <typename T, int SCHEME, int IT>
+template <typename T, int SCHEME, int IT, int NC>
 __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const Cplx<T>* __restrict__ tw, Mesh<T> m,
                                                                 const T* __restrict__ v, Cplx<T>* __restrict__ spec,
-                                                                int lpb, FastDiv dny, FastDiv d2l) {
+                                                                int lpb_, FastDiv dny, FastDiv d2l) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
-    const int N = pl.n, nrows = m.ny * m.nz;
+    const int N = NC > 0 ? NC : pl.n, nrows = m.ny * m.nz;
+    const int lpb = NC > 0 ? lines_per_block<T>(NC) : lpb_;
     Cplx<T>* b1 = b0 + lds_elems(lpb * N);
     Cplx<T>* twl = b1 + lds_elems(lpb * N + 2 * lpb);
     // XCD-swizzled: the y and z taps of the divergence read rows of neighbouring
@@ -409,7 +410,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int t = min((int)threadIdx.x + (it * U + u) * kFftThreads, tot - 1);
-                const int l = pl.dn.div(t), n = t - l * N;
+                const int l = div_n<NC>(pl, t), n = t - l * N;
                 const int ra = min(2 * (q0 + l), nrows - 2);  // rows ra, ra + 1 of one plane
                 ca[it][u] = *reinterpret_cast<const Cell3<T>*>(v + 3 * ((size_t)ra * N + n));
                 cb[it][u] = *reinterpret_cast<const Cell3<T>*>(v + 3 * ((size_t)(ra + 1) * N + n));
@@ -420,7 +421,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
             for (int u = 0; u < U; u++) {
                 const int t = (int)threadIdx.x + (it * U + u) * kFftThreads;
                 if (t < tot) {
-                    const int l = pl.dn.div(t), n = t - l * N;
+                    const int l = div_n<NC>(pl, t), n = t - l * N;
                     ub[2 * l * N + n] = ca[it][u].x;
                     ub[(2 * l + 1) * N + n] = cb[it][u].x;
                 }
@@ -443,7 +444,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = min(t0 + u * kFftThreads, tot - 1);
-            const int l = pl.dn.div(t), n = t - l * N;
+            const int l = div_n<NC>(pl, t), n = t - l * N;
             const int ra = min(2 * (q0 + l), nrows - 2);  // rows ra, ra + 1 of one plane
             const int z = dny.div(ra), iy = ra - z * ny;
             iz[u] = z;
@@ -520,22 +521,23 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
         }
     }
     __syncthreads();
-    lds_fft(b0, pl, lpb, FastDiv(), N, 1, twl, T(-1));
-    store_half_spectra(b0, pl, m.ny, dny, d2l, q0, nrows, spec);
+    fft_lines<T, NC>(b0, pl, lpb, twl, T(-1));
+    store_half_spectra<T, NC>(b0, pl, m.ny, dny, d2l, q0, nrows, spec);
 }
 
 // Inverse x transform: half spectra of rows 2q, 2q+1 -> real rows (times scale).
 // A thread reads the two rows' values of one kx together (consecutive ky), so
 // each load instruction covers runs of lpb value pairs, and writes the
 // complex line A + iB at k and, by the conjugate symmetry, at N - k.
-template <typename T>
+template <typename T, int NC>
 __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cplx<T>* __restrict__ tw,
                                                             const Cplx<T>* __restrict__ spec, T* __restrict__ out,
-                                                            int nrows, int ny, FastDiv dny, int lpb, FastDiv dlpb,
+                                                            int nrows, int ny, FastDiv dny, int lpb_, FastDiv dlpb,
                                                             T scale) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
-    const int N = pl.n, nxh = N / 2 + 1, half = N / 2;
+    const int N = NC > 0 ? NC : pl.n, nxh = N / 2 + 1, half = N / 2;
+    const int lpb = NC > 0 ? lines_per_block<T>(NC) : lpb_;
     Cplx<T>* twl = b0 + lds_elems(lpb * N);
     // XCD-contiguous block order: a block reads 2*lpb values of each kx, part of a
     // cache line whose rest the next block reads, which then runs on the same XCD
@@ -550,7 +552,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = min(t0 + u * kFftThreads, tst - 1);
-            const int kk = dlpb.div(t), l = t - kk * lpb;
+            const int kk = (NC > 0 ? t / lpb : dlpb.div(t)), l = t - kk * lpb;
             const int ra = min(2 * (q0 + l), nrows - 2);  // rows ra, ra + 1: one plane, consecutive ky
             const int iz = dny.div(ra), iy = ra - iz * ny;
             q[u] = *reinterpret_cast<const Pair*>(spec + ((size_t)iz * nxh + kk) * ny + iy);
@@ -559,7 +561,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
         for (int u = 0; u < U; u++) {
             const int t = t0 + u * kFftThreads;
             if (t >= tst) continue;
-            const int kk = dlpb.div(t), l = t - kk * lpb;
+            const int kk = (NC > 0 ? t / lpb : dlpb.div(t)), l = t - kk * lpb;
             const int ra = 2 * (q0 + l);
             Cplx<T> a = ra < nrows ? q[u].a : Cplx<T>{T(0), T(0)};
             Cplx<T> b = ra + 1 < nrows ? q[u].b : Cplx<T>{T(0), T(0)};
@@ -573,11 +575,11 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_inv(FFTPlan pl, const Cpl
     }
     tws.store(twl, tw, N);
     __syncthreads();
-    lds_fft(b0, pl, lpb, FastDiv(), N, 1, twl, T(1));
-    for (int t = threadIdx.x; t < lpb * N; t += blockDim.x) {
-        const int l = pl.dn.div(t), n = t - l * N;
+    fft_lines<T, NC>(b0, pl, lpb, twl, T(1));
+    for (int t = threadIdx.x; t < lpb * N; t += kFftThreads) {
+        const int l = div_n<NC>(pl, t), n = t - l * N;
         const int ra = 2 * (q0 + l), rb = ra + 1;
-        const Cplx<T> z = b0[ldsi(l * N + dif_pos(pl, n))];
+        const Cplx<T> z = b0[ldsi(l * N + fft_pos<NC>(pl, n))];
         if (ra < nrows) out[(size_t)ra * N + n] = z.x * scale;
         if (rb < nrows) out[(size_t)rb * N + n] = z.y * scale;
     }
@@ -853,7 +855,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_q(FFTPlan pl, const C
     tws.store(twl, tw, N);
     __syncthreads();
     lds_fft(b0, pl, lpb, FastDiv(), N, 1, twl, T(-1));
-    store_half_spectra(b0, pl, ny, dny, d2l, q0, nrows, spec);
+    store_half_spectra<T, 0>(b0, pl, ny, dny, d2l, q0, nrows, spec);
 }
 
 // Half spectrum of the coarse grid holding the modes a sharp cut-off keeps:
@@ -878,18 +880,28 @@ __global__ void k_spec_crop(const Cplx<T>* __restrict__ fine, int nxh, int ny, C
 
 }  // namespace
 
+// Batches of 2 elements per thread in the divergence staging: up to 4 (lines of
+// up to 2048 points).
+constexpr int fwd_div_batches(int lpb, int n) { return (lpb * n + 2 * kFftThreads - 1) / (2 * kFftThreads); }
+
 template <typename T, int SCHEME>
 hipError_t launch_fft_x_fwd_div_s(hipStream_t s, int it, dim3 grid, size_t sm, const FFTPlan& pl, const Cplx<T>* tw,
                                   const Mesh<T>& m, const T* v, Cplx<T>* spec, int lpb, FastDiv dny, FastDiv d2l) {
-#define LFD(IT) hipLaunchKernelGGL((k_fft_x_fwd_div<T, SCHEME, IT>), grid, dim3(kFftThreads), sm, s, pl, tw, m, v, spec, \
-                                   lpb, dny, d2l)
-    switch (it) {
-        case 1: LFD(1); break;
-        case 2: LFD(2); break;
-        case 3: LFD(3); break;
-        case 4: LFD(4); break;
-        default: return hipErrorInvalidValue;
+#define LFD(IT, NC_) hipLaunchKernelGGL((k_fft_x_fwd_div<T, SCHEME, IT, NC_>), grid, dim3(kFftThreads), sm, s, pl, tw, m, \
+                                        v, spec, lpb, dny, d2l)
+#define LFDC(NC_) LFD(fwd_div_batches(lines_per_block<T>(NC_), NC_), NC_)
+    switch (pl.n) {
+        TC_FFT_LENGTHS(LFDC)
+        default:
+            switch (it) {
+                case 1: LFD(1, 0); break;
+                case 2: LFD(2, 0); break;
+                case 3: LFD(3, 0); break;
+                case 4: LFD(4, 0); break;
+                default: return hipErrorInvalidValue;
+            }
     }
+#undef LFDC
 #undef LFD
     return hipGetLastError();
 }
@@ -903,8 +915,7 @@ hipError_t launch_fft_x_fwd_div(hipStream_t s, const FFTPlan& pl, const Cplx<T>*
     const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems((size_t)lpb * pl.n + 2 * lpb) + lds_elems(pl.n)) *
                       sizeof(Cplx<T>);
     const FastDiv dny((unsigned)m.ny), d2l((unsigned)(2 * lpb));
-    // batches of 2 elements per thread: up to 4 (lines of up to 2048 points)
-    const int it = (lpb * pl.n + 2 * kFftThreads - 1) / (2 * kFftThreads);
+    const int it = fwd_div_batches(lpb, pl.n);
     if (scheme == TC_SCHEME_HW)
         return launch_fft_x_fwd_div_s<T, TC_SCHEME_HW>(s, it, dim3(grid), sm, pl, tw, m, v, spec, lpb, dny, d2l);
     if (scheme == TC_SCHEME_VV)
@@ -919,8 +930,13 @@ hipError_t launch_fft_x_inv(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw,
     const int nlines = (nrows + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
     const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems(pl.n)) * sizeof(Cplx<T>);
-    hipLaunchKernelGGL((k_fft_x_inv<T>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, spec, out, nrows, ny,
-                       FastDiv((unsigned)ny), lpb, FastDiv((unsigned)lpb), scale);
+#define LFI(NC_) hipLaunchKernelGGL((k_fft_x_inv<T, NC_>), dim3(grid), dim3(kFftThreads), sm, s, pl, tw, spec, out, \
+                                   nrows, ny, FastDiv((unsigned)ny), lpb, FastDiv((unsigned)lpb), scale)
+    switch (pl.n) {
+        TC_FFT_LENGTHS(LFI)
+        default: LFI(0);
+    }
+#undef LFI
     return hipGetLastError();
 }
 
