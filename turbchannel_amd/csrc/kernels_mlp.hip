@@ -31,10 +31,13 @@ typedef __attribute__((ext_vector_type(4))) float f32x4;
 constexpr int kLda = 72;          // LDS row stride of a [64 points][<=64] bfloat16 tile (16 B pad)
 constexpr int kTile = 64 * kLda;  // elements per tile
 
-__device__ __forceinline__ u16 f2bf(float f) {
-    unsigned u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (u16)(u >> 16);
+// FP32 -> bfloat16, nearest even: gfx950's v_cvt_pk_bf16_f32 (two values per
+// instruction when converted in pairs).
+__device__ __forceinline__ u16 f2bf(float f) { return __builtin_bit_cast(u16, (__bf16)f); }
+__device__ __forceinline__ unsigned f2bf2(float a, float b) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, bf16x2));
 }
 __device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((unsigned)h << 16); }
 
@@ -48,11 +51,18 @@ constexpr float kGa = 0.044715f;
 constexpr float kLog2e = 1.4426950408889634f;
 // GELU, tanh form: 0.5 z (1 + tanh u) = z s with s = 1 / (1 + exp(-2u)),
 // u = sqrt(2/pi) (z + 0.044715 z^3), evaluated with the hardware exp2 and
-// reciprocal (no libm call); derivative s + 2 z s (1 - s) u'(z).
+// reciprocal (no libm call) as s = 1 / (1 + 2^(z (A + B z^2))), A = -2 log2(e)
+// sqrt(2/pi), B = 0.044715 A; derivative s + s (1 - s) z (2 sqrt(2/pi) +
+// 6 sqrt(2/pi) 0.044715 z^2).
+constexpr float kGeA = -2.f * kLog2e * kGc, kGeB = kGeA * kGa;
+constexpr float kGeD1 = 2.f * kGc, kGeD3 = 6.f * kGc * kGa;
+__device__ __forceinline__ float gelu_s(float z, float z2) {
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z * __builtin_fmaf(kGeB, z2, kGeA)));
+}
+__device__ __forceinline__ float gelu(float z) { return z * gelu_s(z, z * z); }
 __device__ __forceinline__ float gelu(float z, float& der) {
-    const float u = kGc * (z + kGa * z * z * z);
-    const float s = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-2.f * kLog2e * u));
-    der = s + 2.f * z * s * (1.f - s) * kGc * (1.f + 3.f * kGa * z * z);
+    const float z2 = z * z, s = gelu_s(z, z2);
+    der = __builtin_fmaf(s * (1.f - s), z * __builtin_fmaf(kGeD3, z2, kGeD1), s);
     return z * s;
 }
 
@@ -97,7 +107,8 @@ __device__ __forceinline__ void load4(const u16* p, u16 (&v)[4]) {
     v[3] = (u16)(q.y >> 16);
 }
 
-// Hidden-layer epilogue: act = bf16(gelu(acc + b)), der = bf16(gelu'(acc + b)).
+// Hidden-layer epilogue: act = bf16(gelu(acc + b)), der = bf16(gelu'(acc + b));
+// bias == nullptr: the accumulators already hold the bias.
 template <bool DER>
 __device__ __forceinline__ void hidden_out(const f32x4 (&acc)[4][4], const float* __restrict__ bias, u16* act,
                                            u16* der) {
@@ -107,18 +118,20 @@ __device__ __forceinline__ void hidden_out(const f32x4 (&acc)[4][4], const float
         const int f0 = 16 * mt + 4 * g;
         float bb[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) bb[i] = bias[f0 + i];
+        for (int i = 0; i < 4; i++) bb[i] = bias ? bias[f0 + i] : 0.f;  // folded away without bias
 #pragma unroll
         for (int nt = 0; nt < 4; nt++) {
-            u16 a4[4], d4[4];
+            float a[4], d[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                float d;
-                a4[i] = f2bf(gelu(acc[mt][nt][i] + bb[i], d));
-                d4[i] = f2bf(d);
+                const float z = bias ? acc[mt][nt][i] + bb[i] : acc[mt][nt][i];
+                if (DER)
+                    a[i] = gelu(z, d[i]);
+                else
+                    a[i] = gelu(z);
             }
-            store4(act + (16 * nt + r) * kLda + f0, a4);
-            if (DER) store4(der + (16 * nt + r) * kLda + f0, d4);
+            *(uint2*)(act + (16 * nt + r) * kLda + f0) = make_uint2(f2bf2(a[0], a[1]), f2bf2(a[2], a[3]));
+            if (DER) *(uint2*)(der + (16 * nt + r) * kLda + f0) = make_uint2(f2bf2(d[0], d[1]), f2bf2(d[2], d[3]));
         }
     }
 }
@@ -160,8 +173,10 @@ __device__ __forceinline__ void load_fwd_regs(const MlpDev& w, FwdRegs& R) {
     for (int ks = 0; ks < 2; ks++) R.a4[0][ks] = *(const u16x8*)(w.w + kW4p + r * 64 + 32 * ks + 8 * g);
 }
 
+// The accumulators start from the layer's bias (rows 16 mt + 4 g + i).
 template <int MT, int KS>
-__device__ __forceinline__ void mfma_layer_r(const u16x8 (&a)[MT][KS], const u16* act, f32x4 (&acc)[MT][4]) {
+__device__ __forceinline__ void mfma_layer_r(const u16x8 (&a)[MT][KS], const u16* act, const float* __restrict__ bias,
+                                             f32x4 (&acc)[MT][4]) {
     const int r = lane() & 15, g = lane() >> 4;
     u16x8 b[4][KS];
 #pragma unroll
@@ -172,7 +187,8 @@ __device__ __forceinline__ void mfma_layer_r(const u16x8 (&a)[MT][KS], const u16
     for (int mt = 0; mt < MT; mt++)
 #pragma unroll
         for (int nt = 0; nt < 4; nt++) {
-            f32x4 c = {0.f, 0.f, 0.f, 0.f};
+            const float* bm = bias + 16 * mt + 4 * g;
+            f32x4 c = {bm[0], bm[1], bm[2], bm[3]};
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) c = mfma(a[mt][ks], b[nt][ks], c);
             acc[mt][nt] = c;
@@ -183,25 +199,22 @@ __device__ __forceinline__ void mfma_layer_r(const u16x8 (&a)[MT][KS], const u16
 __device__ __forceinline__ void forward_wave_r(const FwdRegs& R, const float* __restrict__ bias, u16* buf,
                                                f32x4 (&y)[4]) {
     f32x4 acc[4][4];
-    mfma_layer_r<4, 1>(R.a1, buf, acc);
+    mfma_layer_r<4, 1>(R.a1, buf, bias, acc);
     __syncthreads();
-    hidden_out<false>(acc, bias, buf, nullptr);
+    hidden_out<false>(acc, nullptr, buf, nullptr);
     __syncthreads();
-    mfma_layer_r<4, 2>(R.a2, buf, acc);
+    mfma_layer_r<4, 2>(R.a2, buf, bias + 64, acc);
     __syncthreads();
-    hidden_out<false>(acc, bias + 64, buf, nullptr);
+    hidden_out<false>(acc, nullptr, buf, nullptr);
     __syncthreads();
-    mfma_layer_r<4, 2>(R.a3, buf, acc);
+    mfma_layer_r<4, 2>(R.a3, buf, bias + 128, acc);
     __syncthreads();
-    hidden_out<false>(acc, bias + 128, buf, nullptr);
+    hidden_out<false>(acc, nullptr, buf, nullptr);
     __syncthreads();
     f32x4 o[1][4];
-    mfma_layer_r<1, 2>(R.a4, buf, o);
-    const int g = lane() >> 4;
+    mfma_layer_r<1, 2>(R.a4, buf, bias + 192, o);
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) y[nt][i] = o[0][nt][i] + bias[192 + 4 * g + i];
+    for (int nt = 0; nt < 4; nt++) y[nt] = o[0][nt];
 }
 
 __global__ __launch_bounds__(64) void k_mlp_forward(MlpDev w, const float* __restrict__ x, long long n,
