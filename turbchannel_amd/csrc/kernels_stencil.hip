@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <type_traits>
 
+#include <type_traits>
 #include "diag.h"
 #include "kernels.h"
 #include "stencil.h"
@@ -432,14 +433,32 @@ __device__ __forceinline__ void flux_planes(const Mesh<T>& m, const FluxArgs<T>&
                 k[0] += m.sour;
             }
         } else {  // ExplicitTerm (staggered/func_global.h:642-649)
-            T c[3], v[3];
-            if (SCHEME == TC_SCHEME_HW) conv_hw(m, acc, iz, c);
-            else conv_vv(m, acc, iz, c, comp);
-            if (VISC == 1) visc_naive<T, TURB>(m, acc, nu, iz, v, comp);
-            else visc_cross(m, acc, iz, v, comp);
-            k[0] = (m.sour + c[0]) + v[0];
-            k[1] = (T(0) + c[1]) + v[1];
-            k[2] = iz == 0 ? T(0) : (T(0) + c[2]) + v[2];
+            auto explicit_term = [&](auto interior) {
+                constexpr bool INT = decltype(interior)::value;
+                T c[3], v[3];
+                if (SCHEME == TC_SCHEME_HW) conv_hw<INT>(m, acc, iz, c);
+                else conv_vv<INT>(m, acc, iz, c, comp);
+                if (VISC == 1) visc_naive<INT, T, TURB>(m, acc, nu, iz, v, comp);
+                else visc_cross<INT>(m, acc, iz, v, comp);
+                k[0] = (m.sour + c[0]) + v[0];
+                k[1] = (T(0) + c[1]) + v[1];
+                k[2] = !INT && iz == 0 ? T(0) : (T(0) + c[2]) + v[2];
+            };
+            // FP64 constant viscosity: planes 3..nz-4 take a copy of the stencils
+            // with every wall test folded away (a uniform branch)
+            if constexpr (sizeof(T) == 8 && VISC == 0 && !TURB) {
+                if (iz >= 3 && iz < m.nz - 3) explicit_term(std::true_type{});
+                else explicit_term(std::false_type{});
+            } else {
+                T c[3], v[3];
+                if (SCHEME == TC_SCHEME_HW) conv_hw<false>(m, acc, iz, c);
+                else conv_vv<false>(m, acc, iz, c, comp);
+                if (VISC == 1) visc_naive<false, T, TURB>(m, acc, nu, iz, v, comp);
+                else visc_cross<false>(m, acc, iz, v, comp);
+                k[0] = (m.sour + c[0]) + v[0];
+                k[1] = (T(0) + c[1]) + v[1];
+                k[2] = iz == 0 ? T(0) : (T(0) + c[2]) + v[2];
+            }
         }
         if (NC == 3) {  // the kept component moves to slot 0
             k[0] = comp == 0 ? k[0] : comp == 1 ? k[1] : k[2];

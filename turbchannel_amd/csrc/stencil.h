@@ -24,10 +24,12 @@ __device__ __forceinline__ int zcl(int i, int nz) { return min(max(i, 0), nz - 1
 // ---------------------------------------------------------------------------
 // Convection, Harlow-Welch (staggered/func_global.h:345-370)
 // ---------------------------------------------------------------------------
-template <typename T, class A>
+template <bool INT, typename T, class A>
 __device__ __forceinline__ void conv_hw(const Mesh<T>& m, const A& acc, int iz, T f[3]) {
+    // wall tests: on the interior planes (INT) every one is false
+    const int zt = INT ? 3 : iz, NZt = INT ? 7 : m.nz;
     const int NZ = m.nz;
-    const bool bot = iz == 0, top = iz == NZ - 1;
+    const bool bot = zt == 0, top = zt == NZt - 1;
     const T dz = m.DZ[iz], dzm = bot ? T(0) : m.DZ[zcl(iz - 1, NZ)];
     const T inv_hz = m.IDZ[iz];        // 1 / DZ[iz]
     const T inv_hz_alt = m.IHDN[iz];   // 2 / (DZ[iz] + DZ[iz-1]), DZ[-1] = 0
@@ -72,9 +74,11 @@ __device__ __forceinline__ void conv_hw(const Mesh<T>& m, const A& acc, int iz, 
 #define XBAR(A, dy, dz) (T(9) * (A(0, dy, dz) + A(-1, dy, dz)) - (A(-2, dy, dz) + A(1, dy, dz)))
 #define YBAR(A, dx, dz) (T(9) * (A(dx, 0, dz) + A(dx, -1, dz)) - (A(dx, -2, dz) + A(dx, 1, dz)))
 
-template <typename T, class AC>
+template <bool INT, typename T, class AC>
 __device__ __forceinline__ T zbar_vv(const Mesh<T>& m, const AC& acc, int c, int dx, int dy, int iz,
                                      const T* __restrict__ A) {
+    // wall tests: on the interior planes (INT) every one is false
+    const int zt = INT ? 3 : iz, NZt = INT ? 7 : m.nz;
     // 9*(F(0)*A[iz] + F(-1)*A[iz-1]) - F(-2)*A[iz-2] - F(+1)*A[iz+1] with the wall rules
     const int NZ = m.nz;
     const T f0 = F3(vel, c, dx, dy, 0), fm = F3(vel, c, dx, dy, -1);
@@ -82,16 +86,17 @@ __device__ __forceinline__ T zbar_vv(const Mesh<T>& m, const AC& acc, int c, int
     const T amm = A[zcl(iz - 2, NZ)], ap = A[zcl(iz + 1, NZ)], a_first = A[0], a_last = A[NZ - 1];
     T r = T(9) * (f0 * A[iz] + fm * A[iz - 1]);
     const T lo = fmm * amm, lo_wall = -fm * a_first;
-    r -= (iz == 1) ? lo_wall : lo;
+    r -= (zt == 1) ? lo_wall : lo;
     const T hi = fp * ap, hi_wall = -f0 * a_last;
-    r -= (iz == NZ - 1) ? hi_wall : hi;
+    r -= (zt == NZt - 1) ? hi_wall : hi;
     return r;
 }
 
 // only: -1 all components, else the one component (wave-uniform) the caller keeps.
-template <typename T, class A>
+template <bool INT, typename T, class A>
 __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, T f[3], int only = -1) {
-    const int NZ = m.nz;
+    // wall tests: on the interior planes (INT) every one is false
+    const int zt = INT ? 3 : iz, NZt = INT ? 7 : m.nz;
     const T C27 = T(1.0 / 27.0);
     const T hx = m.hx, hy = m.hy, hxy = hx * hy;
     const T dz = m.DZ[iz], dzw = m.DZW[iz];
@@ -118,14 +123,14 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
         const T xm1 = hxy * XBAR(WW, 0, -1), x2 = hxy * XBAR(WW, 0, 2);
         const T p1 = UU(0, 0, 1), p2 = UU(0, 0, 2), p3 = UU(0, 0, 3);
         const T m1 = UU(0, 0, -1), m2 = UU(0, 0, -2), m3 = UU(0, 0, -3);
-        const T bl = iz == 0 ? T(0) : x0;
-        const T br = iz == NZ - 1 ? T(0) : x1;
-        const T bbl = iz == 1 ? T(0) : iz == 0 ? br : xm1;
-        const T bbr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? bl : x2;
-        const T q_r = iz == NZ - 1 ? T(0) : u0 + p1;
-        const T q_rr = u0 + (iz == NZ - 3 ? -p2 : iz == NZ - 2 ? T(0) : iz == NZ - 1 ? m2 : p3);
-        const T q_l = iz == 0 ? T(0) : m1 + u0;
-        const T q_ll = u0 + (iz == 2 ? -m2 : iz == 1 ? T(0) : iz == 0 ? p2 : m3);
+        const T bl = zt == 0 ? T(0) : x0;
+        const T br = zt == NZt - 1 ? T(0) : x1;
+        const T bbl = zt == 1 ? T(0) : zt == 0 ? br : xm1;
+        const T bbr = zt == NZt - 2 ? T(0) : zt == NZt - 1 ? bl : x2;
+        const T q_r = zt == NZt - 1 ? T(0) : u0 + p1;
+        const T q_rr = u0 + (zt == NZt - 3 ? -p2 : zt == NZt - 2 ? T(0) : zt == NZt - 1 ? m2 : p3);
+        const T q_l = zt == 0 ? T(0) : m1 + u0;
+        const T q_ll = u0 + (zt == 2 ? -m2 : zt == 1 ? T(0) : zt == 0 ? p2 : m3);
         f0 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
     }
     }
@@ -151,52 +156,52 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
         const T ym1 = hxy * YBAR(WW, 0, -1), y2 = hxy * YBAR(WW, 0, 2);
         const T p1 = VV(0, 0, 1), p2 = VV(0, 0, 2), p3 = VV(0, 0, 3);
         const T m1 = VV(0, 0, -1), m2 = VV(0, 0, -2), m3 = VV(0, 0, -3);
-        const T bl = iz == 0 ? T(0) : y0;
-        const T br = iz == NZ - 1 ? T(0) : y1;
-        const T bbl = iz == 1 ? T(0) : iz == 0 ? br : ym1;
-        const T bbr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? bl : y2;
-        const T q_r = iz == NZ - 1 ? T(0) : v0 + p1;
-        const T q_rr = v0 + (iz == NZ - 3 ? -p2 : iz == NZ - 2 ? T(0) : iz == NZ - 1 ? m2 : p3);
-        const T q_l = iz == 0 ? T(0) : m1 + v0;
-        const T q_ll = v0 + (iz == 2 ? -m2 : iz == 1 ? T(0) : iz == 0 ? p2 : m3);
+        const T bl = zt == 0 ? T(0) : y0;
+        const T br = zt == NZt - 1 ? T(0) : y1;
+        const T bbl = zt == 1 ? T(0) : zt == 0 ? br : ym1;
+        const T bbr = zt == NZt - 2 ? T(0) : zt == NZt - 1 ? bl : y2;
+        const T q_r = zt == NZt - 1 ? T(0) : v0 + p1;
+        const T q_rr = v0 + (zt == NZt - 3 ? -p2 : zt == NZt - 2 ? T(0) : zt == NZt - 1 ? m2 : p3);
+        const T q_l = zt == 0 ? T(0) : m1 + v0;
+        const T q_ll = v0 + (zt == 2 ? -m2 : zt == 1 ? T(0) : zt == 0 ? p2 : m3);
         f1 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
     }
     }
     __builtin_amdgcn_sched_barrier(0);
     // ---- w-equation (no unknown at iz = 0)
-    if (iz != 0 && (only < 0 || only == 2)) {
+    if (zt != 0 && (only < 0 || only == 2)) {
         {
-            const T br = hy * zbar_vv(m, acc, 0, 1, 0, iz, m.DZ);
-            const T bl = hy * zbar_vv(m, acc, 0, 0, 0, iz, m.DZ);
-            const T bbr = hy * zbar_vv(m, acc, 0, 2, 0, iz, m.DZW);
-            const T bbl = hy * zbar_vv(m, acc, 0, -1, 0, iz, m.DZW);
+            const T br = hy * zbar_vv<INT>(m, acc, 0, 1, 0, iz, m.DZ);
+            const T bl = hy * zbar_vv<INT>(m, acc, 0, 0, 0, iz, m.DZ);
+            const T bbr = hy * zbar_vv<INT>(m, acc, 0, 2, 0, iz, m.DZW);
+            const T bbl = hy * zbar_vv<INT>(m, acc, 0, -1, 0, iz, m.DZW);
             f2 += br * (w0 + WW(1, 0, 0)) - C27 * bbr * (w0 + WW(3, 0, 0)) - bl * (WW(-1, 0, 0) + w0) + C27 * bbl * (WW(-3, 0, 0) + w0);
         }
         {
             // The reference scales this y-face flux by hy (staggered/func_global.h:475-487); kept.
-            const T br = hy * zbar_vv(m, acc, 1, 0, 1, iz, m.DZ);
-            const T bl = hy * zbar_vv(m, acc, 1, 0, 0, iz, m.DZ);
-            const T bbr = hy * zbar_vv(m, acc, 1, 0, 2, iz, m.DZW);
-            const T bbl = hy * zbar_vv(m, acc, 1, 0, -1, iz, m.DZW);
+            const T br = hy * zbar_vv<INT>(m, acc, 1, 0, 1, iz, m.DZ);
+            const T bl = hy * zbar_vv<INT>(m, acc, 1, 0, 0, iz, m.DZ);
+            const T bbr = hy * zbar_vv<INT>(m, acc, 1, 0, 2, iz, m.DZW);
+            const T bbl = hy * zbar_vv<INT>(m, acc, 1, 0, -1, iz, m.DZW);
             f2 += br * (w0 + WW(0, 1, 0)) - C27 * bbr * (w0 + WW(0, 3, 0)) - bl * (WW(0, -1, 0) + w0) + C27 * bbl * (WW(0, -3, 0) + w0);
         }
         {
             const T wzl = WW(0, 0, -1);
             const T p1 = WW(0, 0, 1), p2 = WW(0, 0, 2), p3 = WW(0, 0, 3);
             const T m2 = WW(0, 0, -2), m3 = WW(0, 0, -3);
-            const T wzr = iz == NZ - 1 ? T(0) : p1;
-            const T wzll = iz == 1 ? w0 : m2;
-            const T wzrr = iz == NZ - 2 ? T(0) : iz == NZ - 1 ? w0 : p2;
-            const T wzlll = iz == 1 ? p1 : iz == 2 ? wzl : m3;
-            const T wzrrr = iz == NZ - 3 ? T(0) : iz == NZ - 2 ? p1 : iz == NZ - 1 ? wzl : p3;
+            const T wzr = zt == NZt - 1 ? T(0) : p1;
+            const T wzll = zt == 1 ? w0 : m2;
+            const T wzrr = zt == NZt - 2 ? T(0) : zt == NZt - 1 ? w0 : p2;
+            const T wzlll = zt == 1 ? p1 : zt == 2 ? wzl : m3;
+            const T wzrrr = zt == NZt - 3 ? T(0) : zt == NZt - 2 ? p1 : zt == NZt - 1 ? wzl : p3;
             const T bl = hxy * (T(9) * (wzl + w0) - (wzll + wzr));
             const T br = hxy * (T(9) * (w0 + wzr) - (wzl + wzrr));
             const T bbl = hxy * (T(9) * (wzll + wzl) - (wzlll + w0));
             const T bbr = hxy * (T(9) * (wzr + wzrr) - (w0 + wzrrr));
             f2 += br * (w0 + wzr) - bl * (wzl + w0);
-            const T qr = iz < NZ - 2 ? (w0 + wzrrr) : iz == NZ - 2 ? (-p1 + w0) : (wzl + w0);
+            const T qr = zt < NZt - 2 ? (w0 + wzrrr) : zt == NZt - 2 ? (-p1 + w0) : (wzl + w0);
             f2 -= C27 * bbr * qr;
-            const T ql = iz > 2 ? (wzlll + w0) : iz == 2 ? (-wzl + w0) : (p1 + w0);
+            const T ql = zt > 2 ? (wzlll + w0) : zt == 2 ? (-wzl + w0) : (p1 + w0);
             f2 += C27 * bbl * ql;
         }
     }
@@ -209,10 +214,11 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
 // ---------------------------------------------------------------------------
 // Viscous term, constant viscosity (staggered/func_global.h:533-562)
 // ---------------------------------------------------------------------------
-template <typename T, class A>
+template <bool INT, typename T, class A>
 __device__ __forceinline__ void visc_cross(const Mesh<T>& m, const A& acc, int iz, T f[3], int only = -1) {
-    const int NZ = m.nz;
-    const bool bot = iz == 0, top = iz == NZ - 1;
+    // wall tests: on the interior planes (INT) every one is false
+    const int zt = INT ? 3 : iz, NZt = INT ? 7 : m.nz;
+    const bool bot = zt == 0, top = zt == NZt - 1;
     const T inv_hz = m.IDZ[iz];        // 1 / DZ[iz]
     const T inv_hz_alt = m.IHDN[iz];   // 2 / (DZ[iz] + DZ[iz-1]), DZ[-1] = 0
     const T ihx2 = m.inv_hx * m.inv_hx, ihy2 = m.inv_hy * m.inv_hy;
@@ -249,12 +255,13 @@ __device__ __forceinline__ void visc_cross(const Mesh<T>& m, const A& acc, int i
 // Viscous term, div(2 (nu + nu_t) S) (staggered/func_global.h:565-625),
 // including the edge-average of sxy_190 at :598 as the reference has it.
 // ---------------------------------------------------------------------------
-template <typename T, bool TURB, class A, class NU>
+template <bool INT, typename T, bool TURB, class A, class NU>
 __device__ __forceinline__ void visc_naive(const Mesh<T>& m, const A& acc, const NU& nu, int iz, T f[3], int only = -1) {
-    const int NZ = m.nz;
+    // wall tests: on the interior planes (INT) every one is false
+    const int zt = INT ? 3 : iz, NZt = INT ? 7 : m.nz;
     const T visc = m.visc;
     auto mu = [&](int dx, int dy, int dz) -> T { return TURB ? visc + nu(dx, dy, dz) : visc; };
-    const bool bot = iz == 0, top = iz == NZ - 1;
+    const bool bot = zt == 0, top = zt == NZt - 1;
     const T dz = m.DZ[iz];
     const T inv_hz = m.IDZ[iz];         // 1 / DZ[iz]
     const T inv_hz_up = m.IHUP[iz];     // 2 / (DZ[iz] + DZ[iz+1]), DZ[nz] = 0
@@ -304,7 +311,7 @@ __device__ __forceinline__ void visc_naive(const Mesh<T>& m, const A& acc, const
         const T syz_091 = top ? wall : in;
         f1 += (syz_091 - syz_099()) * inv_hz;
     }
-    if (iz != 0 && (only < 0 || only == 2)) {
+    if (zt != 0 && (only < 0 || only == 2)) {
         const T sxz_109 = ((UU(1, 0, 0) - UU(1, 0, -1)) * inv_hz_down + (WW(1, 0, 0) - w0) * ihx) * q *
                           (mu(0, 0, -1) + mu(1, 0, -1) + mu0 + mu(1, 0, 0));
         f2 += (sxz_109 - sxz_909()) * ihx;
