@@ -768,14 +768,97 @@ __device__ __forceinline__ void lane_solve(T* __restrict__ x, size_t xs, const C
     if (n00) norm.s00[0] = psum;  // applied by the inverse y transform (NormalizePressure)
 }
 
+// lane_solve with nz = NZ known at compile time and the forward intermediate
+// held in registers (NZ reals per lane): the row loops are unrolled, so every
+// intermediate row has a register of its own, and the backward sweep reads
+// them instead of a second pass over the spectrum. Same arithmetic as
+// lane_solve.
+template <typename T, int W, int B, int NB, int NZ>
+__device__ __forceinline__ void lane_solve_reg(T* __restrict__ x, size_t xs, const T* __restrict__ F, size_t fs,
+                                               const T* __restrict__ U, size_t us, bool n00, const ZNorm& norm) {
+    static_assert(NZ % B == 0, "rows in whole batches");
+    constexpr int NBAT = NZ / B;
+    T yr[NZ];
+    T y1 = 0, y2 = 0, y3 = 0;
+    {
+        T b[NB][B], m1[NB][B], m2[NB][B], m3[NB][B], d[NB][B];
+        auto load = [&](int kb, T(&bb)[B], T(&q1)[B], T(&q2)[B], T(&q3)[B], T(&qd)[B]) {
+            const int i0 = (kb < NBAT ? kb : NBAT - 1) * B;
+#pragma unroll
+            for (int k = 0; k < B; k++) {
+                bb[k] = x[(i0 + k) * xs];
+                load_fwd_row<T, W>(F + (i0 + k) * fs, q1[k], q2[k], q3[k], qd[k]);
+            }
+        };
+#pragma unroll
+        for (int sb = 0; sb < NB - 1; sb++) load(sb, b[sb], m1[sb], m2[sb], m3[sb], d[sb]);
+#pragma unroll
+        for (int kb = 0; kb < NBAT; kb++) {
+            const int sb = kb % NB, nx = (kb + NB - 1) % NB;
+            load(kb + NB - 1, b[nx], m1[nx], m2[nx], m3[nx], d[nx]);
+#pragma unroll
+            for (int k = 0; k < B; k++) {
+                T v = b[sb][k];
+                if (W == 3) v = v + m3[sb][k] * y3;
+                if (W == 3) v = v + m2[sb][k] * y2;
+                v = v + m1[sb][k] * y1;
+                v = v * d[sb][k];
+                yr[kb * B + k] = v;
+                y3 = y2;
+                y2 = y1;
+                y1 = v;
+            }
+            __builtin_amdgcn_sched_barrier(0);  // keep each batch's loads in its place
+        }
+    }
+    double psum = 0.0;
+    T z1 = 0, z2 = 0, z3 = 0;
+    // the stores' addresses from a pointer the compiler cannot equate with x, so
+    // that it recomputes them instead of keeping the forward sweep's 2 NZ
+    // address registers alive across the sweeps
+    T* xw = x;
+    asm volatile("" : "+v"(xw));
+    {
+        T u1[NB][B], u2[NB][B], u3[NB][B];
+        auto load = [&](int kb, T(&q1)[B], T(&q2)[B], T(&q3)[B]) {
+            const int i0 = NZ - 1 - (kb < NBAT ? kb : NBAT - 1) * B;
+#pragma unroll
+            for (int k = 0; k < B; k++) load_bwd_row<T, W>(U + (i0 - k) * us, q1[k], q2[k], q3[k]);
+        };
+#pragma unroll
+        for (int sb = 0; sb < NB - 1; sb++) load(sb, u1[sb], u2[sb], u3[sb]);
+#pragma unroll
+        for (int kb = 0; kb < NBAT; kb++) {
+            const int sb = kb % NB, nx = (kb + NB - 1) % NB;
+            load(kb + NB - 1, u1[nx], u2[nx], u3[nx]);
+#pragma unroll
+            for (int k = 0; k < B; k++) {
+                const int i = NZ - 1 - (kb * B + k);
+                T v = yr[i];
+                v = v - u1[sb][k] * z1;
+                if (W == 3) v = v - u2[sb][k] * z2;
+                if (W == 3) v = v - u3[sb][k] * z3;
+                xw[i * xs] = v;
+                if (n00) psum += norm.Wt[i] * (double)v;
+                z3 = z2;
+                z2 = z1;
+                z1 = v;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if (n00) norm.s00[0] = psum;  // applied by the inverse y transform (NormalizePressure)
+}
+
 // Few independent systems (2 per mode) for the latency of every row: one wave
 // per block so the waves spread over all CUs, and pipelined row batches. The
 // FP32-bulk engine's FP64 low modes run in blocks of their own, after the bulk
 // blocks, so that no FP32 wave carries a divergent FP64 lane; their forward
 // intermediate stays in double (ylow: nz * nlow * 2). NBL = 0: the bulk
 // forward intermediate is written over the spectrum; NBL > 0: it stays in LDS
-// (lane-fastest, nz x 64 reals), with NBL batches of rows in the pipeline.
-template <typename T, int W, int NBL>
+// (lane-fastest, nz x 64 reals), with NBL batches of rows in the pipeline;
+// NZR > 0 (nz == NZR): it stays in registers (lane_solve_reg).
+template <typename T, int W, int NBL, int NZR = 0>
 __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFacDev<double> flow, const int* lowidx,
                                                 const int* lowmode, int nlow, double* ylow, ZNorm norm, int nbulk) {
     constexpr int FW = zfac_fw(W), UW = zfac_uw(W);
@@ -800,7 +883,11 @@ __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFac
     T* x = reinterpret_cast<T*>(spec) + 2 * mode + part;
     const int col = zfac_col(mode, f.ny);  // the factors of ky and ny - ky are one column
     const size_t nc = f.ncols;
-    if constexpr (NBL == 0) {
+    if constexpr (NZR > 0) {  // the intermediate in registers (nz == NZR)
+        // one batch of rows ahead: 8 rows (W = 1), 4 rows (W = 3: 5 reals per row)
+        lane_solve_reg<T, W, W == 3 ? 4 : 8, 2, NZR>(x, xs, f.F + (size_t)col * FW, nc * FW, f.U + (size_t)col * UW,
+                                                      nc * UW, mode == 0 && part == 0 && norm.s00 != nullptr, norm);
+    } else if constexpr (NBL == 0) {
         // batches of 8 rows in flight: NB - 1 (FP64 lanes hold 5 reals per row of W = 3)
         constexpr int NB = 3;
         lane_solve<T, T, W, 8, NB>(x, xs, f.F + (size_t)col * FW, nc * FW, f.U + (size_t)col * UW, nc * UW, x, xs, nz,
@@ -990,13 +1077,23 @@ hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, cons
     const bool lds = sm <= 64 * 1024 && (int)grid.x <= (f.W == 3 ? zsolve_slots<T, 3>(sm) : zsolve_slots<T, 1>(sm));
 #define LZS(W_, N_) hipLaunchKernelGGL((k_zsolve<T, W_, N_>), grid, dim3(64), N_ ? sm : 0, s, spec, f, flow, lowidx, \
                                        lowmode, nlow, ylow, norm, nbulk)
+    // FP32 with nz = 192 (the 384 x 384 x 192 grids) when not every block is
+    // resident: the intermediate in registers (192 per lane, 239-247 VGPRs, two
+    // waves per SIMD), which removes the spectrum's second pass (AB3 118 -> 88 us,
+    // fourth order 140 -> 105 us at 384 x 384 x 192)
+    const bool reg = sizeof(T) == 4 && f.nz == 192 && !lds;
+#define LZR(W_) hipLaunchKernelGGL((k_zsolve<T, W_, 0, 192>), grid, dim3(64), 0, s, spec, f, flow, lowidx, lowmode, nlow, \
+                                   ylow, norm, nbulk)
     if (f.W == 3) {
         if (lds) LZS(3, 3);
+        else if (reg) LZR(3);
         else LZS(3, 0);
     } else {
         if (lds) LZS(1, 3);
+        else if (reg) LZR(1);
         else LZS(1, 0);
     }
+#undef LZR
 #undef LZS
     return hipGetLastError();
 }
