@@ -172,7 +172,14 @@ int tc_initial_field(tc_engine* e, int slot, unsigned long long seed, double ubu
 
 /* Mesh data derived on the host from tc_params (staggered/main.cpp:112-156,
  * collocated_AB3/main.cpp:295-300). name: "ZZ","DZ","DZW","GC1","GC2","GCZ",
- * "MAT_A","MAT_D","KERG","KERG_norm2". Returns the length, or -1. No device needed. */
+ * "MAT_A","MAT_D","KERG","KERG_norm2"; and the wall-normal elimination of the
+ * pressure systems (ThomasForPressure, staggered/func_global.h:71-189,
+ * collocated_AB3/func_global.h:8-61) per (kx, |ky|) column: "ZF_M" (W x nz x
+ * columns forward multipliers), "ZF_D" (inverse pivots), "ZF_U" (normalised upper
+ * band), and the constants from which the FP32-bulk z-solve forms the forward
+ * factors of the tridiagonal systems in-lane: "ZF_R" (nz x 4; empty for the
+ * fourth-order scheme), "ZF_KX", "ZF_KY" (wavenumber terms per kx, |ky|). Returns
+ * the length, or -1. No device needed. */
 int tc_mesh_array(const tc_params* p, const char* name, double* out, int cap);
 
 /* ---------------------------------------------------------------------------

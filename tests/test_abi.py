@@ -97,3 +97,47 @@ def test_srk_mesh_data_is_collocated():
     q = tca.make_params(tca.SCHEME_AB3, 16, 12, len(zz), zz, visc=0.01, hx=0.1, hy=0.1)
     for n in ("ZZ", "DZ", "MAT_A"):
         np.testing.assert_array_equal(tca.mesh_array(p, n), tca.mesh_array(q, n))
+
+
+def _inlane_factors(R, KX, KY, nx, ny, nz):
+    """The in-lane forward elimination of the FP32-bulk tridiagonal z-solve
+    (ZInl::row in kernels_spectral.hip), restated in FP64 for every (kx, |ky|)
+    column."""
+    nxh, nyh = nx // 2 + 1, ny // 2 + 1
+    ncols = nxh * nyh
+    kx, ky = np.divmod(np.arange(ncols), nyh)
+    M, D, U = np.zeros((nz, ncols)), np.zeros((nz, ncols)), np.zeros((nz, ncols))
+    mm = KX[kx] + KY[ky]
+    u = np.zeros(ncols)
+    for i, (am, ap, c0, cq) in enumerate(R.reshape(nz, 4)):
+        d = 1.0 / ((c0 + cq * mm) - am * u)
+        u = ap * d
+        M[i], D[i], U[i] = -am, d, u
+    return M, D, U
+
+
+@pytest.mark.parametrize("name,extra", [("hw", {}), ("ab3", {}), ("ab3", {"fd_mode": 0}), ("ab3_stab0", {})])
+def test_zsolve_inlane_constants_reproduce_the_factors(name, extra):
+    """The per-row constants and wavenumber terms from which the FP32-bulk
+    tridiagonal z-solve forms its forward factors in-lane give the host's
+    precomputed elimination (the factors the FP64 engine streams, which the GPU
+    parity tests hold to the oracle's ThomasForPressure) for every column but the
+    pinned (0,0) one, which the FP32-bulk engine solves from the precomputed FP64
+    factors. The fourth-order scheme streams its factors and has no constants."""
+    scheme, nx, ny, nz, zz, kw = parity.problem(name)
+    kw.update(extra)
+    p = tca.make_params(scheme, nx, ny, nz, zz, **kw)
+    ncols = (nx // 2 + 1) * (ny // 2 + 1)
+    Mh = tca.mesh_array(p, "ZF_M").reshape(nz, ncols)
+    Dh = tca.mesh_array(p, "ZF_D").reshape(nz, ncols)
+    Uh = tca.mesh_array(p, "ZF_U").reshape(nz, ncols)
+    with np.errstate(divide="ignore", invalid="ignore"):  # the unpinned (0,0) column may be singular
+        M, D, U = _inlane_factors(tca.mesh_array(p, "ZF_R"), tca.mesh_array(p, "ZF_KX"),
+                                  tca.mesh_array(p, "ZF_KY"), nx, ny, nz)
+    for got, ref, what in ((M, Mh, "M"), (D, Dh, "D"), (U, Uh, "U")):
+        g, r = got[..., 1:], ref[..., 1:]
+        np.testing.assert_allclose(g, r, rtol=1e-11, atol=1e-12 * np.abs(r).max(), err_msg=what)
+    # the (0,0) column differs: its first pivot is doubled by the pin
+    assert not abs(D[0, 0] - Dh[0, 0]) <= 0.1 * abs(Dh[0, 0])
+    vv = tca.make_params(*parity.problem("vv")[:5], **parity.problem("vv")[5])
+    assert tca.mesh_array(vv, "ZF_R").size == 0

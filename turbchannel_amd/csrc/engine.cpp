@@ -89,6 +89,7 @@ struct tc_engine {
     double* d_wplane = nullptr;  // normalisation weights per plane
     void* d_kerg = nullptr;      // KERG in T (plane shift weight, VV)
     void* zfac = nullptr;        // factors in T
+    void* zinl = nullptr;        // in-lane forward constants in T (FP32-bulk): R, KX, KY
     double* zfac_low = nullptr;  // factors of the low modes in double
     int* d_lowidx = nullptr;  // mode -> index in the FP64 low-mode list, or -1
     int* d_lowmode = nullptr; // index in the low-mode list -> mode
@@ -372,6 +373,13 @@ int setup_device(tc_engine* e) {
     if (int r = dalloc(e, &e->d_s00, e->nz)) return r;
     HIPCHK(hipMemcpy(e->d_Wt, zf.Wt.data(), e->nz * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_C, zf.C.data(), e->nz * sizeof(double), hipMemcpyHostToDevice));
+    if (e->fp32 && !zf.R.empty()) {  // the bulk modes form their forward elimination in-lane
+        std::vector<double> all(zf.R);
+        all.insert(all.end(), zf.KX.begin(), zf.KX.end());
+        all.insert(all.end(), zf.KY.begin(), zf.KY.end());
+        if (int r = dalloc_bytes(e, &e->zinl, all.size() * sizeof(T))) return r;
+        if (int r = upload_as<T>(e, (T*)e->zinl, all.data(), all.size())) return r;
+    }
     if (e->fp32) {  // low-mode list and its double factors
         std::vector<int> low;
         for (int kx = 0; kx <= kLowK && kx < e->nxh; kx++)
@@ -422,6 +430,11 @@ ZFacDev<T> zfac_dev(tc_engine* e) {
     const size_t fz = (size_t)e->nz * f.ncols;
     f.F = (const T*)e->zfac;
     f.U = f.F + zfac_fw(f.W) * fz;
+    if (e->zinl) {
+        f.R = (const T*)e->zinl;
+        f.KX = f.R + e->zf.R.size();
+        f.KY = f.KX + e->zf.KX.size();
+    }
     return f;
 }
 
@@ -803,7 +816,18 @@ int tc_mesh_array(const tc_params* pin, const char* name, double* out, int cap) 
     else if (!strcmp(name, "DZ")) v.assign(m.DZ.begin(), m.DZ.begin() + (p->scheme == TC_SCHEME_AB3 ? p->nz - 1 : p->nz));
     else if (!strcmp(name, "MAT_A")) v = m.MAT_A;
     else if (!strcmp(name, "KERG_norm2") && p->scheme == TC_SCHEME_VV) v = {m.KERG_norm2};
-    else if (p->scheme == TC_SCHEME_VV) {
+    else if (!strncmp(name, "ZF_", 3)) {  // wall-normal elimination factors
+        ZFactors f;
+        if (!build_zfactors(*p, m, f).empty()) return -1;
+        const char* q = name + 3;
+        if (!strcmp(q, "M") || !strcmp(q, "U"))
+            for (int c = 0; c < f.W; c++) v.insert(v.end(), (*q == 'M' ? f.M : f.U)[c].begin(), (*q == 'M' ? f.M : f.U)[c].end());
+        else if (!strcmp(q, "D")) v = f.D;
+        else if (!strcmp(q, "R")) v = f.R;
+        else if (!strcmp(q, "KX")) v = f.KX;
+        else if (!strcmp(q, "KY")) v = f.KY;
+        else return -1;
+    } else if (p->scheme == TC_SCHEME_VV) {
         if (!strcmp(name, "DZW")) v = m.DZW;
         else if (!strcmp(name, "GC1")) v = m.GC1;
         else if (!strcmp(name, "GC2")) v = m.GC2;

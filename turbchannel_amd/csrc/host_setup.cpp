@@ -273,6 +273,36 @@ std::string build_zfactors(const tc_params& p, const HostMesh& m, ZFactors& f) {
         }
     }
     if (!err.empty()) return err;
+    // The constants of the in-lane forward elimination of the tridiagonal
+    // systems (the loop above, split into what depends on the row and what on
+    // the wavenumbers)
+    if (W == 1) {
+        const bool hw = p.scheme == TC_SCHEME_HW, fd = p.fd_mode != 0;
+        f.R.assign((size_t)nz * kZRow, 0.0);
+        f.KX.assign(nxh, 0.0);
+        f.KY.assign(nyh, 0.0);
+        for (int i = 0; i < nz; i++) {
+            double* r = &f.R[(size_t)i * kZRow];
+            r[0] = i > 0 ? A[i] : 0.0;
+            r[1] = i < nz - 1 ? A[i + 1] : 0.0;
+            if (i == 0) {
+                r[2] = -A[1];
+                r[3] = hw ? -DZ[0] : (fd ? 0.0 : DZ[0]);
+            } else {
+                r[2] = -A[i] - (i < nz - 1 ? A[i + 1] : 0.0);
+                if (hw) r[3] = -DZ[i];
+                else r[3] = -((fd && i == nz - 1) ? 0.0 : DZ[i - 1]) - (i < nz - 1 ? DZ[i] : 0.0);
+            }
+        }
+        for (int ix = 0; ix < nxh; ix++) {
+            const double c = 1 - std::cos(2. * kPi * ix / nx);
+            f.KX[ix] = hw ? 2 * c * inv_hx * inv_hx : c * p.hy / p.hx;
+        }
+        for (int iy = 0; iy < nyh; iy++) {
+            const double c = 1 - std::cos(2. * kPi * iy / ny);
+            f.KY[iy] = hw ? 2 * c * inv_hy * inv_hy : c * p.hx / p.hy;
+        }
+    }
     // (0,0)-mode normalisation in spectral space (NormalizePressure,
     // staggered/func_global.h:657-675; collocated_AB3/func_global.h:163-170).
     f.Wt.assign(nz, 0.0);

@@ -163,6 +163,13 @@ struct ZFacDev {
     int nmodes = 0, ny = 0;        // spectrum modes, and modes per kx (columns by zfac_col)
     const CT* F = nullptr;
     const CT* U = nullptr;
+    // In-lane forward elimination of the tridiagonal systems (R != nullptr):
+    // per-row constants R[iz][kZRow] and wavenumber terms KX, KY (see
+    // ZFactors); the forward sweep then reads no factors, the backward sweep
+    // still reads U.
+    const CT* R = nullptr;
+    const CT* KX = nullptr;
+    const CT* KY = nullptr;
 };
 constexpr int zfac_fw(int W) { return W == 3 ? 4 : 2; }
 constexpr int zfac_uw(int W) { return W == 3 ? 3 : 1; }

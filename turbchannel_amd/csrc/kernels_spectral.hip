@@ -665,6 +665,26 @@ __device__ __forceinline__ void load_bwd_row(const CT* __restrict__ U, CT& u1, C
     }
 }
 
+// The forward elimination of one tridiagonal system (W = 1) formed in-lane
+// (see ZFactors::R): the row's constants are mode-independent (the same address
+// in every lane, so scalar loads), the mode enters through MM, and the lane
+// carries the previous row's normalised upper entry. Same steps as the host's
+// elimination (build_zfactors); pivots inverted by v_rcp (FP32 lanes only).
+template <typename CT>
+struct ZInl {
+    CT mm = 0, u = 0;  // the mode's MM, U of the previous row
+    __device__ __forceinline__ void init(const CT* __restrict__ KX, const CT* __restrict__ KY, int kx, int aky) {
+        mm = KX[kx] + KY[aky];
+    }
+    // factors of the row whose constants are at R: multiplier and inverse pivot
+    __device__ __forceinline__ void row(const CT* __restrict__ R, CT& m1, CT& d) {
+        static_assert(sizeof(CT) == 4, "in-lane factors for FP32 lanes");
+        d = __builtin_amdgcn_rcpf((R[2] + R[3] * mm) - R[0] * u);
+        m1 = -R[0];
+        u = R[1] * d;
+    }
+};
+
 // One lane's solve: forward sweep x -> y, backward sweep y -> x, in CT
 // arithmetic. Rows move in full batches of B with no per-row conditions (a
 // condition per row splits the code into blocks at whose joins the compiler
@@ -672,10 +692,14 @@ __device__ __forceinline__ void load_bwd_row(const CT* __restrict__ U, CT& u1, C
 // a batch are issued NB-1 batches ahead of the recurrence and before the stores
 // of the batch being finished, since the vector-memory counter counts loads and
 // stores together. The nz % B remaining rows follow one at a time.
-template <typename T, typename CT, int W, int B, int NB>
+// INL (W = 1): the forward factors are formed in-lane by zi from the row
+// constants R (F is not read).
+template <typename T, typename CT, int W, int B, int NB, bool INL = false>
 __device__ __forceinline__ void lane_solve(T* __restrict__ x, size_t xs, const CT* __restrict__ F, size_t fs,
                                            const CT* __restrict__ U, size_t us, CT* __restrict__ y, size_t ys, int nz,
-                                           bool n00, const ZNorm& norm) {
+                                           bool n00, const ZNorm& norm, const CT* __restrict__ R = nullptr,
+                                           ZInl<CT>* zi = nullptr) {
+    static_assert(!INL || W == 1, "in-lane factors for the tridiagonal systems");
     const int nfull = nz / B;
     CT y1 = 0, y2 = 0, y3 = 0;
     auto fwd_row = [&](CT b, CT m1, CT m2, CT m3, CT d, int i) {
@@ -697,7 +721,7 @@ __device__ __forceinline__ void lane_solve(T* __restrict__ x, size_t xs, const C
 #pragma unroll
             for (int k = 0; k < B; k++) {
                 bb[k] = (CT)x[(i0 + k) * xs];
-                load_fwd_row<CT, W>(F + (i0 + k) * fs, q1[k], q2[k], q3[k], qd[k]);
+                if constexpr (!INL) load_fwd_row<CT, W>(F + (i0 + k) * fs, q1[k], q2[k], q3[k], qd[k]);
             }
         };
 #pragma unroll
@@ -710,13 +734,22 @@ __device__ __forceinline__ void lane_solve(T* __restrict__ x, size_t xs, const C
                 const int nx = (sb + NB - 1) % NB;
                 load(kb + NB - 1, b[nx], m1[nx], m2[nx], m3[nx], d[nx]);
 #pragma unroll
-                for (int k = 0; k < B; k++) fwd_row(b[sb][k], m1[sb][k], m2[sb][k], m3[sb][k], d[sb][k], kb * B + k);
+                for (int k = 0; k < B; k++) {
+                    if constexpr (INL) {
+                        CT q1, qd;
+                        zi->row(R + (size_t)(kb * B + k) * kZRow, q1, qd);
+                        fwd_row(b[sb][k], q1, CT(0), CT(0), qd, kb * B + k);
+                    } else {
+                        fwd_row(b[sb][k], m1[sb][k], m2[sb][k], m3[sb][k], d[sb][k], kb * B + k);
+                    }
+                }
             }
         }
     }
     for (int i = nfull * B; i < nz; i++) {
         CT m1, m2 = 0, m3 = 0, d;
-        load_fwd_row<CT, W>(F + i * fs, m1, m2, m3, d);
+        if constexpr (INL) zi->row(R + (size_t)i * kZRow, m1, d);
+        else load_fwd_row<CT, W>(F + i * fs, m1, m2, m3, d);
         fwd_row((CT)x[i * xs], m1, m2, m3, d, i);
     }
     // backward substitution: x[i] = y[i] - U1 x[i+1] - U2 x[i+2] - U3 x[i+3]
@@ -773,10 +806,12 @@ __device__ __forceinline__ void lane_solve(T* __restrict__ x, size_t xs, const C
 // intermediate row has a register of its own, and the backward sweep reads
 // them instead of a second pass over the spectrum. Same arithmetic as
 // lane_solve.
-template <typename T, int W, int B, int NB, int NZ>
+template <typename T, int W, int B, int NB, int NZ, bool INL = false>
 __device__ __forceinline__ void lane_solve_reg(T* __restrict__ x, size_t xs, const T* __restrict__ F, size_t fs,
-                                               const T* __restrict__ U, size_t us, bool n00, const ZNorm& norm) {
+                                               const T* __restrict__ U, size_t us, bool n00, const ZNorm& norm,
+                                               const T* __restrict__ R = nullptr, ZInl<T>* zi = nullptr) {
     static_assert(NZ % B == 0, "rows in whole batches");
+    static_assert(!INL || W == 1, "in-lane factors for the tridiagonal systems");
     constexpr int NBAT = NZ / B;
     T yr[NZ];
     T y1 = 0, y2 = 0, y3 = 0;
@@ -787,7 +822,7 @@ __device__ __forceinline__ void lane_solve_reg(T* __restrict__ x, size_t xs, con
 #pragma unroll
             for (int k = 0; k < B; k++) {
                 bb[k] = x[(i0 + k) * xs];
-                load_fwd_row<T, W>(F + (i0 + k) * fs, q1[k], q2[k], q3[k], qd[k]);
+                if constexpr (!INL) load_fwd_row<T, W>(F + (i0 + k) * fs, q1[k], q2[k], q3[k], qd[k]);
             }
         };
 #pragma unroll
@@ -798,11 +833,20 @@ __device__ __forceinline__ void lane_solve_reg(T* __restrict__ x, size_t xs, con
             load(kb + NB - 1, b[nx], m1[nx], m2[nx], m3[nx], d[nx]);
 #pragma unroll
             for (int k = 0; k < B; k++) {
+                T f1, f2 = 0, f3 = 0, fd;
+                if constexpr (INL) {
+                    zi->row(R + (size_t)(kb * B + k) * kZRow, f1, fd);
+                } else {
+                    f1 = m1[sb][k];
+                    if (W == 3) f2 = m2[sb][k];
+                    if (W == 3) f3 = m3[sb][k];
+                    fd = d[sb][k];
+                }
                 T v = b[sb][k];
-                if (W == 3) v = v + m3[sb][k] * y3;
-                if (W == 3) v = v + m2[sb][k] * y2;
-                v = v + m1[sb][k] * y1;
-                v = v * d[sb][k];
+                if (W == 3) v = v + f3 * y3;
+                if (W == 3) v = v + f2 * y2;
+                v = v + f1 * y1;
+                v = v * fd;
                 yr[kb * B + k] = v;
                 y3 = y2;
                 y2 = y1;
@@ -858,9 +902,10 @@ __device__ __forceinline__ void lane_solve_reg(T* __restrict__ x, size_t xs, con
 // forward intermediate is written over the spectrum; NBL > 0: it stays in LDS
 // (lane-fastest, nz x 64 reals), with NBL batches of rows in the pipeline;
 // NZR > 0 (nz == NZR): it stays in registers (lane_solve_reg).
-template <typename T, int W, int NBL, int NZR = 0>
+template <typename T, int W, int NBL, int NZR = 0, bool INL = false>
 __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFacDev<double> flow, const int* lowidx,
-                                                const int* lowmode, int nlow, double* ylow, ZNorm norm, int nbulk) {
+                                                const int* lowmode, int nlow, double* ylow, ZNorm norm, int nbulk,
+                                                const T* __restrict__ zrow) {
     constexpr int FW = zfac_fw(W), UW = zfac_uw(W);
     const int nm = f.nmodes, nz = f.nz;
     const size_t xs = 2 * (size_t)nm;  // stride between planes, in reals
@@ -883,20 +928,27 @@ __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFac
     T* x = reinterpret_cast<T*>(spec) + 2 * mode + part;
     const int col = zfac_col(mode, f.ny);  // the factors of ky and ny - ky are one column
     const size_t nc = f.ncols;
+    ZInl<T> zi;
+    if constexpr (INL) {
+        const int kx = mode / f.ny, ky = mode - kx * f.ny;
+        zi.init(f.KX, f.KY, kx, min(ky, f.ny - ky));
+    }
     if constexpr (NZR > 0) {  // the intermediate in registers (nz == NZR)
         // one batch of rows ahead: 8 rows (W = 1), 4 rows (W = 3: 5 reals per row)
-        lane_solve_reg<T, W, W == 3 ? 4 : 8, 2, NZR>(x, xs, f.F + (size_t)col * FW, nc * FW, f.U + (size_t)col * UW,
-                                                      nc * UW, mode == 0 && part == 0 && norm.s00 != nullptr, norm);
+        lane_solve_reg<T, W, W == 3 ? 4 : 8, 2, NZR, INL>(x, xs, f.F + (size_t)col * FW, nc * FW,
+                                                           f.U + (size_t)col * UW, nc * UW,
+                                                           mode == 0 && part == 0 && norm.s00 != nullptr, norm, zrow,
+                                                           &zi);
     } else if constexpr (NBL == 0) {
         // batches of 8 rows in flight: NB - 1 (FP64 lanes hold 5 reals per row of W = 3)
         constexpr int NB = 3;
-        lane_solve<T, T, W, 8, NB>(x, xs, f.F + (size_t)col * FW, nc * FW, f.U + (size_t)col * UW, nc * UW, x, xs, nz,
-                                   mode == 0 && part == 0 && norm.s00 != nullptr, norm);
+        lane_solve<T, T, W, 8, NB, INL>(x, xs, f.F + (size_t)col * FW, nc * FW, f.U + (size_t)col * UW, nc * UW, x,
+                                        xs, nz, mode == 0 && part == 0 && norm.s00 != nullptr, norm, zrow, &zi);
     } else {
         extern __shared__ __align__(16) unsigned char smem_raw[];
         T* yl = reinterpret_cast<T*>(smem_raw) + threadIdx.x;
-        lane_solve<T, T, W, 8, NBL>(x, xs, f.F + (size_t)col * FW, nc * FW, f.U + (size_t)col * UW, nc * UW, yl, 64,
-                                    nz, mode == 0 && part == 0 && norm.s00 != nullptr, norm);
+        lane_solve<T, T, W, 8, NBL, INL>(x, xs, f.F + (size_t)col * FW, nc * FW, f.U + (size_t)col * UW, nc * UW, yl,
+                                         64, nz, mode == 0 && part == 0 && norm.s00 != nullptr, norm, zrow, &zi);
     }
 }
 
@@ -1075,15 +1127,43 @@ hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, cons
     // intermediate goes over the spectrum instead.
     const size_t sm = 64 * (size_t)f.nz * sizeof(T);
     const bool lds = sm <= 64 * 1024 && (int)grid.x <= (f.W == 3 ? zsolve_slots<T, 3>(sm) : zsolve_slots<T, 1>(sm));
-#define LZS(W_, N_) hipLaunchKernelGGL((k_zsolve<T, W_, N_>), grid, dim3(64), N_ ? sm : 0, s, spec, f, flow, lowidx, \
-                                       lowmode, nlow, ylow, norm, nbulk)
+    // FP32-bulk engines form the tridiagonal bulk modes' forward factors
+    // in-lane (the (0,0) mode, which the host pins, is one of the FP64 low
+    // modes). AB3 384 x 384 x 192: 90 -> 78 us. The seven-band systems keep the
+    // precomputed factors: the in-lane elimination's chain per row made that
+    // solve slower (105 -> 115-137 us) although it moved 3 instead of 7 factor
+    // reals per row.
+    const bool inl = sizeof(T) == 4 && f.R != nullptr && lowidx != nullptr && f.W == 1;
+#define LZS(W_, N_)                                                                                                     \
+    do {                                                                                                                \
+        if constexpr (sizeof(T) == 4 && W_ == 1) {                                                                      \
+            if (inl) {                                                                                                  \
+                hipLaunchKernelGGL((k_zsolve<T, W_, N_, 0, true>), grid, dim3(64), N_ ? sm : 0, s, spec, f, flow,      \
+                                   lowidx, lowmode, nlow, ylow, norm, nbulk, f.R);                                      \
+                break;                                                                                                  \
+            }                                                                                                           \
+        }                                                                                                               \
+        hipLaunchKernelGGL((k_zsolve<T, W_, N_>), grid, dim3(64), N_ ? sm : 0, s, spec, f, flow, lowidx, lowmode,      \
+                           nlow, ylow, norm, nbulk, (const T*)nullptr);                                                 \
+    } while (0)
     // FP32 with nz = 192 (the 384 x 384 x 192 grids) when not every block is
     // resident: the intermediate in registers (192 per lane, 239-247 VGPRs, two
     // waves per SIMD), which removes the spectrum's second pass (AB3 118 -> 88 us,
     // fourth order 140 -> 105 us at 384 x 384 x 192)
     const bool reg = sizeof(T) == 4 && f.nz == 192 && !lds;
-#define LZR(W_) hipLaunchKernelGGL((k_zsolve<T, W_, 0, 192>), grid, dim3(64), 0, s, spec, f, flow, lowidx, lowmode, nlow, \
-                                   ylow, norm, nbulk)
+#define LZR(W_)                                                                                                         \
+    do {                                                                                                                \
+        if constexpr (sizeof(T) == 4 && W_ == 1) {                                                                      \
+            if (inl) {                                                                                                  \
+                hipLaunchKernelGGL((k_zsolve<T, W_, 0, 192, true>), grid, dim3(64), 0, s, spec, f, flow, lowidx,        \
+                                   lowmode, nlow, ylow, norm, nbulk, f.R);                                              \
+                break;                                                                                                  \
+            }                                                                                                           \
+        }                                                                                                               \
+        if constexpr (sizeof(T) == 4)                                                                                   \
+            hipLaunchKernelGGL((k_zsolve<T, W_, 0, 192>), grid, dim3(64), 0, s, spec, f, flow, lowidx, lowmode, nlow,  \
+                               ylow, norm, nbulk, (const T*)nullptr);                                                   \
+    } while (0)
     if (f.W == 3) {
         if (lds) LZS(3, 3);
         else if (reg) LZR(3);
