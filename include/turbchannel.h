@@ -76,7 +76,9 @@ typedef struct tc_params {
     const double* zz;      /* staggered: nz+1 face coordinates; collocated: nz node coordinates */
     int sgs_model;         /* collocated only, turb_visc = 0: 1 = learned closure (no reference
                               counterpart): each step evaluates tau from the velocity with the
-                              network set by tc_set_sgs_params and subtracts its divergence */
+                              network set by tc_set_sgs_params and subtracts its divergence;
+                              2 = the same with backscatter clipping (tau = 0 at a node where
+                              tau_ij S_ij > 0, S the resolved strain) */
     int imex;              /* TC_SCHEME_SRK: tc_imex method (main_params::IMEXtype, NStages,
                               ButcherE/I, d; collocated_SRK/main.cpp:342-350) */
     double alpha_tau;      /* TC_SCHEME_SRK: alpha = alpha_tau / tau; <= 0 selects the reference's

@@ -202,12 +202,22 @@ def samples(u, nx, ny, nz, zz, hx, hy, u_tau, inv_len, factor):
     return X.reshape(-1, 10), T.reshape(-1, 6)
 
 
-def node_tau(params, u, nx, ny, nz, zz, hx, hy, u_tau, inv_len, emulate=True):
+def stress_strain(y, X):
+    """tau_ij S_ij per sample from the outputs (XX YY ZZ XY XZ YZ) and the gradient
+    inputs g_ij = X[3 i + j], S_ij = (g_ij + g_ji) / 2."""
+    return (y[:, 0] * X[:, 0] + y[:, 1] * X[:, 4] + y[:, 2] * X[:, 8] + y[:, 3] * (X[:, 1] + X[:, 3]) +
+            y[:, 4] * (X[:, 2] + X[:, 6]) + y[:, 5] * (X[:, 5] + X[:, 7]))
+
+
+def node_tau(params, u, nx, ny, nz, zz, hx, hy, u_tau, inv_len, emulate=True, clip=False):
     """Closure stress at every node [N3D, 6] (zero on the wall planes), as the
-    LES step computes it from its own velocity field."""
+    LES step computes it from its own velocity field; clip: zero where
+    tau_ij S_ij > 0 (backscatter)."""
     u = np.asarray(u, dtype=np.float64).reshape(nz, ny, nx, 3)
     X = features([u[..., c] for c in range(3)], zz, hx, hy, u_tau, inv_len).reshape(-1, 10)
     y, _ = forward(params, X, emulate)
+    if clip:
+        y = np.where((stress_strain(y, X) > 0)[:, None], 0.0, y)
     tau = np.zeros((nz, ny, nx, 6))
     tau[1:-1] = (y * u_tau * u_tau).reshape(nz - 2, ny, nx, 6)
     return tau.reshape(-1, 6)

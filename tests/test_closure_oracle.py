@@ -243,3 +243,19 @@ def test_standardizer_fold_is_the_same_network():
     want = y_std * st.st + st.mt
     got, _ = mlp.forward(st.fold(p), X, emulate=False)
     assert np.abs(got - want).max() < 1e-5 * np.abs(want).max()
+
+
+def test_output_scaling_and_filter_width_ratio():
+    """closure.scale_outputs multiplies the network's outputs (the last layer is
+    linear) and filter_width_ratio2 is the squared ratio of the filter widths in
+    wall units, Re_tau L / modes."""
+    from turbchannel_amd import closure
+    p = rand_params(3)
+    X, _ = rand_data(64, seed=4)
+    y = mlp.forward(p, X, emulate=False)[0]
+    y2 = mlp.forward(closure.scale_outputs(p, 2.5), X, emulate=False)[0]
+    np.testing.assert_allclose(y2, 2.5 * y, rtol=1e-6, atol=1e-7)
+    assert closure.filter_width_ratio2(395.0, (96, 96), 395.0, (96, 96)) == pytest.approx(1.0)
+    assert closure.filter_width_ratio2(395.0, (96, 96), 790.0, (96, 96)) == pytest.approx(4.0)
+    assert closure.filter_width_ratio2(395.0, (96, 96), 395.0, (48, 96)) == pytest.approx(2.0)
+    assert closure.filter_width_ratio2(395.0, (96, 96), 590.0, (64, 64)) == pytest.approx((590 / 395 * 1.5) ** 2)

@@ -191,6 +191,44 @@ def test_node_tau_matches_oracle(name):
     assert np.all(got[:nxy] == 0) and np.all(got[-nxy:] == 0)
 
 
+def test_node_tau_with_backscatter_clipping_matches_oracle():
+    """sgs_model = 2: the closure stress is zero at the nodes where tau_ij S_ij > 0.
+    Nodes whose sign of tau_ij S_ij is within the network's bfloat16 rounding of
+    zero may go either way; every other node matches the oracle's clipped stress."""
+    name = "ab3_sgs"
+    nx, ny, nz, zz, kw = sample_case(name)
+    sol = parity.make_oracle(name)
+    scheme, *_, pkw = parity.problem(name)
+    pkw = dict(pkw, sgs_model=2)
+    eng = tca.Engine(scheme, nx, ny, nz, zz, **pkw)
+    p = closure_params()
+    eng.set_sgs_params(p)
+    u = parity.seeded_velocity(sol, seed=31, noise=0.3)
+    eng.set_field(tca.VEL, 2, u)
+    eng.recalc_turb_visc((tca.VEL, 2))
+    got = eng.get_field(tca.TAU, 0)
+    u_tau = kw["visc"] * kw["inv_length_scale"]
+    args = (p, u, nx, ny, nz, zz, kw["hx"], kw["hy"], u_tau, kw["inv_length_scale"])
+    ref = mlp.node_tau(*args, clip=True)
+    raw = mlp.node_tau(*args, clip=False)
+    X = mlp.features([u.reshape(nz, ny, nx, 3)[..., c] for c in range(3)], zz, kw["hx"], kw["hy"], u_tau,
+                     kw["inv_length_scale"]).reshape(-1, 10)
+    inner = slice(nx * ny, -nx * ny)
+    y = raw[inner] / (u_tau * u_tau)
+    ts = mlp.stress_strain(y, X)
+    bound = 2.0 * np.abs(y).sum(axis=1) * np.abs(X[:, :9]).max(axis=1)  # |tau_ij S_ij| <= bound
+    firm = np.abs(ts) > 0.05 * bound
+    clipped = (ts > 0) & firm
+    print(f"clipped {clipped.mean():.2%} of the interior nodes, {firm.mean():.2%} firm")
+    assert 0.05 < clipped.sum() / max(1, firm.sum()) < 0.95
+    g, r = got[inner], ref[inner]
+    e = np.abs(g[firm] - r[firm]).max() / np.abs(r).max()
+    print(f"clipped node tau max-norm rel {e:.2e}")
+    assert e < 2e-2
+    assert np.all(g[clipped] == 0)
+    assert np.all(got[:nx * ny] == 0) and np.all(got[-nx * ny:] == 0)
+
+
 @pytest.mark.parametrize("name", ["ab3_sgs", "ab3_sgs_gal"])
 def test_explicit_term_with_tau(name):
     sol = parity.make_oracle(name)

@@ -34,7 +34,8 @@ def test_closure_protocol_stages_run(tmp_path):
     judge = cp.stage_judge590(cp.QUICK["judge590"], log)
     assert judge["records"] > 0 and judge["filtered_fields"] > 0 and judge["re_tau"] > 0
     res = cp.stage_les590(cp.QUICK["les590"], net, judge, log)
-    assert len(res["les"]) == 4
+    assert len(res["les"]) == len(cp.QUICK["les590"]["cases"])
+    assert res["les"]["learned_16x16x25"]["output_scale"] > 1.0  # a coarser cut-off at a higher Re_tau
     for name, r in res["les"].items():
         assert "diverged" not in r, name
         for k in ("mean_u_err", "mean_u_plus_err", "u_tau_err", "rms_u_err", "uw_err", "seconds_per_flow_through"):

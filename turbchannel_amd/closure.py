@@ -49,6 +49,23 @@ class Standardizer:
         return p.astype(np.float32)
 
 
+def filter_width_ratio2(re_tau_train, modes_train, re_tau_les, modes_les):
+    """(Delta+_les / Delta+_train)^2 for the sharp cut-offs of the training data
+    and of the LES grid on the same box: modes_* are the cut-off mode counts in x
+    and y, the filter width in wall units Re_tau L / modes (geometric mean of x
+    and y)."""
+    r = [(re_tau_les / re_tau_train) * (mt / ml) for mt, ml in zip(modes_train, modes_les)]
+    return float(np.prod(r)) ** (2.0 / len(r))
+
+
+def scale_outputs(params, s):
+    """The network with its outputs multiplied by s (last layer's weights and biases)."""
+    p = np.asarray(params, dtype=np.float64).copy()
+    assert p.size == mlp_param_count()
+    p[_OFF_W4:_OFF_B4 + N_OUT] *= s
+    return p.astype(np.float32)
+
+
 def train(net, dev_x, dev_t, n, *, batch=65536, iters=1000, lr=1e-3, lr_final=None):
     """Adam over contiguous mini-batches of device samples (addresses of float32
     [n][10] inputs and [n][6] targets, already shuffled); returns the losses.
@@ -63,4 +80,4 @@ def train(net, dev_x, dev_t, n, *, batch=65536, iters=1000, lr=1e-3, lr_final=No
     return np.array(losses)
 
 
-__all__ = ["Standardizer", "train", "Mlp"]
+__all__ = ["Standardizer", "train", "Mlp", "filter_width_ratio2", "scale_outputs"]

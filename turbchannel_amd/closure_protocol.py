@@ -28,6 +28,14 @@ each run in one process (device memory does not survive between them):
              count, since the CFL rule halves dt on the finer grids). Errors of mean U+,
              u_tau and the rms/<u'w'> profiles against the judge (interpolated in
              wall units onto the LES nodes) and GPU seconds per flow-through.
+             The network predicts the sub-filter stress in wall units at the
+             training data's filter width; on an LES grid its outputs are scaled
+             by (Delta+_les / Delta+_train)^2 (the Delta^2 of a gradient-type
+             stress at the same gradients in wall units, Delta+ = Re_tau L /
+             cut-off modes): 5.0 for the 64-mode grid at Re_tau 590 against the
+             96-mode cut-off of the Re_tau 395 data. ``learned_unscaled`` runs the
+             network without it; ``learned_clip`` with it and with backscatter
+             clipping (sgs_model 2: no stress where tau_ij S_ij > 0).
 
 Run (from the repository root): ``python -m turbchannel_amd.closure_protocol STAGE
 --out FILE [--net FILE]``; ``--quick`` shrinks every size for a functional check.
@@ -41,7 +49,7 @@ import time
 import numpy as np
 
 from . import FLUX, FP32_BULK, FP64, PRES, SCHEME_AB3, VEL, Engine, Mlp
-from .closure import Standardizer, train
+from .closure import Standardizer, filter_width_ratio2, scale_outputs, train
 from .les_study import Run, plane_stats, profile_stats, spectral_restrict, wall_shear
 
 LX, LY = 2.0 * math.pi, math.pi
@@ -54,16 +62,18 @@ FULL = dict(
                   t_trans_max=60.0, re_tau_turb=450.0, t_develop=15.0, t_avg=25.0, les_cuts=[64, 96],
                   stats_every=20, wall_seconds=960.0),
     les590=dict(re_b=21900.0, re_tau=590.0, gamma=2.0, dt=2e-3, t_wash=40.0, t_avg=40.0, noise=0.3, seed=1,
-                cases=[("learned", 64, 64, 96), ("smagorinsky", 64, 64, 96), ("smagorinsky", 96, 96, 144),
-                       ("no_model", 64, 64, 96)], time_steps=2000),
+                cases=[("learned", 64, 64, 96), ("learned_clip", 64, 64, 96), ("learned_unscaled", 64, 64, 96),
+                       ("smagorinsky", 64, 64, 96), ("smagorinsky", 96, 96, 144), ("no_model", 64, 64, 96)],
+                time_steps=2000),
 )
 QUICK = dict(
     dns395=dict(FULL["dns395"], grid=(64, 48, 33), t_trans_max=0.2, re_tau_turb=1e9, steps_timed=20, n_snap=6,
                 snap_every=10, holdout=2, iters=200, batch=4096, t_develop=0.05),
     judge590=dict(FULL["judge590"], grid=(64, 48, 33), t_trans_max=0.2, re_tau_turb=1e9, t_develop=0.05, t_avg=0.2,
                   les_cuts=[16, 24], wall_seconds=60.0),
-    les590=dict(FULL["les590"], t_wash=0.1, t_avg=0.1, cases=[("learned", 16, 16, 25), ("smagorinsky", 16, 16, 25),
-                                                  ("smagorinsky", 24, 24, 37), ("no_model", 16, 16, 25)],
+    les590=dict(FULL["les590"], t_wash=0.1, t_avg=0.1, cases=[("learned", 16, 16, 25), ("learned_clip", 16, 16, 25),
+                                                  ("smagorinsky", 16, 16, 25), ("smagorinsky", 24, 24, 37),
+                                                  ("no_model", 16, 16, 25)],
                 time_steps=20),
 )
 
@@ -76,12 +86,14 @@ def nodes(nz, gamma):
 
 
 def engine(nx, ny, zz, re_b, re_tau, model, precision=FP32_BULK):
-    """model: 'dns' (FD viscous term), 'no_model', 'smagorinsky', 'learned' (Galerkin viscous term)."""
+    """model: 'dns' (FD viscous term), 'no_model', 'smagorinsky', 'learned', 'learned_clip',
+    'learned_unscaled' (Galerkin viscous term)."""
     visc = 2.0 / re_b
     u_tau = re_tau * visc
     kw = dict(precision=precision, visc=visc, sour_dpdx=u_tau * u_tau, inv_length_scale=re_tau, hx=LX / nx,
               hy=LY / ny, stab_type=1, fd_mode=1, cfr_ubulk=1.0, visc_type=0 if model == "dns" else 1,
-              turb_visc=1 if model == "smagorinsky" else 0, sgs_model=1 if model == "learned" else 0)
+              turb_visc=1 if model == "smagorinsky" else 0,
+              sgs_model=(2 if model == "learned_clip" else 1) if model.startswith("learned") else 0)
     e = Engine(SCHEME_AB3, nx, ny, len(zz), zz, **kw)
     e.graphs(True)
     return e
@@ -142,7 +154,8 @@ def stage_dns395(P, log):
     zz = nodes(nz, P["gamma"])
     visc = 2.0 / P["re_b"]
     f = P["factor"]
-    out = dict(stage="dns395", grid=[nx, ny, nz], re_b=P["re_b"], gamma=P["gamma"], noise=P["noise"])
+    out = dict(stage="dns395", grid=[nx, ny, nz], re_b=P["re_b"], re_tau=P["re_tau"], gamma=P["gamma"],
+               noise=P["noise"], factor=f)
     dns = engine(nx, ny, zz, P["re_b"], P["re_tau"], "dns")
     dns.initial_field(0, seed=P["seed"], ubulk=1.0, noise_rms=P["noise"])
     run = Run(dns, P["dt"])
@@ -289,12 +302,19 @@ def stage_les590(P, net, judge, log):
     for model, lx, ly, lz in P["cases"]:
         zz = nodes(lz, P["gamma"])
         e = engine(lx, ly, zz, P["re_b"], P["re_tau"], model)
-        if model == "learned":
-            e.set_sgs_params(params)
+        if model.startswith("learned"):
+            # the training data's cut-off: the DNS grid over the filter factor
+            gx, gy = net["grid"][0], net["grid"][1]
+            fac = net.get("factor", FULL["dns395"]["factor"])
+            s = filter_width_ratio2(net.get("re_tau", FULL["dns395"]["re_tau"]), (gx // fac, gy // fac),
+                                    P["re_tau"], (lx, ly)) if model != "learned_unscaled" else 1.0
+            e.set_sgs_params(scale_outputs(params, s))
         e.initial_field(0, seed=P["seed"], ubulk=1.0, noise_rms=P["noise"])
         run = Run(e, P["dt"])
         name = f"{model}_{lx}x{ly}x{lz}"
         r = dict(model=model, grid=[lx, ly, lz])
+        if model.startswith("learned"):
+            r["output_scale"] = s
         try:
             run.advance(P["t_wash"], block=500)
             av = stats_over(run, int(round(P["t_avg"] / run.dt)), 10)

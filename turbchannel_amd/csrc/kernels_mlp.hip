@@ -477,10 +477,15 @@ __global__ void k_mlp_reduce_adam(const double* __restrict__ tmp, float* __restr
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-template <typename T>
+// CLIP: backscatter clipping, tau = 0 at a node where the stress would move
+// energy from the sub-filter scales to the resolved ones (tau_ij S_ij > 0, S the
+// resolved strain of the node's own gradient inputs): the outputs of a node,
+// spread over the MFMA accumulator lanes, meet the node's inputs through LDS.
+template <typename T, bool CLIP>
 __global__ __launch_bounds__(64) void k_sgs_tau(Mesh<T> m, MlpDev w, const T* __restrict__ vel, T* __restrict__ tau,
                                                 T u_tau2, T gscale, T yscale) {
     __shared__ __align__(16) u16 buf[kTile];
+    __shared__ float tq[CLIP ? 64 * 7 : 1];
     FwdRegs R;
     load_fwd_regs(w, R);
     const int nxy = m.nxy;
@@ -522,16 +527,36 @@ __global__ __launch_bounds__(64) void k_sgs_tau(Mesh<T> m, MlpDev w, const T* __
         __syncthreads();
         f32x4 o[4];
         forward_wave_r(R, w.b, buf, o);
+        if constexpr (CLIP) {
+            // outputs of point 16 nt + r, feature 4 g + i -> tq[point][feature]
 #pragma unroll
-        for (int nt = 0; nt < 4; nt++) {
-            const long long p = p0 + 16 * nt + r;
-            if (p >= m.n3d) continue;
-            const int pz = (int)(p / nxy);
-            const bool in = pz > 0 && pz < m.nz - 1;
+            for (int nt = 0; nt < 4; nt++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int f = 4 * g + i;
-                if (f < kMlpOut) tau[p * kMlpOut + f] = in ? T(o[nt][i]) * u_tau2 : T(0);
+                for (int i = 0; i < 4; i++)
+                    if (4 * g + i < kMlpOut) tq[(16 * nt + r) * 7 + 4 * g + i] = o[nt][i];
+            __syncthreads();
+            float t6[kMlpOut];
+#pragma unroll
+            for (int f = 0; f < kMlpOut; f++) t6[f] = tq[lane() * 7 + f];
+            // tau_ij S_ij with S_ij = (g_ij + g_ji) / 2, g_ij = xv[3 i + j] (positive scale)
+            const float ts = t6[0] * xv[0] + t6[1] * xv[4] + t6[2] * xv[8] + t6[3] * (xv[1] + xv[3]) +
+                             t6[4] * (xv[2] + xv[6]) + t6[5] * (xv[5] + xv[7]);
+            const bool keep = interior && !(ts > 0.f);
+            if (node < m.n3d)
+#pragma unroll
+                for (int f = 0; f < kMlpOut; f++) tau[node * kMlpOut + f] = keep ? T(t6[f]) * u_tau2 : T(0);
+        } else {
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) {
+                const long long p = p0 + 16 * nt + r;
+                if (p >= m.n3d) continue;
+                const int pz = (int)(p / nxy);
+                const bool in = pz > 0 && pz < m.nz - 1;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int f = 4 * g + i;
+                    if (f < kMlpOut) tau[p * kMlpOut + f] = in ? T(o[nt][i]) * u_tau2 : T(0);
+                }
             }
         }
         __syncthreads();
@@ -652,12 +677,18 @@ hipError_t launch_mlp_reduce(hipStream_t s, const float* partial, int nrec, doub
 }
 
 template <typename T>
-hipError_t launch_sgs_tau(hipStream_t s, const Mesh<T>& m, MlpDev w, const T* vel, T* tau) {
+hipError_t launch_sgs_tau(hipStream_t s, const Mesh<T>& m, MlpDev w, const T* vel, T* tau, bool clip) {
     const double u_tau = (double)m.visc * (double)m.inv_len;
-    static const long long resident = resident_blocks(k_sgs_tau<T>);
-    const long long nb = std::min(((long long)m.n3d + 63) / 64, resident);
-    hipLaunchKernelGGL((k_sgs_tau<T>), dim3((unsigned)nb), dim3(64), 0, s, m, w, vel, tau,
-                       T(u_tau * u_tau), T(1.0 / (u_tau * (double)m.inv_len)), T((double)m.inv_len * kYPlusScale));
+    const T ut2 = T(u_tau * u_tau), gs = T(1.0 / (u_tau * (double)m.inv_len)), ys = T((double)m.inv_len * kYPlusScale);
+    if (clip) {
+        static const long long resident = resident_blocks(k_sgs_tau<T, true>);
+        const long long nb = std::min(((long long)m.n3d + 63) / 64, resident);
+        hipLaunchKernelGGL((k_sgs_tau<T, true>), dim3((unsigned)nb), dim3(64), 0, s, m, w, vel, tau, ut2, gs, ys);
+    } else {
+        static const long long resident = resident_blocks(k_sgs_tau<T, false>);
+        const long long nb = std::min(((long long)m.n3d + 63) / 64, resident);
+        hipLaunchKernelGGL((k_sgs_tau<T, false>), dim3((unsigned)nb), dim3(64), 0, s, m, w, vel, tau, ut2, gs, ys);
+    }
     return hipGetLastError();
 }
 
@@ -676,7 +707,7 @@ template hipError_t launch_sgs_samples<double>(hipStream_t, const Mesh<double>&,
                                                float*, float*);
 template hipError_t launch_sgs_samples<float>(hipStream_t, const Mesh<float>&, const float*, int, int, float, float,
                                               float*, float*);
-template hipError_t launch_sgs_tau<double>(hipStream_t, const Mesh<double>&, MlpDev, const double*, double*);
-template hipError_t launch_sgs_tau<float>(hipStream_t, const Mesh<float>&, MlpDev, const float*, float*);
+template hipError_t launch_sgs_tau<double>(hipStream_t, const Mesh<double>&, MlpDev, const double*, double*, bool);
+template hipError_t launch_sgs_tau<float>(hipStream_t, const Mesh<float>&, MlpDev, const float*, float*, bool);
 
 }  // namespace tc

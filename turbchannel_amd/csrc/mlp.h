@@ -67,7 +67,7 @@ hipError_t launch_mlp_reduce(hipStream_t s, const float* partial, int nrec, doub
 
 // Closure stress at every node of a collocated field (walls: zero), scaled by u_tau^2.
 template <typename T>
-hipError_t launch_sgs_tau(hipStream_t s, const Mesh<T>& m, MlpDev w, const T* vel, T* tau);
+hipError_t launch_sgs_tau(hipStream_t s, const Mesh<T>& m, MlpDev w, const T* vel, T* tau, bool clip);
 
 // Samples at the coarse interior nodes from the 9 filtered fields q (u, v, w, uu,
 // vv, ww, uv, uw, vw; each [nz][nyc][nxc]): x [S][10], y [S][6], S = nxc nyc (nz-2).
