@@ -38,9 +38,15 @@ def hipcc():
     return exe
 
 
+# The z-solve's row loops are unrolled in full (each intermediate row has a
+# place fixed at compile time); their unrolled size exceeds the default limit
+# of a pragma-directed unroll.
+SRC_FLAGS = {"kernels_spectral.hip": ["-mllvm", "-pragma-unroll-threshold=65536"]}
+
+
 def _compile(src):
     out = os.path.join(OBJ, src + ".o")
-    cmd = [hipcc(), *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", out]
+    cmd = [hipcc(), *FLAGS, *SRC_FLAGS.get(src, []), "-x", "hip", "-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compiling {src} failed:\n{r.stdout}\n{r.stderr}")

@@ -802,18 +802,22 @@ __device__ __forceinline__ void lane_solve(T* __restrict__ x, size_t xs, const C
 }
 
 // lane_solve with nz = NZ known at compile time and the forward intermediate
-// held in registers (NZ reals per lane): the row loops are unrolled, so every
-// intermediate row has a register of its own, and the backward sweep reads
-// them instead of a second pass over the spectrum. Same arithmetic as
-// lane_solve.
-template <typename T, int W, int B, int NB, int NZ, bool INL = false>
+// kept on chip: the row loops are unrolled, so each intermediate row has a
+// place fixed at compile time, and the backward sweep reads it there instead of
+// a second pass over the spectrum. The last NR rows (the first the backward
+// sweep needs) are registers, the NL rows before them LDS (yl: lane-fastest,
+// 64 reals a row), the first NZ - NR - NL rows go over the spectrum. Same
+// arithmetic as lane_solve.
+template <typename T, int W, int B, int NB, int NZ, bool INL = false, int NR = NZ, int NL = 0>
 __device__ __forceinline__ void lane_solve_reg(T* __restrict__ x, size_t xs, const T* __restrict__ F, size_t fs,
                                                const T* __restrict__ U, size_t us, bool n00, const ZNorm& norm,
-                                               const T* __restrict__ R = nullptr, ZInl<T>* zi = nullptr) {
-    static_assert(NZ % B == 0, "rows in whole batches");
+                                               const T* __restrict__ R = nullptr, ZInl<T>* zi = nullptr,
+                                               T* __restrict__ yl = nullptr) {
+    static_assert(NZ % B == 0 && NR % B == 0 && NL % B == 0 && NR + NL <= NZ, "rows in whole batches");
+    constexpr int NH = NZ - NR - NL;  // rows over the spectrum
     static_assert(!INL || W == 1, "in-lane factors for the tridiagonal systems");
     constexpr int NBAT = NZ / B;
-    T yr[NZ];
+    T yr[NR];
     T y1 = 0, y2 = 0, y3 = 0;
     {
         T b[NB][B], m1[NB][B], m2[NB][B], m3[NB][B], d[NB][B];
@@ -847,7 +851,10 @@ __device__ __forceinline__ void lane_solve_reg(T* __restrict__ x, size_t xs, con
                 if (W == 3) v = v + f2 * y2;
                 v = v + f1 * y1;
                 v = v * fd;
-                yr[kb * B + k] = v;
+                const int i = kb * B + k;
+                if (i < NH) x[i * xs] = v;
+                else if (i < NH + NL) yl[(i - NH) * 64] = v;
+                else yr[i - NH - NL >= 0 ? i - NH - NL : 0] = v;
                 y3 = y2;
                 y2 = y1;
                 y1 = v;
@@ -862,23 +869,33 @@ __device__ __forceinline__ void lane_solve_reg(T* __restrict__ x, size_t xs, con
     // address registers alive across the sweeps
     T* xw = x;
     asm volatile("" : "+v"(xw));
+    // the intermediate rows over the spectrum come back through another such
+    // pointer, after every store of the forward sweep (memory clobber)
+    const T* xr = x;
+    asm volatile("" : "+v"(xr)::"memory");
     {
-        T u1[NB][B], u2[NB][B], u3[NB][B];
-        auto load = [&](int kb, T(&q1)[B], T(&q2)[B], T(&q3)[B]) {
+        T u1[NB][B], u2[NB][B], u3[NB][B], yy[NB][B];
+        auto load = [&](int kb, T(&q1)[B], T(&q2)[B], T(&q3)[B], T(&qy)[B]) {
             const int i0 = NZ - 1 - (kb < NBAT ? kb : NBAT - 1) * B;
 #pragma unroll
-            for (int k = 0; k < B; k++) load_bwd_row<T, W>(U + (i0 - k) * us, q1[k], q2[k], q3[k]);
+            for (int k = 0; k < B; k++) {
+                load_bwd_row<T, W>(U + (i0 - k) * us, q1[k], q2[k], q3[k]);
+                if (i0 - k < NH) qy[k] = xr[(i0 - k) * xs];
+                else if (i0 - k < NH + NL) qy[k] = yl[(i0 - k - NH) * 64];
+            }
         };
 #pragma unroll
-        for (int sb = 0; sb < NB - 1; sb++) load(sb, u1[sb], u2[sb], u3[sb]);
+        for (int sb = 0; sb < NB - 1; sb++) load(sb, u1[sb], u2[sb], u3[sb], yy[sb]);
 #pragma unroll
         for (int kb = 0; kb < NBAT; kb++) {
             const int sb = kb % NB, nx = (kb + NB - 1) % NB;
-            load(kb + NB - 1, u1[nx], u2[nx], u3[nx]);
+            load(kb + NB - 1, u1[nx], u2[nx], u3[nx], yy[nx]);
 #pragma unroll
             for (int k = 0; k < B; k++) {
                 const int i = NZ - 1 - (kb * B + k);
-                T v = yr[i];
+                T v;
+                if (i < NH + NL) v = yy[sb][k];
+                else v = yr[i - NH - NL >= 0 ? i - NH - NL : 0];
                 v = v - u1[sb][k] * z1;
                 if (W == 3) v = v - u2[sb][k] * z2;
                 if (W == 3) v = v - u3[sb][k] * z3;
@@ -893,6 +910,19 @@ __device__ __forceinline__ void lane_solve_reg(T* __restrict__ x, size_t xs, con
     }
     if (n00) norm.s00[0] = psum;  // applied by the inverse y transform (NormalizePressure)
 }
+
+// Rows of the nz = 192 z-solve's forward intermediate kept in registers and in
+// LDS (the rest go over the spectrum), by bandwidth W. Tridiagonal (in-lane
+// factors): 128 in registers (165 VGPRs, three waves per SIMD, so all 2 317
+// waves of a 384 x 384 x 192 solve are resident at once), 48 in LDS (12 KiB per
+// one-wave block, 144 KiB per CU), 16 over the spectrum: AB3 384 x 384 x 192
+// 80 -> 69 us. Seven-band: all 192 in registers (247 VGPRs, two waves per
+// SIMD); 80 or 96 registers with 48 LDS rows (three waves) measured 104-107 us
+// against 105.
+template <int W>
+constexpr int zreg_rows(int nz) { return W == 1 && nz == 192 ? 128 : nz; }
+template <int W>
+constexpr int zlds_rows(int nz) { return W == 1 && nz == 192 ? 48 : 0; }
 
 // Few independent systems (2 per mode) for the latency of every row: one wave
 // per block so the waves spread over all CUs, and pipelined row batches. The
@@ -933,12 +963,13 @@ __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFac
         const int kx = mode / f.ny, ky = mode - kx * f.ny;
         zi.init(f.KX, f.KY, kx, min(ky, f.ny - ky));
     }
-    if constexpr (NZR > 0) {  // the intermediate in registers (nz == NZR)
+    if constexpr (NZR > 0) {  // the intermediate on chip (nz == NZR)
         // one batch of rows ahead: 8 rows (W = 1), 4 rows (W = 3: 5 reals per row)
-        lane_solve_reg<T, W, W == 3 ? 4 : 8, 2, NZR, INL>(x, xs, f.F + (size_t)col * FW, nc * FW,
-                                                           f.U + (size_t)col * UW, nc * UW,
-                                                           mode == 0 && part == 0 && norm.s00 != nullptr, norm, zrow,
-                                                           &zi);
+        extern __shared__ __align__(16) unsigned char smem_raw[];
+        T* yl = reinterpret_cast<T*>(smem_raw) + threadIdx.x;
+        lane_solve_reg<T, W, W == 3 ? 4 : 8, 2, NZR, INL, zreg_rows<W>(NZR), zlds_rows<W>(NZR)>(
+            x, xs, f.F + (size_t)col * FW, nc * FW, f.U + (size_t)col * UW, nc * UW,
+            mode == 0 && part == 0 && norm.s00 != nullptr, norm, zrow, &zi, yl);
     } else if constexpr (NBL == 0) {
         // batches of 8 rows in flight: NB - 1 (FP64 lanes hold 5 reals per row of W = 3)
         constexpr int NB = 3;
@@ -1147,22 +1178,22 @@ hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, cons
                            nlow, ylow, norm, nbulk, (const T*)nullptr);                                                 \
     } while (0)
     // FP32 with nz = 192 (the 384 x 384 x 192 grids) when not every block is
-    // resident: the intermediate in registers (192 per lane, 239-247 VGPRs, two
-    // waves per SIMD), which removes the spectrum's second pass (AB3 118 -> 88 us,
-    // fourth order 140 -> 105 us at 384 x 384 x 192)
+    // resident: the intermediate on chip (zreg_rows, zlds_rows), which removes
+    // the spectrum's second pass (AB3 118 -> 88 us, fourth order 140 -> 105 us
+    // at 384 x 384 x 192 with all rows in registers)
     const bool reg = sizeof(T) == 4 && f.nz == 192 && !lds;
 #define LZR(W_)                                                                                                         \
     do {                                                                                                                \
         if constexpr (sizeof(T) == 4 && W_ == 1) {                                                                      \
             if (inl) {                                                                                                  \
-                hipLaunchKernelGGL((k_zsolve<T, W_, 0, 192, true>), grid, dim3(64), 0, s, spec, f, flow, lowidx,        \
-                                   lowmode, nlow, ylow, norm, nbulk, f.R);                                              \
+                hipLaunchKernelGGL((k_zsolve<T, W_, 0, 192, true>), grid, dim3(64), zlds_rows<W_>(192) * 64 * sizeof(T), \
+                                   s, spec, f, flow, lowidx, lowmode, nlow, ylow, norm, nbulk, f.R);                    \
                 break;                                                                                                  \
             }                                                                                                           \
         }                                                                                                               \
         if constexpr (sizeof(T) == 4)                                                                                   \
-            hipLaunchKernelGGL((k_zsolve<T, W_, 0, 192>), grid, dim3(64), 0, s, spec, f, flow, lowidx, lowmode, nlow,  \
-                               ylow, norm, nbulk, (const T*)nullptr);                                                   \
+            hipLaunchKernelGGL((k_zsolve<T, W_, 0, 192>), grid, dim3(64), zlds_rows<W_>(192) * 64 * sizeof(T), s,      \
+                               spec, f, flow, lowidx, lowmode, nlow, ylow, norm, nbulk, (const T*)nullptr);             \
     } while (0)
     if (f.W == 3) {
         if (lds) LZS(3, 3);

@@ -254,7 +254,9 @@ __device__ __forceinline__ void sgs_divergence(const Mesh<T>& m, const T q[3][7]
 // The plane loop of K1 for one block; COMP is the momentum component of the
 // calling wave when a cell's three equations go to three waves (else -1), a
 // compile-time value so that each wave's registers hold only its own equation.
-template <typename T, int SCHEME, int VISC, bool TURB, int COMP>
+// ALLINT: every plane of the block's chunk lies in 3..nz-4, so the staggered
+// stencils run with their wall tests folded away.
+template <typename T, int SCHEME, int VISC, bool TURB, int COMP, bool ALLINT = false>
 __device__ __forceinline__ void flux_planes(const Mesh<T>& m, const FluxArgs<T>& a, int ntx, int ntiles, int zchunk,
                                             T* ring, double* sh) {
     constexpr int TY = flux_ty<T, SCHEME, VISC>();
@@ -446,7 +448,9 @@ __device__ __forceinline__ void flux_planes(const Mesh<T>& m, const FluxArgs<T>&
             };
             // FP64 constant viscosity: planes 3..nz-4 take a copy of the stencils
             // with every wall test folded away (a uniform branch)
-            if constexpr (sizeof(T) == 8 && VISC == 0 && !TURB) {
+            if constexpr (ALLINT) {
+                explicit_term(std::true_type{});
+            } else if constexpr (sizeof(T) == 8 && VISC == 0 && !TURB) {
                 if (iz >= 3 && iz < m.nz - 3) explicit_term(std::true_type{});
                 else explicit_term(std::false_type{});
             } else {
@@ -542,6 +546,14 @@ __global__ __launch_bounds__((flux_threads<T, SCHEME, VISC>()), (flux_min_waves<
         if (comp == 0) flux_planes<T, SCHEME, VISC, TURB, 0>(m, a, ntx, ntiles, zchunk, ring, sh);
         else if (comp == 1) flux_planes<T, SCHEME, VISC, TURB, 1>(m, a, ntx, ntiles, zchunk, ring, sh);
         else flux_planes<T, SCHEME, VISC, TURB, 2>(m, a, ntx, ntiles, zchunk, ring, sh);
+    } else if constexpr (sizeof(T) == 4 && SCHEME != TC_SCHEME_AB3 && VISC == 0 && !TURB) {
+        // FP32 constant viscosity: a block whose chunk lies in planes 3..nz-4
+        // runs the copy of the plane loop without wall tests; the choice is per
+        // block, so each copy is scheduled as a loop of its own (a per-plane
+        // choice made the FP32 loop wait for its history loads)
+        const int z0 = xcd_swizzle(blockIdx.x, gridDim.x) / ntiles * zchunk, z1 = min(m.nz, z0 + zchunk);
+        if (z0 >= 3 && z1 <= m.nz - 3) flux_planes<T, SCHEME, VISC, TURB, -1, true>(m, a, ntx, ntiles, zchunk, ring, sh);
+        else flux_planes<T, SCHEME, VISC, TURB, -1>(m, a, ntx, ntiles, zchunk, ring, sh);
     } else {
         flux_planes<T, SCHEME, VISC, TURB, -1>(m, a, ntx, ntiles, zchunk, ring, sh);
     }
