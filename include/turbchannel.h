@@ -179,9 +179,9 @@ int tc_initial_field(tc_engine* e, int slot, unsigned long long seed, double ubu
  * collocated_AB3/func_global.h:8-61) per (kx, |ky|) column: "ZF_M" (W x nz x
  * columns forward multipliers), "ZF_D" (inverse pivots), "ZF_U" (normalised upper
  * band), and the constants from which the FP32-bulk z-solve forms the forward
- * factors of the tridiagonal systems in-lane: "ZF_R" (nz x 4; empty for the
- * fourth-order scheme), "ZF_KX", "ZF_KY" (wavenumber terms per kx, |ky|). Returns
- * the length, or -1. No device needed. */
+ * factors in-lane: "ZF_R" (nz x 12 for the fourth-order scheme, nz x 4
+ * otherwise), "ZF_KX", "ZF_KY" (wavenumber terms per kx, |ky|). Returns the
+ * length, or -1. No device needed. */
 int tc_mesh_array(const tc_params* p, const char* name, double* out, int cap);
 
 /* ---------------------------------------------------------------------------

@@ -99,45 +99,67 @@ def test_srk_mesh_data_is_collocated():
         np.testing.assert_array_equal(tca.mesh_array(p, n), tca.mesh_array(q, n))
 
 
-def _inlane_factors(R, KX, KY, nx, ny, nz):
-    """The in-lane forward elimination of the FP32-bulk tridiagonal z-solve
-    (ZInl::row in kernels_spectral.hip), restated in FP64 for every (kx, |ky|)
-    column."""
+def _inlane_factors(W, R, KX, KY, nx, ny, nz):
+    """The in-lane forward elimination of the FP32-bulk z-solve (ZInl::row in
+    kernels_spectral.hip), restated in FP64 for every (kx, |ky|) column."""
     nxh, nyh = nx // 2 + 1, ny // 2 + 1
     ncols = nxh * nyh
     kx, ky = np.divmod(np.arange(ncols), nyh)
-    M, D, U = np.zeros((nz, ncols)), np.zeros((nz, ncols)), np.zeros((nz, ncols))
-    mm = KX[kx] + KY[ky]
-    u = np.zeros(ncols)
-    for i, (am, ap, c0, cq) in enumerate(R.reshape(nz, 4)):
-        d = 1.0 / ((c0 + cq * mm) - am * u)
-        u = ap * d
-        M[i], D[i], U[i] = -am, d, u
+    M, D, U = np.zeros((W, nz, ncols)), np.zeros((nz, ncols)), np.zeros((W, nz, ncols))
+    if W == 1:
+        mm = KX[kx] + KY[ky]
+        u = np.zeros(ncols)
+        for i, (am, ap, c0, cq) in enumerate(R.reshape(nz, 4)):
+            d = 1.0 / ((c0 + cq * mm) - am * u)
+            u = ap * d
+            M[0, i], D[i], U[0, i] = -am, d, u
+        return M, D, U
+    sx, s3x = KX.reshape(-1, 2)[kx].T
+    sy, s3y = KY.reshape(-1, 2)[ky].T
+    b1, b2, b3 = (np.zeros((3, ncols)) for _ in range(3))
+    for i, r in enumerate(R.reshape(nz, 12)):
+        cx, cy = r[7] * sx - r[8] * s3x, r[7] * sy - r[8] * s3y
+        dg = r[3] - r[9] * cx * cx - r[10] * cy * cy
+        q3 = -r[0] + 0 * dg
+        l2 = r[1] + q3 * b3[0]
+        l3 = r[2] + q3 * b3[1]
+        dg = dg + q3 * b3[2]
+        q2 = -l2
+        l3 = l3 + q2 * b2[0]
+        dg = dg + q2 * b2[1]
+        u1 = r[4] + q2 * b2[2]
+        q1 = -l3
+        dg = dg + q1 * b1[0]
+        u1 = u1 + q1 * b1[1]
+        u2 = r[5] + q1 * b1[2]
+        d = 1.0 / dg
+        b3, b2, b1 = b2, b1, np.array([u1 * d, u2 * d, r[6] * d])
+        M[0, i], M[1, i], M[2, i], D[i] = q1, q2, q3, d
+        U[:, i] = b1
     return M, D, U
 
 
-@pytest.mark.parametrize("name,extra", [("hw", {}), ("ab3", {}), ("ab3", {"fd_mode": 0}), ("ab3_stab0", {})])
+@pytest.mark.parametrize("name,extra", [("hw", {}), ("ab3", {}), ("ab3", {"fd_mode": 0}), ("ab3_stab0", {}),
+                                        ("vv", {}), ("vv_odd", {})])
 def test_zsolve_inlane_constants_reproduce_the_factors(name, extra):
     """The per-row constants and wavenumber terms from which the FP32-bulk
-    tridiagonal z-solve forms its forward factors in-lane give the host's
-    precomputed elimination (the factors the FP64 engine streams, which the GPU
-    parity tests hold to the oracle's ThomasForPressure) for every column but the
-    pinned (0,0) one, which the FP32-bulk engine solves from the precomputed FP64
-    factors. The fourth-order scheme streams its factors and has no constants."""
+    z-solve forms its forward factors in-lane give the host's precomputed
+    elimination (the factors the FP64 engine streams, which the GPU parity tests
+    hold to the oracle's ThomasForPressure) for every column but the pinned (0,0)
+    one, which the FP32-bulk engine solves from the precomputed FP64 factors."""
     scheme, nx, ny, nz, zz, kw = parity.problem(name)
     kw.update(extra)
     p = tca.make_params(scheme, nx, ny, nz, zz, **kw)
+    W = 3 if scheme == tca.SCHEME_VV else 1
     ncols = (nx // 2 + 1) * (ny // 2 + 1)
-    Mh = tca.mesh_array(p, "ZF_M").reshape(nz, ncols)
+    Mh = tca.mesh_array(p, "ZF_M").reshape(W, nz, ncols)
     Dh = tca.mesh_array(p, "ZF_D").reshape(nz, ncols)
-    Uh = tca.mesh_array(p, "ZF_U").reshape(nz, ncols)
+    Uh = tca.mesh_array(p, "ZF_U").reshape(W, nz, ncols)
     with np.errstate(divide="ignore", invalid="ignore"):  # the unpinned (0,0) column may be singular
-        M, D, U = _inlane_factors(tca.mesh_array(p, "ZF_R"), tca.mesh_array(p, "ZF_KX"),
+        M, D, U = _inlane_factors(W, tca.mesh_array(p, "ZF_R"), tca.mesh_array(p, "ZF_KX"),
                                   tca.mesh_array(p, "ZF_KY"), nx, ny, nz)
     for got, ref, what in ((M, Mh, "M"), (D, Dh, "D"), (U, Uh, "U")):
         g, r = got[..., 1:], ref[..., 1:]
         np.testing.assert_allclose(g, r, rtol=1e-11, atol=1e-12 * np.abs(r).max(), err_msg=what)
     # the (0,0) column differs: its first pivot is doubled by the pin
     assert not abs(D[0, 0] - Dh[0, 0]) <= 0.1 * abs(Dh[0, 0])
-    vv = tca.make_params(*parity.problem("vv")[:5], **parity.problem("vv")[5])
-    assert tca.mesh_array(vv, "ZF_R").size == 0

@@ -197,6 +197,12 @@ def test_driver_learned_closure_from_file(tmp_path):
     c = read_vtk(tmp_path / "rand" / "output.vtk")[2]
     np.testing.assert_allclose(b, a, rtol=1e-6, atol=1e-6 * np.abs(a).max())
     assert np.isfinite(c).all() and np.abs(c - a).max() > 1e-6 * np.abs(a).max()
+    # backscatter clipping (sgs_model 2): another run again, finite
+    (tmp_path / "clip").mkdir()
+    r3 = run_driver([*base, "--sgs", "../rand.bin", "--sgs_clip", 1], tmp_path / "clip")
+    assert r3.returncode == 0 and "Time: total" in r3.stdout, r3.stdout + r3.stderr
+    d = read_vtk(tmp_path / "clip" / "output.vtk")[2]
+    assert np.isfinite(d).all() and np.abs(d - c).max() > 1e-6 * np.abs(a).max()
     np.zeros(npar - 1, dtype="<f4").tofile(tmp_path / "short.bin")
     r = run_driver([*base, "--sgs", "short.bin"], tmp_path)
     assert f"expected {npar} float32 network parameters" in r.stdout

@@ -273,16 +273,51 @@ std::string build_zfactors(const tc_params& p, const HostMesh& m, ZFactors& f) {
         }
     }
     if (!err.empty()) return err;
-    // The constants of the in-lane forward elimination of the tridiagonal
-    // systems (the loop above, split into what depends on the row and what on
-    // the wavenumbers)
+    // The constants of the in-lane forward elimination (the loops above, split
+    // into what depends on the row and what on the wavenumbers)
+    if (W == 3) {
+        const int RW = zrow_w(3);
+        f.R.assign((size_t)nz * RW, 0.0);
+        f.KX.assign((size_t)nxh * 2, 0.0);
+        f.KY.assign((size_t)nyh * 2, 0.0);
+        for (int i = 0; i < nz; i++) {
+            double* r = &f.R[(size_t)i * RW];
+            double sum = 0;
+            for (int k = 0; k < 3; k++)
+                if (i + k - 3 >= 0) {
+                    r[k] = A[i * 3 + k];
+                    sum += A[i * 3 + k];
+                }
+            for (int k = 1; k <= 3; k++)
+                if (i + k < nz) {
+                    r[3 + k] = A[(i + k) * 3 + 3 - k];
+                    sum += A[(i + k) * 3 + 3 - k];
+                }
+            r[3] = i <= 2 ? m.MAT_D[i] : (i >= nz - 3 ? m.MAT_D[i - (nz - 3) + 3] : -sum);
+            const double den = DZ[i] - (1. / 9.) * m.DZW[i];
+            r[7] = DZ[i];
+            r[8] = (1. / 27.) * m.DZW[i];
+            r[9] = 4 * inv_hx * inv_hx / den;
+            r[10] = 4 * inv_hy * inv_hy / den;
+        }
+        for (int ix = 0; ix < nxh; ix++) {
+            const double sx = std::sin(kPi * ix / nx);
+            f.KX[(size_t)ix * 2] = sx;
+            f.KX[(size_t)ix * 2 + 1] = 3 * sx - 4 * sx * sx * sx;
+        }
+        for (int iy = 0; iy < nyh; iy++) {
+            const double sy = std::sin(kPi * iy / ny);
+            f.KY[(size_t)iy * 2] = sy;
+            f.KY[(size_t)iy * 2 + 1] = 3 * sy - 4 * sy * sy * sy;
+        }
+    }
     if (W == 1) {
         const bool hw = p.scheme == TC_SCHEME_HW, fd = p.fd_mode != 0;
-        f.R.assign((size_t)nz * kZRow, 0.0);
+        f.R.assign((size_t)nz * zrow_w(1), 0.0);
         f.KX.assign(nxh, 0.0);
         f.KY.assign(nyh, 0.0);
         for (int i = 0; i < nz; i++) {
-            double* r = &f.R[(size_t)i * kZRow];
+            double* r = &f.R[(size_t)i * zrow_w(1)];
             r[0] = i > 0 ? A[i] : 0.0;
             r[1] = i < nz - 1 ? A[i + 1] : 0.0;
             if (i == 0) {

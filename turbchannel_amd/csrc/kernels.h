@@ -163,10 +163,9 @@ struct ZFacDev {
     int nmodes = 0, ny = 0;        // spectrum modes, and modes per kx (columns by zfac_col)
     const CT* F = nullptr;
     const CT* U = nullptr;
-    // In-lane forward elimination of the tridiagonal systems (R != nullptr):
-    // per-row constants R[iz][kZRow] and wavenumber terms KX, KY (see
-    // ZFactors); the forward sweep then reads no factors, the backward sweep
-    // still reads U.
+    // In-lane forward elimination (R != nullptr): per-row constants
+    // R[iz][zrow_w(W)] and wavenumber terms KX, KY (see ZFactors); the forward
+    // sweep then reads no factors, the backward sweep still reads U.
     const CT* R = nullptr;
     const CT* KX = nullptr;
     const CT* KY = nullptr;

@@ -665,23 +665,64 @@ __device__ __forceinline__ void load_bwd_row(const CT* __restrict__ U, CT& u1, C
     }
 }
 
-// The forward elimination of one tridiagonal system (W = 1) formed in-lane
-// (see ZFactors::R): the row's constants are mode-independent (the same address
-// in every lane, so scalar loads), the mode enters through MM, and the lane
-// carries the previous row's normalised upper entry. Same steps as the host's
-// elimination (build_zfactors); pivots inverted by v_rcp (FP32 lanes only).
-template <typename CT>
+// The forward elimination of one mode formed in-lane (see ZFactors::R): the
+// row's constants are mode-independent (the same address in every lane, so
+// scalar loads), the mode enters through its wavenumber terms, and the lane
+// carries the normalised upper band of its last W rows. Same steps as the
+// host's elimination (build_zfactors); pivots inverted by v_rcp (FP32 lanes).
+template <typename CT, int W>
 struct ZInl {
-    CT mm = 0, u = 0;  // the mode's MM, U of the previous row
+    CT k0 = 0, k1 = 0, k2 = 0, k3 = 0;  // W = 1: MM (k0); W = 3: sx, s3x, sy, s3y
+    CT b1[W] = {}, b2[W] = {}, b3[W] = {};  // upper band of rows i-1 (W = 1: U), i-2, i-3
     __device__ __forceinline__ void init(const CT* __restrict__ KX, const CT* __restrict__ KY, int kx, int aky) {
-        mm = KX[kx] + KY[aky];
+        if constexpr (W == 3) {
+            k0 = KX[2 * kx];
+            k1 = KX[2 * kx + 1];
+            k2 = KY[2 * aky];
+            k3 = KY[2 * aky + 1];
+        } else {
+            k0 = KX[kx] + KY[aky];
+        }
     }
-    // factors of the row whose constants are at R: multiplier and inverse pivot
-    __device__ __forceinline__ void row(const CT* __restrict__ R, CT& m1, CT& d) {
+    // factors of the row whose constants are at R: multipliers and inverse pivot
+    __device__ __forceinline__ void row(const CT* __restrict__ R, CT& m1, CT& m2, CT& m3, CT& d) {
         static_assert(sizeof(CT) == 4, "in-lane factors for FP32 lanes");
-        d = __builtin_amdgcn_rcpf((R[2] + R[3] * mm) - R[0] * u);
-        m1 = -R[0];
-        u = R[1] * d;
+        if constexpr (W == 3) {
+            CT l2 = R[1], l3 = R[2];
+            const CT cx = R[7] * k0 - R[8] * k1, cy = R[7] * k2 - R[8] * k3;
+            CT dg = R[3] - R[9] * cx * cx;
+            dg = dg - R[10] * cy * cy;
+            CT u1 = R[4], u2 = R[5];
+            const CT q3 = -R[0];
+            l2 = l2 + q3 * b3[0];
+            l3 = l3 + q3 * b3[1];
+            dg = dg + q3 * b3[2];
+            const CT q2 = -l2;
+            l3 = l3 + q2 * b2[0];
+            dg = dg + q2 * b2[1];
+            u1 = u1 + q2 * b2[2];
+            const CT q1 = -l3;
+            dg = dg + q1 * b1[0];
+            u1 = u1 + q1 * b1[1];
+            u2 = u2 + q1 * b1[2];
+            d = __builtin_amdgcn_rcpf(dg);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                b3[k] = b2[k];
+                b2[k] = b1[k];
+            }
+            b1[0] = u1 * d;
+            b1[1] = u2 * d;
+            b1[2] = R[6] * d;
+            m1 = q1;
+            m2 = q2;
+            m3 = q3;
+        } else {
+            d = __builtin_amdgcn_rcpf((R[2] + R[3] * k0) - R[0] * b1[0]);
+            m1 = -R[0];
+            m2 = m3 = 0;
+            b1[0] = R[1] * d;
+        }
     }
 };
 
@@ -698,8 +739,7 @@ template <typename T, typename CT, int W, int B, int NB, bool INL = false>
 __device__ __forceinline__ void lane_solve(T* __restrict__ x, size_t xs, const CT* __restrict__ F, size_t fs,
                                            const CT* __restrict__ U, size_t us, CT* __restrict__ y, size_t ys, int nz,
                                            bool n00, const ZNorm& norm, const CT* __restrict__ R = nullptr,
-                                           ZInl<CT>* zi = nullptr) {
-    static_assert(!INL || W == 1, "in-lane factors for the tridiagonal systems");
+                                           ZInl<CT, W>* zi = nullptr) {
     const int nfull = nz / B;
     CT y1 = 0, y2 = 0, y3 = 0;
     auto fwd_row = [&](CT b, CT m1, CT m2, CT m3, CT d, int i) {
@@ -736,9 +776,9 @@ __device__ __forceinline__ void lane_solve(T* __restrict__ x, size_t xs, const C
 #pragma unroll
                 for (int k = 0; k < B; k++) {
                     if constexpr (INL) {
-                        CT q1, qd;
-                        zi->row(R + (size_t)(kb * B + k) * kZRow, q1, qd);
-                        fwd_row(b[sb][k], q1, CT(0), CT(0), qd, kb * B + k);
+                        CT q1, q2, q3, qd;
+                        zi->row(R + (size_t)(kb * B + k) * zrow_w(W), q1, q2, q3, qd);
+                        fwd_row(b[sb][k], q1, q2, q3, qd, kb * B + k);
                     } else {
                         fwd_row(b[sb][k], m1[sb][k], m2[sb][k], m3[sb][k], d[sb][k], kb * B + k);
                     }
@@ -748,7 +788,7 @@ __device__ __forceinline__ void lane_solve(T* __restrict__ x, size_t xs, const C
     }
     for (int i = nfull * B; i < nz; i++) {
         CT m1, m2 = 0, m3 = 0, d;
-        if constexpr (INL) zi->row(R + (size_t)i * kZRow, m1, d);
+        if constexpr (INL) zi->row(R + (size_t)i * zrow_w(W), m1, m2, m3, d);
         else load_fwd_row<CT, W>(F + i * fs, m1, m2, m3, d);
         fwd_row((CT)x[i * xs], m1, m2, m3, d, i);
     }
@@ -811,11 +851,10 @@ __device__ __forceinline__ void lane_solve(T* __restrict__ x, size_t xs, const C
 template <typename T, int W, int B, int NB, int NZ, bool INL = false, int NR = NZ, int NL = 0>
 __device__ __forceinline__ void lane_solve_reg(T* __restrict__ x, size_t xs, const T* __restrict__ F, size_t fs,
                                                const T* __restrict__ U, size_t us, bool n00, const ZNorm& norm,
-                                               const T* __restrict__ R = nullptr, ZInl<T>* zi = nullptr,
+                                               const T* __restrict__ R = nullptr, ZInl<T, W>* zi = nullptr,
                                                T* __restrict__ yl = nullptr) {
     static_assert(NZ % B == 0 && NR % B == 0 && NL % B == 0 && NR + NL <= NZ, "rows in whole batches");
     constexpr int NH = NZ - NR - NL;  // rows over the spectrum
-    static_assert(!INL || W == 1, "in-lane factors for the tridiagonal systems");
     constexpr int NBAT = NZ / B;
     T yr[NR];
     T y1 = 0, y2 = 0, y3 = 0;
@@ -839,7 +878,7 @@ __device__ __forceinline__ void lane_solve_reg(T* __restrict__ x, size_t xs, con
             for (int k = 0; k < B; k++) {
                 T f1, f2 = 0, f3 = 0, fd;
                 if constexpr (INL) {
-                    zi->row(R + (size_t)(kb * B + k) * kZRow, f1, fd);
+                    zi->row(R + (size_t)(kb * B + k) * zrow_w(W), f1, f2, f3, fd);
                 } else {
                     f1 = m1[sb][k];
                     if (W == 3) f2 = m2[sb][k];
@@ -912,17 +951,15 @@ __device__ __forceinline__ void lane_solve_reg(T* __restrict__ x, size_t xs, con
 }
 
 // Rows of the nz = 192 z-solve's forward intermediate kept in registers and in
-// LDS (the rest go over the spectrum), by bandwidth W. Tridiagonal (in-lane
-// factors): 128 in registers (165 VGPRs, three waves per SIMD, so all 2 317
-// waves of a 384 x 384 x 192 solve are resident at once), 48 in LDS (12 KiB per
-// one-wave block, 144 KiB per CU), 16 over the spectrum: AB3 384 x 384 x 192
-// 80 -> 69 us. Seven-band: all 192 in registers (247 VGPRs, two waves per
-// SIMD); 80 or 96 registers with 48 LDS rows (three waves) measured 104-107 us
-// against 105.
+// LDS (the rest go over the spectrum). With the factors formed in-lane: 128 in
+// registers (165-166 VGPRs, three waves per SIMD, so all 2 317 waves of a
+// 384 x 384 x 192 solve are resident at once), 48 in LDS (12 KiB per one-wave
+// block, 144 KiB per CU), 16 over the spectrum (AB3 384 x 384 x 192 80 -> 69 us;
+// seven-band 105 -> 90 us).
 template <int W>
-constexpr int zreg_rows(int nz) { return W == 1 && nz == 192 ? 128 : nz; }
+constexpr int zreg_rows(int nz) { return nz == 192 ? 128 : nz; }
 template <int W>
-constexpr int zlds_rows(int nz) { return W == 1 && nz == 192 ? 48 : 0; }
+constexpr int zlds_rows(int nz) { return nz == 192 ? 48 : 0; }
 
 // Few independent systems (2 per mode) for the latency of every row: one wave
 // per block so the waves spread over all CUs, and pipelined row batches. The
@@ -958,7 +995,7 @@ __global__ __launch_bounds__(64) void k_zsolve(Cplx<T>* spec, ZFacDev<T> f, ZFac
     T* x = reinterpret_cast<T*>(spec) + 2 * mode + part;
     const int col = zfac_col(mode, f.ny);  // the factors of ky and ny - ky are one column
     const size_t nc = f.ncols;
-    ZInl<T> zi;
+    ZInl<T, W> zi;
     if constexpr (INL) {
         const int kx = mode / f.ny, ky = mode - kx * f.ny;
         zi.init(f.KX, f.KY, kx, min(ky, f.ny - ky));
@@ -1158,13 +1195,13 @@ hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, cons
     // intermediate goes over the spectrum instead.
     const size_t sm = 64 * (size_t)f.nz * sizeof(T);
     const bool lds = sm <= 64 * 1024 && (int)grid.x <= (f.W == 3 ? zsolve_slots<T, 3>(sm) : zsolve_slots<T, 1>(sm));
-    // FP32-bulk engines form the tridiagonal bulk modes' forward factors
-    // in-lane (the (0,0) mode, which the host pins, is one of the FP64 low
-    // modes). AB3 384 x 384 x 192: 90 -> 78 us. The seven-band systems keep the
-    // precomputed factors: the in-lane elimination's chain per row made that
-    // solve slower (105 -> 115-137 us) although it moved 3 instead of 7 factor
-    // reals per row.
-    const bool inl = sizeof(T) == 4 && f.R != nullptr && lowidx != nullptr && f.W == 1;
+    // FP32-bulk engines form the bulk modes' forward factors in-lane (the (0,0)
+    // mode, which the host pins, is one of the FP64 low modes): the
+    // tridiagonal systems on every path (AB3 384 x 384 x 192: 90 -> 78 us), the
+    // seven-band ones with the on-chip intermediate at nz = 192 (three waves
+    // per SIMD: 105 -> 90 us; with two waves per SIMD the elimination's chain
+    // per row made the solve slower, 105 -> 115-137 us).
+    const bool inl = sizeof(T) == 4 && f.R != nullptr && lowidx != nullptr && (f.W == 1 || f.nz == 192);
 #define LZS(W_, N_)                                                                                                     \
     do {                                                                                                                \
         if constexpr (sizeof(T) == 4 && W_ == 1) {                                                                      \
@@ -1184,7 +1221,7 @@ hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, cons
     const bool reg = sizeof(T) == 4 && f.nz == 192 && !lds;
 #define LZR(W_)                                                                                                         \
     do {                                                                                                                \
-        if constexpr (sizeof(T) == 4 && W_ == 1) {                                                                      \
+        if constexpr (sizeof(T) == 4) {                                                                                 \
             if (inl) {                                                                                                  \
                 hipLaunchKernelGGL((k_zsolve<T, W_, 0, 192, true>), grid, dim3(64), zlds_rows<W_>(192) * 64 * sizeof(T), \
                                    s, spec, f, flow, lowidx, lowmode, nlow, ylow, norm, nbulk, f.R);                    \

@@ -39,20 +39,28 @@ int solve_banded(int N, int NB, double* M, double* x, double tiny);
 struct ZFactors {
     int W = 1, nz = 0, ncols = 0;
     std::vector<double> M[3], D, U[3];
-    // The tridiagonal forward elimination formed in-lane instead (FP32-bulk
-    // engines, W = 1): the mode-independent per-row constants R[iz][4] and the
-    // wavenumber terms KX[kx], KY[|ky|] (layout below). Empty for W = 3. Not
-    // used for the pinned (0,0) mode.
+    // The forward elimination formed in-lane instead (FP32-bulk engines): the
+    // mode-independent per-row constants R[iz][zrow_w(W)] and the wavenumber
+    // terms KX[kx][zk_w(W)], KY[|ky|][zk_w(W)] (layout below). Not used for the
+    // pinned (0,0) mode.
     std::vector<double> R, KX, KY;
     // normalisation of the (0,0) mode in spectral space: X00[iz] -= psum * C[iz],
     // psum = sum_iz Wt[iz] * X00[iz]
     std::vector<double> Wt, C;
 };
-// In-lane forward elimination of the tridiagonal systems, per row iz of R:
-// AM (entry at iz-1, 0 on row 0), AP (entry at iz+1, 0 on the last row), C0,
-// CQ; KX[kx] + KY[|ky|] = the mode's MM (build_zfactors); with U[-1] = 0:
-// D = 1 / ((C0 + CQ MM) - AM U[iz-1]), M1 = -AM, U = AP D.
-constexpr int kZRow = 4;
+// In-lane forward elimination, per row iz of R (zrow_w(W) reals):
+//   W = 1: AM (entry at iz-1, 0 on row 0), AP (entry at iz+1, 0 on the last
+//          row), C0, CQ; KX[kx] + KY[|ky|] = the mode's MM; with U[-1] = 0:
+//          D = 1 / ((C0 + CQ MM) - AM U[iz-1]), M1 = -AM, U = AP D.
+//   W = 3: LO1, LO2, LO3 (the row's entries at iz-3, iz-2, iz-1), DG (diagonal
+//          without the wavenumber term), UP1, UP2, UP3 (entries at iz+1..iz+3),
+//          DZ, DZW / 27, EX = 4 / (hx^2 (DZ - DZW / 9)), EY (with hy), 0;
+//          KX[kx] = (sin(pi kx / nx), sin(3 pi kx / nx)), KY likewise: the
+//          diagonal is DG - EX cx^2 - EY cy^2 with c = DZ s - DZW / 27 s3, and
+//          the elimination with the normalised upper band of the last 3 rows
+//          follows build_zfactors.
+constexpr int zrow_w(int W) { return W == 3 ? 12 : 4; }
+constexpr int zk_w(int W) { return W == 3 ? 2 : 1; }
 std::string build_zfactors(const tc_params& p, const HostMesh& m, ZFactors& f);
 
 void make_initial_field(const tc_params& p, const HostMesh& m, unsigned long long seed, double ubulk,
