@@ -1215,22 +1215,16 @@ hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, cons
                            nlow, ylow, norm, nbulk, (const T*)nullptr);                                                 \
     } while (0)
     // FP32 with nz = 192 (the 384 x 384 x 192 grids) when not every block is
-    // resident: the intermediate on chip (zreg_rows, zlds_rows), which removes
-    // the spectrum's second pass (AB3 118 -> 88 us, fourth order 140 -> 105 us
-    // at 384 x 384 x 192 with all rows in registers)
-    const bool reg = sizeof(T) == 4 && f.nz == 192 && !lds;
+    // resident: the factors in-lane and the intermediate on chip (zreg_rows,
+    // zlds_rows), which removes the spectrum's second pass (AB3 118 -> 88 us,
+    // fourth order 140 -> 105 us at 384 x 384 x 192 with all rows in
+    // registers and streamed factors)
+    const bool reg = inl && f.nz == 192 && !lds;
 #define LZR(W_)                                                                                                         \
     do {                                                                                                                \
-        if constexpr (sizeof(T) == 4) {                                                                                 \
-            if (inl) {                                                                                                  \
-                hipLaunchKernelGGL((k_zsolve<T, W_, 0, 192, true>), grid, dim3(64), zlds_rows<W_>(192) * 64 * sizeof(T), \
-                                   s, spec, f, flow, lowidx, lowmode, nlow, ylow, norm, nbulk, f.R);                    \
-                break;                                                                                                  \
-            }                                                                                                           \
-        }                                                                                                               \
         if constexpr (sizeof(T) == 4)                                                                                   \
-            hipLaunchKernelGGL((k_zsolve<T, W_, 0, 192>), grid, dim3(64), zlds_rows<W_>(192) * 64 * sizeof(T), s,      \
-                               spec, f, flow, lowidx, lowmode, nlow, ylow, norm, nbulk, (const T*)nullptr);             \
+            hipLaunchKernelGGL((k_zsolve<T, W_, 0, 192, true>), grid, dim3(64), zlds_rows<W_>(192) * 64 * sizeof(T), s, \
+                               spec, f, flow, lowidx, lowmode, nlow, ylow, norm, nbulk, f.R);                           \
     } while (0)
     if (f.W == 3) {
         if (lds) LZS(3, 3);
