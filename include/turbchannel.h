@@ -78,7 +78,9 @@ typedef struct tc_params {
                               counterpart): each step evaluates tau from the velocity with the
                               network set by tc_set_sgs_params and subtracts its divergence;
                               2 = the same with backscatter clipping (tau = 0 at a node where
-                              tau_ij S_ij > 0, S the resolved strain) */
+                              tau_ij S_ij > 0, S the resolved strain); 3 = the dissipative part
+                              of the network's stress along the deviatoric strain D:
+                              tau' = (tau_kl D_kl / D_kl D_kl) D where tau_kl D_kl < 0, else 0 */
     int imex;              /* TC_SCHEME_SRK: tc_imex method (main_params::IMEXtype, NStages,
                               ButcherE/I, d; collocated_SRK/main.cpp:342-350) */
     double alpha_tau;      /* TC_SCHEME_SRK: alpha = alpha_tau / tau; <= 0 selects the reference's

@@ -209,15 +209,33 @@ def stress_strain(y, X):
             y[:, 4] * (X[:, 2] + X[:, 6]) + y[:, 5] * (X[:, 5] + X[:, 7]))
 
 
-def node_tau(params, u, nx, ny, nz, zz, hx, hy, u_tau, inv_len, emulate=True, clip=False):
+def strain_projection(y, X):
+    """The dissipative part of the stress y (XX YY ZZ XY XZ YZ) along the
+    deviatoric strain rate D of the gradient inputs: (y:D / D:D) D where
+    y:D < 0, else 0, as six components."""
+    G = X[:, :9].reshape(-1, 3, 3)
+    S = 0.5 * (G + np.transpose(G, (0, 2, 1)))
+    D = S - (np.trace(S, axis1=1, axis2=2) / 3.0)[:, None, None] * np.eye(3)
+    d6 = np.stack([D[:, 0, 0], D[:, 1, 1], D[:, 2, 2], D[:, 0, 1], D[:, 0, 2], D[:, 1, 2]], axis=-1)
+    w = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0])  # off-diagonal entries appear twice in a:b
+    yd = np.sum(w * y * d6, axis=1)
+    dd = np.sum(w * d6 * d6, axis=1)
+    c = np.where((yd < 0) & (dd > 0), yd / np.where(dd > 0, dd, 1.0), 0.0)
+    return c[:, None] * d6
+
+
+def node_tau(params, u, nx, ny, nz, zz, hx, hy, u_tau, inv_len, emulate=True, clip=False, project=False):
     """Closure stress at every node [N3D, 6] (zero on the wall planes), as the
     LES step computes it from its own velocity field; clip: zero where
-    tau_ij S_ij > 0 (backscatter)."""
+    tau_ij S_ij > 0 (backscatter); project: the dissipative part along the
+    deviatoric strain rate (strain_projection)."""
     u = np.asarray(u, dtype=np.float64).reshape(nz, ny, nx, 3)
     X = features([u[..., c] for c in range(3)], zz, hx, hy, u_tau, inv_len).reshape(-1, 10)
     y, _ = forward(params, X, emulate)
     if clip:
         y = np.where((stress_strain(y, X) > 0)[:, None], 0.0, y)
+    if project:
+        y = strain_projection(y, X)
     tau = np.zeros((nz, ny, nx, 6))
     tau[1:-1] = (y * u_tau * u_tau).reshape(nz - 2, ny, nx, 6)
     return tau.reshape(-1, 6)

@@ -259,3 +259,54 @@ def test_output_scaling_and_filter_width_ratio():
     assert closure.filter_width_ratio2(395.0, (96, 96), 790.0, (96, 96)) == pytest.approx(4.0)
     assert closure.filter_width_ratio2(395.0, (96, 96), 395.0, (48, 96)) == pytest.approx(2.0)
     assert closure.filter_width_ratio2(395.0, (96, 96), 590.0, (64, 64)) == pytest.approx((590 / 395 * 1.5) ** 2)
+
+
+def _sym6(y):
+    """[n, 6] (XX YY ZZ XY XZ YZ) -> [n, 3, 3]"""
+    return np.stack([np.stack([y[:, 0], y[:, 3], y[:, 4]], -1), np.stack([y[:, 3], y[:, 1], y[:, 5]], -1),
+                     np.stack([y[:, 4], y[:, 5], y[:, 2]], -1)], 1)
+
+
+def test_strain_projection_is_the_dissipative_eddy_viscosity_part():
+    """mlp.strain_projection keeps c D with c = (y:D)/(D:D) <= 0, D the deviatoric
+    strain rate of the gradient inputs: an eddy-viscosity stress -2 nu D comes back
+    unchanged for nu > 0 and is dropped for nu < 0; otherwise the result is
+    traceless, parallel to D, carries the same dissipation y:D, leaves a remainder
+    orthogonal to D, and projecting again changes nothing. The gradients have a
+    trace and off-diagonal parts, so a strain that is not made deviatoric or an
+    off-diagonal entry counted once fails."""
+    rng = np.random.default_rng(12)
+    X, _ = rand_data(400, seed=13)
+    G = X[:, :9].reshape(-1, 3, 3)
+    S = 0.5 * (G + np.transpose(G, (0, 2, 1)))
+    D = S - (np.trace(S, axis1=1, axis2=2) / 3.0)[:, None, None] * np.eye(3)
+    d6 = np.stack([D[:, 0, 0], D[:, 1, 1], D[:, 2, 2], D[:, 0, 1], D[:, 0, 2], D[:, 1, 2]], -1)
+    nu = rng.uniform(0.1, 2.0, 400)
+    np.testing.assert_allclose(mlp.strain_projection(-2 * nu[:, None] * d6, X), -2 * nu[:, None] * d6, rtol=1e-12,
+                               atol=1e-13)
+    assert np.all(mlp.strain_projection(2 * nu[:, None] * d6, X) == 0)
+    y = rng.normal(size=(400, 6))
+    p = mlp.strain_projection(y, X)
+    Y, P = _sym6(y), _sym6(p)
+    yd = np.einsum("nij,nij->n", Y, D)
+    dis = yd < 0
+    assert 0.2 < dis.mean() < 0.8
+    assert np.all(p[~dis] == 0)
+    np.testing.assert_allclose(np.trace(P, axis1=1, axis2=2), 0, atol=1e-12)
+    c = np.einsum("nij,nij->n", P, D) / np.einsum("nij,nij->n", D, D)
+    np.testing.assert_allclose(P[dis], c[dis, None, None] * D[dis], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(np.einsum("nij,nij->n", P, D)[dis], yd[dis], rtol=1e-10)
+    np.testing.assert_allclose(np.einsum("nij,nij->n", Y - P, D)[dis], 0, atol=1e-10)
+    np.testing.assert_allclose(mlp.strain_projection(p, X), p, rtol=1e-12, atol=1e-13)
+
+
+def test_training_target_projection_matches_oracle():
+    """closure.strain_projection (the training targets of the eddy-viscosity
+    network, chunked, float32 out) equals the oracle's projection."""
+    from turbchannel_amd import closure
+    X, T = rand_data(1000, seed=21)
+    got = closure.strain_projection(T.astype(np.float32), X.astype(np.float32), chunk=300)
+    ref = mlp.strain_projection(T.astype(np.float32).astype(np.float64), X.astype(np.float32).astype(np.float64))
+    assert got.dtype == np.float32
+    np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-7)
+    assert 0.2 < np.mean(np.all(got == 0, axis=1)) < 0.8

@@ -229,6 +229,49 @@ def test_node_tau_with_backscatter_clipping_matches_oracle():
     assert np.all(got[:nx * ny] == 0) and np.all(got[-nx * ny:] == 0)
 
 
+def test_node_tau_projected_on_the_strain_rate_matches_oracle():
+    """sgs_model = 3: the closure stress is its dissipative part along the
+    deviatoric strain rate. Nodes whose sign of tau:D is within the network's
+    bfloat16 rounding of zero may go either way; every other node matches the
+    oracle's projected stress, and the dropped ones are zero."""
+    name = "ab3_sgs"
+    nx, ny, nz, zz, kw = sample_case(name)
+    sol = parity.make_oracle(name)
+    scheme, *_, pkw = parity.problem(name)
+    eng = tca.Engine(scheme, nx, ny, nz, zz, **dict(pkw, sgs_model=3))
+    p = closure_params()
+    eng.set_sgs_params(p)
+    u = parity.seeded_velocity(sol, seed=32, noise=0.3)
+    eng.set_field(tca.VEL, 2, u)
+    eng.recalc_turb_visc((tca.VEL, 2))
+    got = eng.get_field(tca.TAU, 0)
+    u_tau = kw["visc"] * kw["inv_length_scale"]
+    args = (p, u, nx, ny, nz, zz, kw["hx"], kw["hy"], u_tau, kw["inv_length_scale"])
+    ref = mlp.node_tau(*args, project=True)
+    raw = mlp.node_tau(*args)
+    X = mlp.features([u.reshape(nz, ny, nx, 3)[..., c] for c in range(3)], zz, kw["hx"], kw["hy"], u_tau,
+                     kw["inv_length_scale"]).reshape(-1, 10)
+    inner = slice(nx * ny, -nx * ny)
+    y = raw[inner] / (u_tau * u_tau)
+    G = X[:, :9].reshape(-1, 3, 3)
+    S = 0.5 * (G + np.transpose(G, (0, 2, 1)))
+    D = S - (np.trace(S, axis1=1, axis2=2) / 3.0)[:, None, None] * np.eye(3)
+    d6 = np.stack([D[:, 0, 0], D[:, 1, 1], D[:, 2, 2], D[:, 0, 1], D[:, 0, 2], D[:, 1, 2]], -1)
+    yd = np.sum(np.array([1, 1, 1, 2, 2, 2]) * y * d6, axis=1)  # tau:D
+    bound = 2.0 * np.abs(y).sum(axis=1) * np.abs(d6).max(axis=1)  # |tau:D| <= bound
+    firm = np.abs(yd) > 0.05 * bound
+    kept = (yd < 0) & firm
+    dropped = (yd > 0) & firm
+    print(f"kept {kept.mean():.2%}, dropped {dropped.mean():.2%} of the interior nodes")
+    assert kept.sum() > 0.05 * firm.sum() and dropped.sum() > 0.05 * firm.sum()
+    g, r = got[inner], ref[inner]
+    e = np.abs(g[firm] - r[firm]).max() / np.abs(r).max()
+    print(f"projected node tau max-norm rel {e:.2e}")
+    assert e < 2e-2
+    assert np.all(g[dropped] == 0)
+    assert np.all(got[:nx * ny] == 0) and np.all(got[-nx * ny:] == 0)
+
+
 @pytest.mark.parametrize("name", ["ab3_sgs", "ab3_sgs_gal"])
 def test_explicit_term_with_tau(name):
     sol = parity.make_oracle(name)

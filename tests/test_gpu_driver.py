@@ -203,6 +203,12 @@ def test_driver_learned_closure_from_file(tmp_path):
     assert r3.returncode == 0 and "Time: total" in r3.stdout, r3.stdout + r3.stderr
     d = read_vtk(tmp_path / "clip" / "output.vtk")[2]
     assert np.isfinite(d).all() and np.abs(d - c).max() > 1e-6 * np.abs(a).max()
+    # dissipative part along the strain rate (sgs_model 3): finite, differs from both
+    (tmp_path / "proj").mkdir()
+    r4 = run_driver([*base, "--sgs", "../rand.bin", "--sgs_proj", 1], tmp_path / "proj")
+    assert r4.returncode == 0 and "Time: total" in r4.stdout, r4.stdout + r4.stderr
+    f = read_vtk(tmp_path / "proj" / "output.vtk")[2]
+    assert np.isfinite(f).all() and min(np.abs(f - c).max(), np.abs(f - d).max()) > 1e-6 * np.abs(a).max()
     np.zeros(npar - 1, dtype="<f4").tofile(tmp_path / "short.bin")
     r = run_driver([*base, "--sgs", "short.bin"], tmp_path)
     assert f"expected {npar} float32 network parameters" in r.stdout

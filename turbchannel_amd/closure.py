@@ -66,6 +66,28 @@ def scale_outputs(params, s):
     return p.astype(np.float32)
 
 
+def strain_projection(T, X, chunk=1 << 22):
+    """The dissipative part of each stress T[n] (XX YY ZZ XY XZ YZ) along the
+    deviatoric strain rate D of the sample's gradient inputs X[n, :9]
+    (g_ij = X[3 i + j], any positive scale): (T:D / D:D) D where T:D < 0, else
+    0 -- the stress of an eddy viscosity -T:D / (2 D:D) >= 0, the part an LES
+    engine with sgs_model 3 keeps of the network's output. float32 [n, 6]."""
+    T = np.asarray(T)
+    X = np.asarray(X)
+    out = np.empty((T.shape[0], 6), dtype=np.float32)
+    w = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0])  # off-diagonal entries appear twice in a:b
+    for a in range(0, T.shape[0], chunk):
+        g = X[a:a + chunk, :9].astype(np.float64)
+        tr = (g[:, 0] + g[:, 4] + g[:, 8]) / 3.0
+        d = np.stack([g[:, 0] - tr, g[:, 4] - tr, g[:, 8] - tr, 0.5 * (g[:, 1] + g[:, 3]), 0.5 * (g[:, 2] + g[:, 6]),
+                      0.5 * (g[:, 5] + g[:, 7])], axis=-1)
+        td = np.sum(w * T[a:a + chunk].astype(np.float64) * d, axis=1)
+        dd = np.sum(w * d * d, axis=1)
+        c = np.where((td < 0) & (dd > 0), td / np.where(dd > 0, dd, 1.0), 0.0)
+        out[a:a + chunk] = c[:, None] * d
+    return out
+
+
 def train(net, dev_x, dev_t, n, *, batch=65536, iters=1000, lr=1e-3, lr_final=None):
     """Adam over contiguous mini-batches of device samples (addresses of float32
     [n][10] inputs and [n][6] targets, already shuffled); returns the losses.

@@ -12,7 +12,10 @@ each run in one process (device memory does not survive between them):
              4x sharp spectral cut-off in x and y (96 x 96 coarse planes, exact
              sub-filter stress, ~1.75 M samples); 40 snapshots train the network
              (Adam, lr 1e-3 decaying to 1e-4, batch 65536), 10 are held out.
-             Writes the trained network (raw inputs/outputs) and the statistics.
+             A second network, trained the same way on the dissipative part of
+             each target along the sample's deviatoric strain rate
+             (closure.strain_projection), is the eddy-viscosity variant.
+             Writes the trained networks (raw inputs/outputs) and the statistics.
 ``judge590`` the judge: DNS at Re_tau ~590 (bulk Re 21900) on 512 x 512 x 256 nodes,
              tanh 1.8, seeded start, FP32 bulk: transition, 15 time units to
              settle, then profiles of
@@ -35,7 +38,14 @@ each run in one process (device memory does not survive between them):
              cut-off modes): 5.0 for the 64-mode grid at Re_tau 590 against the
              96-mode cut-off of the Re_tau 395 data. ``learned_unscaled`` runs the
              network without it; ``learned_clip`` with it and with backscatter
-             clipping (sgs_model 2: no stress where tau_ij S_ij > 0).
+             clipping (sgs_model 2: no stress where tau_ij S_ij > 0);
+             ``learned_proj`` with it, keeping only the stress's dissipative part
+             along the deviatoric strain rate D (sgs_model 3: an eddy viscosity
+             -tau_kl D_kl / (2 D_kl D_kl) >= 0 read off the network), and
+             ``learned_proj_unscaled`` the same without the scaling;
+             ``learned_eddy`` and ``learned_eddy_unscaled`` run the second network
+             of the dns395 stage (trained on that dissipative part of the
+             filtered-DNS stress) the same way.
 
 Run (from the repository root): ``python -m turbchannel_amd.closure_protocol STAGE
 --out FILE [--net FILE]``; ``--quick`` shrinks every size for a functional check.
@@ -49,7 +59,7 @@ import time
 import numpy as np
 
 from . import FLUX, FP32_BULK, FP64, PRES, SCHEME_AB3, VEL, Engine, Mlp
-from .closure import Standardizer, filter_width_ratio2, scale_outputs, train
+from .closure import Standardizer, filter_width_ratio2, scale_outputs, strain_projection, train
 from .les_study import Run, plane_stats, profile_stats, spectral_restrict, wall_shear
 
 LX, LY = 2.0 * math.pi, math.pi
@@ -63,6 +73,8 @@ FULL = dict(
                   stats_every=20, wall_seconds=960.0),
     les590=dict(re_b=21900.0, re_tau=590.0, gamma=2.0, dt=2e-3, t_wash=40.0, t_avg=40.0, noise=0.3, seed=1,
                 cases=[("learned", 64, 64, 96), ("learned_clip", 64, 64, 96), ("learned_unscaled", 64, 64, 96),
+                       ("learned_proj", 64, 64, 96), ("learned_proj_unscaled", 64, 64, 96),
+                       ("learned_eddy", 64, 64, 96), ("learned_eddy_unscaled", 64, 64, 96),
                        ("smagorinsky", 64, 64, 96), ("smagorinsky", 96, 96, 144), ("no_model", 64, 64, 96)],
                 time_steps=2000),
 )
@@ -72,6 +84,7 @@ QUICK = dict(
     judge590=dict(FULL["judge590"], grid=(64, 48, 33), t_trans_max=0.2, re_tau_turb=1e9, t_develop=0.05, t_avg=0.2,
                   les_cuts=[16, 24], wall_seconds=60.0),
     les590=dict(FULL["les590"], t_wash=0.1, t_avg=0.1, cases=[("learned", 16, 16, 25), ("learned_clip", 16, 16, 25),
+                                                  ("learned_proj", 16, 16, 25), ("learned_eddy", 16, 16, 25),
                                                   ("smagorinsky", 16, 16, 25), ("smagorinsky", 24, 24, 37),
                                                   ("no_model", 16, 16, 25)],
                 time_steps=20),
@@ -85,15 +98,24 @@ def nodes(nz, gamma):
     return z
 
 
+def learned_post(model):
+    """tc_params.sgs_model of a model name: 0 (not learned), 1 (the network's
+    stress), 2 (backscatter clipped), 3 (dissipative part along the strain rate)."""
+    if not model.startswith("learned"):
+        return 0
+    return 3 if model.startswith(("learned_proj", "learned_eddy")) else 2 if model == "learned_clip" else 1
+
+
 def engine(nx, ny, zz, re_b, re_tau, model, precision=FP32_BULK):
     """model: 'dns' (FD viscous term), 'no_model', 'smagorinsky', 'learned', 'learned_clip',
-    'learned_unscaled' (Galerkin viscous term)."""
+    'learned_unscaled', 'learned_proj', 'learned_proj_unscaled', 'learned_eddy',
+    'learned_eddy_unscaled' (Galerkin viscous term)."""
     visc = 2.0 / re_b
     u_tau = re_tau * visc
     kw = dict(precision=precision, visc=visc, sour_dpdx=u_tau * u_tau, inv_length_scale=re_tau, hx=LX / nx,
               hy=LY / ny, stab_type=1, fd_mode=1, cfr_ubulk=1.0, visc_type=0 if model == "dns" else 1,
               turb_visc=1 if model == "smagorinsky" else 0,
-              sgs_model=(2 if model == "learned_clip" else 1) if model.startswith("learned") else 0)
+              sgs_model=learned_post(model))
     e = Engine(SCHEME_AB3, nx, ny, len(zz), zz, **kw)
     e.graphs(True)
     return e
@@ -204,43 +226,54 @@ def stage_dns395(P, log):
     d64.close()
     dns.close()
 
-    # training set and training (cfg 3)
+    # training set and training (cfg 3): the network on the sub-filter stress, and
+    # a second one on its dissipative part along the strain rate (the part an
+    # engine with sgs_model 3 keeps)
     ntr = P["n_snap"] - P["holdout"]
-    X = torch.cat(snaps_x[:ntr]).view(-1, 10)
-    T = torch.cat(snaps_t[:ntr]).view(-1, 6)
+    Xn = torch.cat(snaps_x[:ntr]).view(-1, 10).cpu().numpy()
+    Tn = torch.cat(snaps_t[:ntr]).view(-1, 6).cpu().numpy()
     Xh = torch.cat(snaps_x[ntr:]).view(-1, 10)
-    Th = torch.cat(snaps_t[ntr:]).view(-1, 6)
+    Thn = torch.cat(snaps_t[ntr:]).view(-1, 6).cpu().numpy().astype(np.float64)
     del snaps_x, snaps_t
-    Xn, Tn = X.cpu().numpy(), T.cpu().numpy()
-    st = Standardizer(Xn, Tn)
-    perm = torch.from_numpy(np.random.default_rng(0).permutation(len(Xn))).cuda()
-    xs = torch.from_numpy(st.inputs(Xn)).cuda()[perm].contiguous()
-    ts = torch.from_numpy(st.targets(Tn)).cuda()[perm].contiguous()
-    del X, T, Xn, Tn
-    torch.cuda.synchronize()
-    net = Mlp(seed=0)
-    t0 = time.perf_counter()
-    losses = train(net, xs.data_ptr(), ts.data_ptr(), xs.shape[0], batch=P["batch"], iters=P["iters"], lr=1e-3,
-                   lr_final=1e-4)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    params = st.fold(net.get_params())
-    raw = Mlp(seed=0)
-    raw.set_params(params)
-    yh = torch.empty(6 * Xh.shape[0], dtype=torch.float32, device="cuda")
-    torch.cuda.synchronize()
-    raw.forward(Xh.data_ptr(), Xh.shape[0], yh.data_ptr())
-    torch.cuda.synchronize()
-    pred = yh.view(-1, 6).cpu().numpy().astype(np.float64)
-    Thn = Th.cpu().numpy().astype(np.float64)
-    mse, mse0 = float(np.mean((pred - Thn) ** 2)), float(np.mean(Thn ** 2))
-    out.update(samples_per_snapshot=n_samp, train_samples=int(xs.shape[0]), holdout_samples=int(Thn.shape[0]),
-               train_loss=[float(losses[0]), float(np.mean(losses[-100:]))], train_seconds=t_train,
-               train_samples_per_s=P["iters"] * P["batch"] / t_train, holdout_mse_learned=mse, holdout_mse_zero=mse0,
-               holdout_explained=1.0 - mse / mse0, params=[float(v) for v in params])
-    log(f"training: {xs.shape[0]} samples, {P['iters']} iterations in {t_train:.1f} s, loss {losses[0]:.3f} -> "
-        f"{np.mean(losses[-100:]):.3f}; held-out MSE learned {mse:.4e} vs no model {mse0:.4e} "
-        f"({100 * (1 - mse / mse0):.1f}% explained)")
+    out.update(samples_per_snapshot=n_samp, train_samples=int(Xn.shape[0]), holdout_samples=int(Thn.shape[0]))
+    perm = np.random.default_rng(0).permutation(len(Xn))
+    Xhn = Xh.cpu().numpy()
+    for key, tgt, tgt_h in (("", Tn, Thn), ("proj_", None, None)):
+        if tgt is None:  # the dissipative part of the same targets
+            tgt = strain_projection(Tn, Xn)
+            tgt_h = strain_projection(Thn, Xhn).astype(np.float64)
+        st = Standardizer(Xn, tgt)
+        xs = torch.from_numpy(st.inputs(Xn)[perm]).cuda().contiguous()
+        ts = torch.from_numpy(st.targets(tgt)[perm]).cuda().contiguous()
+        torch.cuda.synchronize()
+        net = Mlp(seed=0)
+        t0 = time.perf_counter()
+        losses = train(net, xs.data_ptr(), ts.data_ptr(), xs.shape[0], batch=P["batch"], iters=P["iters"], lr=1e-3,
+                       lr_final=1e-4)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        params = st.fold(net.get_params())
+        net.close()
+        del xs, ts
+        raw = Mlp(seed=0)
+        raw.set_params(params)
+        yh = torch.empty(6 * Xh.shape[0], dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        raw.forward(Xh.data_ptr(), Xh.shape[0], yh.data_ptr())
+        torch.cuda.synchronize()
+        raw.close()
+        pred = yh.view(-1, 6).cpu().numpy().astype(np.float64)
+        del yh
+        if key:  # what the engine keeps of the prediction
+            pred = strain_projection(pred, Xhn).astype(np.float64)
+        mse, mse0 = float(np.mean((pred - tgt_h) ** 2)), float(np.mean(tgt_h ** 2))
+        out.update({key + "train_loss": [float(losses[0]), float(np.mean(losses[-100:]))],
+                    key + "train_seconds": t_train, key + "train_samples_per_s": P["iters"] * P["batch"] / t_train,
+                    key + "holdout_mse_learned": mse, key + "holdout_mse_zero": mse0,
+                    key + "holdout_explained": 1.0 - mse / mse0, key + "params": [float(v) for v in params]})
+        log(f"training {key or 'stress'}: {Xn.shape[0]} samples, {P['iters']} iterations in {t_train:.1f} s, loss "
+            f"{losses[0]:.3f} -> {np.mean(losses[-100:]):.3f}; held-out MSE learned {mse:.4e} vs no model {mse0:.4e} "
+            f"({100 * (1 - mse / mse0):.1f}% explained)")
     return out
 
 
@@ -295,6 +328,7 @@ def stage_judge590(P, log):
 def stage_les590(P, net, judge, log):
     visc = 2.0 / P["re_b"]
     params = np.asarray(net["params"], dtype=np.float32)
+    params_eddy = np.asarray(net["proj_params"], dtype=np.float32) if "proj_params" in net else None
     zj = np.asarray(judge["z"])
     jprof = np.asarray(judge["profile"])
     ut_j = judge["u_tau"]
@@ -307,8 +341,8 @@ def stage_les590(P, net, judge, log):
             gx, gy = net["grid"][0], net["grid"][1]
             fac = net.get("factor", FULL["dns395"]["factor"])
             s = filter_width_ratio2(net.get("re_tau", FULL["dns395"]["re_tau"]), (gx // fac, gy // fac),
-                                    P["re_tau"], (lx, ly)) if model != "learned_unscaled" else 1.0
-            e.set_sgs_params(scale_outputs(params, s))
+                                    P["re_tau"], (lx, ly)) if not model.endswith("unscaled") else 1.0
+            e.set_sgs_params(scale_outputs(params_eddy if model.startswith("learned_eddy") else params, s))
         e.initial_field(0, seed=P["seed"], ubulk=1.0, noise_rms=P["noise"])
         run = Run(e, P["dt"])
         name = f"{model}_{lx}x{ly}x{lz}"
@@ -356,7 +390,7 @@ def stage_les590(P, net, judge, log):
         e.close()
         res[name] = r
         log(f"LES {name}: " + ", ".join(f"{k} {v:.4g}" for k, v in r.items() if isinstance(v, float)))
-    return dict(stage="les590", judge_re_tau=judge["re_tau"], les=res)
+    return dict(stage="les590", judge_re_tau=judge["re_tau"], seed=P["seed"], les=res)
 
 
 def main():
@@ -366,8 +400,14 @@ def main():
     ap.add_argument("--net", help="dns395 output (trained network) for les590")
     ap.add_argument("--judge", help="judge590 output for les590")
     ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--seed", type=int, help="les590: seed of the LES starts (a replicate of the study)")
+    ap.add_argument("--cases", help="les590: comma-separated model:nx:ny:nz in place of the default cases")
     a = ap.parse_args()
-    P = (QUICK if a.quick else FULL)[a.stage]
+    P = dict((QUICK if a.quick else FULL)[a.stage])
+    if a.seed is not None:
+        P["seed"] = a.seed
+    if a.cases:
+        P["cases"] = [(c.split(":")[0], *map(int, c.split(":")[1:])) for c in a.cases.split(",")]
     log = lambda s: print(s, flush=True)  # noqa: E731
     if a.stage == "dns395":
         res = stage_dns395(P, log)

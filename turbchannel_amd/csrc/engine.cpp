@@ -117,7 +117,7 @@ struct tc_engine {
     int diag_rec[2] = {-1, -1};
     int diag_cur = 0;  // the buffer the next step writes
     void drop_records() { diag_rec[0] = diag_rec[1] = -1; }
-    // learned closure (collocated, sgs_model 1 or 2)
+    // learned closure (collocated, sgs_model 1, 2 or 3)
     void* tau = nullptr;                // [node][6] closure stress
     unsigned short* sgs_w = nullptr;    // bfloat16 weight block
     float* sgs_b = nullptr;             // FP32 biases
@@ -507,7 +507,7 @@ int step(tc_engine* e, double tau, int cur, int order) {
     if (e->P.sgs_model) {
         if (!e->sgs_ready) return fail(TC_ERR_STATE, "learned closure weights are not set (tc_set_sgs_params)");
         e->tbeg(TC_K_TURB_VISC, ev);
-        HIPCHK(launch_sgs_tau<T>(e->s, m, sgs_view(e), (const T*)e->vel[cur], (T*)e->tau, e->P.sgs_model == 2));
+        HIPCHK(launch_sgs_tau<T>(e->s, m, sgs_view(e), (const T*)e->vel[cur], (T*)e->tau, e->P.sgs_model - 1));
         e->tend(TC_K_TURB_VISC, ev);
     }
     FluxArgs<T> a;
@@ -867,7 +867,7 @@ int tc_create(const tc_params* pin, int device, tc_engine** out) {
     if (p->visc_type < 0 || p->visc_type > 1) return fail(TC_ERR_ARG, "visc_type must be 0 or 1");
     if (p->scheme != TC_SCHEME_AB3 && p->turb_visc && p->visc_type == 0)
         return fail(TC_ERR_ARG, "tViscApproximation::CROSS does not work for a turbulence model");
-    if (p->sgs_model < 0 || p->sgs_model > 2) return fail(TC_ERR_ARG, "sgs_model must be 0, 1 or 2");
+    if (p->sgs_model < 0 || p->sgs_model > 3) return fail(TC_ERR_ARG, "sgs_model must be 0, 1, 2 or 3");
     if (p->sgs_model && (p->scheme != TC_SCHEME_AB3 || p->turb_visc))
         return fail(TC_ERR_ARG, "the learned closure runs on the collocated scheme, without nu_t");
     const double n3 = (double)p->nx * p->ny * p->nz;
@@ -1146,11 +1146,11 @@ int tc_recalc_turb_visc(tc_engine* e, int kind, int slot) {
     if (!v || !is_vec(kind)) return fail(TC_ERR_ARG, "bad field");
     if (e->P.sgs_model) {  // the learned closure's counterpart: tau from this field
         if (!e->sgs_ready) return fail(TC_ERR_STATE, "learned closure weights are not set (tc_set_sgs_params)");
-        const bool clip = e->P.sgs_model == 2;
+        const int post = e->P.sgs_model - 1;
         if (e->fp32)
-            HIPCHK(launch_sgs_tau<float>(e->s, e->mesh<float>(), sgs_view(e), (const float*)v, (float*)e->tau, clip));
+            HIPCHK(launch_sgs_tau<float>(e->s, e->mesh<float>(), sgs_view(e), (const float*)v, (float*)e->tau, post));
         else
-            HIPCHK(launch_sgs_tau<double>(e->s, e->mesh<double>(), sgs_view(e), (const double*)v, (double*)e->tau, clip));
+            HIPCHK(launch_sgs_tau<double>(e->s, e->mesh<double>(), sgs_view(e), (const double*)v, (double*)e->tau, post));
         return TC_OK;
     }
     if (!e->nut) return TC_OK;  // RecalcTurbVisc returns when nu_t is not allocated

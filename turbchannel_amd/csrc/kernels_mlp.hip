@@ -477,15 +477,20 @@ __global__ void k_mlp_reduce_adam(const double* __restrict__ tmp, float* __restr
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-// CLIP: backscatter clipping, tau = 0 at a node where the stress would move
-// energy from the sub-filter scales to the resolved ones (tau_ij S_ij > 0, S the
-// resolved strain of the node's own gradient inputs): the outputs of a node,
-// spread over the MFMA accumulator lanes, meet the node's inputs through LDS.
-template <typename T, bool CLIP>
+// POST (kSgsPost*): what is kept of the network's stress at a node.
+// kSgsPostClip: backscatter clipping, tau = 0 at a node where the stress would
+// move energy from the sub-filter scales to the resolved ones (tau_ij S_ij > 0,
+// S the resolved strain of the node's own gradient inputs).
+// kSgsPostProject: the dissipative part along the deviatoric strain D, an eddy
+// viscosity read off the network: tau' = (tau_kl D_kl / D_kl D_kl) D where
+// tau_kl D_kl < 0, else 0 (tau' = -2 nu_t D with nu_t >= 0).
+// Both need a node's six outputs, spread over the MFMA accumulator lanes, in the
+// lane of the node's inputs: they meet through LDS.
+template <typename T, int POST>
 __global__ __launch_bounds__(64) void k_sgs_tau(Mesh<T> m, MlpDev w, const T* __restrict__ vel, T* __restrict__ tau,
                                                 T u_tau2, T gscale, T yscale) {
     __shared__ __align__(16) u16 buf[kTile];
-    __shared__ float tq[CLIP ? 64 * 7 : 1];
+    __shared__ float tq[POST != kSgsPostNone ? 64 * 7 : 1];
     FwdRegs R;
     load_fwd_regs(w, R);
     const int nxy = m.nxy;
@@ -527,7 +532,7 @@ __global__ __launch_bounds__(64) void k_sgs_tau(Mesh<T> m, MlpDev w, const T* __
         __syncthreads();
         f32x4 o[4];
         forward_wave_r(R, w.b, buf, o);
-        if constexpr (CLIP) {
+        if constexpr (POST != kSgsPostNone) {
             // outputs of point 16 nt + r, feature 4 g + i -> tq[point][feature]
 #pragma unroll
             for (int nt = 0; nt < 4; nt++)
@@ -538,13 +543,34 @@ __global__ __launch_bounds__(64) void k_sgs_tau(Mesh<T> m, MlpDev w, const T* __
             float t6[kMlpOut];
 #pragma unroll
             for (int f = 0; f < kMlpOut; f++) t6[f] = tq[lane() * 7 + f];
-            // tau_ij S_ij with S_ij = (g_ij + g_ji) / 2, g_ij = xv[3 i + j] (positive scale)
-            const float ts = t6[0] * xv[0] + t6[1] * xv[4] + t6[2] * xv[8] + t6[3] * (xv[1] + xv[3]) +
-                             t6[4] * (xv[2] + xv[6]) + t6[5] * (xv[5] + xv[7]);
-            const bool keep = interior && !(ts > 0.f);
-            if (node < m.n3d)
+            if constexpr (POST == kSgsPostClip) {
+                // tau_ij S_ij with S_ij = (g_ij + g_ji) / 2, g_ij = xv[3 i + j] (positive scale)
+                const float ts = t6[0] * xv[0] + t6[1] * xv[4] + t6[2] * xv[8] + t6[3] * (xv[1] + xv[3]) +
+                                 t6[4] * (xv[2] + xv[6]) + t6[5] * (xv[5] + xv[7]);
+                const bool keep = interior && !(ts > 0.f);
+                if (node < m.n3d)
 #pragma unroll
-                for (int f = 0; f < kMlpOut; f++) tau[node * kMlpOut + f] = keep ? T(t6[f]) * u_tau2 : T(0);
+                    for (int f = 0; f < kMlpOut; f++) tau[node * kMlpOut + f] = keep ? T(t6[f]) * u_tau2 : T(0);
+            } else {
+                // D = S - tr(S) / 3 I in the inputs' (positive) gradient scale, which
+                // cancels in (tau:D / D:D) D
+                const float tr = (xv[0] + xv[4] + xv[8]) * (1.f / 3.f);
+                float d[kMlpOut];
+                d[0] = xv[0] - tr;
+                d[1] = xv[4] - tr;
+                d[2] = xv[8] - tr;
+                d[3] = 0.5f * (xv[1] + xv[3]);
+                d[4] = 0.5f * (xv[2] + xv[6]);
+                d[5] = 0.5f * (xv[5] + xv[7]);
+                const float td = t6[0] * d[0] + t6[1] * d[1] + t6[2] * d[2] +
+                                 2.f * (t6[3] * d[3] + t6[4] * d[4] + t6[5] * d[5]);
+                const float dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] +
+                                 2.f * (d[3] * d[3] + d[4] * d[4] + d[5] * d[5]);
+                const float c = (interior && td < 0.f && dd > 0.f) ? td / dd : 0.f;
+                if (node < m.n3d)
+#pragma unroll
+                    for (int f = 0; f < kMlpOut; f++) tau[node * kMlpOut + f] = T(c * d[f]) * u_tau2;
+            }
         } else {
 #pragma unroll
             for (int nt = 0; nt < 4; nt++) {
@@ -677,18 +703,20 @@ hipError_t launch_mlp_reduce(hipStream_t s, const float* partial, int nrec, doub
 }
 
 template <typename T>
-hipError_t launch_sgs_tau(hipStream_t s, const Mesh<T>& m, MlpDev w, const T* vel, T* tau, bool clip) {
+hipError_t launch_sgs_tau(hipStream_t s, const Mesh<T>& m, MlpDev w, const T* vel, T* tau, int post) {
     const double u_tau = (double)m.visc * (double)m.inv_len;
     const T ut2 = T(u_tau * u_tau), gs = T(1.0 / (u_tau * (double)m.inv_len)), ys = T((double)m.inv_len * kYPlusScale);
-    if (clip) {
-        static const long long resident = resident_blocks(k_sgs_tau<T, true>);
-        const long long nb = std::min(((long long)m.n3d + 63) / 64, resident);
-        hipLaunchKernelGGL((k_sgs_tau<T, true>), dim3((unsigned)nb), dim3(64), 0, s, m, w, vel, tau, ut2, gs, ys);
-    } else {
-        static const long long resident = resident_blocks(k_sgs_tau<T, false>);
-        const long long nb = std::min(((long long)m.n3d + 63) / 64, resident);
-        hipLaunchKernelGGL((k_sgs_tau<T, false>), dim3((unsigned)nb), dim3(64), 0, s, m, w, vel, tau, ut2, gs, ys);
+    const long long nch = ((long long)m.n3d + 63) / 64;
+#define LSGS(P)                                                                                                   \
+    {                                                                                                             \
+        static const long long resident = resident_blocks(k_sgs_tau<T, P>);                                       \
+        hipLaunchKernelGGL((k_sgs_tau<T, P>), dim3((unsigned)std::min(nch, resident)), dim3(64), 0, s, m, w, vel, \
+                           tau, ut2, gs, ys);                                                                     \
     }
+    if (post == kSgsPostClip) LSGS(kSgsPostClip)
+    else if (post == kSgsPostProject) LSGS(kSgsPostProject)
+    else LSGS(kSgsPostNone)
+#undef LSGS
     return hipGetLastError();
 }
 
@@ -707,7 +735,7 @@ template hipError_t launch_sgs_samples<double>(hipStream_t, const Mesh<double>&,
                                                float*, float*);
 template hipError_t launch_sgs_samples<float>(hipStream_t, const Mesh<float>&, const float*, int, int, float, float,
                                               float*, float*);
-template hipError_t launch_sgs_tau<double>(hipStream_t, const Mesh<double>&, MlpDev, const double*, double*, bool);
-template hipError_t launch_sgs_tau<float>(hipStream_t, const Mesh<float>&, MlpDev, const float*, float*, bool);
+template hipError_t launch_sgs_tau<double>(hipStream_t, const Mesh<double>&, MlpDev, const double*, double*, int);
+template hipError_t launch_sgs_tau<float>(hipStream_t, const Mesh<float>&, MlpDev, const float*, float*, int);
 
 }  // namespace tc

@@ -65,9 +65,13 @@ constexpr int kMlpReduceSplit = 16;
 hipError_t launch_mlp_reduce(hipStream_t s, const float* partial, int nrec, double* tmp, float* grad, double* loss,
                              const AdamArgs& a, float* params, float* m, float* v, unsigned short* wdev, float* bdev);
 
-// Closure stress at every node of a collocated field (walls: zero), scaled by u_tau^2.
+// Closure stress at every node of a collocated field (walls: zero), scaled by
+// u_tau^2; post: what is kept of the network's output (tc_params::sgs_model - 1):
+// all of it, the stress without backscatter, or its dissipative part along the
+// deviatoric strain rate.
+constexpr int kSgsPostNone = 0, kSgsPostClip = 1, kSgsPostProject = 2;
 template <typename T>
-hipError_t launch_sgs_tau(hipStream_t s, const Mesh<T>& m, MlpDev w, const T* vel, T* tau, bool clip);
+hipError_t launch_sgs_tau(hipStream_t s, const Mesh<T>& m, MlpDev w, const T* vel, T* tau, int post);
 
 // Samples at the coarse interior nodes from the 9 filtered fields q (u, v, w, uu,
 // vv, ww, uv, uw, vw; each [nz][nyc][nxc]): x [S][10], y [S][6], S = nxc nyc (nz-2).

@@ -82,6 +82,7 @@ struct Options {
     double ubulk = 1.0, noise = 0.1;
     std::string sgs_file;
     int sgs_clip = 0;  // 1: backscatter clipping of the learned closure
+    int sgs_proj = 0;  // 1: the learned closure's dissipative part along the strain rate
     long long checkpoint_every = 0;
     std::string checkpoint = "restart.bin";
     std::string restart;
@@ -135,6 +136,7 @@ const OptSpec kOpts[] = {
     {"--noise", "rms of the seeded perturbation (default 0.1)"},
     {"--sgs", "learned closure: file of float32 network parameters (collocated, --les 0)"},
     {"--sgs_clip", "1: the learned closure's stress is zero where tau_ij S_ij > 0 (backscatter clipping)"},
+    {"--sgs_proj", "1: only the learned closure's dissipative part along the deviatoric strain rate (eddy viscosity)"},
     {"--checkpoint-every", "write the restart file every this many steps (0: never)"},
     {"--checkpoint", "restart file to write (default restart.bin)"},
     {"--restart", "continue exactly from this restart file"},
@@ -245,6 +247,7 @@ bool parse(int argc, char** argv, Options& o) {
         else if (is("--noise")) parse_num(key, val, o.noise);
         else if (is("--sgs")) o.sgs_file = val;
         else if (is("--sgs_clip")) parse_num(key, val, o.sgs_clip);
+        else if (is("--sgs_proj")) parse_num(key, val, o.sgs_proj);
         else if (is("--checkpoint-every")) parse_num(key, val, o.checkpoint_every);
         else if (is("--checkpoint")) o.checkpoint = val;
         else if (is("--restart")) o.restart = val;
@@ -398,7 +401,7 @@ Problem setup(const Options& o) {
     p.ymin = P.ymin;
     p.cfr_ubulk = o.cfr;
     p.zz = P.zz.data();
-    p.sgs_model = o.sgs_file.empty() ? 0 : (o.sgs_clip ? 2 : 1);
+    p.sgs_model = o.sgs_file.empty() ? 0 : o.sgs_proj ? 3 : o.sgs_clip ? 2 : 1;
     p.imex = imex;
     p.alpha_tau = o.alpha_tau;
     return P;
