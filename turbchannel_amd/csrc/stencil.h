@@ -93,34 +93,37 @@ __device__ __forceinline__ T zbar_vv(const Mesh<T>& m, const AC& acc, int c, int
 }
 
 // only: -1 all components, else the one component (wave-uniform) the caller keeps.
+// Each direction's term is a difference of face fluxes (interpolated mass flux
+// times the sum of the two velocities), the regular faces weighted by DZ (or hy,
+// hx hy), the 3h-faces by DZW / 27; those weights are per-plane constants and are
+// applied once per direction, after the differences.
 template <bool INT, typename T, class A>
 __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, T f[3], int only = -1) {
     // wall tests: on the interior planes (INT) every one is false
     const int zt = INT ? 3 : iz, NZt = INT ? 7 : m.nz;
     const T C27 = T(1.0 / 27.0);
-    const T hx = m.hx, hy = m.hy, hxy = hx * hy;
+    const T hx = m.hx, hy = m.hy, hxy = hx * hy, hxy27 = C27 * hxy;
     const T dz = m.DZ[iz], dzw = m.DZW[iz];
+    const T dzhy = dz * hy, dzhx = dz * hx, dzwhy27 = C27 * (dzw * hy), dzwhx27 = C27 * (dzw * hx);
     const T u0 = UU(0, 0, 0), v0 = VV(0, 0, 0), w0 = WW(0, 0, 0);
     T f0 = 0, f1 = 0, f2 = 0;
     // ---- u-equation
     if (only < 0 || only == 0) {
     {
-        const T br = dz * hy * (T(9) * (u0 + UU(1, 0, 0)) - (UU(-1, 0, 0) + UU(2, 0, 0)));
-        const T bl = dz * hy * (T(9) * (UU(-1, 0, 0) + u0) - (UU(-2, 0, 0) + UU(1, 0, 0)));
-        const T bbr = dzw * hy * (T(9) * (UU(1, 0, 0) + UU(2, 0, 0)) - (u0 + UU(3, 0, 0)));
-        const T bbl = dzw * hy * (T(9) * (UU(-2, 0, 0) + UU(-1, 0, 0)) - (UU(-3, 0, 0) + u0));
-        f0 += br * (u0 + UU(1, 0, 0)) - C27 * bbr * (u0 + UU(3, 0, 0)) - bl * (UU(-1, 0, 0) + u0) + C27 * bbl * (UU(-3, 0, 0) + u0);
+        const T sp = u0 + UU(1, 0, 0), sm = UU(-1, 0, 0) + u0;
+        const T s3p = u0 + UU(3, 0, 0), s3m = UU(-3, 0, 0) + u0;
+        const T er = sp * (T(9) * sp - (UU(-1, 0, 0) + UU(2, 0, 0))) - sm * (T(9) * sm - (UU(-2, 0, 0) + UU(1, 0, 0)));
+        const T ew = s3p * (T(9) * (UU(1, 0, 0) + UU(2, 0, 0)) - s3p) - s3m * (T(9) * (UU(-2, 0, 0) + UU(-1, 0, 0)) - s3m);
+        f0 += dzhy * er - dzwhy27 * ew;
     }
     {
-        const T br = dz * hx * XBAR(VV, 1, 0);
-        const T bl = dz * hx * XBAR(VV, 0, 0);
-        const T bbr = dzw * hx * XBAR(VV, 2, 0);
-        const T bbl = dzw * hx * XBAR(VV, -1, 0);
-        f0 += br * (u0 + UU(0, 1, 0)) - C27 * bbr * (u0 + UU(0, 3, 0)) - bl * (UU(0, -1, 0) + u0) + C27 * bbl * (UU(0, -3, 0) + u0);
+        const T er = XBAR(VV, 1, 0) * (u0 + UU(0, 1, 0)) - XBAR(VV, 0, 0) * (UU(0, -1, 0) + u0);
+        const T ew = XBAR(VV, 2, 0) * (u0 + UU(0, 3, 0)) - XBAR(VV, -1, 0) * (UU(0, -3, 0) + u0);
+        f0 += dzhx * er - dzwhx27 * ew;
     }
     {
-        const T x0 = hxy * XBAR(WW, 0, 0), x1 = hxy * XBAR(WW, 0, 1);
-        const T xm1 = hxy * XBAR(WW, 0, -1), x2 = hxy * XBAR(WW, 0, 2);
+        const T x0 = XBAR(WW, 0, 0), x1 = XBAR(WW, 0, 1);
+        const T xm1 = XBAR(WW, 0, -1), x2 = XBAR(WW, 0, 2);
         const T p1 = UU(0, 0, 1), p2 = UU(0, 0, 2), p3 = UU(0, 0, 3);
         const T m1 = UU(0, 0, -1), m2 = UU(0, 0, -2), m3 = UU(0, 0, -3);
         const T bl = zt == 0 ? T(0) : x0;
@@ -131,29 +134,27 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
         const T q_rr = u0 + (zt == NZt - 3 ? -p2 : zt == NZt - 2 ? T(0) : zt == NZt - 1 ? m2 : p3);
         const T q_l = zt == 0 ? T(0) : m1 + u0;
         const T q_ll = u0 + (zt == 2 ? -m2 : zt == 1 ? T(0) : zt == 0 ? p2 : m3);
-        f0 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
+        f0 += hxy * (br * q_r - bl * q_l) - hxy27 * (bbr * q_rr - bbl * q_ll);
     }
     }
     __builtin_amdgcn_sched_barrier(0);  // keep the equations' live ranges apart
     // ---- v-equation
     if (only < 0 || only == 1) {
     {
-        const T br = dz * hy * YBAR(UU, 1, 0);
-        const T bl = dz * hy * YBAR(UU, 0, 0);
-        const T bbr = dzw * hy * YBAR(UU, 2, 0);
-        const T bbl = dzw * hy * YBAR(UU, -1, 0);
-        f1 += br * (v0 + VV(1, 0, 0)) - C27 * bbr * (v0 + VV(3, 0, 0)) - bl * (VV(-1, 0, 0) + v0) + C27 * bbl * (VV(-3, 0, 0) + v0);
+        const T er = YBAR(UU, 1, 0) * (v0 + VV(1, 0, 0)) - YBAR(UU, 0, 0) * (VV(-1, 0, 0) + v0);
+        const T ew = YBAR(UU, 2, 0) * (v0 + VV(3, 0, 0)) - YBAR(UU, -1, 0) * (VV(-3, 0, 0) + v0);
+        f1 += dzhy * er - dzwhy27 * ew;
     }
     {
-        const T br = dz * hx * (T(9) * (v0 + VV(0, 1, 0)) - (VV(0, -1, 0) + VV(0, 2, 0)));
-        const T bl = dz * hx * (T(9) * (VV(0, -1, 0) + v0) - (VV(0, -2, 0) + VV(0, 1, 0)));
-        const T bbr = dzw * hx * (T(9) * (VV(0, 1, 0) + VV(0, 2, 0)) - (v0 + VV(0, 3, 0)));
-        const T bbl = dzw * hx * (T(9) * (VV(0, -2, 0) + VV(0, -1, 0)) - (VV(0, -3, 0) + v0));
-        f1 += br * (v0 + VV(0, 1, 0)) - C27 * bbr * (v0 + VV(0, 3, 0)) - bl * (VV(0, -1, 0) + v0) + C27 * bbl * (VV(0, -3, 0) + v0);
+        const T sp = v0 + VV(0, 1, 0), sm = VV(0, -1, 0) + v0;
+        const T s3p = v0 + VV(0, 3, 0), s3m = VV(0, -3, 0) + v0;
+        const T er = sp * (T(9) * sp - (VV(0, -1, 0) + VV(0, 2, 0))) - sm * (T(9) * sm - (VV(0, -2, 0) + VV(0, 1, 0)));
+        const T ew = s3p * (T(9) * (VV(0, 1, 0) + VV(0, 2, 0)) - s3p) - s3m * (T(9) * (VV(0, -2, 0) + VV(0, -1, 0)) - s3m);
+        f1 += dzhx * er - dzwhx27 * ew;
     }
     {
-        const T y0 = hxy * YBAR(WW, 0, 0), y1 = hxy * YBAR(WW, 0, 1);
-        const T ym1 = hxy * YBAR(WW, 0, -1), y2 = hxy * YBAR(WW, 0, 2);
+        const T y0 = YBAR(WW, 0, 0), y1 = YBAR(WW, 0, 1);
+        const T ym1 = YBAR(WW, 0, -1), y2 = YBAR(WW, 0, 2);
         const T p1 = VV(0, 0, 1), p2 = VV(0, 0, 2), p3 = VV(0, 0, 3);
         const T m1 = VV(0, 0, -1), m2 = VV(0, 0, -2), m3 = VV(0, 0, -3);
         const T bl = zt == 0 ? T(0) : y0;
@@ -164,26 +165,28 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
         const T q_rr = v0 + (zt == NZt - 3 ? -p2 : zt == NZt - 2 ? T(0) : zt == NZt - 1 ? m2 : p3);
         const T q_l = zt == 0 ? T(0) : m1 + v0;
         const T q_ll = v0 + (zt == 2 ? -m2 : zt == 1 ? T(0) : zt == 0 ? p2 : m3);
-        f1 += br * q_r - C27 * bbr * q_rr - bl * q_l + C27 * bbl * q_ll;
+        f1 += hxy * (br * q_r - bl * q_l) - hxy27 * (bbr * q_rr - bbl * q_ll);
     }
     }
     __builtin_amdgcn_sched_barrier(0);
     // ---- w-equation (no unknown at iz = 0)
     if (zt != 0 && (only < 0 || only == 2)) {
         {
-            const T br = hy * zbar_vv<INT>(m, acc, 0, 1, 0, iz, m.DZ);
-            const T bl = hy * zbar_vv<INT>(m, acc, 0, 0, 0, iz, m.DZ);
-            const T bbr = hy * zbar_vv<INT>(m, acc, 0, 2, 0, iz, m.DZW);
-            const T bbl = hy * zbar_vv<INT>(m, acc, 0, -1, 0, iz, m.DZW);
-            f2 += br * (w0 + WW(1, 0, 0)) - C27 * bbr * (w0 + WW(3, 0, 0)) - bl * (WW(-1, 0, 0) + w0) + C27 * bbl * (WW(-3, 0, 0) + w0);
+            const T br = zbar_vv<INT>(m, acc, 0, 1, 0, iz, m.DZ);
+            const T bl = zbar_vv<INT>(m, acc, 0, 0, 0, iz, m.DZ);
+            const T bbr = zbar_vv<INT>(m, acc, 0, 2, 0, iz, m.DZW);
+            const T bbl = zbar_vv<INT>(m, acc, 0, -1, 0, iz, m.DZW);
+            f2 += hy * ((br * (w0 + WW(1, 0, 0)) - bl * (WW(-1, 0, 0) + w0)) -
+                        C27 * (bbr * (w0 + WW(3, 0, 0)) - bbl * (WW(-3, 0, 0) + w0)));
         }
         {
             // The reference scales this y-face flux by hy (staggered/func_global.h:475-487); kept.
-            const T br = hy * zbar_vv<INT>(m, acc, 1, 0, 1, iz, m.DZ);
-            const T bl = hy * zbar_vv<INT>(m, acc, 1, 0, 0, iz, m.DZ);
-            const T bbr = hy * zbar_vv<INT>(m, acc, 1, 0, 2, iz, m.DZW);
-            const T bbl = hy * zbar_vv<INT>(m, acc, 1, 0, -1, iz, m.DZW);
-            f2 += br * (w0 + WW(0, 1, 0)) - C27 * bbr * (w0 + WW(0, 3, 0)) - bl * (WW(0, -1, 0) + w0) + C27 * bbl * (WW(0, -3, 0) + w0);
+            const T br = zbar_vv<INT>(m, acc, 1, 0, 1, iz, m.DZ);
+            const T bl = zbar_vv<INT>(m, acc, 1, 0, 0, iz, m.DZ);
+            const T bbr = zbar_vv<INT>(m, acc, 1, 0, 2, iz, m.DZW);
+            const T bbl = zbar_vv<INT>(m, acc, 1, 0, -1, iz, m.DZW);
+            f2 += hy * ((br * (w0 + WW(0, 1, 0)) - bl * (WW(0, -1, 0) + w0)) -
+                        C27 * (bbr * (w0 + WW(0, 3, 0)) - bbl * (WW(0, -3, 0) + w0)));
         }
         {
             const T wzl = WW(0, 0, -1);
@@ -194,15 +197,14 @@ __device__ __forceinline__ void conv_vv(const Mesh<T>& m, const A& acc, int iz, 
             const T wzrr = zt == NZt - 2 ? T(0) : zt == NZt - 1 ? w0 : p2;
             const T wzlll = zt == 1 ? p1 : zt == 2 ? wzl : m3;
             const T wzrrr = zt == NZt - 3 ? T(0) : zt == NZt - 2 ? p1 : zt == NZt - 1 ? wzl : p3;
-            const T bl = hxy * (T(9) * (wzl + w0) - (wzll + wzr));
-            const T br = hxy * (T(9) * (w0 + wzr) - (wzl + wzrr));
-            const T bbl = hxy * (T(9) * (wzll + wzl) - (wzlll + w0));
-            const T bbr = hxy * (T(9) * (wzr + wzrr) - (w0 + wzrrr));
-            f2 += br * (w0 + wzr) - bl * (wzl + w0);
+            const T sl = wzl + w0, sr = w0 + wzr;
+            const T bl = T(9) * sl - (wzll + wzr);
+            const T br = T(9) * sr - (wzl + wzrr);
+            const T bbl = T(9) * (wzll + wzl) - (wzlll + w0);
+            const T bbr = T(9) * (wzr + wzrr) - (w0 + wzrrr);
             const T qr = zt < NZt - 2 ? (w0 + wzrrr) : zt == NZt - 2 ? (-p1 + w0) : (wzl + w0);
-            f2 -= C27 * bbr * qr;
             const T ql = zt > 2 ? (wzlll + w0) : zt == 2 ? (-wzl + w0) : (p1 + w0);
-            f2 += C27 * bbl * ql;
+            f2 += hxy * (br * sr - bl * sl) - hxy27 * (bbr * qr - bbl * ql);
         }
     }
     const T minus_inv_vol = m.IVOL[iz];  // -1 / (32 (dz - dzw/9) hx hy)
