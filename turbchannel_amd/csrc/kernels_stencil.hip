@@ -846,14 +846,26 @@ __global__ __launch_bounds__(kBlock) void k_project(Mesh<T> m, ProjectArgs<T> a)
 #pragma unroll
     for (int k = 0; k < kProjDiagCells; k++)
         if (act[k]) project_store<T, SCHEME, STAB, true>(a, c[k], &acc, dp);
-    double v[P];
-#pragma unroll
-    for (int k = 0; k < P; k++) v[k] = k < D::NS ? acc.s[k] : k < D::NX ? acc.mx[k - D::NS] : 0.0;
-    const unsigned long long summask = (1ull << D::NS) - 1;  // then 3 maxima (and padding)
-    const double r = wave_reduce_transpose<P>(v, summask);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    constexpr int LS = 64 / P;  // lanes per value
-    if (lane % LS == 0) shr[wid][lane / LS] = r;
+    if (P > 16 && D::NX == 17 && !a.nut) {
+        // Without nu_t every cell's viscous CFL term is kv * visc, so that maximum
+        // needs no exchange, and the other 14 sums and 2 maxima fit 16 slots: half
+        // the exchanges of the 32-slot reduction. Each block has an active cell.
+        double v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = k < D::NS ? acc.s[k] : k == D::NS ? acc.mx[0] : acc.mx[2];
+        const double r = wave_reduce_transpose<16>(v, (1ull << D::NS) - 1);
+        if (lane % 4 == 0) shr[wid][lane / 4 + (lane / 4 > D::NS)] = r;  // slot 15 -> record value NS + 2
+        if (lane == 0) shr[wid][D::NS + 1] = dp.kv * (double)m.visc;
+    } else {
+        double v[P];
+#pragma unroll
+        for (int k = 0; k < P; k++) v[k] = k < D::NS ? acc.s[k] : k < D::NX ? acc.mx[k - D::NS] : 0.0;
+        const unsigned long long summask = (1ull << D::NS) - 1;  // then 3 maxima (and padding)
+        const double r = wave_reduce_transpose<P>(v, summask);
+        constexpr int LS = 64 / P;  // lanes per value
+        if (lane % LS == 0) shr[wid][lane / LS] = r;
+    }
     __syncthreads();
     if (threadIdx.x < D::NX) {
         const int k = threadIdx.x;
