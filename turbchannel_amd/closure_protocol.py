@@ -45,7 +45,9 @@ each run in one process (device memory does not survive between them):
              ``learned_proj_unscaled`` the same without the scaling;
              ``learned_eddy`` and ``learned_eddy_unscaled`` run the second network
              of the dns395 stage (trained on that dissipative part of the
-             filtered-DNS stress) the same way.
+             filtered-DNS stress) the same way; ``learned_eddy_k41`` scales it by
+             (Delta+_les / Delta+_train)^(4/3), the inertial-range scaling of an
+             eddy viscosity (eps^(1/3) Delta^(4/3)) in place of Delta^2.
 
 Run (from the repository root): ``python -m turbchannel_amd.closure_protocol STAGE
 --out FILE [--net FILE]``; ``--quick`` shrinks every size for a functional check.
@@ -109,7 +111,7 @@ def learned_post(model):
 def engine(nx, ny, zz, re_b, re_tau, model, precision=FP32_BULK):
     """model: 'dns' (FD viscous term), 'no_model', 'smagorinsky', 'learned', 'learned_clip',
     'learned_unscaled', 'learned_proj', 'learned_proj_unscaled', 'learned_eddy',
-    'learned_eddy_unscaled' (Galerkin viscous term)."""
+    'learned_eddy_unscaled', 'learned_eddy_k41' (Galerkin viscous term)."""
     visc = 2.0 / re_b
     u_tau = re_tau * visc
     kw = dict(precision=precision, visc=visc, sour_dpdx=u_tau * u_tau, inv_length_scale=re_tau, hx=LX / nx,
@@ -342,6 +344,8 @@ def stage_les590(P, net, judge, log):
             fac = net.get("factor", FULL["dns395"]["factor"])
             s = filter_width_ratio2(net.get("re_tau", FULL["dns395"]["re_tau"]), (gx // fac, gy // fac),
                                     P["re_tau"], (lx, ly)) if not model.endswith("unscaled") else 1.0
+            if model.endswith("_k41"):  # an eddy viscosity in the inertial range: eps^(1/3) Delta^(4/3)
+                s = s ** (2.0 / 3.0)
             e.set_sgs_params(scale_outputs(params_eddy if model.startswith("learned_eddy") else params, s))
         e.initial_field(0, seed=P["seed"], ubulk=1.0, noise_rms=P["noise"])
         run = Run(e, P["dt"])
