@@ -16,6 +16,11 @@ CASES = {
     "ab3": (tca.SCHEME_AB3, 16, 12, 18, "tanh", dict(stab_type=1, fd_mode=1, visc_type=0)),
     "ab3_les": (tca.SCHEME_AB3, 16, 12, 18, "tanh", dict(stab_type=1, fd_mode=1, visc_type=1, turb_visc=1)),
     "ab3_stab0": (tca.SCHEME_AB3, 12, 20, 14, "tanh", dict(stab_type=0, fd_mode=1, visc_type=0)),
+    # 192 planes: the FP64 z-solve keeps its forward intermediate over the
+    # spectrum instead of in LDS, on the launch path of the 384 x 384 x 192 grids
+    "hw_tall": (tca.SCHEME_HW, 8, 6, 192, "sinh", dict()),
+    "vv_tall": (tca.SCHEME_VV, 8, 6, 192, "tanh", dict()),
+    "ab3_tall": (tca.SCHEME_AB3, 8, 6, 192, "tanh", dict(stab_type=1, fd_mode=1, visc_type=0)),
     # learned closure (collocated only), FD and Galerkin viscous terms
     "ab3_sgs": (tca.SCHEME_AB3, 32, 16, 18, "tanh", dict(stab_type=1, fd_mode=1, visc_type=0, sgs_model=1)),
     "ab3_sgs_gal": (tca.SCHEME_AB3, 24, 16, 14, "tanh", dict(stab_type=1, fd_mode=1, visc_type=1, sgs_model=1)),

@@ -1219,12 +1219,14 @@ hipError_t launch_zsolve(hipStream_t s, Cplx<T>* spec, const ZFacDev<T>& f, cons
     // zlds_rows), which removes the spectrum's second pass (AB3 118 -> 88 us,
     // fourth order 140 -> 105 us at 384 x 384 x 192 with all rows in
     // registers and streamed factors)
-    const bool reg = inl && f.nz == 192 && !lds;
+    const bool reg = sizeof(T) == 4 && inl && f.nz == 192 && !lds;
 #define LZR(W_)                                                                                                         \
     do {                                                                                                                \
         if constexpr (sizeof(T) == 4)                                                                                   \
             hipLaunchKernelGGL((k_zsolve<T, W_, 0, 192, true>), grid, dim3(64), zlds_rows<W_>(192) * 64 * sizeof(T), s, \
                                spec, f, flow, lowidx, lowmode, nlow, ylow, norm, nbulk, f.R);                           \
+        else                                                                                                            \
+            LZS(W_, 0);                                                                                                 \
     } while (0)
     if (f.W == 3) {
         if (lds) LZS(3, 3);
