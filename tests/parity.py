@@ -21,6 +21,10 @@ CASES = {
     "hw_tall": (tca.SCHEME_HW, 8, 6, 192, "sinh", dict()),
     "vv_tall": (tca.SCHEME_VV, 8, 6, 192, "tanh", dict()),
     "ab3_tall": (tca.SCHEME_AB3, 8, 6, 192, "tanh", dict(stab_type=1, fd_mode=1, visc_type=0)),
+    # 264 planes: the FP32-bulk z-solve forms its factors in-lane with the forward
+    # intermediate over the spectrum (as the 512 x 512 x 256 grid does); FP32 tests only
+    "hw_deep": (tca.SCHEME_HW, 8, 6, 264, "tanh", dict()),
+    "ab3_deep": (tca.SCHEME_AB3, 8, 6, 264, "tanh", dict(stab_type=1, fd_mode=1, visc_type=0)),
     # learned closure (collocated only), FD and Galerkin viscous terms
     "ab3_sgs": (tca.SCHEME_AB3, 32, 16, 18, "tanh", dict(stab_type=1, fd_mode=1, visc_type=0, sgs_model=1)),
     "ab3_sgs_gal": (tca.SCHEME_AB3, 24, 16, 14, "tanh", dict(stab_type=1, fd_mode=1, visc_type=1, sgs_model=1)),

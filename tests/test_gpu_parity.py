@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 TOL64 = 1e-12
 TOL_TRAJ = 1e-10
 TOL32 = 1e-5
-ALL = [c for c in parity.CASES if "sgs" not in c]  # closure cases: test_gpu_closure.py
+# closure cases: test_gpu_closure.py; the deep cases: FP32-bulk per-step error only
+ALL = [c for c in parity.CASES if "sgs" not in c and "deep" not in c]
 
 
 def fields(sol, seed=3):
@@ -207,7 +208,7 @@ def test_projection_is_divergence_free(name):
     assert np.abs(sol.apply_divergence(v)).max() < 1e-11
 
 
-@pytest.mark.parametrize("name", ["hw", "vv", "vv_les", "ab3", "ab3_les"])
+@pytest.mark.parametrize("name", ["hw", "vv", "vv_les", "ab3", "ab3_les", "hw_deep", "ab3_deep"])
 def test_fp32_bulk_per_step_error(name):
     e64 = parity.make_engine(name, precision=tca.FP64)
     e32 = parity.make_engine(name, precision=tca.FP32_BULK)
