@@ -254,6 +254,7 @@ def cpu_baseline(name, seconds_target=15.0, threads=None, shipped_threads=0, shi
     for it in range(3):  # order ramp, untimed
         step(it, cur, it + 1)
         cur = (cur + 1) % 4
+    sol.timers(reset=True)
     t0 = time.perf_counter()
     n = 0
     while True:
@@ -264,7 +265,9 @@ def cpu_baseline(name, seconds_target=15.0, threads=None, shipped_threads=0, shi
         if el > seconds_target:
             break
     out = {"value": nx * ny * nz * n / el, "unit": "grid-pt steps/s", "cores": threads, "kind": "port",
-           "sample": f"{n} BDF3 steps of {name} (FP64 oracle, OpenMP {threads} threads, {el:.1f} s)"}
+           "sample": f"{n} BDF3 steps of {name} (FP64 oracle, OpenMP {threads} threads, {el:.1f} s)",
+           # the reference's timer channels (explicit, grad, pressure F/T/N), seconds per step
+           "channels_s_per_step": {k: round(v / n, 6) for k, v in sol.timers().items()}}
     if shipped_threads:
         # the reference's own OpenMP setting, timed on the same state for at least one step
         oracle.set_threads(shipped_threads)

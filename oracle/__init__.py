@@ -67,6 +67,7 @@ def lib():
         L.or_flux_term.argtypes = [vp, dp, ctypes.c_int, ctypes.c_int, dp]
         L.or_solve_pressure.argtypes = [vp, dp, dp]
         L.or_normalize_pressure.argtypes = [vp, dp]
+        L.or_timers.argtypes = [vp, dp, ctypes.c_int]
         L.or_recalc_turb_visc.argtypes = [vp, dp]
         L.or_integral.argtypes = [vp, dp, dp]
         L.or_kinetic_energy.argtypes = [vp, dp]
@@ -220,6 +221,15 @@ class Solver:
         p = np.array(p, dtype=np.float64, copy=True)
         lib().or_normalize_pressure(self._h, _p(p))
         return p
+
+    TIMER_CHANNELS = ("explicit", "gradient", "divergence", "pressure_fft", "pressure_thomas",
+                      "pressure_normalize", "other")
+
+    def timers(self, reset=False):
+        """Seconds per timer channel since the last reset (or_timers)."""
+        out = np.zeros(len(self.TIMER_CHANNELS))
+        lib().or_timers(self._h, _p(out), int(reset))
+        return dict(zip(self.TIMER_CHANNELS, out.tolist()))
 
     def recalc_turb_visc(self, u):
         lib().or_recalc_turb_visc(self._h, _p(np.ascontiguousarray(u)))

@@ -134,6 +134,7 @@ struct or_solver {
     cplx* spec;
     double* cosphi[2];
     fft_plan px, py;
+    double tmr[OR_NTIMERS]; /* seconds per channel (or_timers) */
 };
 
 /* Generalised periodic neighbours (staggered/data_global_inc.h:4-9). */
@@ -1139,8 +1140,11 @@ void or_normalize_pressure(or_solver* s, double* f) {
  * collocated_AB3/func_global.h:127-147). */
 void or_solve_pressure(or_solver* s, double* XX, const double* f) {
     const int N3D = s->N3D, NXY = s->NXY, NZ = s->NZ;
+    double t0 = omp_get_wtime();
     for (int in = 0; in < N3D; in++) s->spec[in] = f[in];
     fft2d_planes(s, 0);
+    double t1 = omp_get_wtime();
+    s->tmr[OR_TMR_FFT] += t1 - t0;
 #pragma omp parallel
     {
         double* buf = (double*)malloc(sizeof(double) * 3 * (size_t)NZ);
@@ -1152,10 +1156,22 @@ void or_solve_pressure(or_solver* s, double* XX, const double* f) {
         }
         free(buf);
     }
+    t0 = omp_get_wtime();
+    s->tmr[OR_TMR_THOMAS] += t0 - t1;
     fft2d_planes(s, 1);
     double inv_NN = 1. / NXY;
     for (int in = 0; in < N3D; in++) XX[in] = creal(s->spec[in]) * inv_NN;
+    t1 = omp_get_wtime();
+    s->tmr[OR_TMR_FFT] += t1 - t0;
     or_normalize_pressure(s, XX);
+    s->tmr[OR_TMR_NORMALIZE] += omp_get_wtime() - t1;
+}
+
+void or_timers(or_solver* s, double* out, int reset) {
+    for (int k = 0; k < OR_NTIMERS; k++) {
+        if (out) out[k] = s->tmr[k];
+        if (reset) s->tmr[k] = 0.0;
+    }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1231,7 +1247,9 @@ void or_apply_divergence(or_solver* s, const double* v, double* dv) {
 
 /* UnapplyGradient (staggered/func_global.h:290-293). */
 void or_unapply_gradient(or_solver* s, const double* v, double* p) {
+    double t0 = omp_get_wtime();
     or_apply_divergence(s, v, s->divv);
+    s->tmr[OR_TMR_DIVERGENCE] += omp_get_wtime() - t0;
     or_solve_pressure(s, p, s->divv);
 }
 
@@ -1392,8 +1410,12 @@ int or_step(or_solver* s, double t_start, double tau, int cur, int bdf_order) {
     const double inv_tau = 1. / tau;
     const double bd0 = (bdf_order == 3) ? (1.5 + C1_3) : (bdf_order == 2) ? 1.5 : 1.0;
     const double tau_over_bd0 = tau / bd0;
+    /* timer channels: the reference's explicit / grad / pressure F, T, N */
+    double t0 = omp_get_wtime(), t1;
+#define OR_LAP(ch) (t1 = omp_get_wtime(), s->tmr[ch] += t1 - t0, t0 = t1)
     if (s->nu_t) or_recalc_turb_visc(s, s->vel[cur]);
     or_explicit_term(s, s->vel[cur], s->flux[cur]);
+    OR_LAP(OR_TMR_EXPLICIT);
     double *Fn = s->flux[next], *Fc = s->flux[cur], *Fp = s->flux[prev], *Fp2 = s->flux[prev2];
     double *Vc = s->vel[cur], *Vp = s->vel[prev], *Vp2 = s->vel[prev2], *Vn = s->vel[next];
 #pragma omp parallel for
@@ -1402,21 +1424,29 @@ int or_step(or_solver* s, double t_start, double tau, int cur, int bdf_order) {
         else if (bdf_order == 2) Fn[k] = 2. * Fc[k] - Fp[k] + inv_tau * (2 * Vc[k] - 0.5 * Vp[k]);
         else Fn[k] = 3 * Fc[k] - 3 * Fp[k] + Fp2[k] + inv_tau * (3 * Vc[k] - 1.5 * Vp[k] + C1_3 * Vp2[k]);
     }
+    OR_LAP(OR_TMR_OTHER);
     if (IS_COLL(s->P.scheme) && s->P.stab_type != 0) {
         or_apply_gradient(s, s->pres[cur], Vn);
 #pragma omp parallel for
         for (int k = 0; k < 3 * N3D; k++) Fn[k] -= Vn[k];
-        or_unapply_gradient(s, Fn, s->pres[next]);
+        OR_LAP(OR_TMR_GRADIENT);
+        or_unapply_gradient(s, Fn, s->pres[next]); /* divergence and pressure channels inside */
+        t0 = omp_get_wtime();
         or_apply_gradient(s, s->pres[next], Vn);
 #pragma omp parallel for
         for (int i = 0; i < N3D; i++) s->pres[next][i] += s->pres[cur][i];
+        OR_LAP(OR_TMR_GRADIENT);
     } else {
         or_unapply_gradient(s, Fn, s->pres[next]);
+        t0 = omp_get_wtime();
         or_apply_gradient(s, s->pres[next], Vn);
+        OR_LAP(OR_TMR_GRADIENT);
     }
 #pragma omp parallel for
     for (int k = 0; k < 3 * N3D; k++) Vn[k] = (Fn[k] - Vn[k]) * tau_over_bd0;
     if (s->P.cfr_ubulk > 0) or_cfr_correct(s, Vn);
+    OR_LAP(OR_TMR_OTHER);
+#undef OR_LAP
     return 0;
 }
 

@@ -72,6 +72,14 @@ void or_apply_divergence(or_solver* s, const double* v, double* dv);
 void or_solve_pressure(or_solver* s, double* x, const double* f);   /* LinearSystemSolveFFT */
 void or_unapply_gradient(or_solver* s, const double* v, double* p);
 void or_normalize_pressure(or_solver* s, double* p);
+/* Seconds spent per channel since the last reset (or creation), mirroring the
+ * reference's timers (staggered/main.cpp:376-378): explicit term (with nu_t),
+ * gradient (the step's ApplyGradient calls), divergence, pressure FFT (with the
+ * copies around it), pressure Thomas, pressure normalisation, and the rest of
+ * or_step (BDF combination, projection, flow rate). out: OR_NTIMERS doubles. */
+enum { OR_TMR_EXPLICIT, OR_TMR_GRADIENT, OR_TMR_DIVERGENCE, OR_TMR_FFT, OR_TMR_THOMAS, OR_TMR_NORMALIZE,
+       OR_TMR_OTHER, OR_NTIMERS };
+void or_timers(or_solver* s, double* out, int reset);
 void or_recalc_turb_visc(or_solver* s, const double* u);            /* writes the internal nu_t */
 void or_calc_abs_s(or_solver* s, const double* u, double* absS);    /* staggered CalcAbsS */
 void or_integral(or_solver* s, const double* u, double out[3]);
