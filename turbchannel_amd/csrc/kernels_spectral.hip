@@ -98,10 +98,15 @@ __device__ __forceinline__ void bfly(Cplx<T>* v, T s) {
     }
 }
 
-// LDS element index with one pad element every 16: the early Stockham passes
-// write with a stride of R elements, which would otherwise fall on few banks.
-__device__ __forceinline__ int ldsi(int i) { return i + (i >> 4); }
-__host__ __device__ constexpr size_t lds_elems(size_t n) { return n + (n >> 4) + 1; }
+// LDS element index with one pad element every 4: the passes read and write
+// with strides of the sub-transform spans (48, 6 and 1 elements at 384 =
+// 8 * 8 * 6), which would otherwise fall on few banks. One pad every 16 left
+// 2-4 bank-conflict cycles per LDS instruction (SQ counters); every 4 takes the
+// 384 x 384 x 192 FP32 transforms 141 -> 135 us (forward x with the divergence,
+// collocated), 168 -> 158 us (fourth order), 40.5 -> 38.5 us (each y), 43.5 ->
+// 41.4 us (inverse x); every 8 and every 32 gained less.
+__device__ __forceinline__ int ldsi(int i) { return i + (i >> 2); }
+__host__ __device__ constexpr size_t lds_elems(size_t n) { return n + (n >> 2) + 1; }
 
 // One in-place decimation-in-frequency pass of radix R over nlines lines of
 // length N. Element e of line l lives at buf[ldsi(l*lstride + e*estride)];
