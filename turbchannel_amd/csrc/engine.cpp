@@ -204,17 +204,17 @@ int dalloc(tc_engine* e, T** p, size_t n) {
     void* q = nullptr;
     if (n == 0) n = 1;
     HIPCHK(hipMalloc(&q, n * sizeof(T)));
-    HIPCHK(hipMemset(q, 0, n * sizeof(T)));
-    e->allocs.push_back(q);
+    e->allocs.push_back(q);  // freed with the engine, also if the clear below fails
     *p = (T*)q;
+    HIPCHK(hipMemset(q, 0, n * sizeof(T)));
     return TC_OK;
 }
 
 int dalloc_bytes(tc_engine* e, void** p, size_t bytes) {
     if (bytes == 0) bytes = 8;
     HIPCHK(hipMalloc(p, bytes));
-    HIPCHK(hipMemset(*p, 0, bytes));
     e->allocs.push_back(*p);
+    HIPCHK(hipMemset(*p, 0, bytes));
     return TC_OK;
 }
 
@@ -914,7 +914,10 @@ int tc_create(const tc_params* pin, int device, tc_engine** out) {
         tc_destroy(e.release());
         return r;
     }
-    HIPCHK(hipDeviceSynchronize());
+    if (const hipError_t se = hipDeviceSynchronize(); se != hipSuccess) {
+        tc_destroy(e.release());  // frees what setup_device allocated
+        return fail(TC_ERR_HIP, std::string("hipDeviceSynchronize: ") + hipGetErrorString(se));
+    }
     *out = e.release();
     return TC_OK;
 }
