@@ -452,12 +452,12 @@ ZFacDev<double> zfac_low_dev(tc_engine* e) {
 // UnapplyGradient: p = L^-1 D v (ApplyDivergence + LinearSystemSolveFFT incl.
 // NormalizePressure, staggered/func_global.h:214-248, :290-293).
 template <typename T>
-int unapply(tc_engine* e, const T* v, T* x) {
+int unapply(tc_engine* e, const T* v, T* x, const CfrShiftArgs& cfr = CfrShiftArgs()) {
     hipEvent_t ev;
     Cplx<T>* spec = (Cplx<T>*)e->spec;
     const int nrows = e->ny * e->nz;
     e->tbeg(TC_K_FFT_X, ev);
-    HIPCHK(launch_fft_x_fwd_div<T>(e->s, e->plx, (const Cplx<T>*)e->twx, e->mesh<T>(), e->P.scheme, v, spec));
+    HIPCHK(launch_fft_x_fwd_div<T>(e->s, e->plx, (const Cplx<T>*)e->twx, e->mesh<T>(), e->P.scheme, v, spec, cfr));
     e->tend(TC_K_FFT_X, ev);
     e->tbeg(TC_K_FFT_Y, ev);
     HIPCHK(launch_fft_y<T>(e->s, e->ply, (const Cplx<T>*)e->twy, spec, e->nxh, e->nz, 0, nullptr, nullptr));
@@ -533,13 +533,16 @@ int step(tc_engine* e, double tau, int cur, int order) {
     HIPCHK(launch_flux<T>(e->s, m, a, &nblk));
     e->tend(TC_K_FLUX, ev);
     T* ptarget = stab ? (T*)e->dp : (T*)e->pres[next];
-    if (int r = unapply<T>(e, (const T*)e->flux[next], ptarget)) return r;
+    CfrShiftArgs cs;  // the flow-rate shift, formed by the divergence transform's block 0
     if (cfr) {
-        e->tbeg(TC_K_OTHER, ev);
-        HIPCHK(launch_cfr_shift(e->s, e->d_partial, nblk, tau / bd0,
-                                e->P.cfr_ubulk * e->hm.area, e->hm.vol_u, e->d_shift));
-        e->tend(TC_K_OTHER, ev);
+        cs.partial = e->d_partial;
+        cs.nb = nblk;
+        cs.scale = tau / bd0;
+        cs.ub_vol = e->P.cfr_ubulk * e->hm.area;
+        cs.vol_u = e->hm.vol_u;
+        cs.shift = e->d_shift;
     }
+    if (int r = unapply<T>(e, (const T*)e->flux[next], ptarget, cs)) return r;
     ProjectArgs<T> pa;
     pa.scheme = e->P.scheme;
     pa.p = ptarget;

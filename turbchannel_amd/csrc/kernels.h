@@ -126,10 +126,20 @@ hipError_t launch_srk_pre(hipStream_t s, const Mesh<T>& m, const SrkPre<T>& a);
 // ---------------------------------------------------------------------------
 // Spectral pressure solve
 // ---------------------------------------------------------------------------
+// The flow-rate correction's shift = (ub V - q scale) / V_u, q the sum of the
+// explicit-term kernel's per-block partial sums. The divergence transform that
+// follows that kernel forms it in its block 0 (partial != nullptr), so it needs
+// no launch of its own.
+struct CfrShiftArgs {
+    const double* partial = nullptr;
+    int nb = 0;
+    double scale = 0.0, ub_vol = 0.0, vol_u = 1.0;
+    double* shift = nullptr;
+};
 // Forward x transform of the discrete divergence of v (computed on the fly; ApplyDivergence).
 template <typename T>
 hipError_t launch_fft_x_fwd_div(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const Mesh<T>& m, int scheme,
-                                const T* v, Cplx<T>* spec);
+                                const T* v, Cplx<T>* spec, const CfrShiftArgs& cfr = CfrShiftArgs());
 // psum/C (inverse only, may be nullptr): X00[iz] -= psum[0] * C[iz] before the transform.
 template <typename T>
 hipError_t launch_fft_y(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, Cplx<T>* spec, int nxh, int nz,
@@ -203,9 +213,6 @@ constexpr int kRedBlocks = 1024;
 // out: 3 doubles (integral), 1 (KE, PSUM, CFL). partial: kRedBlocks*3 doubles.
 template <typename T>
 hipError_t launch_reduce(hipStream_t s, const Mesh<T>& m, const RedArgs<T>& a, double* partial, double* out);
-// Final stage of the flow-rate correction: shift = (ub*V - q*scale)/V_u from partial sums.
-hipError_t launch_cfr_shift(hipStream_t s, const double* partial, int nblocks, double scale, double ub_vol,
-                            double vol_u, double* shift);
 // Plane statistics: out[iz*NV + k], NV = 12 (staggered) or 10 (collocated).
 template <typename T>
 hipError_t launch_profiles(hipStream_t s, const Mesh<T>& m, int scheme, const T* v, const T* p, const T* nut,

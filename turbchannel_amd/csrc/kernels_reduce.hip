@@ -112,29 +112,6 @@ __global__ __launch_bounds__(kThreads) void k_reduce_final(const double* partial
         for (int k = 0; k < 3; k++) out[k] = acc[k];
 }
 
-// The flow-rate shift from the explicit-term kernel's per-block partial sums
-// (one per block, several thousand at 384 x 384 x 192). Each thread issues the
-// loads of its first kCfrBatch partials (clamped addresses) before adding any of
-// them, then adds them in the order b = tid, tid + kThreads, ...: the same sums
-// as a loop that waits for each load in turn, without its chain of memory
-// round trips (AB3 384 x 384 x 192 FP32: the step's small kernels 10.7 -> 7.0 us).
-constexpr int kCfrBatch = 32;
-__global__ __launch_bounds__(kThreads) void k_cfr_shift(const double* partial, int nb, double scale, double ub_vol,
-                                                         double vol_u, double* shift) {
-    __shared__ double sh[1][kThreads / 64];
-    double acc[1] = {0.0};
-    for (int b0 = 0; b0 < nb; b0 += kCfrBatch * kThreads) {
-        double v[kCfrBatch];
-#pragma unroll
-        for (int i = 0; i < kCfrBatch; i++) v[i] = partial[min(b0 + (int)threadIdx.x + i * kThreads, nb - 1)];
-#pragma unroll
-        for (int i = 0; i < kCfrBatch; i++)
-            if (b0 + (int)threadIdx.x + i * kThreads < nb) acc[0] += v[i];
-    }
-    block_reduce<1>(acc, sh, false);
-    if (threadIdx.x == 0) shift[0] = (ub_vol - acc[0] * scale) / vol_u;
-}
-
 // Per-plane statistics (CalcProfiles / CalcVelocityProfile).
 template <typename T, int SCHEME>
 __global__ __launch_bounds__(kThreads) void k_profiles(Mesh<T> m, const T* __restrict__ vel, const T* __restrict__ p,
@@ -349,11 +326,6 @@ hipError_t launch_reduce(hipStream_t s, const Mesh<T>& m, const RedArgs<T>& a, d
     return hipGetLastError();
 }
 
-hipError_t launch_cfr_shift(hipStream_t s, const double* partial, int nblocks, double scale, double ub_vol, double vol_u,
-                            double* shift) {
-    hipLaunchKernelGGL(k_cfr_shift, dim3(1), dim3(kThreads), 0, s, partial, nblocks, scale, ub_vol, vol_u, shift);
-    return hipGetLastError();
-}
 
 template <typename T>
 hipError_t launch_profiles(hipStream_t s, const Mesh<T>& m, int scheme, const T* v, const T* p, const T* nut,

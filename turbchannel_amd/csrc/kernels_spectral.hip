@@ -382,11 +382,39 @@ struct Cell3 {
     T x, y, z;
 };
 
+// Block 0 of the divergence transform: the flow-rate shift (CfrShiftArgs). Thread
+// t adds partials t, t + 256, ... in that order, their loads issued in batches of
+// 32 before any is added (a loop that waits for each load in turn is a chain of
+// memory round trips); then the waves' sums by shuffles, and those in wave order.
+__device__ __forceinline__ void cfr_shift_block(const CfrShiftArgs& c) {
+    __shared__ double sh[kFftThreads / 64];
+    constexpr int B = 32;
+    double acc = 0.0;
+    for (int b0 = 0; b0 < c.nb; b0 += B * kFftThreads) {
+        double q[B];
+#pragma unroll
+        for (int i = 0; i < B; i++) q[i] = c.partial[min(b0 + (int)threadIdx.x + i * kFftThreads, c.nb - 1)];
+#pragma unroll
+        for (int i = 0; i < B; i++)
+            if (b0 + (int)threadIdx.x + i * kFftThreads < c.nb) acc += q[i];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = sh[0];
+        for (int w = 1; w < kFftThreads / 64; w++) r += sh[w];
+        c.shift[0] = (c.ub_vol - r * c.scale) / c.vol_u;
+    }
+}
+
 template <typename T, int SCHEME, int IT, int NC>
 __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const Cplx<T>* __restrict__ tw, Mesh<T> m,
                                                                 const T* __restrict__ v, Cplx<T>* __restrict__ spec,
-                                                                int lpb_, FastDiv dny, FastDiv d2l) {
+                                                                int lpb_, FastDiv dny, FastDiv d2l, CfrShiftArgs cfr) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    if (cfr.partial != nullptr && blockIdx.x == 0) cfr_shift_block(cfr);  // block-uniform
     Cplx<T>* b0 = reinterpret_cast<Cplx<T>*>(smem_raw);
     const int N = NC > 0 ? NC : pl.n, nrows = m.ny * m.nz;
     const int lpb = NC > 0 ? lines_per_block<T>(NC) : lpb_;
@@ -1098,9 +1126,10 @@ constexpr int fwd_div_batches(int lpb, int n) { return (lpb * n + 2 * kFftThread
 
 template <typename T, int SCHEME>
 hipError_t launch_fft_x_fwd_div_s(hipStream_t s, int it, dim3 grid, size_t sm, const FFTPlan& pl, const Cplx<T>* tw,
-                                  const Mesh<T>& m, const T* v, Cplx<T>* spec, int lpb, FastDiv dny, FastDiv d2l) {
+                                  const Mesh<T>& m, const T* v, Cplx<T>* spec, int lpb, FastDiv dny, FastDiv d2l,
+                                  const CfrShiftArgs& cfr) {
 #define LFD(IT, NC_) hipLaunchKernelGGL((k_fft_x_fwd_div<T, SCHEME, IT, NC_>), grid, dim3(kFftThreads), sm, s, pl, tw, m, \
-                                        v, spec, lpb, dny, d2l)
+                                        v, spec, lpb, dny, d2l, cfr)
 #define LFDC(NC_) LFD(fwd_div_batches(lines_per_block<T>(NC_), NC_), NC_)
     switch (pl.n) {
         TC_FFT_LENGTHS(LFDC)
@@ -1120,7 +1149,7 @@ hipError_t launch_fft_x_fwd_div_s(hipStream_t s, int it, dim3 grid, size_t sm, c
 
 template <typename T>
 hipError_t launch_fft_x_fwd_div(hipStream_t s, const FFTPlan& pl, const Cplx<T>* tw, const Mesh<T>& m, int scheme,
-                                const T* v, Cplx<T>* spec) {
+                                const T* v, Cplx<T>* spec, const CfrShiftArgs& cfr) {
     const int lpb = lines_per_block<T>(pl.n);
     const int nlines = (m.ny * m.nz + 1) / 2;
     const int grid = (nlines + lpb - 1) / lpb;
@@ -1129,10 +1158,10 @@ hipError_t launch_fft_x_fwd_div(hipStream_t s, const FFTPlan& pl, const Cplx<T>*
     const FastDiv dny((unsigned)m.ny), d2l((unsigned)(2 * lpb));
     const int it = fwd_div_batches(lpb, pl.n);
     if (scheme == TC_SCHEME_HW)
-        return launch_fft_x_fwd_div_s<T, TC_SCHEME_HW>(s, it, dim3(grid), sm, pl, tw, m, v, spec, lpb, dny, d2l);
+        return launch_fft_x_fwd_div_s<T, TC_SCHEME_HW>(s, it, dim3(grid), sm, pl, tw, m, v, spec, lpb, dny, d2l, cfr);
     if (scheme == TC_SCHEME_VV)
-        return launch_fft_x_fwd_div_s<T, TC_SCHEME_VV>(s, it, dim3(grid), sm, pl, tw, m, v, spec, lpb, dny, d2l);
-    return launch_fft_x_fwd_div_s<T, TC_SCHEME_AB3>(s, it, dim3(grid), sm, pl, tw, m, v, spec, lpb, dny, d2l);
+        return launch_fft_x_fwd_div_s<T, TC_SCHEME_VV>(s, it, dim3(grid), sm, pl, tw, m, v, spec, lpb, dny, d2l, cfr);
+    return launch_fft_x_fwd_div_s<T, TC_SCHEME_AB3>(s, it, dim3(grid), sm, pl, tw, m, v, spec, lpb, dny, d2l, cfr);
 }
 
 template <typename T>
@@ -1271,7 +1300,7 @@ hipError_t launch_spec_crop(hipStream_t s, const Cplx<T>* fine, int nxh, int ny,
 
 #define INST(T)                                                                                                      \
     template hipError_t launch_fft_x_fwd_div<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const Mesh<T>&, int,    \
-                                                const T*, Cplx<T>*);                                                 \
+                                                const T*, Cplx<T>*, const CfrShiftArgs&);                            \
     template hipError_t launch_fft_x_inv<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, const Cplx<T>*, T*, int, int, T); \
     template hipError_t launch_fft_y<T>(hipStream_t, const FFTPlan&, const Cplx<T>*, Cplx<T>*, int, int, int,       \
                                         const double*, const double*);                                              \
