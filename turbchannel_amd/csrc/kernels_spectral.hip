@@ -409,6 +409,12 @@ __device__ __forceinline__ void cfr_shift_block(const CfrShiftArgs& c) {
     }
 }
 
+// Elements per batch of the divergence's tap loads. Four waves per SIMD (the
+// block's LDS) leave 128 VGPRs: the second-order collocated taps (6 loads per
+// element) keep 6 elements' loads in flight there; the staggered ones 2
+// (measured: 6 at fourth order is slower).
+constexpr int div_batch_width(int scheme) { return scheme == TC_SCHEME_AB3 ? 6 : 2; }
+
 template <typename T, int SCHEME, int IT, int NC>
 __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const Cplx<T>* __restrict__ tw, Mesh<T> m,
                                                                 const T* __restrict__ v, Cplx<T>* __restrict__ spec,
@@ -433,7 +439,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_x_fwd_div(FFTPlan pl, const
     // y+3 around the pair (the pair shares its own v as each other's y taps) and
     // the w of the z taps, all loaded before any is used.
     T* ub = reinterpret_cast<T*>(b1);
-    constexpr int U = 2;  // elements per batch of the divergence
+    constexpr int U = div_batch_width(SCHEME);  // elements per batch of the divergence
     const int ny = m.ny, nz = m.nz;
     T va[IT][U], wa[IT][U], vb[IT][U], wb[IT][U];
     {
@@ -1122,7 +1128,7 @@ __global__ void k_spec_crop(const Cplx<T>* __restrict__ fine, int nxh, int ny, C
 
 // Batches of 2 elements per thread in the divergence staging: up to 4 (lines of
 // up to 2048 points).
-constexpr int fwd_div_batches(int lpb, int n) { return (lpb * n + 2 * kFftThreads - 1) / (2 * kFftThreads); }
+constexpr int fwd_div_batches(int lpb, int n, int u) { return (lpb * n + u * kFftThreads - 1) / (u * kFftThreads); }
 
 template <typename T, int SCHEME>
 hipError_t launch_fft_x_fwd_div_s(hipStream_t s, int it, dim3 grid, size_t sm, const FFTPlan& pl, const Cplx<T>* tw,
@@ -1130,7 +1136,7 @@ hipError_t launch_fft_x_fwd_div_s(hipStream_t s, int it, dim3 grid, size_t sm, c
                                   const CfrShiftArgs& cfr) {
 #define LFD(IT, NC_) hipLaunchKernelGGL((k_fft_x_fwd_div<T, SCHEME, IT, NC_>), grid, dim3(kFftThreads), sm, s, pl, tw, m, \
                                         v, spec, lpb, dny, d2l, cfr)
-#define LFDC(NC_) LFD(fwd_div_batches(lines_per_block<T>(NC_), NC_), NC_)
+#define LFDC(NC_) LFD(fwd_div_batches(lines_per_block<T>(NC_), NC_, div_batch_width(SCHEME)), NC_)
     switch (pl.n) {
         TC_FFT_LENGTHS(LFDC)
         default:
@@ -1156,7 +1162,7 @@ hipError_t launch_fft_x_fwd_div(hipStream_t s, const FFTPlan& pl, const Cplx<T>*
     const size_t sm = (lds_elems((size_t)lpb * pl.n) + lds_elems((size_t)lpb * pl.n + 2 * lpb) + lds_elems(pl.n)) *
                       sizeof(Cplx<T>);
     const FastDiv dny((unsigned)m.ny), d2l((unsigned)(2 * lpb));
-    const int it = fwd_div_batches(lpb, pl.n);
+    const int it = fwd_div_batches(lpb, pl.n, div_batch_width(scheme));
     if (scheme == TC_SCHEME_HW)
         return launch_fft_x_fwd_div_s<T, TC_SCHEME_HW>(s, it, dim3(grid), sm, pl, tw, m, v, spec, lpb, dny, d2l, cfr);
     if (scheme == TC_SCHEME_VV)
